@@ -1,0 +1,204 @@
+#!/usr/bin/env python3
+"""Benchmark: 16-D SD-KDE Gpair-evals/s on one MI355X.
+
+One step = the plain-FP32 SD-KDE pass at n = 1,048,576 training points drawn
+from an 8-component isotropic Gaussian mixture (means ~ N(0, 9 I), sigma 1,
+seed 0) and m = 65,536 queries from the same mixture (seed 1):
+  1. empirical-score accumulators over all n^2 ordered pairs (symmetric kernel,
+     each unordered pair evaluated once),
+  2. FP64 debias x + (h^2/2) score,
+  3. KDE of the m queries against the n debiased points.
+value = (m n + n^2) / step time, inputs resident in HBM.
+
+Prints ONE JSON line. --gpus N with N > 1 exits: this build measures one GPU.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "flash-sd-kde_amd"))
+
+METRIC = "SD-KDE 16D Gpair-evals/s at n=1M vs FP32 plain, at max rel err ≤1e-3 vs FP64"
+PEAK_FP32_MFMA_TFLOPS = 157.3
+# vector issue rates (whole chip) used for the exp/VALU roofline: 256 CU x 2.4 GHz,
+# 64 lanes/clk/CU for plain FP32 VALU ops, 16 lanes/clk/CU for transcendental (v_exp_f32).
+R_VALU = 256 * 2.4e9 * 64
+R_EXP = 256 * 2.4e9 * 16
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=1 << 20)
+    ap.add_argument("--m", type=int, default=65536)
+    ap.add_argument("--variant", choices=["symmetric", "ordered"], default="symmetric")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    return ap.parse_args()
+
+
+def cpu_baseline(x, q, h, budget_s):
+    """Time the CPU oracle (numpy FP32 restatement of the reference's
+    reference.py) on a bounded sample of the same workload: blocks of 64 score
+    rows and 64 KDE queries against all n points, spread over a thread pool
+    (BLAS pinned to one thread per worker), until the time budget is spent."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    from threadpoolctl import threadpool_limits
+
+    from oracle import reference_np as R
+
+    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    blk = 64
+    nblocks = threads * 64
+    pairs = 0
+    t0 = time.perf_counter()
+    with threadpool_limits(limits=1):
+        with ThreadPoolExecutor(threads) as ex:
+            def work(b):
+                if time.perf_counter() - t0 > budget_s:
+                    return 0
+                a = (b * blk) % (x.shape[0] - blk)
+                if b % 2 == 0:
+                    R.score_rows(x[a:a + blk], x, h, block=blk)
+                else:
+                    qa = (b * blk) % max(1, q.shape[0] - blk)
+                    R.kde_sums_nd(q[qa:qa + blk], x, h, block=blk)
+                return blk * x.shape[0]
+            for p in ex.map(work, range(nblocks)):
+                pairs += p
+    dt = time.perf_counter() - t0
+    return {
+        "value": pairs / dt / 1e9,
+        "unit": "Gpair-evals/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"{pairs / (blk * x.shape[0]):.0f} blocks of {blk} rows (score rows and KDE "
+                   f"queries alternating) x all n={x.shape[0]} points, {pairs:.3g} pairs in "
+                   f"{dt:.1f} s; numpy FP32 oracle, {threads} threads, BLAS 1 thread each"),
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (--gpus {args.gpus} requested): this build has "
+              "no multi-GPU path; run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from flash_sd_kde import _native, _ops
+    from flash_sd_kde.bandwidth import scott_mean_std_bandwidth
+    from flash_sd_kde.datasets import gmm_isotropic_split
+
+    dev = torch.device("cuda:0")
+    _native.require_gpu(dev, "bench")
+    n, m = args.n, args.m
+    x, q, _ = gmm_isotropic_split(n, m, k=8, mean_scale=3.0, seed=0)
+    h = scott_mean_std_bandwidth(x)
+    xd = torch.as_tensor(x, device=dev)
+    qd = torch.as_tensor(q, device=dev)
+
+    def step():
+        pdf, w = _ops.score16(xd, h, variant=args.variant)
+        deb = _ops.debias(xd, pdf, w, h, 1e-12)
+        return _ops.kde16_sums(deb, qd, h)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+
+    lib = _native.lib()
+    lib.fsk_profile_enable(1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        dens = step()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    op_score = _native.OP_SCORE16_SYM if args.variant == "symmetric" else _native.OP_SCORE16_ORDERED
+    score_ms = lib.fsk_profile_last_ms(op_score)
+    kde_ms = lib.fsk_profile_last_ms(_native.OP_KDE16_SPLITK)
+    lib.fsk_profile_enable(0)
+
+    ms_per_step = dt / args.steps * 1e3
+    pairs = float(m) * n + float(n) * n
+    value = pairs / (dt / args.steps) / 1e9
+
+    # dominant kernel: the score pass. Algorithmic MFMA work per launch
+    # (symmetric): n(n+1)/2 unique pairs x 96 flop (cross term 2d, phi X both
+    # ways 4d); ordered: n^2 x 64 flop.
+    if args.variant == "symmetric":
+        units = n * (n + 1) / 2.0
+        flops = units * 96.0
+        exp_valu_s = units / R_EXP + units * 5.0 / R_VALU
+    else:
+        units = float(n) * n
+        flops = units * 64.0
+        exp_valu_s = units / R_EXP + units * 4.0 / R_VALU
+    achieved = flops / (score_ms * 1e-3) / 1e12 if score_ms > 0 else None
+    kde_flops = float(m) * n * 32.0
+    kde_tflops = kde_flops / (kde_ms * 1e-3) / 1e12 if kde_ms > 0 else None
+    kde_exp_valu_s = float(m) * n / R_EXP + float(m) * n * 3.0 / R_VALU
+
+    finite = bool(torch.isfinite(dens).all().item())
+    line = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "Gpair-evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"sdkde16_gmm8_n{n}_m{m}",
+            "n_train": n,
+            "n_query": m,
+            "bandwidth": h,
+            "bandwidth_rule": "mean_std*(4/(d+2))^(1/(d+4))*n^(-1/(d+4))",
+            "score_kernel": args.variant,
+            "precision": "fp32 (MFMA f32 cross term and phi.X, FP64 combine/debias)",
+            "parallelism": "single_gpu",
+        },
+        "roofline": {
+            "bound": "mfma",
+            "kernel": f"score16_{args.variant}",
+            "achieved": round(achieved, 3) if achieved else None,
+            "peak": PEAK_FP32_MFMA_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": round(achieved / PEAK_FP32_MFMA_TFLOPS, 4) if achieved else None,
+            "traffic": None,
+            "kernel_ms": round(score_ms, 3),
+            "exp_valu_frac": round(exp_valu_s / (score_ms * 1e-3), 4) if score_ms > 0 else None,
+        },
+        "kde_eval_kernel": {
+            "kernel_ms": round(kde_ms, 3),
+            "mfma_tflops": round(kde_tflops, 3) if kde_tflops else None,
+            "mfma_frac": round(kde_tflops / PEAK_FP32_MFMA_TFLOPS, 4) if kde_tflops else None,
+            "exp_valu_frac": round(kde_exp_valu_s / (kde_ms * 1e-3), 4) if kde_ms > 0 else None,
+        },
+        "outputs_finite": finite,
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(x, q, h, args.cpu_seconds)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,103 @@
+// fsk_common.h - shared device/host helpers for the gfx950 kernels.
+//
+// MFMA operand layout used throughout (v_mfma_f32_16x16x4_f32, one block):
+//   lane l, g = l / 16, c = l % 16
+//   A (16 x 4):  lane holds A[row = c][k = g]
+//   B (4 x 16):  lane holds B[k = g][col = c]
+//   D (16 x 16): lane holds D[row = 4 g + r][col = c], r = 0..3
+// The pair kernels put data points j on the A rows and queries i on the B
+// columns, so a lane's four D values belong to ONE query (i = c) and four data
+// points (j = 4 g + r): the row sum over data stays inside the lane until the
+// final cross-lane reduction.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+#include <math.h>
+
+#include "flashsdkde.h"
+
+namespace fsk {
+
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+constexpr int kD = 16;        // feature count of the N-D path
+constexpr int kTile = 16;     // MFMA tile edge
+constexpr int kWG = 256;      // threads per workgroup (4 wave64)
+constexpr int kXCD = 8;       // MI355X: 8 accelerator complex dies
+constexpr double kLog2e = 1.4426950408889634;
+constexpr double kLn2 = 0.6931471805599453;
+// c|x|^2 of padded tail points: exp2 of anything this negative is exactly 0,
+// and 0 * (finite) stays finite (no inf - inf or 0 * inf).
+constexpr float kPadNeg = -1.0e30f;
+// Norm arrays are padded to this many rows so any tile / block read is in bounds.
+constexpr int64_t kNormPad = 256;
+
+// ---- error plumbing (host) ----
+void set_error(const char *fmt, ...);
+int launch_status(const char *what);  // FSK_OK or FSK_ELAUNCH after a launch
+int check_points16(const char *what, const float *p, int64_t ld, int64_t rows);
+
+inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
+inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
+inline int64_t round_up(int64_t a, int64_t b) { return cdiv(a, b) * b; }
+inline size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+
+// exp2 scale: exp(-d2 / (2 h^2)) == exp2(c * d2) with c = -log2(e) / (2 h^2)
+inline double exp2_coef(double h) { return -kLog2e / (2.0 * h * h); }
+
+// Map a linear block id to a task so that the blocks one XCD runs at the same
+// time get consecutive tasks (blocks are dealt round-robin to the 8 XCDs).
+__device__ __forceinline__ int64_t xcd_task(int64_t b, int64_t per_xcd) {
+  return (b % kXCD) * per_xcd + (b / kXCD);
+}
+inline int64_t xcd_grid(int64_t tasks, int64_t *per_xcd) {
+  *per_xcd = cdiv(tasks, kXCD);
+  return *per_xcd * kXCD;
+}
+
+__device__ __forceinline__ f4 mfma_f32(float a, float b, f4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ float fexp2(float x) { return __builtin_amdgcn_exp2f(x); }
+
+__device__ __forceinline__ f4 ld4(const float *p) {
+  return *reinterpret_cast<const f4 *>(p);
+}
+
+// Sum over the four lane groups (g) of a wave; every lane gets the total of
+// lanes {c, c+16, c+32, c+48}.
+__device__ __forceinline__ float sum_groups(float v) {
+  v += __shfl_xor(v, 16);
+  v += __shfl_xor(v, 32);
+  return v;
+}
+__device__ __forceinline__ double sum_groups(double v) {
+  v += __shfl_xor(v, 16);
+  v += __shfl_xor(v, 32);
+  return v;
+}
+
+// Optional event timing around the main kernel of an op (see fsk_profile_enable).
+void prof_begin(int op, hipStream_t s);
+void prof_end(int op, hipStream_t s);
+
+// Workspace sizes of the individual operations (defined next to each op).
+size_t kde16_ws_bytes(int64_t n, int64_t m, bool with_partials);
+size_t kde16_lse_ws_bytes(int64_t n, int64_t m, unsigned flags);
+size_t score16_ws_bytes(int64_t n, bool ordered);
+size_t kde1_ws_bytes(int64_t n, int64_t m, bool partials, bool two);
+
+// Fill kernel used instead of hipMemsetAsync (safe under graph capture).
+__global__ void fill_f32(float *p, int64_t n, float v);
+__global__ void fill_f64(double *p, int64_t n, double v);
+
+// Per-row scaled squared norms: out[j] = coef * |x_j|^2 (FP64 accumulation),
+// rows [n, n_pad) get kPadNeg so padded tail points contribute exactly 0.
+__global__ void prep_norms16(const float *__restrict__ x, int64_t ld, int64_t n,
+                             int64_t n_pad, double coef, float *__restrict__ out);
+
+}  // namespace fsk

@@ -1,0 +1,215 @@
+// fsk_kde16.hip - 16-D Gaussian KDE sums on MFMA (split-K and atomic forms).
+//
+// Per pair (i, j) the kernel evaluates
+//     t_ij = c |q_i|^2 + c |x_j|^2 - 2 c q_i . x_j,   c = -log2(e) / (2 h^2)
+//     phi  = exp2(min(t_ij, 0))
+// which is exp(-max(|q|^2 + |x|^2 - 2 q.x, 0) / (2 h^2)), the reference's
+// clamped norm expansion (kernels/kde_eval_16d_stream_splitk.py:71-75).
+// The two norm terms initialise the MFMA accumulator (one VALU add), the
+// cross term is four chained v_mfma_f32_16x16x4_f32 with the query side
+// pre-scaled by -2c, and min / exp2 / add finish the pair on the VALU.
+// Padded tail points carry c|x|^2 = -1e30, so they contribute exactly 0.
+#include "fsk_common.h"
+#include "fsk_plan.h"
+
+namespace fsk {
+
+constexpr int kFlushTiles = 32;  // FP32 inner sums of 4*32 terms, then FP64
+
+// One workgroup = 4 waves; a wave owns QT row tiles of 16 queries and streams
+// the data range of its split in tiles of 16 points.
+template <int QT, bool LAPL, bool ATOMIC>
+__global__ __launch_bounds__(kWG) void kde16_f32_kernel(
+    const float *__restrict__ data, int64_t ld, int64_t n, const float *__restrict__ bn,
+    int64_t n_pad, const float *__restrict__ query, int64_t ldq, int64_t m,
+    const float *__restrict__ qa, float qscale, int64_t split_len, int64_t qblocks,
+    int64_t ntasks, int64_t per_xcd, float *__restrict__ out, int64_t ld_out) {
+  const int64_t task = xcd_task(blockIdx.x, per_xcd);
+  if (task >= ntasks) return;
+  const int64_t split = task / qblocks;
+  const int64_t qb = task - split * qblocks;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int g = lane >> 4, c = lane & 15;
+  const int64_t row0 = qb * (64 * QT) + int64_t(wave) * (16 * QT);
+
+  f4 qv[QT];
+  float qav[QT], acc[QT], accT[QT];
+  double acc64[QT], accT64[QT];
+#pragma unroll
+  for (int R = 0; R < QT; ++R) {
+    int64_t i = row0 + 16 * R + c;
+    if (i > m - 1) i = m - 1;
+    qv[R] = ld4(query + i * ldq + 4 * g) * qscale;
+    qav[R] = qa[i];
+    acc[R] = 0.f;
+    accT[R] = 0.f;
+    acc64[R] = 0.0;
+    accT64[R] = 0.0;
+  }
+
+  const int64_t j_begin = split * split_len;
+  int64_t j_end = j_begin + split_len;
+  if (j_end > n_pad) j_end = n_pad;
+  int tc = 0;
+  for (int64_t j0 = j_begin; j0 < j_end; j0 += kTile) {
+    int64_t jr = j0 + c;
+    if (jr > n - 1) jr = n - 1;
+    const f4 xa = ld4(data + jr * ld + 4 * g);
+    const f4 bb = ld4(bn + j0 + 4 * g);
+    f4 d[QT];
+#pragma unroll
+    for (int R = 0; R < QT; ++R) {
+      d[R][0] = qav[R] + bb[0];
+      d[R][1] = qav[R] + bb[1];
+      d[R][2] = qav[R] + bb[2];
+      d[R][3] = qav[R] + bb[3];
+    }
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+#pragma unroll
+      for (int R = 0; R < QT; ++R) d[R] = mfma_f32(xa[s], qv[R][s], d[R]);
+    }
+#pragma unroll
+    for (int R = 0; R < QT; ++R) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        float t = fminf(d[R][r], 0.f);
+        float e = fexp2(t);
+        acc[R] += e;
+        if (LAPL) accT[R] = fmaf(e, t, accT[R]);
+      }
+    }
+    if (++tc == kFlushTiles) {
+      tc = 0;
+#pragma unroll
+      for (int R = 0; R < QT; ++R) {
+        acc64[R] += acc[R];
+        acc[R] = 0.f;
+        if (LAPL) {
+          accT64[R] += accT[R];
+          accT[R] = 0.f;
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int R = 0; R < QT; ++R) {
+    double v = acc64[R] + acc[R];
+    if (LAPL) v = 9.0 * v + kLn2 * (accT64[R] + accT[R]);
+    v = sum_groups(v);
+    int64_t i = row0 + 16 * R + c;
+    if (g == 0 && i < m) {
+      if (ATOMIC)
+        atomicAdd(out + i, float(v));
+      else
+        out[split * ld_out + i] = float(v);
+    }
+  }
+}
+
+struct Kde16Layout {
+  float *bn;
+  float *qa;
+  float *partials;
+};
+
+static size_t kde16_layout(void *ws, int64_t n, int64_t m, bool with_partials,
+                           Kde16Layout *L) {
+  Carve cv(ws);
+  SplitPlan p = kde16_plan(n, m);
+  L->bn = cv.take<float>(round_up(n, kNormPad));
+  L->qa = cv.take<float>(m);
+  L->partials = with_partials ? cv.take<float>(p.splits * m) : nullptr;
+  return cv.used;
+}
+
+size_t kde16_ws_bytes(int64_t n, int64_t m, bool with_partials) {
+  Kde16Layout L;
+  return kde16_layout(nullptr, n, m, with_partials, &L);
+}
+
+static int kde16_run(const char *what, bool atomic, const float *data, int64_t ld, int64_t n,
+                     const float *query, int64_t ldq, int64_t m, double h, unsigned flags,
+                     void *ws, size_t ws_bytes, float *out, void *stream) {
+  int st;
+  if ((st = check_points16(what, data, ld, n)) != FSK_OK) return st;
+  if ((st = check_points16(what, query, ldq, m)) != FSK_OK) return st;
+  if (!(h > 0.0) || !isfinite(h)) {
+    set_error("%s: bandwidth must be positive, got %g", what, h);
+    return FSK_EINVAL;
+  }
+  if (!out) {
+    set_error("%s: null output", what);
+    return FSK_EINVAL;
+  }
+  const unsigned prec = flags & FSK_PREC_MASK;
+  if (prec != FSK_PREC_F32) {
+    set_error("%s: precision %u not available on this path", what, prec);
+    return FSK_EUNSUPPORTED;
+  }
+  const bool lapl = (flags & FSK_FLAG_LAPLACIAN) != 0;
+  Kde16Layout L;
+  size_t need = kde16_layout(ws, n, m, !atomic, &L);
+  if (!ws || ws_bytes < need) {
+    set_error("%s: workspace of %zu bytes needed, got %zu", what, need, ws_bytes);
+    return FSK_EWORKSPACE;
+  }
+  hipStream_t s = as_stream(stream);
+  const double coef = exp2_coef(h);
+  const int64_t n_pad = round_up(n, kTile);
+  hipLaunchKernelGGL(prep_norms16, dim3(cdiv(round_up(n, kNormPad), 256)), dim3(256), 0, s,
+                     data, ld, n, round_up(n, kNormPad), coef, L.bn);
+  hipLaunchKernelGGL(prep_norms16, dim3(cdiv(m, 256)), dim3(256), 0, s, query, ldq, m, m,
+                     coef, L.qa);
+  SplitPlan p = kde16_plan(n, m);
+  const int64_t ntasks = p.qblocks * p.splits;
+  int64_t per_xcd;
+  const int64_t grid = xcd_grid(ntasks, &per_xcd);
+  const float qscale = float(-2.0 * coef);
+  const int op = atomic ? FSK_OP_KDE16_ATOMIC : FSK_OP_KDE16_SPLITK;
+  if (atomic) {
+    hipLaunchKernelGGL(fill_f32, dim3(cdiv(m, 256)), dim3(256), 0, s, out, m, 0.f);
+    prof_begin(op, s);
+    if (lapl)
+      hipLaunchKernelGGL((kde16_f32_kernel<kKdeQT, true, true>), dim3(grid), dim3(kWG), 0, s,
+                         data, ld, n, L.bn, n_pad, query, ldq, m, L.qa, qscale, p.split_len,
+                         p.qblocks, ntasks, per_xcd, out, int64_t(0));
+    else
+      hipLaunchKernelGGL((kde16_f32_kernel<kKdeQT, false, true>), dim3(grid), dim3(kWG), 0, s,
+                         data, ld, n, L.bn, n_pad, query, ldq, m, L.qa, qscale, p.split_len,
+                         p.qblocks, ntasks, per_xcd, out, int64_t(0));
+    prof_end(op, s);
+    return launch_status(what);
+  }
+  prof_begin(op, s);
+  if (lapl)
+    hipLaunchKernelGGL((kde16_f32_kernel<kKdeQT, true, false>), dim3(grid), dim3(kWG), 0, s,
+                       data, ld, n, L.bn, n_pad, query, ldq, m, L.qa, qscale, p.split_len,
+                       p.qblocks, ntasks, per_xcd, L.partials, m);
+  else
+    hipLaunchKernelGGL((kde16_f32_kernel<kKdeQT, false, false>), dim3(grid), dim3(kWG), 0, s,
+                       data, ld, n, L.bn, n_pad, query, ldq, m, L.qa, qscale, p.split_len,
+                       p.qblocks, ntasks, per_xcd, L.partials, m);
+  prof_end(op, s);
+  if ((st = launch_status(what)) != FSK_OK) return st;
+  return fsk_reduce_partials_f64(L.partials, p.splits, m, m, out, stream);
+}
+
+}  // namespace fsk
+
+using namespace fsk;
+
+extern "C" int fsk_kde16_splitk(const float *data, int64_t ld, int64_t n, const float *query,
+                                int64_t ldq, int64_t m, double bandwidth, unsigned flags,
+                                void *ws, size_t ws_bytes, float *out, void *stream) {
+  return kde16_run("fsk_kde16_splitk", false, data, ld, n, query, ldq, m, bandwidth, flags, ws,
+                   ws_bytes, out, stream);
+}
+
+extern "C" int fsk_kde16_atomic(const float *data, int64_t ld, int64_t n, const float *query,
+                                int64_t ldq, int64_t m, double bandwidth, unsigned flags,
+                                void *ws, size_t ws_bytes, float *out, void *stream) {
+  return kde16_run("fsk_kde16_atomic", true, data, ld, n, query, ldq, m, bandwidth, flags, ws,
+                   ws_bytes, out, stream);
+}

@@ -1,0 +1,551 @@
+// fsk_kde16_lse.hip - 16-D log-density by log-sum-exp, with a choice of
+// cross-term precision (the precision form).
+//
+// Data and queries are first centred on the data mean (distances are
+// translation invariant; centring shrinks |x|^2 and with it the cancellation
+// in |q|^2 + |x|^2 - 2 q.x). In base 2, with c = -log2(e)/(2h^2),
+//     t_ij = c|q_i|^2 + c|x_j|^2 - 2c q_i.x_j  (<= 0 up to rounding)
+// and each (query, split) keeps a running maximum m_i (shared by the four
+// lanes that hold query i) and s_i = sum_j exp2(min(t_ij, 0) - m_i). The
+// maximum is found exactly on the first tile of the split and only ever
+// raised afterwards (rare, wave-uniform branch). Partials (m, s) are combined
+// in FP64: log p_i = ln2 (M + log2 sum_s s_s 2^(m_s - M)) + log_const.
+//
+// 16-bit forms use v_mfma_f32_16x16x32_{bf16,f16}. K = 32 holds the 16
+// features plus a "fold" block that adds the norm terms and -m_i inside the
+// MFMA, so no per-pair VALU add is needed:
+//   data  record (48 x 16-bit): [ xh[16] | xl[16] | 1 1 1 bh bm bl 1 1 1 0... ]
+//   query record (48 x 16-bit): [ qh[16] | ql[16] | ah am al 1 1 1 mh mm ml 0... ]
+// (x = xh + xl, q' = -2c q = qh + ql, a = c|q|^2, b = c|x|^2 each split into
+// three 16-bit pieces; -m_i is split into three pieces and patched in place).
+//   BF16 / F16 : one MFMA,  A = [xh | fold_b],  B = [qh | fold_a]
+//   BF16X3     : two MFMAs, A1 = [xh | xl] x B1 = [qh | qh]  and
+//                           A2 = [xh | fold_b] x B2 = [ql | fold_a]
+//                i.e. xh.qh + xl.qh + xh.ql + norms - m  (the lo.lo term dropped)
+// FP32 forms: MFMA f32 16x16x4 with the norm terms (and -m) in the
+// accumulator initialiser, or plain FP32 FMAs on the VALU.
+#include "fsk_common.h"
+#include "fsk_plan.h"
+
+namespace fsk {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 h16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
+
+constexpr int kRec = 48;            // 16-bit values per packed record
+constexpr float kLseSlack = 8.0f;   // raise m only when a term exceeds 2^8 * 2^m
+constexpr int kLseFlush = 32;
+
+// ---- 16-bit helpers (round to nearest even) ----
+template <int PREC>
+__device__ __forceinline__ unsigned short to16(float v);
+template <>
+__device__ __forceinline__ unsigned short to16<FSK_PREC_BF16>(float v) {
+  __bf16 b = (__bf16)v;
+  return __builtin_bit_cast(unsigned short, b);
+}
+template <>
+__device__ __forceinline__ unsigned short to16<FSK_PREC_F16>(float v) {
+  _Float16 h = (_Float16)v;
+  return __builtin_bit_cast(unsigned short, h);
+}
+template <int PREC>
+__device__ __forceinline__ float from16(unsigned short u);
+template <>
+__device__ __forceinline__ float from16<FSK_PREC_BF16>(unsigned short u) {
+  return float(__builtin_bit_cast(__bf16, u));
+}
+template <>
+__device__ __forceinline__ float from16<FSK_PREC_F16>(unsigned short u) {
+  return float(__builtin_bit_cast(_Float16, u));
+}
+// Split v into three 16-bit pieces (hi, mid, lo) whose sum approximates v.
+template <int P16>
+__device__ __forceinline__ void split3(double v, unsigned short *p) {
+  float r = float(v);
+  p[0] = to16<P16>(r);
+  double rem = v - double(from16<P16>(p[0]));
+  p[1] = to16<P16>(float(rem));
+  rem -= double(from16<P16>(p[1]));
+  p[2] = to16<P16>(float(rem));
+}
+
+__global__ void mean16_kernel(const float *__restrict__ x, int64_t ld, int64_t n,
+                              double *__restrict__ sum) {
+  __shared__ double red[256][kD + 1];
+  double acc[kD];
+#pragma unroll
+  for (int f = 0; f < kD; ++f) acc[f] = 0.0;
+  for (int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; j < n;
+       j += int64_t(gridDim.x) * blockDim.x) {
+#pragma unroll
+    for (int f = 0; f < kD; f += 4) {
+      f4 v = ld4(x + j * ld + f);
+      acc[f] += v[0];
+      acc[f + 1] += v[1];
+      acc[f + 2] += v[2];
+      acc[f + 3] += v[3];
+    }
+  }
+#pragma unroll
+  for (int f = 0; f < kD; ++f) red[threadIdx.x][f] = acc[f];
+  __syncthreads();
+  if (threadIdx.x < kD) {
+    double s = 0.0;
+    for (int t = 0; t < 256; ++t) s += red[t][threadIdx.x];
+    atomicAdd(&sum[threadIdx.x], s);
+  }
+}
+
+// Centred, scaled operands. side 0 = data (A side), 1 = query (B side).
+// FP32 forms write float rows (data: x - mu, query: -2c (q - mu)) plus the
+// norm term; 16-bit forms write 48-value records. Rows in [rows, rows_pad)
+// are padding: zero features and a norm term that drives exp2 to 0.
+template <int PREC>
+__global__ void prep_lse_kernel(const float *__restrict__ x, int64_t ld, int64_t rows,
+                                int64_t rows_pad, const double *__restrict__ mu_sum, double inv_n,
+                                double coef, int side, float *__restrict__ f32_out,
+                                float *__restrict__ norm_out,
+                                unsigned short *__restrict__ rec_out) {
+  int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (j >= rows_pad) return;
+  const bool pad = j >= rows;
+  double xc[kD];
+  double nrm = 0.0;
+#pragma unroll
+  for (int f = 0; f < kD; ++f) {
+    double v = pad ? 0.0 : double(x[j * ld + f]) - mu_sum[f] * inv_n;
+    xc[f] = v;
+    nrm += v * v;
+  }
+  const double nt = coef * nrm;  // c |x - mu|^2
+  const double sc = side == 0 ? 1.0 : -2.0 * coef;
+  if (PREC == FSK_PREC_F32 || PREC == FSK_PREC_F32_VALU) {
+#pragma unroll
+    for (int f = 0; f < kD; ++f) f32_out[j * kD + f] = float(sc * xc[f]);
+    norm_out[j] = pad ? kPadNeg : float(nt);
+    return;
+  }
+  constexpr int P16 = (PREC == FSK_PREC_F16) ? FSK_PREC_F16 : FSK_PREC_BF16;
+  unsigned short r[kRec];
+#pragma unroll
+  for (int f = 0; f < kD; ++f) {
+    float v = float(sc * xc[f]);
+    r[f] = to16<P16>(v);
+    r[kD + f] = to16<P16>(v - from16<P16>(r[f]));
+  }
+  const unsigned short one = to16<P16>(1.0f), zero = to16<P16>(0.0f);
+#pragma unroll
+  for (int k = 0; k < kD; ++k) r[2 * kD + k] = zero;
+  unsigned short piece[3];
+  if (side == 0) {
+    if (pad) {
+      piece[0] = to16<P16>(P16 == FSK_PREC_F16 ? -60000.0f : kPadNeg);
+      piece[1] = piece[2] = zero;
+    } else {
+      split3<P16>(nt, piece);
+    }
+    r[32] = r[33] = r[34] = one;
+    r[35] = piece[0];
+    r[36] = piece[1];
+    r[37] = piece[2];
+    r[38] = r[39] = r[40] = one;
+  } else {
+    split3<P16>(nt, piece);
+    r[32] = piece[0];
+    r[33] = piece[1];
+    r[34] = piece[2];
+    r[35] = r[36] = r[37] = one;
+    // r[38..40] = pieces of -m, patched inside the kernel
+  }
+#pragma unroll
+  for (int k = 0; k < kRec; ++k) rec_out[j * kRec + k] = r[k];
+}
+
+__device__ __forceinline__ f4 mfma16(u16x8 a, u16x8 b, f4 c, int prec) {
+  if (prec == FSK_PREC_F16)
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(h16x8, a),
+                                                  __builtin_bit_cast(h16x8, b), c, 0, 0, 0);
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a),
+                                                 __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
+}
+
+__device__ __forceinline__ float max4(f4 v) { return fmaxf(fmaxf(v[0], v[1]), fmaxf(v[2], v[3])); }
+__device__ __forceinline__ float gmax(float v) {
+  v = fmaxf(v, __shfl_xor(v, 16));
+  v = fmaxf(v, __shfl_xor(v, 32));
+  return v;
+}
+
+template <int PREC, int QT>
+__global__ __launch_bounds__(kWG) void kde16_lse_mfma_kernel(
+    const float *__restrict__ xf, const float *__restrict__ bn,
+    const unsigned short *__restrict__ xrec, int64_t n_pad, const float *__restrict__ qf,
+    const float *__restrict__ qa, const unsigned short *__restrict__ qrec, int64_t m,
+    int64_t split_len, int64_t qblocks, int64_t ntasks, int64_t per_xcd,
+    float2 *__restrict__ part) {
+  constexpr bool F32 = (PREC == FSK_PREC_F32);
+  constexpr int P16 = (PREC == FSK_PREC_F16) ? FSK_PREC_F16 : FSK_PREC_BF16;
+  const int64_t task = xcd_task(blockIdx.x, per_xcd);
+  if (task >= ntasks) return;
+  const int64_t split = task / qblocks;
+  const int64_t qb = task - split * qblocks;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int g = lane >> 4, c = lane & 15;
+  const int64_t row0 = qb * (64 * QT) + int64_t(wave) * (16 * QT);
+
+  f4 qv[QT];
+  u16x8 B1[QT], B2[QT];
+  float qav[QT], negm[QT], s[QT];
+  double s64[QT];
+  double mrec[QT];  // exactly the m subtracted inside the MFMA
+#pragma unroll
+  for (int R = 0; R < QT; ++R) {
+    int64_t i = row0 + 16 * R + c;
+    if (i > m - 1) i = m - 1;
+    if (F32) {
+      qv[R] = ld4(qf + i * kD + 4 * g);
+      qav[R] = qa[i];
+    } else {
+      const unsigned short *rq = qrec + i * kRec;
+      B1[R] = *reinterpret_cast<const u16x8 *>(rq + 8 * (g & 1));
+      B2[R] = *reinterpret_cast<const u16x8 *>(rq + 16 + 8 * g);
+      if (PREC != FSK_PREC_BF16X3)
+        B1[R] = *reinterpret_cast<const u16x8 *>(rq + (g < 2 ? 8 * g : 16 + 8 * g));
+    }
+    negm[R] = 0.f;
+    mrec[R] = 0.0;
+    s[R] = 0.f;
+    s64[R] = 0.0;
+  }
+
+  auto tile = [&](int64_t j0, f4 *d) {
+    if (F32) {
+      const f4 xa = ld4(xf + (j0 + c) * kD + 4 * g);
+      const f4 bb = ld4(bn + j0 + 4 * g);
+#pragma unroll
+      for (int R = 0; R < QT; ++R) {
+        d[R][0] = qav[R] + bb[0];
+        d[R][1] = qav[R] + bb[1];
+        d[R][2] = qav[R] + bb[2];
+        d[R][3] = qav[R] + bb[3];
+      }
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+#pragma unroll
+        for (int R = 0; R < QT; ++R) d[R] = mfma_f32(xa[k], qv[R][k], d[R]);
+      }
+    } else {
+      const unsigned short *rx = xrec + (j0 + c) * kRec;
+      const u16x8 A2 = *reinterpret_cast<const u16x8 *>(rx + (g < 2 ? 8 * g : 16 + 8 * g));
+      if (PREC == FSK_PREC_BF16X3) {
+        const u16x8 A1 = *reinterpret_cast<const u16x8 *>(rx + 8 * g);
+#pragma unroll
+        for (int R = 0; R < QT; ++R) {
+          d[R] = mfma16(A1, B1[R], f4{0.f, 0.f, 0.f, 0.f}, P16);
+          d[R] = mfma16(A2, B2[R], d[R], P16);
+        }
+      } else {
+#pragma unroll
+        for (int R = 0; R < QT; ++R) d[R] = mfma16(A2, B1[R], f4{0.f, 0.f, 0.f, 0.f}, P16);
+      }
+    }
+  };
+
+  // Install a new maximum for query row tile R (uniform over the 4 lanes of a query).
+  auto set_m = [&](int R, float mnew) {
+    if (F32) {
+      const int64_t i0 = row0 + 16 * R + c;
+      const float base = qa[i0 > m - 1 ? m - 1 : i0];
+      qav[R] = base - mnew;
+      mrec[R] = double(mnew);
+      negm[R] = -mnew;
+    } else {
+      unsigned short pc[3];
+      split3<P16>(-double(mnew), pc);
+      double mr = -(double(from16<P16>(pc[0])) + double(from16<P16>(pc[1])) +
+                    double(from16<P16>(pc[2])));
+      // record slots 38, 39 live in lane group 2 (elements 6, 7), slot 40 in group 3 (element 0)
+      if (PREC == FSK_PREC_BF16X3) {
+        if (g == 2) {
+          B2[R][6] = pc[0];
+          B2[R][7] = pc[1];
+        }
+        if (g == 3) B2[R][0] = pc[2];
+      } else {
+        if (g == 2) {
+          B1[R][6] = pc[0];
+          B1[R][7] = pc[1];
+        }
+        if (g == 3) B1[R][0] = pc[2];
+      }
+      mrec[R] = mr;
+      negm[R] = float(-mr);
+    }
+  };
+
+  const int64_t j_begin = split * split_len;
+  int64_t j_end = j_begin + split_len;
+  if (j_end > n_pad) j_end = n_pad;
+
+  f4 d[QT];
+  // First tile: exact per-query maximum of this split's first 16 points.
+  tile(j_begin, d);
+#pragma unroll
+  for (int R = 0; R < QT; ++R) {
+    float mx = fminf(gmax(max4(d[R])), 0.f);
+    set_m(R, mx);
+    const float dm = float(mrec[R]);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) s[R] += fexp2(fminf(d[R][r] - dm, negm[R]));
+  }
+  int tc = 1;
+  for (int64_t j0 = j_begin + kTile; j0 < j_end; j0 += kTile) {
+    tile(j0, d);
+    bool hi = false;
+#pragma unroll
+    for (int R = 0; R < QT; ++R) hi |= max4(d[R]) > kLseSlack;
+    if (__any(hi)) {
+#pragma unroll
+      for (int R = 0; R < QT; ++R) {
+        float up = gmax(max4(d[R]));
+        if (up > 0.f) {
+          // raise m by `up` (bounded by the clamp t <= 0)
+          float mnew = fminf(float(mrec[R]) + up, 0.f);
+          double delta = double(mnew) - mrec[R];
+          float old = float(mrec[R]);
+          set_m(R, mnew);
+          const float sc = fexp2(float(-(mrec[R] - double(old))));
+          s[R] *= sc;
+          s64[R] *= double(sc);
+          const float dd = float(mrec[R] - double(old));
+          d[R] -= dd;
+          (void)delta;
+        }
+      }
+    }
+#pragma unroll
+    for (int R = 0; R < QT; ++R) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) s[R] += fexp2(fminf(d[R][r], negm[R]));
+    }
+    if (++tc == kLseFlush) {
+      tc = 0;
+#pragma unroll
+      for (int R = 0; R < QT; ++R) {
+        s64[R] += s[R];
+        s[R] = 0.f;
+      }
+    }
+  }
+#pragma unroll
+  for (int R = 0; R < QT; ++R) {
+    double v = sum_groups(s64[R] + s[R]);
+    int64_t i = row0 + 16 * R + c;
+    if (g == 0 && i < m) part[split * m + i] = make_float2(float(mrec[R]), float(v));
+  }
+}
+
+// FP32 cross term on the VALU: one thread per query, data staged in LDS.
+constexpr int kValuChunk = 128;
+__global__ __launch_bounds__(256) void kde16_lse_valu_kernel(
+    const float *__restrict__ xf, const float *__restrict__ bn, int64_t n_pad,
+    const float *__restrict__ qf, const float *__restrict__ qa, int64_t m, int64_t split_len,
+    int64_t qblocks, int64_t ntasks, int64_t per_xcd, float2 *__restrict__ part) {
+  __shared__ float xs[kValuChunk][kD + 1];
+  __shared__ float bs[kValuChunk];
+  const int64_t task = xcd_task(blockIdx.x, per_xcd);
+  if (task >= ntasks) return;
+  const int64_t split = task / qblocks;
+  const int64_t qb = task - split * qblocks;
+  const int tid = threadIdx.x;
+  int64_t i = qb * 256 + tid;
+  const int64_t ic = i > m - 1 ? m - 1 : i;
+  float q[kD];
+#pragma unroll
+  for (int f = 0; f < kD; ++f) q[f] = qf[ic * kD + f];
+  const float a = qa[ic];
+  float mx = -INFINITY, s = 0.f;
+  double s64 = 0.0;
+  const int64_t j_begin = split * split_len;
+  int64_t j_end = j_begin + split_len;
+  if (j_end > n_pad) j_end = n_pad;
+  for (int64_t j0 = j_begin; j0 < j_end; j0 += kValuChunk) {
+    __syncthreads();
+    for (int e = tid; e < kValuChunk * kD; e += 256) {
+      int64_t j = j0 + e / kD;
+      xs[e / kD][e % kD] = j < n_pad ? xf[j * kD + (e % kD)] : 0.f;
+    }
+    for (int e = tid; e < kValuChunk; e += 256) bs[e] = (j0 + e) < n_pad ? bn[j0 + e] : kPadNeg;
+    __syncthreads();
+    for (int jj = 0; jj < kValuChunk; jj += 16) {
+      float t[16];
+      float tm = -INFINITY;
+#pragma unroll
+      for (int u = 0; u < 16; ++u) {
+        float acc = a + bs[jj + u];
+#pragma unroll
+        for (int f = 0; f < kD; ++f) acc = fmaf(q[f], xs[jj + u][f], acc);
+        t[u] = fminf(acc, 0.f);
+        tm = fmaxf(tm, t[u]);
+      }
+      if (tm > mx) {
+        float sc = fexp2(mx - tm);  // mx = -inf on the first group -> 0
+        s *= sc;
+        s64 *= double(sc);
+        mx = tm;
+      }
+#pragma unroll
+      for (int u = 0; u < 16; ++u) s += fexp2(t[u] - mx);
+    }
+    s64 += s;
+    s = 0.f;
+  }
+  if (i < m) part[split * m + i] = make_float2(mx, float(s64));
+}
+
+// FP64 combine of (m, s) partials into natural-log densities.
+__global__ void lse_combine_kernel(const float2 *__restrict__ part, int64_t S, int64_t m,
+                                   double log_const, double *__restrict__ out) {
+  int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  double M = -INFINITY;
+  for (int64_t s = 0; s < S; ++s) {
+    float2 p = part[s * m + i];
+    if (p.y > 0.f && double(p.x) > M) M = p.x;
+  }
+  double acc = 0.0;
+  for (int64_t s = 0; s < S; ++s) {
+    float2 p = part[s * m + i];
+    if (p.y > 0.f) acc += double(p.y) * exp2(double(p.x) - M);
+  }
+  out[i] = (acc > 0.0) ? kLn2 * (M + log2(acc)) + log_const : -INFINITY;
+}
+
+struct LseLayout {
+  double *mu;
+  float *xf, *bn, *qf, *qa;
+  unsigned short *xrec, *qrec;
+  float2 *part;
+};
+
+static bool lse_is16(unsigned prec) {
+  return prec == FSK_PREC_BF16 || prec == FSK_PREC_F16 || prec == FSK_PREC_BF16X3;
+}
+
+static size_t lse_layout(void *ws, int64_t n, int64_t m, unsigned flags, LseLayout *L) {
+  Carve cv(ws);
+  const unsigned prec = flags & FSK_PREC_MASK;
+  SplitPlan p = kde16_plan(n, m);
+  const int64_t n_pad = round_up(n, kValuChunk);
+  L->mu = cv.take<double>(kD);
+  L->xf = L->bn = L->qf = L->qa = nullptr;
+  L->xrec = L->qrec = nullptr;
+  if (lse_is16(prec)) {
+    L->xrec = cv.take<unsigned short>(n_pad * kRec);
+    L->qrec = cv.take<unsigned short>(m * kRec);
+  } else {
+    L->xf = cv.take<float>(n_pad * kD);
+    L->bn = cv.take<float>(n_pad);
+    L->qf = cv.take<float>(m * kD);
+    L->qa = cv.take<float>(m);
+  }
+  L->part = cv.take<float2>(p.splits * m);
+  return cv.used;
+}
+
+size_t kde16_lse_ws_bytes(int64_t n, int64_t m, unsigned flags) {
+  LseLayout L;
+  return lse_layout(nullptr, n, m, flags, &L);
+}
+
+}  // namespace fsk
+
+using namespace fsk;
+
+extern "C" int fsk_kde16_logdensity(const float *data, int64_t ld, int64_t n,
+                                    const float *query, int64_t ldq, int64_t m,
+                                    double bandwidth, unsigned flags, void *ws,
+                                    size_t ws_bytes, double *out, void *stream) {
+  const char *what = "fsk_kde16_logdensity";
+  int st;
+  if ((st = check_points16(what, data, ld, n)) != FSK_OK) return st;
+  if ((st = check_points16(what, query, ldq, m)) != FSK_OK) return st;
+  if (!(bandwidth > 0.0) || !isfinite(bandwidth)) {
+    set_error("%s: bandwidth must be positive, got %g", what, bandwidth);
+    return FSK_EINVAL;
+  }
+  if (!out) {
+    set_error("%s: null output", what);
+    return FSK_EINVAL;
+  }
+  const unsigned prec = flags & FSK_PREC_MASK;
+  if (prec > FSK_PREC_BF16X3 || (flags & ~FSK_PREC_MASK)) {
+    set_error("%s: flags 0x%x not available (Laplacian has no log form)", what, flags);
+    return FSK_EUNSUPPORTED;
+  }
+  LseLayout L;
+  size_t need = lse_layout(ws, n, m, flags, &L);
+  if (!ws || ws_bytes < need) {
+    set_error("%s: workspace of %zu bytes needed, got %zu", what, need, ws_bytes);
+    return FSK_EWORKSPACE;
+  }
+  hipStream_t s = as_stream(stream);
+  const double coef = exp2_coef(bandwidth);
+  const int64_t n_pad = round_up(n, kValuChunk);
+  hipLaunchKernelGGL(fill_f64, dim3(1), dim3(64), 0, s, L.mu, int64_t(kD), 0.0);
+  {
+    int64_t blocks = cdiv(n, 256);
+    if (blocks > 1024) blocks = 1024;
+    hipLaunchKernelGGL(mean16_kernel, dim3(blocks), dim3(256), 0, s, data, ld, n, L.mu);
+  }
+  const double inv_n = 1.0 / double(n);
+#define FSK_PREP(P)                                                                             \
+  do {                                                                                          \
+    hipLaunchKernelGGL(prep_lse_kernel<P>, dim3(cdiv(n_pad, 256)), dim3(256), 0, s, data, ld, n, \
+                       n_pad, L.mu, inv_n, coef, 0, L.xf, L.bn, L.xrec);                        \
+    hipLaunchKernelGGL(prep_lse_kernel<P>, dim3(cdiv(m, 256)), dim3(256), 0, s, query, ldq, m,  \
+                       m, L.mu, inv_n, coef, 1, L.qf, L.qa, L.qrec);                            \
+  } while (0)
+  switch (prec) {
+    case FSK_PREC_F32: FSK_PREP(FSK_PREC_F32); break;
+    case FSK_PREC_F32_VALU: FSK_PREP(FSK_PREC_F32_VALU); break;
+    case FSK_PREC_BF16: FSK_PREP(FSK_PREC_BF16); break;
+    case FSK_PREC_F16: FSK_PREP(FSK_PREC_F16); break;
+    default: FSK_PREP(FSK_PREC_BF16X3); break;
+  }
+#undef FSK_PREP
+  SplitPlan p = kde16_plan(n, m);
+  int64_t per_xcd;
+  prof_begin(FSK_OP_KDE16_LOGDENSITY, s);
+  if (prec == FSK_PREC_F32_VALU) {
+    const int64_t qblocks = cdiv(m, 256);
+    const int64_t ntasks = qblocks * p.splits;
+    const int64_t grid = xcd_grid(ntasks, &per_xcd);
+    hipLaunchKernelGGL(kde16_lse_valu_kernel, dim3(grid), dim3(256), 0, s, L.xf, L.bn,
+                       round_up(n, kTile), L.qf, L.qa, m, p.split_len, qblocks, ntasks, per_xcd,
+                       L.part);
+  } else {
+    const int64_t ntasks = p.qblocks * p.splits;
+    const int64_t grid = xcd_grid(ntasks, &per_xcd);
+#define FSK_LSE(P)                                                                         \
+  hipLaunchKernelGGL((kde16_lse_mfma_kernel<P, kKdeQT>), dim3(grid), dim3(kWG), 0, s, L.xf, \
+                     L.bn, L.xrec, round_up(n, kTile), L.qf, L.qa, L.qrec, m, p.split_len,  \
+                     p.qblocks, ntasks, per_xcd, L.part)
+    switch (prec) {
+      case FSK_PREC_F32: FSK_LSE(FSK_PREC_F32); break;
+      case FSK_PREC_BF16: FSK_LSE(FSK_PREC_BF16); break;
+      case FSK_PREC_F16: FSK_LSE(FSK_PREC_F16); break;
+      default: FSK_LSE(FSK_PREC_BF16X3); break;
+    }
+#undef FSK_LSE
+  }
+  prof_end(FSK_OP_KDE16_LOGDENSITY, s);
+  if ((st = launch_status(what)) != FSK_OK) return st;
+  const double log_const = -log(double(n)) - kD * log(bandwidth) - 0.5 * kD * log(2.0 * M_PI);
+  hipLaunchKernelGGL(lse_combine_kernel, dim3(cdiv(m, 256)), dim3(256), 0, s, L.part, p.splits,
+                     m, log_const, out);
+  return launch_status(what);
+}

@@ -1,0 +1,80 @@
+// fsk_plan.h - launch plans (tile sizes, split counts, workspace carving).
+// Everything here depends only on (n, m, flags), so a plan - and with it the
+// FP32 partial boundaries and the FP64 combine order - is identical for every
+// call with the same sizes: the split-K paths are bitwise reproducible.
+#pragma once
+
+#include <stdint.h>
+#include <stddef.h>
+
+namespace fsk {
+
+// 16-D KDE split-K: a workgroup owns 4 waves x QT16 row tiles of 16 queries.
+constexpr int kKdeQT = 4;
+constexpr int kKdeRowsPerWG = 4 * 16 * kKdeQT;  // 256 queries
+// 16-D score (ordered): rows per workgroup.
+constexpr int kScoreQT = 4;
+constexpr int kScoreRowsPerWG = 4 * 16 * kScoreQT;  // 256 rows
+// Symmetric score: I block = rows of one workgroup (4 waves x kSymQT tiles).
+constexpr int kSymQT = 4;
+// Target number of (query block, split) tasks per launch: a few per CU slot.
+constexpr int64_t kTargetTasks = 8192;
+// Cap on split-K partial workspace for the ordered score (bytes).
+constexpr int64_t kOrderedPartialCap = int64_t(2) << 30;
+// 1-D paths: queries per thread and threads per workgroup.
+constexpr int k1dQPT = 8;
+constexpr int k1dRowsPerWG = 256 * k1dQPT;
+constexpr int k1dChunk = 1024;  // data points staged per LDS round
+
+struct SplitPlan {
+  int64_t qblocks;    // query blocks (workgroups along queries)
+  int64_t splits;     // data splits
+  int64_t split_len;  // data points per split (multiple of the tile)
+};
+
+inline int64_t p_cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+inline SplitPlan make_split_plan(int64_t n, int64_t m, int64_t rows_per_wg,
+                                 int64_t tile, int64_t min_split_len,
+                                 int64_t max_splits) {
+  SplitPlan p;
+  p.qblocks = p_cdiv(m, rows_per_wg);
+  int64_t s = p_cdiv(kTargetTasks, p.qblocks);
+  int64_t s_max_len = p_cdiv(n, min_split_len);
+  if (s > s_max_len) s = s_max_len;
+  if (s > max_splits) s = max_splits;
+  if (s < 1) s = 1;
+  int64_t len = p_cdiv(p_cdiv(n, s), tile) * tile;
+  p.split_len = len;
+  p.splits = p_cdiv(n, len);
+  return p;
+}
+
+inline SplitPlan kde16_plan(int64_t n, int64_t m) {
+  return make_split_plan(n, m, kKdeRowsPerWG, 16, 1024, 1 << 20);
+}
+
+inline SplitPlan score16_ordered_plan(int64_t n) {
+  int64_t cap = kOrderedPartialCap / (n * 17 * 4 > 0 ? n * 17 * 4 : 1);
+  if (cap < 1) cap = 1;
+  return make_split_plan(n, n, kScoreRowsPerWG, 16, 2048, cap);
+}
+
+inline SplitPlan kde1_plan(int64_t n, int64_t m) {
+  return make_split_plan(n, m, k1dRowsPerWG, k1dChunk, 4096, 1 << 20);
+}
+
+// Bump allocator over the caller's workspace (256-byte aligned slices).
+struct Carve {
+  char *base;
+  size_t used;
+  explicit Carve(void *b) : base(static_cast<char *>(b)), used(0) {}
+  template <class T>
+  T *take(int64_t count) {
+    size_t off = used;
+    used += (size_t(count) * sizeof(T) + 255) & ~size_t(255);
+    return base ? reinterpret_cast<T *>(base + off) : nullptr;
+  }
+};
+
+}  // namespace fsk

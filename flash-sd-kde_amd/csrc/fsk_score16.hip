@@ -1,0 +1,406 @@
+// fsk_score16.hip - 16-D empirical-score accumulators on MFMA.
+//
+//   pdf[i] = sum_j phi_ij,   w[i,:] = sum_j phi_ij x_j,   phi_ij = exp(-|x_i-x_j|^2/(2h^2))
+// over all ordered pairs, j = i included (phi_ii = 1), as in
+// flash_sd_kde/reference.py:263-284 (empirical_score_nd_numpy).
+//
+// phi tiles come out of the cross-term MFMA with lane l holding
+// phi[i = l%16][j = 4(l/16) + r], r = 0..3. That is exactly the A-operand
+// layout of v_mfma_f32_16x16x4_f32 for  W_I += phi . X_J  (k = l/16 <-> j),
+// so the phi-weighted data sum needs no data movement: MFMA r consumes the
+// r-th phi value of every lane against X_J rows 4(l/16)+r.
+//
+// Ordered variant: deterministic split-K, FP32 partials per split, FP64
+// combine (plus FP64 outer sums inside each split).
+// Symmetric variant: each unordered 256x256 block pair once; the column side
+// (W_J += phi^T X_I) needs phi transposed, done through a per-wave LDS tile;
+// column sums are gathered in LDS per J block and scattered with FP64 global
+// atomics, then converted to the FP32 outputs.
+#include "fsk_common.h"
+#include "fsk_plan.h"
+
+namespace fsk {
+
+constexpr int kScoreFlush = 32;
+
+template <int QT>
+__global__ __launch_bounds__(kWG) void score16_ordered_kernel(
+    const float *__restrict__ data, int64_t ld, int64_t n, const float *__restrict__ bn,
+    int64_t n_pad, float qscale, int64_t split_len, int64_t qblocks, int64_t ntasks,
+    int64_t per_xcd, float *__restrict__ pdf_part, float *__restrict__ w_part) {
+  const int64_t task = xcd_task(blockIdx.x, per_xcd);
+  if (task >= ntasks) return;
+  const int64_t split = task / qblocks;
+  const int64_t qb = task - split * qblocks;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int g = lane >> 4, c = lane & 15;
+  const int64_t row0 = qb * (64 * QT) + int64_t(wave) * (16 * QT);
+
+  f4 qv[QT], w[QT];
+  d4 w64[QT];
+  float qav[QT], pdf[QT];
+  double pdf64[QT];
+#pragma unroll
+  for (int R = 0; R < QT; ++R) {
+    int64_t i = row0 + 16 * R + c;
+    int64_t ic = i > n - 1 ? n - 1 : i;
+    qv[R] = ld4(data + ic * ld + 4 * g) * qscale;
+    qav[R] = bn[i];  // bn is padded to a multiple of 256 rows
+    w[R] = f4{0.f, 0.f, 0.f, 0.f};
+    w64[R] = d4{0.0, 0.0, 0.0, 0.0};
+    pdf[R] = 0.f;
+    pdf64[R] = 0.0;
+  }
+
+  const int64_t j_begin = split * split_len;
+  int64_t j_end = j_begin + split_len;
+  if (j_end > n_pad) j_end = n_pad;
+  int tc = 0;
+  for (int64_t j0 = j_begin; j0 < j_end; j0 += kTile) {
+    int64_t jr = j0 + c;
+    if (jr > n - 1) jr = n - 1;
+    const f4 xa = ld4(data + jr * ld + 4 * g);
+    const f4 bb = ld4(bn + j0 + 4 * g);
+    float xt[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      int64_t jt = j0 + 4 * g + r;
+      if (jt > n - 1) jt = n - 1;
+      xt[r] = data[jt * ld + c];
+    }
+    f4 d[QT];
+#pragma unroll
+    for (int R = 0; R < QT; ++R) {
+      d[R][0] = qav[R] + bb[0];
+      d[R][1] = qav[R] + bb[1];
+      d[R][2] = qav[R] + bb[2];
+      d[R][3] = qav[R] + bb[3];
+    }
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+#pragma unroll
+      for (int R = 0; R < QT; ++R) d[R] = mfma_f32(xa[s], qv[R][s], d[R]);
+    }
+#pragma unroll
+    for (int R = 0; R < QT; ++R) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        float e = fexp2(fminf(d[R][r], 0.f));
+        d[R][r] = e;
+        pdf[R] += e;
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+#pragma unroll
+      for (int R = 0; R < QT; ++R) w[R] = mfma_f32(d[R][r], xt[r], w[R]);
+    }
+    if (++tc == kScoreFlush) {
+      tc = 0;
+#pragma unroll
+      for (int R = 0; R < QT; ++R) {
+        pdf64[R] += pdf[R];
+        pdf[R] = 0.f;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) w64[R][r] += w[R][r];
+        w[R] = f4{0.f, 0.f, 0.f, 0.f};
+      }
+    }
+  }
+#pragma unroll
+  for (int R = 0; R < QT; ++R) {
+    double p = sum_groups(pdf64[R] + pdf[R]);
+    int64_t i = row0 + 16 * R + c;
+    if (g == 0 && i < n) pdf_part[split * n + i] = float(p);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      int64_t iw = row0 + 16 * R + 4 * g + r;
+      if (iw < n) w_part[(split * n + iw) * kD + c] = float(w64[R][r] + w[R][r]);
+    }
+  }
+}
+
+// Symmetric variant. Task = (I block, chunk of J blocks), blocks of B rows,
+// only J >= I. For J > I both sides accumulate; J == I runs the I side over
+// the whole diagonal block (every ordered pair inside it exactly once).
+template <int QT, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void score16_sym_kernel(
+    const float *__restrict__ data, int64_t ld, int64_t n, const float *__restrict__ bn,
+    float qscale, int64_t nb, int64_t jc, int64_t ntasks, int64_t per_xcd,
+    double *__restrict__ pdf64, double *__restrict__ w64) {
+  constexpr int B = WAVES * 16 * QT;
+  constexpr int TS = 17;  // padded transpose row
+  __shared__ float tbuf[WAVES][16 * TS];
+  __shared__ float jw[B * kD];
+  __shared__ float jp[B];
+
+  const int64_t task = xcd_task(blockIdx.x, per_xcd);
+  if (task >= ntasks) return;
+  const int64_t chunk = task / nb;
+  const int64_t I = task - chunk * nb;
+  int64_t Jlo = chunk * jc;
+  int64_t Jhi = Jlo + jc;
+  if (Jhi > nb) Jhi = nb;
+  if (Jlo < I) Jlo = I;
+  if (Jlo >= Jhi) return;
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int g = lane >> 4, c = lane & 15;
+  for (int e = tid; e < B * kD; e += 64 * WAVES) jw[e] = 0.f;
+  for (int e = tid; e < B; e += 64 * WAVES) jp[e] = 0.f;
+  __syncthreads();
+
+  const int64_t row0 = I * B + int64_t(wave) * (16 * QT);
+  f4 qv[QT], wI[QT];
+  float qav[QT], pI[QT], xIt[QT][4];
+#pragma unroll
+  for (int R = 0; R < QT; ++R) {
+    int64_t i = row0 + 16 * R + c;
+    int64_t ic = i > n - 1 ? n - 1 : i;
+    qv[R] = ld4(data + ic * ld + 4 * g) * qscale;
+    qav[R] = bn[i];  // kPadNeg for padded rows: their phi is exactly 0
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      int64_t it = row0 + 16 * R + 4 * g + r;
+      if (it > n - 1) it = n - 1;
+      xIt[R][r] = data[it * ld + c];
+    }
+    wI[R] = f4{0.f, 0.f, 0.f, 0.f};
+    pI[R] = 0.f;
+  }
+  float *tb = tbuf[wave];
+
+  for (int64_t J = Jlo; J < Jhi; ++J) {
+    const bool diag = (J == I);
+    for (int jt = 0; jt < B / kTile; ++jt) {
+      const int64_t j0 = J * B + jt * kTile;
+      if (j0 >= n) break;  // uniform across the workgroup
+      int64_t jr = j0 + c;
+      if (jr > n - 1) jr = n - 1;
+      const f4 xa = ld4(data + jr * ld + 4 * g);
+      const f4 bb = ld4(bn + j0 + 4 * g);
+      float xt[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        int64_t jj = j0 + 4 * g + r;
+        if (jj > n - 1) jj = n - 1;
+        xt[r] = data[jj * ld + c];
+      }
+      f4 d[QT];
+#pragma unroll
+      for (int R = 0; R < QT; ++R) {
+        d[R][0] = qav[R] + bb[0];
+        d[R][1] = qav[R] + bb[1];
+        d[R][2] = qav[R] + bb[2];
+        d[R][3] = qav[R] + bb[3];
+      }
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+#pragma unroll
+        for (int R = 0; R < QT; ++R) d[R] = mfma_f32(xa[s], qv[R][s], d[R]);
+      }
+#pragma unroll
+      for (int R = 0; R < QT; ++R) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          float e = fexp2(fminf(d[R][r], 0.f));
+          d[R][r] = e;
+          pI[R] += e;
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+#pragma unroll
+        for (int R = 0; R < QT; ++R) wI[R] = mfma_f32(d[R][r], xt[r], wI[R]);
+      }
+      if (!diag) {
+        f4 wJ = f4{0.f, 0.f, 0.f, 0.f};
+        float pj = 0.f;
+#pragma unroll
+        for (int R = 0; R < QT; ++R) {
+          // lane holds phi[i=c][j=4g+r]; W_J needs phi[i=4g+r'][j=c]
+#pragma unroll
+          for (int r = 0; r < 4; ++r) tb[c * TS + 4 * g + r] = d[R][r];
+          __builtin_amdgcn_wave_barrier();
+          float et[4];
+#pragma unroll
+          for (int r = 0; r < 4; ++r) et[r] = tb[(4 * g + r) * TS + c];
+          __builtin_amdgcn_wave_barrier();
+          pj += (et[0] + et[1]) + (et[2] + et[3]);
+#pragma unroll
+          for (int r = 0; r < 4; ++r) wJ = mfma_f32(et[r], xIt[R][r], wJ);
+        }
+        atomicAdd(&jp[jt * kTile + c], pj);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) atomicAdd(&jw[(jt * kTile + 4 * g + r) * kD + c], wJ[r]);
+      }
+    }
+    if (!diag) {
+      __syncthreads();
+      const int64_t jbase = J * B;
+      for (int e = tid; e < B * kD; e += 64 * WAVES) {
+        int64_t j = jbase + e / kD;
+        if (j < n) atomicAdd(&w64[j * kD + (e % kD)], double(jw[e]));
+        jw[e] = 0.f;
+      }
+      for (int e = tid; e < B; e += 64 * WAVES) {
+        int64_t j = jbase + e;
+        if (j < n) atomicAdd(&pdf64[j], double(jp[e]));
+        jp[e] = 0.f;
+      }
+      __syncthreads();
+    }
+  }
+#pragma unroll
+  for (int R = 0; R < QT; ++R) {
+    float p = sum_groups(pI[R]);
+    int64_t i = row0 + 16 * R + c;
+    if (g == 0 && i < n) atomicAdd(&pdf64[i], double(p));
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      int64_t iw = row0 + 16 * R + 4 * g + r;
+      if (iw < n) atomicAdd(&w64[iw * kD + c], double(wI[R][r]));
+    }
+  }
+}
+
+__global__ void score_finish_kernel(const double *__restrict__ pdf64,
+                                    const double *__restrict__ w64, int64_t n,
+                                    float *__restrict__ pdf, float *__restrict__ w,
+                                    int64_t ldw_row, int64_t ldw_col) {
+  int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= n * kD) return;
+  int64_t i = e / kD, f = e % kD;
+  w[i * ldw_row + f * ldw_col] = float(w64[e]);
+  if (f == 0) pdf[i] = float(pdf64[i]);
+}
+
+constexpr int kSymWaves = 4;
+constexpr int kSymB = kSymWaves * 16 * kSymQT;
+constexpr int64_t kSymJC = 8;  // J blocks per task
+
+struct ScoreLayout {
+  float *bn;
+  float *pdf_part;
+  float *w_part;
+  double *pdf64;
+  double *w64;
+};
+
+static size_t score_layout(void *ws, int64_t n, bool ordered, ScoreLayout *L) {
+  Carve cv(ws);
+  L->bn = cv.take<float>(round_up(n, kNormPad));
+  L->pdf_part = L->w_part = nullptr;
+  L->pdf64 = L->w64 = nullptr;
+  if (ordered) {
+    SplitPlan p = score16_ordered_plan(n);
+    L->pdf_part = cv.take<float>(p.splits * n);
+    L->w_part = cv.take<float>(p.splits * n * kD);
+  } else {
+    L->pdf64 = cv.take<double>(n);
+    L->w64 = cv.take<double>(n * kD);
+  }
+  return cv.used;
+}
+
+size_t score16_ws_bytes(int64_t n, bool ordered) {
+  ScoreLayout L;
+  return score_layout(nullptr, n, ordered, &L);
+}
+
+static int score_check(const char *what, const float *data, int64_t ld, int64_t n, double h,
+                       unsigned flags, const float *pdf, const float *w) {
+  int st;
+  if ((st = check_points16(what, data, ld, n)) != FSK_OK) return st;
+  if (!(h > 0.0) || !isfinite(h)) {
+    set_error("%s: bandwidth must be positive, got %g", what, h);
+    return FSK_EINVAL;
+  }
+  if (!pdf || !w) {
+    set_error("%s: null output", what);
+    return FSK_EINVAL;
+  }
+  if ((flags & FSK_PREC_MASK) != FSK_PREC_F32 || (flags & FSK_FLAG_LAPLACIAN)) {
+    set_error("%s: flags 0x%x not available on this path", what, flags);
+    return FSK_EUNSUPPORTED;
+  }
+  return FSK_OK;
+}
+
+}  // namespace fsk
+
+using namespace fsk;
+
+extern "C" int fsk_score16_ordered(const float *data, int64_t ld, int64_t n, double bandwidth,
+                                   unsigned flags, void *ws, size_t ws_bytes, float *pdf,
+                                   float *w, int64_t ldw_row, int64_t ldw_col, void *stream) {
+  const char *what = "fsk_score16_ordered";
+  int st = score_check(what, data, ld, n, bandwidth, flags, pdf, w);
+  if (st != FSK_OK) return st;
+  ScoreLayout L;
+  size_t need = score_layout(ws, n, true, &L);
+  if (!ws || ws_bytes < need) {
+    set_error("%s: workspace of %zu bytes needed, got %zu", what, need, ws_bytes);
+    return FSK_EWORKSPACE;
+  }
+  hipStream_t s = as_stream(stream);
+  const double coef = exp2_coef(bandwidth);
+  const int64_t n_norm = round_up(n, kNormPad);
+  hipLaunchKernelGGL(prep_norms16, dim3(cdiv(n_norm, 256)), dim3(256), 0, s, data, ld, n, n_norm,
+                     coef, L.bn);
+  SplitPlan p = score16_ordered_plan(n);
+  const int64_t ntasks = p.qblocks * p.splits;
+  int64_t per_xcd;
+  const int64_t grid = xcd_grid(ntasks, &per_xcd);
+  prof_begin(FSK_OP_SCORE16_ORDERED, s);
+  hipLaunchKernelGGL((score16_ordered_kernel<kScoreQT>), dim3(grid), dim3(kWG), 0, s, data, ld,
+                     n, L.bn, round_up(n, kTile), float(-2.0 * coef), p.split_len, p.qblocks,
+                     ntasks, per_xcd, L.pdf_part, L.w_part);
+  prof_end(FSK_OP_SCORE16_ORDERED, s);
+  if ((st = launch_status(what)) != FSK_OK) return st;
+  if ((st = fsk_reduce_partials_f64(L.pdf_part, p.splits, n, n, pdf, stream)) != FSK_OK)
+    return st;
+  return fsk_reduce_weighted_partials_f64(L.w_part, p.splits, n, n * kD, w, ldw_row, ldw_col,
+                                          stream);
+}
+
+extern "C" int fsk_score16_sym_atomic(const float *data, int64_t ld, int64_t n,
+                                      double bandwidth, unsigned flags, void *ws,
+                                      size_t ws_bytes, float *pdf, float *w, int64_t ldw_row,
+                                      int64_t ldw_col, void *stream) {
+  const char *what = "fsk_score16_sym_atomic";
+  int st = score_check(what, data, ld, n, bandwidth, flags, pdf, w);
+  if (st != FSK_OK) return st;
+  ScoreLayout L;
+  size_t need = score_layout(ws, n, false, &L);
+  if (!ws || ws_bytes < need) {
+    set_error("%s: workspace of %zu bytes needed, got %zu", what, need, ws_bytes);
+    return FSK_EWORKSPACE;
+  }
+  hipStream_t s = as_stream(stream);
+  const double coef = exp2_coef(bandwidth);
+  const int64_t n_norm = round_up(n, kNormPad);
+  hipLaunchKernelGGL(prep_norms16, dim3(cdiv(n_norm, 256)), dim3(256), 0, s, data, ld, n, n_norm,
+                     coef, L.bn);
+  hipLaunchKernelGGL(fill_f64, dim3(cdiv(n, 256) < 4096 ? cdiv(n, 256) : 4096), dim3(256), 0, s,
+                     L.pdf64, n, 0.0);
+  hipLaunchKernelGGL(fill_f64, dim3(cdiv(n * kD, 256) < 4096 ? cdiv(n * kD, 256) : 4096),
+                     dim3(256), 0, s, L.w64, n * kD, 0.0);
+  const int64_t nb = cdiv(n, kSymB);
+  const int64_t nchunks = cdiv(nb, kSymJC);
+  const int64_t ntasks = nb * nchunks;
+  int64_t per_xcd;
+  const int64_t grid = xcd_grid(ntasks, &per_xcd);
+  prof_begin(FSK_OP_SCORE16_SYM, s);
+  hipLaunchKernelGGL((score16_sym_kernel<kSymQT, kSymWaves>), dim3(grid), dim3(64 * kSymWaves), 0,
+                     s, data, ld, n, L.bn, float(-2.0 * coef), nb, kSymJC, ntasks, per_xcd,
+                     L.pdf64, L.w64);
+  prof_end(FSK_OP_SCORE16_SYM, s);
+  if ((st = launch_status(what)) != FSK_OK) return st;
+  hipLaunchKernelGGL(score_finish_kernel, dim3(cdiv(n * kD, 256)), dim3(256), 0, s, L.pdf64,
+                     L.w64, n, pdf, w, ldw_row, ldw_col);
+  return launch_status(what);
+}

@@ -1,0 +1,140 @@
+"""ctypes binding of libflashsdkde.so, the gfx950 HIP library (C ABI in
+include/flashsdkde.h).
+
+Every public function in this package ends in one of these entry points; there
+is no other compute path. If the library is missing, or no GPU is visible,
+calls raise instead of falling back to anything else.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+from typing import Optional
+
+import torch
+
+_PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(_PKG_ROOT, "lib", "libflashsdkde.so")
+
+FSK_OK = 0
+FSK_EINVAL = -1
+FSK_EUNSUPPORTED = -2
+FSK_ELAUNCH = -3
+FSK_EWORKSPACE = -4
+
+PREC_F32 = 0
+PREC_F32_VALU = 1
+PREC_BF16 = 2
+PREC_F16 = 3
+PREC_BF16X3 = 4
+FLAG_LAPLACIAN = 1 << 8
+
+OP_KDE16_SPLITK = 1
+OP_KDE16_ATOMIC = 2
+OP_KDE1_SPLITK = 3
+OP_KDE1_ATOMIC = 4
+OP_SCORE16_SYM = 5
+OP_SCORE16_ORDERED = 6
+OP_KDE16_LOGDENSITY = 7
+OP_KDE1_SCORE = 8
+
+_p = ctypes.c_void_p
+_i64 = ctypes.c_int64
+_d = ctypes.c_double
+_u = ctypes.c_uint
+_sz = ctypes.c_size_t
+
+# name -> argument types (all return int status unless listed in _RESTYPES)
+SIGNATURES = {
+    "fsk_last_error": [],
+    "fsk_version": [],
+    "fsk_workspace_bytes": [ctypes.c_int, _i64, _i64, _u],
+    "fsk_profile_enable": [ctypes.c_int],
+    "fsk_profile_last_ms": [ctypes.c_int],
+    "fsk_kde16_splitk": [_p, _i64, _i64, _p, _i64, _i64, _d, _u, _p, _sz, _p, _p],
+    "fsk_kde16_atomic": [_p, _i64, _i64, _p, _i64, _i64, _d, _u, _p, _sz, _p, _p],
+    "fsk_kde16_logdensity": [_p, _i64, _i64, _p, _i64, _i64, _d, _u, _p, _sz, _p, _p],
+    "fsk_kde1_splitk": [_p, _i64, _p, _i64, _d, _u, _p, _sz, _p, _p],
+    "fsk_kde1_atomic": [_p, _i64, _p, _i64, _d, _u, _p, _sz, _p, _p],
+    "fsk_kde1_score": [_p, _i64, _p, _i64, _d, _p, _sz, _p, _p, _p],
+    "fsk_score16_sym_atomic": [_p, _i64, _i64, _d, _u, _p, _sz, _p, _p, _i64, _i64, _p],
+    "fsk_score16_ordered": [_p, _i64, _i64, _d, _u, _p, _sz, _p, _p, _i64, _i64, _p],
+    "fsk_reduce_partials_f64": [_p, _i64, _i64, _i64, _p, _p],
+    "fsk_reduce_weighted_partials_f64": [_p, _i64, _i64, _i64, _p, _i64, _i64, _p],
+    "fsk_sdkde_debias_f64": [_p, _i64, _i64, _p, _p, _i64, _i64, _d, _d, _p, _i64, _p, _p],
+}
+_RESTYPES = {
+    "fsk_last_error": ctypes.c_char_p,
+    "fsk_version": ctypes.c_int,
+    "fsk_workspace_bytes": _sz,
+    "fsk_profile_last_ms": _d,
+}
+
+_lib: Optional[ctypes.CDLL] = None
+_lock = threading.Lock()
+
+
+class NativeLibraryError(RuntimeError):
+    """The HIP library could not be loaded."""
+
+
+def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
+    """Load (once) and type the shared library. Raises if it is absent."""
+    global _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(path):
+            raise NativeLibraryError(
+                f"libflashsdkde.so not found at {path}; build it with "
+                "`make -C flash-sd-kde_amd/csrc` (hipcc, gfx950)."
+            )
+        lib = ctypes.CDLL(path)
+        for name, argtypes in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.argtypes = argtypes
+            fn.restype = _RESTYPES.get(name, ctypes.c_int)
+        _lib = lib
+        return lib
+
+
+def lib() -> ctypes.CDLL:
+    return load_library()
+
+
+def last_error() -> str:
+    msg = lib().fsk_last_error()
+    return msg.decode() if msg else ""
+
+
+def check(status: int, what: str) -> None:
+    if status == FSK_OK:
+        return
+    msg = last_error()
+    if status == FSK_EINVAL:
+        raise ValueError(f"{what}: {msg}")
+    raise RuntimeError(f"{what} failed with status {status}: {msg}")
+
+
+def require_gpu(device: torch.device, what: str) -> None:
+    if device.type != "cuda":
+        raise ValueError(f"{what} requires a CUDA (ROCm/HIP) device, got {device}.")
+    if not torch.cuda.is_available():
+        raise RuntimeError(f"CUDA is not available but was requested for {what}.")
+    load_library()
+
+
+def stream_handle(device: torch.device) -> int:
+    return torch.cuda.current_stream(device).cuda_stream
+
+
+def workspace(op: int, n: int, m: int, flags: int, device: torch.device) -> tuple[torch.Tensor, int]:
+    nbytes = int(lib().fsk_workspace_bytes(op, n, m, flags))
+    buf = torch.empty(max(nbytes, 256), dtype=torch.uint8, device=device)
+    return buf, nbytes
+
+
+def ptr(t: Optional[torch.Tensor]) -> Optional[int]:
+    return None if t is None else t.data_ptr()

@@ -1,0 +1,249 @@
+"""Tensor-level operations: validate, allocate outputs/workspace with torch,
+and enqueue the HIP entry points on torch's current stream.
+
+These are the only callers of the C ABI. Higher layers (kernels/*,
+flash_sd_kde.kde, the estimator) mirror the reference's Python signatures on
+top of them.
+"""
+
+from __future__ import annotations
+
+from typing import Sequence
+
+import torch
+
+from . import _native as nat
+
+ND = 16
+
+PRECISION_FLAGS = {
+    # drop-in precision modes of the reference (globals.py:2-3). gfx950 has no
+    # TF32 MFMA (the xf32 MFMA of gfx942 is gone), so both modes run the cross
+    # term as FP32 MFMA: "fast_tf32" is at least as accurate as the reference's.
+    "fp32_ieee": nat.PREC_F32,
+    "fast_tf32": nat.PREC_F32,
+    # log-density precision forms
+    "fp32": nat.PREC_F32,
+    "fp32_valu": nat.PREC_F32_VALU,
+    "bf16": nat.PREC_BF16,
+    "fp16": nat.PREC_F16,
+    "bf16x3": nat.PREC_BF16X3,
+}
+
+
+def precision_flag(precision_mode: str) -> int:
+    try:
+        return PRECISION_FLAGS[precision_mode]
+    except KeyError:
+        raise ValueError(
+            f"precision_mode must be one of {sorted(PRECISION_FLAGS)}, got {precision_mode}."
+        ) from None
+
+
+def as_points16(array_like, device: torch.device) -> torch.Tensor:
+    """(n, 16) float32 contiguous tensor on `device` (reference _to_matrix_tensor,
+    kernels/flash_sd_kde/common.py:28-43)."""
+    if isinstance(array_like, torch.Tensor):
+        t = array_like.to(device=device, dtype=torch.float32, copy=False)
+    else:
+        t = torch.as_tensor(array_like, dtype=torch.float32, device=device)
+    if t.ndim != 2 or t.shape[1] != ND:
+        raise ValueError(f"expected tensor with shape (n, {ND}), got {tuple(t.shape)}")
+    return t.contiguous()
+
+
+def as_points1(array_like, device: torch.device) -> torch.Tensor:
+    if isinstance(array_like, torch.Tensor):
+        t = array_like.to(device=device, dtype=torch.float32, copy=False)
+    else:
+        t = torch.as_tensor(array_like, dtype=torch.float32, device=device)
+    if t.ndim != 1:
+        raise ValueError("expected 1D tensor for KDE eval.")
+    return t.contiguous()
+
+
+def _check_h(bandwidth: float) -> float:
+    h = float(bandwidth)
+    if not (h > 0.0):
+        raise ValueError("bandwidth must be positive.")
+    return h
+
+
+def kde16_sums(train: torch.Tensor, query: torch.Tensor, bandwidth: float, *,
+               precision_mode: str = "fp32_ieee", laplacian: bool = False,
+               atomic: bool = False) -> torch.Tensor:
+    """sum_j exp(-|q_i - x_j|^2 / 2h^2) for each query (unnormalised)."""
+    h = _check_h(bandwidth)
+    dev = train.device
+    nat.require_gpu(dev, "kde16_sums")
+    n, m = train.shape[0], query.shape[0]
+    if n == 0 or m == 0:
+        raise ValueError("data and queries must be non-empty.")
+    flags = precision_flag(precision_mode) | (nat.FLAG_LAPLACIAN if laplacian else 0)
+    if (flags & 0xFF) != nat.PREC_F32:
+        raise ValueError(f"precision_mode {precision_mode} is only available for log-densities")
+    out = torch.empty(m, dtype=torch.float32, device=dev)
+    op = nat.OP_KDE16_ATOMIC if atomic else nat.OP_KDE16_SPLITK
+    ws, nbytes = nat.workspace(op, n, m, flags, dev)
+    fn = nat.lib().fsk_kde16_atomic if atomic else nat.lib().fsk_kde16_splitk
+    st = fn(train.data_ptr(), train.stride(0), n, query.data_ptr(), query.stride(0), m, h,
+            flags, ws.data_ptr(), nbytes, out.data_ptr(), nat.stream_handle(dev))
+    nat.check(st, "fsk_kde16_atomic" if atomic else "fsk_kde16_splitk")
+    return out
+
+
+def kde16_logdensity(train: torch.Tensor, query: torch.Tensor, bandwidth: float, *,
+                     precision: str = "fp32") -> torch.Tensor:
+    """Normalised log-density by log-sum-exp, float64 (m,)."""
+    h = _check_h(bandwidth)
+    dev = train.device
+    nat.require_gpu(dev, "kde16_logdensity")
+    n, m = train.shape[0], query.shape[0]
+    if n == 0 or m == 0:
+        raise ValueError("data and queries must be non-empty.")
+    flags = precision_flag(precision)
+    out = torch.empty(m, dtype=torch.float64, device=dev)
+    ws, nbytes = nat.workspace(nat.OP_KDE16_LOGDENSITY, n, m, flags, dev)
+    st = nat.lib().fsk_kde16_logdensity(train.data_ptr(), train.stride(0), n, query.data_ptr(),
+                                        query.stride(0), m, h, flags, ws.data_ptr(), nbytes,
+                                        out.data_ptr(), nat.stream_handle(dev))
+    nat.check(st, "fsk_kde16_logdensity")
+    return out
+
+
+def kde1_sums(train: torch.Tensor, query: torch.Tensor, bandwidth: float, *,
+              laplacian: bool = False, atomic: bool = False) -> torch.Tensor:
+    h = _check_h(bandwidth)
+    dev = train.device
+    nat.require_gpu(dev, "kde1_sums")
+    n, m = train.numel(), query.numel()
+    if n == 0 or m == 0:
+        raise ValueError("data and queries must be non-empty.")
+    flags = nat.FLAG_LAPLACIAN if laplacian else 0
+    out = torch.empty(m, dtype=torch.float32, device=dev)
+    op = nat.OP_KDE1_ATOMIC if atomic else nat.OP_KDE1_SPLITK
+    ws, nbytes = nat.workspace(op, n, m, flags, dev)
+    fn = nat.lib().fsk_kde1_atomic if atomic else nat.lib().fsk_kde1_splitk
+    st = fn(train.data_ptr(), n, query.data_ptr(), m, h, flags, ws.data_ptr(), nbytes,
+            out.data_ptr(), nat.stream_handle(dev))
+    nat.check(st, "fsk_kde1_atomic" if atomic else "fsk_kde1_splitk")
+    return out
+
+
+def kde1_score_sums(train: torch.Tensor, query: torch.Tensor, bandwidth: float):
+    """(sum_j phi(z_ij), sum_j -z_ij phi(z_ij)) with z = (q - x)/h, phi = exp(-z^2/2)."""
+    h = _check_h(bandwidth)
+    dev = train.device
+    nat.require_gpu(dev, "kde1_score_sums")
+    n, m = train.numel(), query.numel()
+    if n == 0 or m == 0:
+        raise ValueError("data and queries must be non-empty.")
+    pdf = torch.empty(m, dtype=torch.float32, device=dev)
+    der = torch.empty(m, dtype=torch.float32, device=dev)
+    ws, nbytes = nat.workspace(nat.OP_KDE1_SCORE, n, m, 0, dev)
+    st = nat.lib().fsk_kde1_score(train.data_ptr(), n, query.data_ptr(), m, h, ws.data_ptr(),
+                                  nbytes, pdf.data_ptr(), der.data_ptr(),
+                                  nat.stream_handle(dev))
+    nat.check(st, "fsk_kde1_score")
+    return pdf, der
+
+
+def score16(train: torch.Tensor, bandwidth: float, *, variant: str = "symmetric",
+            transposed_w: bool = False):
+    """Empirical-score accumulators (pdf (n,), weighted (n,16)).
+
+    transposed_w=True stores the weighted sums in a (16, n) buffer and returns
+    its transpose, the layout of the reference's flash backend
+    (kernels/flash_sd_kde/empirical.py:180,223)."""
+    h = _check_h(bandwidth)
+    dev = train.device
+    nat.require_gpu(dev, "score16")
+    n = train.shape[0]
+    if n == 0:
+        raise ValueError("data must be non-empty.")
+    pdf = torch.empty(n, dtype=torch.float32, device=dev)
+    if transposed_w:
+        wbuf = torch.empty((ND, n), dtype=torch.float32, device=dev)
+        ld_row, ld_col = 1, n
+    else:
+        wbuf = torch.empty((n, ND), dtype=torch.float32, device=dev)
+        ld_row, ld_col = ND, 1
+    if variant == "symmetric":
+        op, fn, name = nat.OP_SCORE16_SYM, nat.lib().fsk_score16_sym_atomic, "fsk_score16_sym_atomic"
+    elif variant == "ordered":
+        op, fn, name = nat.OP_SCORE16_ORDERED, nat.lib().fsk_score16_ordered, "fsk_score16_ordered"
+    else:
+        raise ValueError(f"unknown score variant {variant}")
+    ws, nbytes = nat.workspace(op, n, n, 0, dev)
+    st = fn(train.data_ptr(), train.stride(0), n, h, 0, ws.data_ptr(), nbytes, pdf.data_ptr(),
+            wbuf.data_ptr(), ld_row, ld_col, nat.stream_handle(dev))
+    nat.check(st, name)
+    return pdf, (wbuf.t() if transposed_w else wbuf)
+
+
+def debias(train: torch.Tensor, pdf: torch.Tensor, weighted: torch.Tensor, bandwidth: float,
+           eps: float, *, return_score: bool = False):
+    """x + (h^2/2) * ((w/(pdf+eps) - x)/h^2), evaluated in FP64 on the GPU."""
+    h = _check_h(bandwidth)
+    dev = train.device
+    n = train.shape[0]
+    if pdf.dtype != torch.float32 or weighted.dtype != torch.float32:
+        raise ValueError("pdf and weighted sums must be float32")
+    pdf = pdf.contiguous()
+    out = torch.empty((n, ND), dtype=torch.float32, device=dev)
+    score = torch.empty((n, ND), dtype=torch.float32, device=dev) if return_score else None
+    st = nat.lib().fsk_sdkde_debias_f64(
+        train.data_ptr(), train.stride(0), n, pdf.data_ptr(), weighted.data_ptr(),
+        weighted.stride(0), weighted.stride(1), h, float(eps), out.data_ptr(), ND,
+        nat.ptr(score), nat.stream_handle(dev))
+    nat.check(st, "fsk_sdkde_debias_f64")
+    return (out, score) if return_score else out
+
+
+def reduce_partials(partials: torch.Tensor) -> torch.Tensor:
+    if partials.ndim != 2:
+        raise ValueError("partials must be 2D (n_splits, n_query).")
+    nat.require_gpu(partials.device, "reduce_pdf_partials")
+    p = partials.to(torch.float32)
+    if p.stride(1) != 1:
+        p = p.contiguous()
+    S, m = p.shape
+    out = torch.empty(m, dtype=torch.float32, device=p.device)
+    st = nat.lib().fsk_reduce_partials_f64(p.data_ptr(), S, m, p.stride(0), out.data_ptr(),
+                                           nat.stream_handle(p.device))
+    nat.check(st, "fsk_reduce_partials_f64")
+    return out
+
+
+def reduce_weighted(partials: torch.Tensor) -> torch.Tensor:
+    if partials.ndim != 3:
+        raise ValueError("partials must be 3D (n_splits, n_query, features).")
+    nat.require_gpu(partials.device, "reduce_weighted_partials")
+    if partials.shape[2] != ND:
+        raise ValueError(f"weighted partials must have {ND} features")
+    p = partials.to(torch.float32)
+    if p.stride(2) != 1 or p.stride(1) != ND:
+        p = p.contiguous()
+    S, m, _ = p.shape
+    out = torch.empty((m, ND), dtype=torch.float32, device=p.device)
+    st = nat.lib().fsk_reduce_weighted_partials_f64(p.data_ptr(), S, m, p.stride(0),
+                                                    out.data_ptr(), ND, 1,
+                                                    nat.stream_handle(p.device))
+    nat.check(st, "fsk_reduce_weighted_partials_f64")
+    return out
+
+
+def infer_ndim(data: Sequence | torch.Tensor) -> int:
+    """Reference _infer_ndim, flash_sd_kde/kde.py:36-46."""
+    if isinstance(data, torch.Tensor):
+        return data.ndim
+    import numpy as np
+
+    if isinstance(data, np.ndarray):
+        return data.ndim
+    if len(data) == 0:
+        return 1
+    first = data[0]
+    if isinstance(first, (list, tuple, np.ndarray)):
+        return 2
+    return 1

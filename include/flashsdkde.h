@@ -1,0 +1,178 @@
+/*
+ * flashsdkde.h - C ABI of the MI355X (gfx950) pairwise Gaussian-kernel hot path.
+ *
+ * Conventions shared by every entry point:
+ *   - All array pointers are DEVICE pointers, owned by the caller. Point sets
+ *     are row-major float32 (rows, 16) with an explicit row stride `ld`
+ *     (in elements, ld >= 16, ld % 4 == 0, base 16-byte aligned); 1-D sets are
+ *     contiguous float32.
+ *   - `stream` is a hipStream_t passed as void* (NULL = the default stream).
+ *     Every call only enqueues work on that stream; nothing synchronizes.
+ *   - No entry point allocates device memory. Scratch comes from the caller
+ *     through (ws, ws_bytes); fsk_workspace_bytes() gives the size needed.
+ *   - Return value: FSK_OK (0) or a negative FSK_E* code. fsk_last_error()
+ *     returns a thread-local message describing the last failure.
+ *   - Accumulator outputs (pdf sums, weighted sums) are fully overwritten:
+ *     callers never pre-zero them (the reference zero-fills with torch first,
+ *     kernels/emp_score_16d_symmetric_atomic.py:189-190).
+ *
+ * The reference (Flash-SD-KDE, Python + Triton) has no C interface; each entry
+ * point below replaces the Python launch wrapper cited next to it, which is
+ * where the reference crosses from host to device.
+ */
+#ifndef FLASHSDKDE_H
+#define FLASHSDKDE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---- */
+#define FSK_OK 0
+#define FSK_EINVAL -1       /* bad argument (shape, stride, bandwidth, pointer) */
+#define FSK_EUNSUPPORTED -2 /* flag combination not implemented */
+#define FSK_ELAUNCH -3      /* HIP launch / runtime error */
+#define FSK_EWORKSPACE -4   /* workspace missing or too small */
+
+/* ---- flags: low byte = cross-term precision ---- */
+#define FSK_PREC_F32 0u      /* FP32 MFMA (v_mfma_f32_16x16x4_f32): the plain FP32 form */
+#define FSK_PREC_F32_VALU 1u /* FP32 FMA on the vector ALU */
+#define FSK_PREC_BF16 2u     /* BF16 MFMA, FP32 accumulate */
+#define FSK_PREC_F16 3u      /* FP16 MFMA, FP32 accumulate */
+#define FSK_PREC_BF16X3 4u   /* BF16 hi/lo split, 3 products, FP32 accumulate */
+#define FSK_PREC_MASK 0xffu
+#define FSK_FLAG_LAPLACIAN (1u << 8) /* phi * (1 + d/2 - |q-x|^2/(2h^2)) */
+
+/* ---- operation ids for fsk_workspace_bytes ---- */
+#define FSK_OP_KDE16_SPLITK 1
+#define FSK_OP_KDE16_ATOMIC 2
+#define FSK_OP_KDE1_SPLITK 3
+#define FSK_OP_KDE1_ATOMIC 4
+#define FSK_OP_SCORE16_SYM 5
+#define FSK_OP_SCORE16_ORDERED 6
+#define FSK_OP_KDE16_LOGDENSITY 7
+#define FSK_OP_KDE1_SCORE 8
+
+const char *fsk_last_error(void);
+int fsk_version(void);
+
+/* Measurement hook (no reference counterpart): when enabled, each entry point
+ * records a pair of HIP events on the caller's stream around its main pair
+ * kernel (not the small prep / combine kernels). fsk_profile_enable resets the
+ * counts; fsk_profile_last_ms waits for the recorded launches of `op` (the
+ * last 64 at most) and returns their mean duration in ms (negative if none).
+ * Single-threaded use; events belong to the current device. */
+int fsk_profile_enable(int on);
+double fsk_profile_last_ms(int op);
+
+/* Bytes of scratch the operation needs for the given sizes (0 if none). */
+size_t fsk_workspace_bytes(int op, int64_t n, int64_t m, unsigned flags);
+
+/* 16-D Gaussian KDE sums, deterministic split-K.
+ * out[i] = sum_j exp(-|q_i - x_j|^2 / (2 h^2))            (no normalisation)
+ * (with FSK_FLAG_LAPLACIAN each term is multiplied by 1 + 8 - |q-x|^2/(2h^2)).
+ * Per-split FP32 partials are combined in FP64.
+ * Replaces kde_eval_16d_splitk, kernels/kde_eval_16d_stream_splitk.py:263-370
+ * (pass A :14-81 and reduce_pdf_partials, kernels/reduce_partials.py:61-79). */
+int fsk_kde16_splitk(const float *data, int64_t ld, int64_t n,
+                     const float *query, int64_t ldq, int64_t m,
+                     double bandwidth, unsigned flags,
+                     void *ws, size_t ws_bytes, float *out, void *stream);
+
+/* Same sums, one pass, per-tile atomics into out (zeroed by the call itself).
+ * Replaces kde_eval_16d_atomic, kernels/kde_eval_16d_stream_splitk.py:373-452,
+ * and gaussian_kde_triton_nd's launch, kernels/flash_sd_kde/kde.py:242-331. */
+int fsk_kde16_atomic(const float *data, int64_t ld, int64_t n,
+                     const float *query, int64_t ldq, int64_t m,
+                     double bandwidth, unsigned flags,
+                     void *ws, size_t ws_bytes, float *out, void *stream);
+
+/* 16-D log-density by log-sum-exp (precision form):
+ * out[i] = log( 1/(n (2 pi)^8 h^16) * sum_j exp(-|q_i - x_j|^2/(2h^2)) ),
+ * computed as per-split (max, sum) pairs in base 2 and an FP64 combine.
+ * Data are centred on the data mean before the cross term.
+ * No reference counterpart (the reference returns log(density + eps),
+ * flash_sd_kde/estimator.py:114); FP64 semantics follow
+ * experiments/error_suite_16d/reference.py:32-82. */
+int fsk_kde16_logdensity(const float *data, int64_t ld, int64_t n,
+                         const float *query, int64_t ldq, int64_t m,
+                         double bandwidth, unsigned flags,
+                         void *ws, size_t ws_bytes, double *out, void *stream);
+
+/* 1-D Gaussian KDE sums, split-K with FP64 combine.
+ * out[i] = sum_j exp(-(q_i - x_j)^2 / (2 h^2))  (FSK_FLAG_LAPLACIAN: times 1.5 - (q-x)^2/(2h^2)).
+ * Replaces kde_eval_1d_splitk, kernels/kde_eval_1d_splitk.py:141-219. */
+int fsk_kde1_splitk(const float *data, int64_t n, const float *query, int64_t m,
+                    double bandwidth, unsigned flags,
+                    void *ws, size_t ws_bytes, float *out, void *stream);
+
+/* 1-D sums with per-tile atomics (out zeroed by the call).
+ * Replaces kde_eval_1d_atomic, kernels/kde_eval_1d_splitk.py:222-256, and the
+ * launch in gaussian_kde_triton, kernels/flash_sd_kde/kde.py:147-225. */
+int fsk_kde1_atomic(const float *data, int64_t n, const float *query, int64_t m,
+                    double bandwidth, unsigned flags,
+                    void *ws, size_t ws_bytes, float *out, void *stream);
+
+/* 1-D KDE density-derivative sums: pdf_sum[i] = sum_j phi(z_ij),
+ * deriv_sum[i] = sum_j (-z_ij) phi(z_ij), z = (q - x)/h, phi = exp(-z^2/2).
+ * Replaces the launch in gaussian_kde_score_triton,
+ * kernels/flash_sd_kde/kde.py:348-438 (kernel :59-100). */
+int fsk_kde1_score(const float *data, int64_t n, const float *query, int64_t m,
+                   double bandwidth, void *ws, size_t ws_bytes,
+                   float *pdf_sum, float *deriv_sum, void *stream);
+
+/* Empirical-score accumulators over all ordered pairs (j = i included):
+ *   pdf[i] = sum_j exp(-|x_i - x_j|^2/(2h^2)),  w[i,:] = sum_j exp(...) x_j.
+ * w is written at w[i*ldw_row + f*ldw_col].
+ * Symmetric variant: each unordered tile pair is evaluated once and scattered
+ * to both sides with atomics (non-deterministic summation order).
+ * Replaces emp_score_16d_symmetric_atomic,
+ * kernels/emp_score_16d_symmetric_atomic.py:163-234 (kernel :10-103), and the
+ * default flash backend emp_score_16d_flash_sd_kde,
+ * kernels/flash_sd_kde/empirical.py:134-223. */
+int fsk_score16_sym_atomic(const float *data, int64_t ld, int64_t n,
+                           double bandwidth, unsigned flags,
+                           void *ws, size_t ws_bytes,
+                           float *pdf, float *w, int64_t ldw_row, int64_t ldw_col,
+                           void *stream);
+
+/* Same accumulators, deterministic: bounded split-K over the data with
+ * FP32 partials and an FP64 combine; bitwise reproducible for fixed n.
+ * Replaces emp_score_16d_ordered_splitk,
+ * kernels/emp_score_16d_ordered_splitk.py:162-263 (pass A :14-92, reduce
+ * kernels/reduce_partials.py:82-108). */
+int fsk_score16_ordered(const float *data, int64_t ld, int64_t n,
+                        double bandwidth, unsigned flags,
+                        void *ws, size_t ws_bytes,
+                        float *pdf, float *w, int64_t ldw_row, int64_t ldw_col,
+                        void *stream);
+
+/* FP64 split-K combine: out[i] = (float) sum_s (double) partials[s*ld_split + i].
+ * Replaces reduce_pdf_partials, kernels/reduce_partials.py:61-79. */
+int fsk_reduce_partials_f64(const float *partials, int64_t n_splits, int64_t m,
+                            int64_t ld_split, float *out, void *stream);
+
+/* out[i*ldo_row + f*ldo_col] = (float) sum_s (double) partials[s*ld_split + i*16 + f].
+ * Replaces reduce_weighted_partials, kernels/reduce_partials.py:82-108. */
+int fsk_reduce_weighted_partials_f64(const float *partials, int64_t n_splits,
+                                     int64_t m, int64_t ld_split, float *out,
+                                     int64_t ldo_row, int64_t ldo_col, void *stream);
+
+/* One-step SD-KDE debias, in FP64:
+ *   score = (w/(pdf + eps) - x) / h^2 ;  out = x + (h^2/2) score
+ * out has row stride ldo; score_out (nullable) is (n,16) contiguous.
+ * Replaces the torch epilogue of emp_sd_kde_fit_transform,
+ * flash_sd_kde/kde.py:316-322, and of empirical_sd_kde_triton_nd,
+ * kernels/flash_sd_kde/empirical.py:280-283. */
+int fsk_sdkde_debias_f64(const float *x, int64_t ldx, int64_t n,
+                         const float *pdf, const float *w, int64_t ldw_row,
+                         int64_t ldw_col, double bandwidth, double eps,
+                         float *out, int64_t ldo, float *score_out, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FLASHSDKDE_H */

@@ -1,0 +1,311 @@
+"""GPU parity: the HIP kernels against the reference's golden vectors and the
+FP64 oracle, on the reference tests' seeds/shapes and the small workloads.
+
+Tolerances (FP32 forms): relative error vs the reference's FP32 outputs
+<= 2e-5, and relative error vs the FP64 oracle <= 2x the reference's own
+FP32 error vs FP64 (+1e-6). Both evaluate the clamped norm expansion
+|q|^2 + |x|^2 - 2 q.x in FP32, whose rounding at magnitude |q|^2/h^2 sets a
+noise floor of ~1e-5 relative for 16-D N(0, I) data; the test requires the
+GPU to sit on the same floor, and every case records both errors.
+"""
+
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+import torch
+
+import flash_sd_kde
+from flash_sd_kde import _ops
+from kernels.emp_score_16d_ordered_splitk import emp_score_16d_ordered_splitk
+from kernels.emp_score_16d_symmetric_atomic import emp_score_16d_symmetric_atomic
+from kernels.flash_sd_kde import (
+    emp_score_16d_flash_sd_kde,
+    empirical_sd_kde_triton,
+    empirical_sd_kde_triton_nd,
+    gaussian_kde_score_triton,
+    gaussian_kde_triton,
+    gaussian_kde_triton_nd,
+)
+from kernels.kde_eval_16d_stream_splitk import kde_eval_16d_atomic, kde_eval_16d_splitk
+from oracle import fp64
+from oracle import reference_np as R
+
+pytestmark = pytest.mark.gpu
+
+IEEE = "fp32_ieee"
+
+
+def rel(a, b):
+    a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+    return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-300)))
+
+
+def record(name, **vals):
+    path = os.environ.get("FSK_METRICS_OUT")
+    print(name, vals, flush=True)
+    if path:
+        with open(path, "a") as f:
+            f.write(json.dumps({"test": name, **vals}) + "\n")
+
+
+def cpu(t):
+    return t.detach().cpu().numpy()
+
+
+def check_vs_ref(name, got, ref, exact, tol=2e-5, slack=1e-6):
+    e_ref = rel(got, ref)
+    e_gpu64 = rel(got, exact)
+    e_ref64 = rel(ref, exact)
+    record(name, rel_vs_reference=e_ref, rel_vs_fp64=e_gpu64, reference_rel_vs_fp64=e_ref64)
+    assert e_ref <= tol, (e_ref, tol)
+    assert e_gpu64 <= 2.0 * e_ref64 + slack, (e_gpu64, e_ref64)
+
+
+# ---------------- 16-D KDE ----------------
+
+@pytest.mark.small
+@pytest.mark.parametrize("backend", ["splitk_stream", "atomic"])
+def test_kde16_matches_reference(gpu, golden, backend):
+    x, q, h = golden("kde16", "x"), golden("kde16", "q"), golden("kde16", "h")
+    out = flash_sd_kde.kde_eval(x, q, h, device=gpu, precision_mode=IEEE, kde_backend=backend)
+    check_vs_ref(f"kde16_{backend}", cpu(out), golden("kde16", "density"),
+                 fp64.kde_density(q, x, h))
+
+
+@pytest.mark.small
+@pytest.mark.parametrize("case", ["kde16_lin", "kde16_lin_b"])
+def test_kde16_linearized_matches_reference(gpu, golden, case):
+    x, q, h = golden(case, "x"), golden(case, "q"), golden(case, "h")
+    out = flash_sd_kde.kde_eval_linearized(x, q, h, device=gpu, precision_mode=IEEE)
+    check_vs_ref(f"{case}_fused", cpu(out), golden(case, "density"),
+                 fp64.kde_density(q, x, h, laplacian=True), tol=5e-5, slack=5e-6)
+    out2 = flash_sd_kde.kde_eval_linearized_nonfused(x, q, h, device=gpu, precision_mode=IEEE,
+                                                     chunk_size=64)
+    np.testing.assert_allclose(cpu(out2), golden(case, "density"), rtol=2e-3, atol=2e-3)
+
+
+@pytest.mark.large
+def test_kde16_chunking_equivalence(gpu, golden):
+    x, q, h = golden("kde_chunk", "x"), golden("kde_chunk", "q"), golden("kde_chunk", "h")
+    full = cpu(flash_sd_kde.kde_eval(x, q, h, device=gpu, precision_mode=IEEE))
+    a = cpu(flash_sd_kde.kde_eval(x, q[:512], h, device=gpu, precision_mode=IEEE))
+    b = cpu(flash_sd_kde.kde_eval(x, q[512:], h, device=gpu, precision_mode=IEEE))
+    np.testing.assert_array_equal(full, np.concatenate([a, b]))  # rows are independent
+    check_vs_ref("kde_chunk", full, golden("kde_chunk", "density"), fp64.kde_density(q, x, h))
+
+
+def test_kde16_plain_workload(gpu, golden):
+    """4096 x 16 train, 1024 queries, seed 0, both bandwidth rules."""
+    c = "cfg_plain"
+    x, q = golden(c, "x"), golden(c, "q")
+    for key in ("baseline", "silverman"):
+        h = golden(c, f"h_{key}")
+        exact = fp64.kde_density(q, x, h)
+        for backend in ("splitk_stream", "atomic"):
+            out = cpu(flash_sd_kde.kde_eval(x, q, h, device=gpu, precision_mode=IEEE,
+                                            kde_backend=backend))
+            check_vs_ref(f"plain_{key}_{backend}", out, golden(c, f"density_{key}"), exact)
+        out = cpu(gaussian_kde_triton_nd(x, q, h, device=gpu))
+        check_vs_ref(f"plain_{key}_flash_nd", out, golden(c, f"density_{key}"), exact)
+
+
+@pytest.mark.parametrize("n,m", [(1, 1), (1, 5), (7, 3), (17, 33), (255, 257), (1000, 1),
+                                 (4097, 300)])
+def test_kde16_ragged_sizes(gpu, n, m):
+    rng = np.random.default_rng(n * 7919 + m)
+    x = rng.normal(size=(n, 16)).astype(np.float32)
+    q = rng.normal(size=(m, 16)).astype(np.float32)
+    h = 1.3
+    ref = R.kde_eval_nd_numpy(q, x, h)
+    exact = fp64.kde_density(q, x, h)
+    for backend in ("splitk_stream", "atomic"):
+        out = cpu(flash_sd_kde.kde_eval(x, q, h, device=gpu, precision_mode=IEEE,
+                                        kde_backend=backend))
+        check_vs_ref(f"ragged_{n}_{m}_{backend}", out, ref, exact)
+    lap = cpu(flash_sd_kde.kde_eval_linearized(x, q, h, device=gpu))
+    np.testing.assert_allclose(lap, fp64.kde_density(q, x, h, laplacian=True), rtol=1e-4,
+                               atol=1e-30)
+
+
+def test_kde16_single_point_closed_form(gpu):
+    x = np.random.default_rng(5).normal(size=(1, 16)).astype(np.float32)
+    h = 0.8
+    out = cpu(flash_sd_kde.kde_eval(x, x, h, device=gpu))
+    assert abs(out[0] / ((2 * math.pi) ** -8 * h ** -16) - 1) < 1e-6
+
+
+def test_kde16_strided_and_tensor_inputs(gpu, golden):
+    x, q, h = golden("kde16", "x"), golden("kde16", "q"), golden("kde16", "h")
+    xt = torch.zeros((x.shape[0], 20), device=gpu)
+    xt[:, :16] = torch.as_tensor(x, device=gpu)
+    out = _ops.kde16_sums(xt[:, :16], torch.as_tensor(q, device=gpu), h)  # row stride 20
+    norm = 1.0 / ((2 * math.pi) ** 8 * h ** 16 * x.shape[0])
+    check_vs_ref("kde16_strided", cpu(out) * norm, golden("kde16", "density"),
+                 fp64.kde_density(q, x, h))
+
+
+# ---------------- 1-D KDE ----------------
+
+@pytest.mark.small
+@pytest.mark.parametrize("backend", ["splitk_stream", "atomic"])
+def test_kde1d_matches_reference(gpu, golden, backend):
+    x, q, h = golden("kde1d", "x"), golden("kde1d", "q"), golden("kde1d", "h")
+    out = cpu(flash_sd_kde.kde_eval(x, q, h, device=gpu, kde_backend=backend))
+    check_vs_ref(f"kde1d_{backend}", out, golden("kde1d", "density"), fp64.kde_density(q, x, h))
+    x, q, h = golden("kde1d_lin", "x"), golden("kde1d_lin", "q"), golden("kde1d_lin", "h")
+    out = cpu(flash_sd_kde.kde_eval_linearized(x, q, h, device=gpu, kde_backend=backend))
+    check_vs_ref(f"kde1d_lin_{backend}", out, golden("kde1d_lin", "density"),
+                 fp64.kde_density(q, x, h, laplacian=True), tol=5e-5, slack=5e-6)
+    out = cpu(flash_sd_kde.kde_eval_linearized_nonfused(x, q, h, device=gpu, chunk_size=64))
+    np.testing.assert_allclose(out, golden("kde1d_lin", "density"), rtol=2e-3, atol=2e-3)
+
+
+def test_kde1d_flash_and_score(gpu, golden):
+    x, q, h = golden("kde1d", "x"), golden("kde1d", "q"), golden("kde1d", "h")
+    out = cpu(gaussian_kde_triton(x, q, h, device=gpu))
+    check_vs_ref("kde1d_flash", out, golden("kde1d", "density"), fp64.kde_density(q, x, h))
+    x1, h1 = golden("one_d", "x"), golden("one_d", "h_silverman")
+    score, pdf, _ = gaussian_kde_score_triton(x1, x1, h1, device=gpu)
+    np.testing.assert_allclose(cpu(pdf), golden("one_d", "pdf"), rtol=2e-5)
+    np.testing.assert_allclose(cpu(score), golden("one_d", "score"), rtol=1e-4, atol=1e-4)
+    deb, hd = empirical_sd_kde_triton(x1, device=gpu)
+    assert abs(hd - golden("one_d_sd", "h")) < 1e-6
+    np.testing.assert_allclose(deb, golden("one_d_sd", "debiased"), rtol=1e-5, atol=1e-5)
+
+
+# ---------------- empirical score ----------------
+
+def score_exact(x, h):
+    return fp64.score_sums(x, h)
+
+
+@pytest.mark.small
+@pytest.mark.parametrize("case", ["score_a", "score_b", "score_large"])
+@pytest.mark.parametrize("variant", ["ordered", "symmetric", "flash"])
+def test_score_matches_reference(gpu, golden, case, variant):
+    x, h = golden(case, "x"), golden(case, "h")
+    if variant == "ordered":
+        pdf, w = emp_score_16d_ordered_splitk(x, h, device=gpu, precision_mode=IEEE,
+                                              use_precomputed_norms=True, autotune=False)
+    elif variant == "symmetric":
+        pdf, w = emp_score_16d_symmetric_atomic(x, h, device=gpu, precision_mode=IEEE,
+                                                use_precomputed_norms=True, block_size=64)
+    else:
+        pdf, w = emp_score_16d_flash_sd_kde(x, h, device=gpu)
+        assert w.shape == (x.shape[0], 16) and w.stride() == (1, x.shape[0])
+    pe, we = score_exact(x, h)
+    check_vs_ref(f"{case}_{variant}_pdf", cpu(pdf), golden(case, "pdf"), pe)
+    wg = cpu(w)
+    err_w = float(np.max(np.abs(wg - we)) / np.max(np.abs(we)))
+    err_w_ref = float(np.max(np.abs(golden(case, "weighted") - we)) / np.max(np.abs(we)))
+    record(f"{case}_{variant}_w", err_vs_fp64=err_w, reference_err_vs_fp64=err_w_ref)
+    np.testing.assert_allclose(wg, golden(case, "weighted"), rtol=1e-4, atol=2e-5)
+    assert err_w <= 2.0 * err_w_ref + 1e-6
+
+
+def test_score_ordered_is_deterministic(gpu):
+    x = np.random.default_rng(8).normal(size=(3000, 16)).astype(np.float32)
+    a = emp_score_16d_ordered_splitk(x, 1.1, device=gpu, precision_mode=IEEE,
+                                     use_precomputed_norms=True, autotune=False)
+    b = emp_score_16d_ordered_splitk(x, 1.1, device=gpu, precision_mode=IEEE,
+                                     use_precomputed_norms=True, autotune=False)
+    assert torch.equal(a[0], b[0]) and torch.equal(a[1], b[1])
+
+
+@pytest.mark.parametrize("n", [1, 2, 15, 130, 257, 1000, 2500])
+def test_score_ragged_and_symmetric_equals_ordered(gpu, n):
+    x = np.random.default_rng(n).normal(size=(n, 16)).astype(np.float32)
+    h = 0.95
+    po, wo = _ops.score16(torch.as_tensor(x, device=gpu), h, variant="ordered")
+    ps, ws = _ops.score16(torch.as_tensor(x, device=gpu), h, variant="symmetric")
+    pe, we = score_exact(x, h)
+    np.testing.assert_allclose(cpu(po), pe, rtol=2e-5)
+    np.testing.assert_allclose(cpu(ps), pe, rtol=2e-5)
+    scale = max(1.0, float(np.max(np.abs(we))))
+    assert np.max(np.abs(cpu(wo) - we)) / scale < 2e-5
+    assert np.max(np.abs(cpu(ws) - we)) / scale < 2e-5
+    if n == 1:
+        assert cpu(po)[0] == 1.0 and np.allclose(cpu(wo), x)
+
+
+def test_score_permutation_equivariance(gpu):
+    x = np.random.default_rng(9).normal(size=(1500, 16)).astype(np.float32)
+    perm = np.random.default_rng(10).permutation(1500)
+    p1, w1 = _ops.score16(torch.as_tensor(x, device=gpu), 1.0, variant="ordered")
+    p2, w2 = _ops.score16(torch.as_tensor(x[perm], device=gpu), 1.0, variant="ordered")
+    np.testing.assert_allclose(cpu(p1)[perm], cpu(p2), rtol=1e-5)
+    np.testing.assert_allclose(cpu(w1)[perm], cpu(w2), rtol=1e-5, atol=1e-5)
+
+
+# ---------------- debias / SD-KDE pipeline ----------------
+
+@pytest.mark.small
+@pytest.mark.parametrize("backend", [None, "ordered_splitk", "symmetric_atomic"])
+def test_fit_transform_matches_reference(gpu, golden, backend):
+    x, h = golden("transform", "x"), golden("transform", "h")
+    deb = cpu(flash_sd_kde.emp_sd_kde_fit_transform(x, h, device=gpu, precision_mode=IEEE,
+                                                    emp_score_backend=backend))
+    pe, we = score_exact(x, h)
+    exact, _ = fp64.debias(x, pe, we, h)
+    e_gpu = float(np.max(np.abs(deb - exact)))
+    e_ref = float(np.max(np.abs(golden("transform", "debiased") - exact)))
+    record(f"transform_{backend}", abs_err_vs_fp64=e_gpu, reference_abs_err_vs_fp64=e_ref)
+    np.testing.assert_allclose(deb, golden("transform", "debiased"), rtol=1e-5, atol=1e-5)
+    assert e_gpu <= 2.0 * e_ref + 1e-6
+
+
+def test_sdkde_pipeline_gmm_small(gpu, golden):
+    c = "cfg_gmm_small"
+    x, q, h = golden(c, "x"), golden(c, "q"), golden(c, "h")
+    deb, pdf, w, score = flash_sd_kde.emp_sd_kde_fit_transform(
+        x, h, device=gpu, precision_mode=IEEE, emp_score_backend="ordered_splitk",
+        return_debug=True)
+    pe, we = score_exact(x, h)
+    check_vs_ref("gmm_small_pdf", cpu(pdf), golden(c, "pdf"), pe)
+    exact_deb, _ = fp64.debias(x, pe, we, h)
+    e_gpu = float(np.max(np.abs(cpu(deb) - exact_deb)))
+    e_ref = float(np.max(np.abs(golden(c, "debiased") - exact_deb)))
+    record("gmm_small_debias", abs_err_vs_fp64=e_gpu, reference_abs_err_vs_fp64=e_ref)
+    assert e_gpu <= 2.0 * e_ref + 1e-6
+    dens = cpu(flash_sd_kde.kde_eval(deb, q, h, device=gpu, precision_mode=IEEE))
+    check_vs_ref("gmm_small_density", dens, golden(c, "density"),
+                 fp64.kde_density(q, exact_deb.astype(np.float32), h), tol=5e-5, slack=5e-6)
+    deb2, h2 = empirical_sd_kde_triton_nd(x, h, device=gpu)
+    np.testing.assert_allclose(deb2, golden(c, "debiased"), rtol=1e-5, atol=1e-5)
+
+
+def test_estimator_end_to_end(gpu, golden):
+    c = "cfg_gmm_small"
+    x, q, h = golden(c, "x"), golden(c, "q"), golden(c, "h")
+    est = flash_sd_kde.FlashSDKDE(mode="sd_kde", bandwidth=h, device=gpu).fit(x)
+    ls = est.score_samples(q)
+    np.testing.assert_allclose(ls, np.log(golden(c, "density") + 1e-12), rtol=1e-6, atol=1e-6)
+    est = flash_sd_kde.FlashSDKDE(mode="kde", device=gpu).fit(golden("cfg_plain", "x"))
+    assert abs(est.bandwidth_ - golden("cfg_plain", "h_silverman")) < 1e-9
+    ls = est.score_samples(golden("cfg_plain", "q"))
+    np.testing.assert_allclose(ls, np.log(golden("cfg_plain", "density_silverman") + 1e-12),
+                               rtol=1e-6, atol=1e-6)
+    x1 = golden("kde1d", "x")[:, None]
+    est1 = flash_sd_kde.FlashSDKDE(mode="kde", device=gpu).fit(x1)
+    d1 = np.exp(est1.score_samples(golden("kde1d", "q")[:, None])) - 1e-12
+    np.testing.assert_allclose(d1, R.kde_eval_1d_numpy(golden("kde1d", "q"), x1[:, 0],
+                                                       est1.bandwidth_), rtol=1e-5)
+    assert np.isfinite(flash_sd_kde.FlashSDKDE(mode="sdkde", device=gpu).fit(x).score(q))
+
+
+# ---------------- log-density precision forms ----------------
+
+@pytest.mark.parametrize("precision,tol", [("fp32", 2e-6), ("fp32_valu", 2e-6),
+                                           ("bf16x3", 1e-4), ("fp16", 1e-1), ("bf16", 1.0)])
+def test_logdensity_precision_forms(gpu, golden, precision, tol):
+    c = "cfg_gmm_small"
+    x, q, h = golden(c, "x"), golden(c, "q"), golden(c, "h")
+    exact = fp64.log_density(q, x, h)
+    lp = cpu(_ops.kde16_logdensity(torch.as_tensor(x, device=gpu),
+                                   torch.as_tensor(q, device=gpu), h, precision=precision))
+    mx, med = fp64.rel_log_error(lp, exact)
+    record(f"logdensity_{precision}", max_rel=mx, median_rel=med)
+    assert mx <= tol

@@ -1,0 +1,85 @@
+"""Host-side API behaviour that needs no GPU: validation, error paths,
+bandwidth rules, backend selection (reference error behaviour, SURVEY-listed
+in flash_sd_kde/kde.py:64-71, estimator.py:93-157, config.py:46-75)."""
+
+import numpy as np
+import pytest
+import torch
+
+import flash_sd_kde
+from flash_sd_kde import bandwidth as bw
+from flash_sd_kde.config import (
+    select_emp_score_backend,
+    validate_emp_score_backend,
+    validate_kde_backend,
+    validate_precision_mode,
+)
+from globals import EMP_SCORE_BACKEND_FLASH_SD_KDE
+
+
+def test_validators():
+    validate_precision_mode("fast_tf32")
+    validate_precision_mode("fp32_ieee")
+    with pytest.raises(ValueError):
+        validate_precision_mode("tf16")
+    validate_kde_backend("atomic")
+    with pytest.raises(ValueError):
+        validate_kde_backend("dense")
+    validate_emp_score_backend("ordered_splitk")
+    with pytest.raises(ValueError):
+        validate_emp_score_backend("nope")
+    assert select_emp_score_backend(10) == EMP_SCORE_BACKEND_FLASH_SD_KDE
+
+
+def test_bandwidth_rules_match_reference(golden):
+    x = golden("cfg_plain", "x")
+    assert abs(bw.silverman_bandwidth_nd(x) - golden("cfg_plain", "h_silverman")) < 1e-12
+    assert abs(bw.scott_mean_std_bandwidth(x) - golden("cfg_plain", "h_baseline")) < 1e-12
+    x1 = golden("one_d", "x")
+    assert abs(bw.silverman_bandwidth(x1) - golden("one_d", "h_silverman")) < 1e-12
+    with pytest.raises(ValueError):
+        bw.silverman_bandwidth_nd(np.zeros((0, 16), np.float32))
+
+
+def test_kde_eval_argument_errors():
+    x = np.zeros((4, 16), np.float32)
+    with pytest.raises(ValueError, match="bandwidth"):
+        flash_sd_kde.kde_eval(x, x, 0.0)
+    with pytest.raises(ValueError, match="precision_mode"):
+        flash_sd_kde.kde_eval(x, x, 1.0, precision_mode="tf16")
+    with pytest.raises(ValueError, match="kde_backend"):
+        flash_sd_kde.kde_eval(x, x, 1.0, kde_backend="dense")
+    with pytest.raises(ValueError, match="CUDA"):
+        flash_sd_kde.kde_eval(x, x, 1.0, device="cpu")
+    if not torch.cuda.is_available():
+        with pytest.raises(RuntimeError, match="CUDA is not available"):
+            flash_sd_kde.kde_eval(x, x, 1.0)
+    with pytest.raises(ValueError, match="bandwidth"):
+        flash_sd_kde.emp_sd_kde_fit_transform(x, -1.0)
+    with pytest.raises(ValueError, match="emp_score_backend"):
+        flash_sd_kde.emp_sd_kde_fit_transform(x, 1.0, emp_score_backend="x")
+
+
+def test_estimator_errors():
+    est = flash_sd_kde.FlashSDKDE(mode="kde")
+    with pytest.raises(ValueError, match="not fitted"):
+        est.score_samples(np.zeros((2, 16)))
+    with pytest.raises(ValueError, match="shape"):
+        est.fit(np.zeros(5))
+    with pytest.raises(ValueError, match="at least one sample"):
+        est.fit(np.zeros((0, 16)))
+    with pytest.raises(ValueError, match="mode"):
+        flash_sd_kde.FlashSDKDE(mode="bogus", device="cpu").fit(np.zeros((3, 16)))
+    with pytest.raises(ValueError, match="16 features"):
+        flash_sd_kde.FlashSDKDE(mode="sd_kde").fit(np.zeros((3, 4)))
+    with pytest.raises(ValueError, match="bandwidth"):
+        flash_sd_kde.FlashSDKDE(bandwidth=-1.0).fit(np.zeros((3, 16)))
+    with pytest.raises(ValueError, match="Unsupported bandwidth"):
+        flash_sd_kde.FlashSDKDE(bandwidth="scott").fit(np.zeros((3, 16)))
+
+
+def test_cpu_tensors_are_rejected_not_computed():
+    from kernels.reduce_partials import reduce_pdf_partials
+
+    with pytest.raises((ValueError, RuntimeError)):
+        reduce_pdf_partials(torch.ones(3, 5))
