@@ -4,8 +4,9 @@
 One step = the plain-FP32 SD-KDE pass at n = 1,048,576 training points drawn
 from an 8-component isotropic Gaussian mixture (means ~ N(0, 9 I), sigma 1,
 seed 0) and m = 65,536 queries from the same mixture (seed 1):
-  1. empirical-score accumulators over all n^2 ordered pairs (symmetric kernel,
-     each unordered pair evaluated once),
+  1. empirical-score accumulators over all n^2 ordered pairs (deterministic
+     ordered split-K kernel by default; --variant symmetric evaluates each
+     unordered pair once and scatters with atomics),
   2. FP64 debias x + (h^2/2) score,
   3. KDE of the m queries against the n debiased points.
 value = (m n + n^2) / step time, inputs resident in HBM.
@@ -40,7 +41,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=1 << 20)
     ap.add_argument("--m", type=int, default=65536)
-    ap.add_argument("--variant", choices=["symmetric", "ordered"], default="symmetric")
+    ap.add_argument("--variant", choices=["symmetric", "ordered"], default="ordered")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     return ap.parse_args()

@@ -11,6 +11,7 @@
 // Padded tail points carry c|x|^2 = -1e30, so they contribute exactly 0.
 #include "fsk_common.h"
 #include "fsk_plan.h"
+#include "fsk_tile.h"
 
 namespace fsk {
 
@@ -52,34 +53,10 @@ __global__ __launch_bounds__(kWG) void kde16_f32_kernel(
   int64_t j_end = j_begin + split_len;
   if (j_end > n_pad) j_end = n_pad;
   int tc = 0;
-  for (int64_t j0 = j_begin; j0 < j_end; j0 += kTile) {
-    int64_t jr = j0 + c;
-    if (jr > n - 1) jr = n - 1;
-    const f4 xa = ld4(data + jr * ld + 4 * g);
-    const f4 bb = ld4(bn + j0 + 4 * g);
+  for_each_tile<false>(data, ld, n, bn, j_begin, j_end, g, c, [&](const TileData &t) {
     f4 d[QT];
-#pragma unroll
-    for (int R = 0; R < QT; ++R) {
-      d[R][0] = qav[R] + bb[0];
-      d[R][1] = qav[R] + bb[1];
-      d[R][2] = qav[R] + bb[2];
-      d[R][3] = qav[R] + bb[3];
-    }
-#pragma unroll
-    for (int s = 0; s < 4; ++s) {
-#pragma unroll
-      for (int R = 0; R < QT; ++R) d[R] = mfma_f32(xa[s], qv[R][s], d[R]);
-    }
-#pragma unroll
-    for (int R = 0; R < QT; ++R) {
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        float t = fminf(d[R][r], 0.f);
-        float e = fexp2(t);
-        acc[R] += e;
-        if (LAPL) accT[R] = fmaf(e, t, accT[R]);
-      }
-    }
+    cross_tile<QT>(qv, qav, t, d);
+    kernel_tile<QT, LAPL>(d, acc, accT);
     if (++tc == kFlushTiles) {
       tc = 0;
 #pragma unroll
@@ -92,7 +69,7 @@ __global__ __launch_bounds__(kWG) void kde16_f32_kernel(
         }
       }
     }
-  }
+  });
 #pragma unroll
   for (int R = 0; R < QT; ++R) {
     double v = acc64[R] + acc[R];

@@ -1,0 +1,139 @@
+// fsk_microbench.hip - issue-rate microbenchmarks for the roofline of the pair
+// kernels: FP32 MFMA 16x16x4, BF16 MFMA 16x16x32, v_exp_f32, v_fma_f32, and
+// an MFMA + exp mix that shows whether the two pipes overlap.
+//
+// Every kernel runs independent dependency chains per lane in a loop and
+// writes one value per thread to the caller's buffer so nothing is removed
+// by the compiler. Rates are whole-chip operations per second.
+#include "fsk_common.h"
+
+namespace fsk {
+
+typedef __bf16 bf16x8_mb __attribute__((ext_vector_type(8)));
+
+__global__ __launch_bounds__(256) void mb_mfma_f32(float *out, int iters) {
+  const float a = 1.0f + threadIdx.x * 1e-7f, b = 0.999f;
+  f4 c0 = {0, 0, 0, 0}, c1 = c0, c2 = c0, c3 = c0;
+  for (int i = 0; i < iters; ++i) {
+    c0 = mfma_f32(a, b, c0);
+    c1 = mfma_f32(a, b, c1);
+    c2 = mfma_f32(a, b, c2);
+    c3 = mfma_f32(a, b, c3);
+  }
+  out[blockIdx.x * blockDim.x + threadIdx.x] = c0[0] + c1[1] + c2[2] + c3[3];
+}
+
+__global__ __launch_bounds__(256) void mb_mfma_bf16(float *out, int iters) {
+  bf16x8_mb a, b;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    a[k] = (__bf16)(1.0f + k * 0.01f);
+    b[k] = (__bf16)(0.5f);
+  }
+  f4 c0 = {0, 0, 0, 0}, c1 = c0, c2 = c0, c3 = c0;
+  for (int i = 0; i < iters; ++i) {
+    c0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c0, 0, 0, 0);
+    c1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c1, 0, 0, 0);
+    c2 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c2, 0, 0, 0);
+    c3 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c3, 0, 0, 0);
+  }
+  out[blockIdx.x * blockDim.x + threadIdx.x] = c0[0] + c1[1] + c2[2] + c3[3];
+}
+
+__global__ __launch_bounds__(256) void mb_exp(float *out, int iters) {
+  float v[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) v[k] = (threadIdx.x + k) * 1e-3f;
+  for (int i = 0; i < iters; ++i) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] = fexp2(-v[k]);
+  }
+  float s = 0.f;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) s += v[k];
+  out[blockIdx.x * blockDim.x + threadIdx.x] = s;
+}
+
+__global__ __launch_bounds__(256) void mb_fma(float *out, int iters) {
+  float v[8];
+  const float a = 0.999f, b = 1e-3f;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) v[k] = (threadIdx.x + k) * 1e-3f;
+  for (int i = 0; i < iters; ++i) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] = fmaf(v[k], a, b);
+  }
+  float s = 0.f;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) s += v[k];
+  out[blockIdx.x * blockDim.x + threadIdx.x] = s;
+}
+
+// 4 FP32 MFMAs (4 x 2048 flop) and 16 exps per lane per iteration: the
+// pair-kernel ratio (one 16x16 tile = 256 pairs = 4 exps per lane per 4 MFMAs
+// for the cross term, times 4 row tiles).
+__global__ __launch_bounds__(256) void mb_mix(float *out, int iters) {
+  const float a = 1.0f + threadIdx.x * 1e-7f, b = 0.999f;
+  f4 c0 = {0, 0, 0, 0}, c1 = c0, c2 = c0, c3 = c0;
+  float v[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) v[k] = (threadIdx.x + k) * 1e-3f;
+  for (int i = 0; i < iters; ++i) {
+    c0 = mfma_f32(a, b, c0);
+    c1 = mfma_f32(a, b, c1);
+    c2 = mfma_f32(a, b, c2);
+    c3 = mfma_f32(a, b, c3);
+#pragma unroll
+    for (int k = 0; k < 16; ++k) v[k] = fexp2(-v[k]);
+  }
+  float s = c0[0] + c1[1] + c2[2] + c3[3];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) s += v[k];
+  out[blockIdx.x * blockDim.x + threadIdx.x] = s;
+}
+
+}  // namespace fsk
+
+using namespace fsk;
+
+// which: 0 = FP32 MFMA (flop/s), 1 = BF16 MFMA (flop/s), 2 = v_exp_f32 (exp/s),
+// 3 = v_fma_f32 (fma/s), 4 = mix (returns seconds per iteration-block, see header).
+extern "C" int fsk_microbench(int which, int blocks, int iters, float *out, double *result,
+                              void *stream) {
+  if (which < 0 || which > 4 || blocks <= 0 || iters <= 0 || !out || !result) {
+    set_error("fsk_microbench: bad arguments");
+    return FSK_EINVAL;
+  }
+  hipStream_t s = as_stream(stream);
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+    set_error("fsk_microbench: event creation failed");
+    return FSK_ELAUNCH;
+  }
+  (void)hipEventRecord(e0, s);
+  switch (which) {
+    case 0: hipLaunchKernelGGL(mb_mfma_f32, dim3(blocks), dim3(256), 0, s, out, iters); break;
+    case 1: hipLaunchKernelGGL(mb_mfma_bf16, dim3(blocks), dim3(256), 0, s, out, iters); break;
+    case 2: hipLaunchKernelGGL(mb_exp, dim3(blocks), dim3(256), 0, s, out, iters); break;
+    case 3: hipLaunchKernelGGL(mb_fma, dim3(blocks), dim3(256), 0, s, out, iters); break;
+    default: hipLaunchKernelGGL(mb_mix, dim3(blocks), dim3(256), 0, s, out, iters); break;
+  }
+  (void)hipEventRecord(e1, s);
+  int st = launch_status("fsk_microbench");
+  if (st == FSK_OK && hipEventSynchronize(e1) != hipSuccess) st = FSK_ELAUNCH;
+  float ms = 0.f;
+  if (st == FSK_OK) (void)hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (st != FSK_OK) return st;
+  const double waves = double(blocks) * 4.0;
+  const double sec = ms * 1e-3;
+  switch (which) {
+    case 0: *result = waves * iters * 4.0 * 2048.0 / sec; break;
+    case 1: *result = waves * iters * 4.0 * 16384.0 / sec; break;
+    case 2: *result = waves * 64.0 * iters * 8.0 / sec; break;
+    case 3: *result = waves * 64.0 * iters * 8.0 / sec; break;
+    default: *result = sec; break;
+  }
+  return FSK_OK;
+}

@@ -13,7 +13,10 @@ namespace fsk {
 constexpr int kKdeQT = 4;
 constexpr int kKdeRowsPerWG = 4 * 16 * kKdeQT;  // 256 queries
 // 16-D score (ordered): rows per workgroup.
-constexpr int kScoreQT = 4;
+#ifndef FSK_SCORE_QT
+#define FSK_SCORE_QT 4
+#endif
+constexpr int kScoreQT = FSK_SCORE_QT;
 constexpr int kScoreRowsPerWG = 4 * 16 * kScoreQT;  // 256 rows
 // Symmetric score: I block = rows of one workgroup (4 waves x kSymQT tiles).
 constexpr int kSymQT = 4;

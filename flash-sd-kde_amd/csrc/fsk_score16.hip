@@ -18,10 +18,18 @@
 // atomics, then converted to the FP32 outputs.
 #include "fsk_common.h"
 #include "fsk_plan.h"
+#include "fsk_tile.h"
 
 namespace fsk {
 
 constexpr int kScoreFlush = 32;
+
+#ifndef FSK_SYM_JC
+#define FSK_SYM_JC 8
+#endif
+#ifndef FSK_SYM_BAND
+#define FSK_SYM_BAND 1
+#endif
 
 template <int QT>
 __global__ __launch_bounds__(kWG) void score16_ordered_kernel(
@@ -56,46 +64,13 @@ __global__ __launch_bounds__(kWG) void score16_ordered_kernel(
   const int64_t j_begin = split * split_len;
   int64_t j_end = j_begin + split_len;
   if (j_end > n_pad) j_end = n_pad;
+  float unused[QT];
   int tc = 0;
-  for (int64_t j0 = j_begin; j0 < j_end; j0 += kTile) {
-    int64_t jr = j0 + c;
-    if (jr > n - 1) jr = n - 1;
-    const f4 xa = ld4(data + jr * ld + 4 * g);
-    const f4 bb = ld4(bn + j0 + 4 * g);
-    float xt[4];
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      int64_t jt = j0 + 4 * g + r;
-      if (jt > n - 1) jt = n - 1;
-      xt[r] = data[jt * ld + c];
-    }
+  for_each_tile<true>(data, ld, n, bn, j_begin, j_end, g, c, [&](const TileData &t) {
     f4 d[QT];
-#pragma unroll
-    for (int R = 0; R < QT; ++R) {
-      d[R][0] = qav[R] + bb[0];
-      d[R][1] = qav[R] + bb[1];
-      d[R][2] = qav[R] + bb[2];
-      d[R][3] = qav[R] + bb[3];
-    }
-#pragma unroll
-    for (int s = 0; s < 4; ++s) {
-#pragma unroll
-      for (int R = 0; R < QT; ++R) d[R] = mfma_f32(xa[s], qv[R][s], d[R]);
-    }
-#pragma unroll
-    for (int R = 0; R < QT; ++R) {
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        float e = fexp2(fminf(d[R][r], 0.f));
-        d[R][r] = e;
-        pdf[R] += e;
-      }
-    }
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-#pragma unroll
-      for (int R = 0; R < QT; ++R) w[R] = mfma_f32(d[R][r], xt[r], w[R]);
-    }
+    cross_tile<QT>(qv, qav, t, d);
+    kernel_tile<QT, false>(d, pdf, unused);
+    phi_x_tile<QT>(d, t, w);
     if (++tc == kScoreFlush) {
       tc = 0;
 #pragma unroll
@@ -107,7 +82,7 @@ __global__ __launch_bounds__(kWG) void score16_ordered_kernel(
         w[R] = f4{0.f, 0.f, 0.f, 0.f};
       }
     }
-  }
+  });
 #pragma unroll
   for (int R = 0; R < QT; ++R) {
     double p = sum_groups(pdf64[R] + pdf[R]);
@@ -121,9 +96,12 @@ __global__ __launch_bounds__(kWG) void score16_ordered_kernel(
   }
 }
 
-// Symmetric variant. Task = (I block, chunk of J blocks), blocks of B rows,
-// only J >= I. For J > I both sides accumulate; J == I runs the I side over
-// the whole diagonal block (every ordered pair inside it exactly once).
+// Symmetric variant. Task = (I block, window of jc J blocks), blocks of B
+// rows, only J >= I. For J > I both sides accumulate; J == I runs the I side
+// over the whole diagonal block (every ordered pair inside it exactly once).
+// Windows are placed relative to I (chunk k covers J in [I + k jc, I + (k+1) jc)),
+// so the workgroups an XCD runs together (consecutive I, same k) scatter
+// their column sums into staggered J blocks instead of the same ones.
 template <int QT, int WAVES>
 __global__ __launch_bounds__(64 * WAVES) void score16_sym_kernel(
     const float *__restrict__ data, int64_t ld, int64_t n, const float *__restrict__ bn,
@@ -139,10 +117,16 @@ __global__ __launch_bounds__(64 * WAVES) void score16_sym_kernel(
   if (task >= ntasks) return;
   const int64_t chunk = task / nb;
   const int64_t I = task - chunk * nb;
+#if FSK_SYM_BAND
+  const int64_t Jlo = I + chunk * jc;
+  int64_t Jhi = Jlo + jc;
+  if (Jhi > nb) Jhi = nb;
+#else
   int64_t Jlo = chunk * jc;
   int64_t Jhi = Jlo + jc;
   if (Jhi > nb) Jhi = nb;
   if (Jlo < I) Jlo = I;
+#endif
   if (Jlo >= Jhi) return;
 
   const int tid = threadIdx.x;
@@ -280,7 +264,7 @@ __global__ void score_finish_kernel(const double *__restrict__ pdf64,
 
 constexpr int kSymWaves = 4;
 constexpr int kSymB = kSymWaves * 16 * kSymQT;
-constexpr int64_t kSymJC = 8;  // J blocks per task
+constexpr int64_t kSymJC = FSK_SYM_JC;  // J blocks per task
 
 struct ScoreLayout {
   float *bn;

@@ -53,6 +53,7 @@ SIGNATURES = {
     "fsk_workspace_bytes": [ctypes.c_int, _i64, _i64, _u],
     "fsk_profile_enable": [ctypes.c_int],
     "fsk_profile_last_ms": [ctypes.c_int],
+    "fsk_microbench": [ctypes.c_int, ctypes.c_int, ctypes.c_int, _p, _p, _p],
     "fsk_kde16_splitk": [_p, _i64, _i64, _p, _i64, _i64, _d, _u, _p, _sz, _p, _p],
     "fsk_kde16_atomic": [_p, _i64, _i64, _p, _i64, _i64, _d, _u, _p, _sz, _p, _p],
     "fsk_kde16_logdensity": [_p, _i64, _i64, _p, _i64, _i64, _d, _u, _p, _sz, _p, _p],
