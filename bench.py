@@ -44,6 +44,8 @@ def parse():
     ap.add_argument("--variant", choices=["symmetric", "ordered"], default="ordered")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-precision-forms", action="store_true")
+    ap.add_argument("--error-queries", type=int, default=4096)
     return ap.parse_args()
 
 
@@ -87,6 +89,58 @@ def cpu_baseline(x, q, h, budget_s):
                    f"queries alternating) x all n={x.shape[0]} points, {pairs:.3g} pairs in "
                    f"{dt:.1f} s; numpy FP32 oracle, {threads} threads, BLAS 1 thread each"),
     }
+
+
+def precision_forms(xd, qd, h, fp32_ms, fp32_dens, steps, warmup, n_err):
+    """Reduced-precision SD-KDE forms on the same workload: score with 16-bit
+    MFMA operands (FP32 accumulate), FP64 debias, log-sum-exp KDE with the same
+    cross-term precision. Errors: max / median relative log-density error on
+    the first n_err queries against the FP64 GPU reference pipeline (FP64
+    direct-difference score, debias and log-sum-exp)."""
+    import math
+
+    import torch
+
+    from flash_sd_kde import _native, _ops, precision
+
+    n, m = xd.shape[0], qd.shape[0]
+    sub = qd[:n_err]
+    t0 = time.perf_counter()
+    ref = precision.ref64_sdkde_logdensity(xd, sub, h, device=xd.device)
+    torch.cuda.synchronize()
+    ref_s = time.perf_counter() - t0
+    log_norm = -math.log(n) - 16 * math.log(h) - 8 * math.log(2 * math.pi)
+    lp32 = torch.log(fp32_dens[:n_err].double()) + log_norm
+    mx, med = precision.rel_log_error(lp32, ref)
+    out = {"reference": f"FP64 GPU pipeline on {n_err} queries ({ref_s:.1f} s)",
+           "fp32": {"ms_per_step": round(fp32_ms, 3), "speedup_vs_fp32": 1.0,
+                    "max_rel_logp_err": mx, "median_rel_logp_err": med,
+                    "within_1e-3": mx <= 1e-3}}
+    lib = _native.lib()
+    pairs = float(m) * n + float(n) * n
+    for prec in ("bf16x3", "fp16", "bf16"):
+        def step():
+            pdf, w = _ops.score16(xd, h, variant="ordered", precision=prec)
+            deb = _ops.debias(xd, pdf, w, h, 1e-12)
+            return _ops.kde16_logdensity(deb, qd, h, precision=prec)
+        for _ in range(warmup):
+            step()
+        torch.cuda.synchronize()
+        lib.fsk_profile_enable(1)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            lp = step()
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) / steps * 1e3
+        score_ms = lib.fsk_profile_last_ms(_native.OP_SCORE16_ORDERED)
+        kde_ms = lib.fsk_profile_last_ms(_native.OP_KDE16_LOGDENSITY)
+        lib.fsk_profile_enable(0)
+        mx, med = precision.rel_log_error(lp[:n_err], ref)
+        out[prec] = {"ms_per_step": round(ms, 3), "gpair_evals_per_s": round(pairs / ms / 1e6, 3),
+                     "speedup_vs_fp32": round(fp32_ms / ms, 3), "score_kernel_ms": round(score_ms, 3),
+                     "kde_kernel_ms": round(kde_ms, 3), "max_rel_logp_err": mx,
+                     "median_rel_logp_err": med, "within_1e-3": mx <= 1e-3}
+    return out
 
 
 def main():
@@ -196,6 +250,9 @@ def main():
         },
         "outputs_finite": finite,
     }
+    if not args.no_precision_forms:
+        line["precision_forms"] = precision_forms(xd, qd, h, ms_per_step, dens, args.steps,
+                                                  args.warmup, args.error_queries)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(x, q, h, args.cpu_seconds)
     print(json.dumps(line), flush=True)

@@ -296,7 +296,7 @@ size_t score16_ws_bytes(int64_t n, bool ordered) {
 }
 
 static int score_check(const char *what, const float *data, int64_t ld, int64_t n, double h,
-                       unsigned flags, const float *pdf, const float *w) {
+                       unsigned flags, const float *pdf, const float *w, bool lowp_ok) {
   int st;
   if ((st = check_points16(what, data, ld, n)) != FSK_OK) return st;
   if (!(h > 0.0) || !isfinite(h)) {
@@ -307,7 +307,9 @@ static int score_check(const char *what, const float *data, int64_t ld, int64_t 
     set_error("%s: null output", what);
     return FSK_EINVAL;
   }
-  if ((flags & FSK_PREC_MASK) != FSK_PREC_F32 || (flags & FSK_FLAG_LAPLACIAN)) {
+  const unsigned prec = flags & FSK_PREC_MASK;
+  const bool lowp = prec == FSK_PREC_BF16 || prec == FSK_PREC_F16 || prec == FSK_PREC_BF16X3;
+  if ((prec != FSK_PREC_F32 && !(lowp && lowp_ok)) || (flags & ~FSK_PREC_MASK)) {
     set_error("%s: flags 0x%x not available on this path", what, flags);
     return FSK_EUNSUPPORTED;
   }
@@ -322,8 +324,11 @@ extern "C" int fsk_score16_ordered(const float *data, int64_t ld, int64_t n, dou
                                    unsigned flags, void *ws, size_t ws_bytes, float *pdf,
                                    float *w, int64_t ldw_row, int64_t ldw_col, void *stream) {
   const char *what = "fsk_score16_ordered";
-  int st = score_check(what, data, ld, n, bandwidth, flags, pdf, w);
+  int st = score_check(what, data, ld, n, bandwidth, flags, pdf, w, true);
   if (st != FSK_OK) return st;
+  if ((flags & FSK_PREC_MASK) != FSK_PREC_F32)
+    return score16_lp_run(data, ld, n, bandwidth, flags & FSK_PREC_MASK, ws, ws_bytes, pdf, w,
+                          ldw_row, ldw_col, stream);
   ScoreLayout L;
   size_t need = score_layout(ws, n, true, &L);
   if (!ws || ws_bytes < need) {
@@ -356,7 +361,7 @@ extern "C" int fsk_score16_sym_atomic(const float *data, int64_t ld, int64_t n,
                                       size_t ws_bytes, float *pdf, float *w, int64_t ldw_row,
                                       int64_t ldw_col, void *stream) {
   const char *what = "fsk_score16_sym_atomic";
-  int st = score_check(what, data, ld, n, bandwidth, flags, pdf, w);
+  int st = score_check(what, data, ld, n, bandwidth, flags, pdf, w, false);
   if (st != FSK_OK) return st;
   ScoreLayout L;
   size_t need = score_layout(ws, n, false, &L);

@@ -1,0 +1,259 @@
+// fsk_score16_lp.hip - empirical-score accumulators with 16-bit MFMA operands
+// (BF16 x3 split, single BF16, single FP16), ordered split-K, deterministic.
+//
+// Cross term: data / query 48-value records (fsk_lowp.h); BF16x3 uses
+//   A1 = [xh | xl] . B1 = [qh | qh]  and  A2 = [xh | fold_b] . B2 = [ql | fold_a]
+// (xh.qh + xl.qh + xh.ql + c|x|^2 + c|q|^2); single forms use A2 . [qh | fold_a].
+// phi = exp2(min(t, 0)) comes out with lane l holding phi(i = l%16, j = 4(l/16)+u),
+// u = 0..3, which is exactly four consecutive K slots of the 16x16x32 A operand,
+// so phi.X needs no data movement:
+//   A = [phi_h(u=0..3) | phi_l(u=0..3)] per lane (phi = phi_h + phi_l),
+//   W += A . [Xh | Xh]  +  A . [Xl | 0]      (BF16x3: drops phi_l.Xl)
+//   W += [phi | 0] . [Xh | 0]                (single forms)
+// X for phi.X comes from a tile-transposed copy: per 16-point tile, per feature
+// f and lane group g, [Xh(4g+u, f) u=0..3 | Xl(4g+u, f) u=0..3] (16 bytes).
+// Points are centred on the data mean: w' = sum phi (x - mu); the finish
+// kernel returns w = w' + mu * pdf in FP64.
+#include "fsk_common.h"
+#include "fsk_lowp.h"
+#include "fsk_plan.h"
+
+namespace fsk {
+
+constexpr int kLpFlush = 32;
+
+template <int P16>
+__global__ void prep_xt16_kernel(const float *__restrict__ x, int64_t ld, int64_t n,
+                                 int64_t n_pad, const double *__restrict__ mu_sum, double inv_n,
+                                 unsigned short *__restrict__ xt) {
+  const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= n_pad * kD) return;
+  const int64_t j = e / kD;
+  const int f = int(e % kD);
+  const float v = j < n ? float(double(x[j * ld + f]) - mu_sum[f] * inv_n) : 0.f;
+  const unsigned short hi = to16<P16>(v);
+  const unsigned short lo = to16<P16>(v - from16<P16>(hi));
+  const int64_t T = j / kTile;
+  const int g = int((j % kTile) / 4), u = int(j % 4);
+  const int64_t base = ((T * kD + f) * 4 + g) * 8;
+  xt[base + u] = hi;
+  xt[base + 4 + u] = lo;
+}
+
+template <int P16>
+__device__ __forceinline__ unsigned pack2(float a, float b) {
+  return unsigned(to16<P16>(a)) | (unsigned(to16<P16>(b)) << 16);
+}
+
+template <int PREC, int QT>
+__global__ __launch_bounds__(kWG) void score16_lp_kernel(
+    const unsigned short *__restrict__ xrec, const unsigned short *__restrict__ qrec,
+    const unsigned short *__restrict__ xt, int64_t n, int64_t n_pad16, int64_t split_len,
+    int64_t qblocks, int64_t ntasks, int64_t per_xcd, float *__restrict__ pdf_part,
+    float *__restrict__ w_part) {
+  constexpr int P16 = (PREC == FSK_PREC_F16) ? FSK_PREC_F16 : FSK_PREC_BF16;
+  constexpr bool X3 = (PREC == FSK_PREC_BF16X3);
+  const int64_t task = xcd_task(blockIdx.x, per_xcd);
+  if (task >= ntasks) return;
+  const int64_t split = task / qblocks;
+  const int64_t qb = task - split * qblocks;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int g = lane >> 4, c = lane & 15;
+  const int64_t row0 = qb * (64 * QT) + int64_t(wave) * (16 * QT);
+
+  u16x8 B1[QT], B2[QT];
+  f4 w[QT];
+  d4 w64[QT];
+  float pdf[QT];
+  double pdf64[QT];
+#pragma unroll
+  for (int R = 0; R < QT; ++R) {
+    const unsigned short *rq = qrec + (row0 + 16 * R + c) * kRec;  // records padded to 256 rows
+    if (X3) {
+      B1[R] = *reinterpret_cast<const u16x8 *>(rq + 8 * (g & 1));
+      B2[R] = *reinterpret_cast<const u16x8 *>(rq + 16 + 8 * g);
+    } else {
+      B1[R] = *reinterpret_cast<const u16x8 *>(rq + (g < 2 ? 8 * g : 16 + 8 * g));
+    }
+    w[R] = f4{0.f, 0.f, 0.f, 0.f};
+    w64[R] = d4{0.0, 0.0, 0.0, 0.0};
+    pdf[R] = 0.f;
+    pdf64[R] = 0.0;
+  }
+
+  const int64_t j_begin = split * split_len;
+  int64_t j_end = j_begin + split_len;
+  if (j_end > n_pad16) j_end = n_pad16;
+  int tc = 0;
+  for (int64_t j0 = j_begin; j0 < j_end; j0 += kTile) {
+    const unsigned short *rx = xrec + (j0 + c) * kRec;
+    const u16x8 A2 = *reinterpret_cast<const u16x8 *>(rx + (g < 2 ? 8 * g : 16 + 8 * g));
+    u16x8 A1;
+    if (X3) A1 = *reinterpret_cast<const u16x8 *>(rx + 8 * g);
+    const u16x8 XT = *reinterpret_cast<const u16x8 *>(xt + (((j0 / kTile) * kD + c) * 4 + g) * 8);
+    u16x8 BX1, BX2;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      BX1[u] = XT[u];
+      BX1[4 + u] = X3 ? XT[u] : (unsigned short)0;
+      BX2[u] = XT[4 + u];
+      BX2[4 + u] = 0;
+    }
+#pragma unroll
+    for (int R = 0; R < QT; ++R) {
+      f4 d;
+      if (X3) {
+        d = mfma16(A1, B1[R], f4{0.f, 0.f, 0.f, 0.f}, P16);
+        d = mfma16(A2, B2[R], d, P16);
+      } else {
+        d = mfma16(A2, B1[R], f4{0.f, 0.f, 0.f, 0.f}, P16);
+      }
+      float e[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        e[u] = fexp2(fminf(d[u], 0.f));
+        pdf[R] += e[u];
+      }
+      u16x8 A;
+      const unsigned h01 = pack2<P16>(e[0], e[1]);
+      const unsigned h23 = pack2<P16>(e[2], e[3]);
+      A[0] = (unsigned short)(h01 & 0xffffu);
+      A[1] = (unsigned short)(h01 >> 16);
+      A[2] = (unsigned short)(h23 & 0xffffu);
+      A[3] = (unsigned short)(h23 >> 16);
+      if (X3) {
+        const unsigned l01 = pack2<P16>(e[0] - from16<P16>(A[0]), e[1] - from16<P16>(A[1]));
+        const unsigned l23 = pack2<P16>(e[2] - from16<P16>(A[2]), e[3] - from16<P16>(A[3]));
+        A[4] = (unsigned short)(l01 & 0xffffu);
+        A[5] = (unsigned short)(l01 >> 16);
+        A[6] = (unsigned short)(l23 & 0xffffu);
+        A[7] = (unsigned short)(l23 >> 16);
+        w[R] = mfma16(A, BX1, w[R], P16);
+        w[R] = mfma16(A, BX2, w[R], P16);
+      } else {
+        A[4] = A[5] = A[6] = A[7] = 0;
+        w[R] = mfma16(A, BX1, w[R], P16);
+      }
+    }
+    if (++tc == kLpFlush) {
+      tc = 0;
+#pragma unroll
+      for (int R = 0; R < QT; ++R) {
+        pdf64[R] += pdf[R];
+        pdf[R] = 0.f;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) w64[R][r] += w[R][r];
+        w[R] = f4{0.f, 0.f, 0.f, 0.f};
+      }
+    }
+  }
+#pragma unroll
+  for (int R = 0; R < QT; ++R) {
+    double p = sum_groups(pdf64[R] + pdf[R]);
+    int64_t i = row0 + 16 * R + c;
+    if (g == 0 && i < n) pdf_part[split * n + i] = float(p);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      int64_t iw = row0 + 16 * R + 4 * g + r;
+      if (iw < n) w_part[(split * n + iw) * kD + c] = float(w64[R][r] + w[R][r]);
+    }
+  }
+}
+
+// pdf[i] = sum_s pdf_part; w[i,f] = sum_s w_part + mu_f pdf[i]  (FP64).
+__global__ void score_lp_finish_kernel(const float *__restrict__ pdf_part,
+                                       const float *__restrict__ w_part, int64_t S, int64_t n,
+                                       const double *__restrict__ mu_sum, double inv_n,
+                                       float *__restrict__ pdf, float *__restrict__ w,
+                                       int64_t ldw_row, int64_t ldw_col) {
+  const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= n * kD) return;
+  const int64_t i = e / kD;
+  const int f = int(e % kD);
+  double p = 0.0, s = 0.0;
+  for (int64_t k = 0; k < S; ++k) {
+    p += double(pdf_part[k * n + i]);
+    s += double(w_part[(k * n + i) * kD + f]);
+  }
+  w[i * ldw_row + f * ldw_col] = float(s + mu_sum[f] * inv_n * p);
+  if (f == 0) pdf[i] = float(p);
+}
+
+struct ScoreLpLayout {
+  double *mu;
+  unsigned short *xrec, *qrec, *xt;
+  float *pdf_part, *w_part;
+};
+
+static size_t score_lp_layout(void *ws, int64_t n, ScoreLpLayout *L) {
+  Carve cv(ws);
+  SplitPlan p = score16_ordered_plan(n);
+  const int64_t rows = round_up(n, kNormPad);
+  L->mu = cv.take<double>(kD);
+  L->xrec = cv.take<unsigned short>(rows * kRec);
+  L->qrec = cv.take<unsigned short>(rows * kRec);
+  L->xt = cv.take<unsigned short>(round_up(n, kTile) * kD * 2);
+  L->pdf_part = cv.take<float>(p.splits * n);
+  L->w_part = cv.take<float>(p.splits * n * kD);
+  return cv.used;
+}
+
+size_t score16_lp_ws_bytes(int64_t n) {
+  ScoreLpLayout L;
+  return score_lp_layout(nullptr, n, &L);
+}
+
+int score16_lp_run(const float *data, int64_t ld, int64_t n, double h, unsigned prec, void *ws,
+                   size_t ws_bytes, float *pdf, float *w, int64_t ldw_row, int64_t ldw_col,
+                   void *stream) {
+  const char *what = "fsk_score16_ordered";
+  ScoreLpLayout L;
+  size_t need = score_lp_layout(ws, n, &L);
+  if (!ws || ws_bytes < need) {
+    set_error("%s: workspace of %zu bytes needed, got %zu", what, need, ws_bytes);
+    return FSK_EWORKSPACE;
+  }
+  hipStream_t s = as_stream(stream);
+  const double coef = exp2_coef(h);
+  const int64_t rows = round_up(n, kNormPad);
+  const int64_t n16 = round_up(n, kTile);
+  const double inv_n = 1.0 / double(n);
+  hipLaunchKernelGGL(fill_f64, dim3(1), dim3(64), 0, s, L.mu, int64_t(kD), 0.0);
+  {
+    int64_t blocks = cdiv(n, 256);
+    if (blocks > 1024) blocks = 1024;
+    hipLaunchKernelGGL(mean16_kernel, dim3(blocks), dim3(256), 0, s, data, ld, n, L.mu);
+  }
+  SplitPlan p = score16_ordered_plan(n);
+  const int64_t ntasks = p.qblocks * p.splits;
+  int64_t per_xcd;
+  const int64_t grid = xcd_grid(ntasks, &per_xcd);
+#define FSK_LP(P, P16)                                                                          \
+  do {                                                                                          \
+    hipLaunchKernelGGL(prep_lse_kernel<P>, dim3(cdiv(rows, 256)), dim3(256), 0, s, data, ld, n, \
+                       rows, L.mu, inv_n, coef, 0, (float *)nullptr, (float *)nullptr, L.xrec);  \
+    hipLaunchKernelGGL(prep_lse_kernel<P>, dim3(cdiv(rows, 256)), dim3(256), 0, s, data, ld, n, \
+                       rows, L.mu, inv_n, coef, 1, (float *)nullptr, (float *)nullptr, L.qrec);  \
+    hipLaunchKernelGGL(prep_xt16_kernel<P16>, dim3(cdiv(n16 * kD, 256)), dim3(256), 0, s, data, \
+                       ld, n, n16, L.mu, inv_n, L.xt);                                          \
+    prof_begin(FSK_OP_SCORE16_ORDERED, s);                                                      \
+    hipLaunchKernelGGL((score16_lp_kernel<P, kScoreQT>), dim3(grid), dim3(kWG), 0, s, L.xrec,   \
+                       L.qrec, L.xt, n, n16, p.split_len, p.qblocks, ntasks, per_xcd,           \
+                       L.pdf_part, L.w_part);                                                   \
+    prof_end(FSK_OP_SCORE16_ORDERED, s);                                                        \
+  } while (0)
+  switch (prec) {
+    case FSK_PREC_BF16: FSK_LP(FSK_PREC_BF16, FSK_PREC_BF16); break;
+    case FSK_PREC_F16: FSK_LP(FSK_PREC_F16, FSK_PREC_F16); break;
+    default: FSK_LP(FSK_PREC_BF16X3, FSK_PREC_BF16); break;
+  }
+#undef FSK_LP
+  int st = launch_status(what);
+  if (st != FSK_OK) return st;
+  hipLaunchKernelGGL(score_lp_finish_kernel, dim3(cdiv(n * kD, 256)), dim3(256), 0, s,
+                     L.pdf_part, L.w_part, p.splits, n, L.mu, inv_n, pdf, w, ldw_row, ldw_col);
+  return launch_status(what);
+}
+
+}  // namespace fsk

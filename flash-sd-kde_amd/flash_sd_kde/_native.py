@@ -65,12 +65,16 @@ SIGNATURES = {
     "fsk_reduce_partials_f64": [_p, _i64, _i64, _i64, _p, _p],
     "fsk_reduce_weighted_partials_f64": [_p, _i64, _i64, _i64, _p, _i64, _i64, _p],
     "fsk_sdkde_debias_f64": [_p, _i64, _i64, _p, _p, _i64, _i64, _d, _d, _p, _i64, _p, _p],
+    "fsk_ref64_workspace_bytes": [_i64, _i64],
+    "fsk_ref64_logdensity16": [_p, _i64, _p, _i64, _d, _p, _sz, _p, _p],
+    "fsk_ref64_score16": [_p, _i64, _d, _p, _p, _p],
 }
 _RESTYPES = {
     "fsk_last_error": ctypes.c_char_p,
     "fsk_version": ctypes.c_int,
     "fsk_workspace_bytes": _sz,
     "fsk_profile_last_ms": _d,
+    "fsk_ref64_workspace_bytes": _sz,
 }
 
 _lib: Optional[ctypes.CDLL] = None

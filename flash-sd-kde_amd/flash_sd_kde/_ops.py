@@ -149,7 +149,7 @@ def kde1_score_sums(train: torch.Tensor, query: torch.Tensor, bandwidth: float):
 
 
 def score16(train: torch.Tensor, bandwidth: float, *, variant: str = "symmetric",
-            transposed_w: bool = False):
+            transposed_w: bool = False, precision: str = "fp32"):
     """Empirical-score accumulators (pdf (n,), weighted (n,16)).
 
     transposed_w=True stores the weighted sums in a (16, n) buffer and returns
@@ -174,8 +174,13 @@ def score16(train: torch.Tensor, bandwidth: float, *, variant: str = "symmetric"
         op, fn, name = nat.OP_SCORE16_ORDERED, nat.lib().fsk_score16_ordered, "fsk_score16_ordered"
     else:
         raise ValueError(f"unknown score variant {variant}")
-    ws, nbytes = nat.workspace(op, n, n, 0, dev)
-    st = fn(train.data_ptr(), train.stride(0), n, h, 0, ws.data_ptr(), nbytes, pdf.data_ptr(),
+    flags = precision_flag(precision)
+    if flags not in (nat.PREC_F32, nat.PREC_BF16, nat.PREC_F16, nat.PREC_BF16X3):
+        raise ValueError(f"precision {precision} is not available for the score pass")
+    if flags != nat.PREC_F32 and variant != "ordered":
+        raise ValueError("reduced-precision score accumulators use the ordered kernel")
+    ws, nbytes = nat.workspace(op, n, n, flags, dev)
+    st = fn(train.data_ptr(), train.stride(0), n, h, flags, ws.data_ptr(), nbytes, pdf.data_ptr(),
             wbuf.data_ptr(), ld_row, ld_col, nat.stream_handle(dev))
     nat.check(st, name)
     return pdf, (wbuf.t() if transposed_w else wbuf)
@@ -247,3 +252,35 @@ def infer_ndim(data: Sequence | torch.Tensor) -> int:
     if isinstance(first, (list, tuple, np.ndarray)):
         return 2
     return 1
+
+
+def ref64_score(train64: torch.Tensor, bandwidth: float):
+    """FP64 reference accumulators (pdf (n,), w (n,16)), float64 on the GPU."""
+    h = _check_h(bandwidth)
+    dev = train64.device
+    nat.require_gpu(dev, "ref64_score")
+    x = train64.to(torch.float64).contiguous()
+    n = x.shape[0]
+    pdf = torch.empty(n, dtype=torch.float64, device=dev)
+    w = torch.empty((n, ND), dtype=torch.float64, device=dev)
+    st = nat.lib().fsk_ref64_score16(x.data_ptr(), n, h, pdf.data_ptr(), w.data_ptr(),
+                                     nat.stream_handle(dev))
+    nat.check(st, "fsk_ref64_score16")
+    return pdf, w
+
+
+def ref64_logdensity(train64: torch.Tensor, query64: torch.Tensor, bandwidth: float):
+    """FP64 reference log-densities (m,), float64 on the GPU."""
+    h = _check_h(bandwidth)
+    dev = train64.device
+    nat.require_gpu(dev, "ref64_logdensity")
+    x = train64.to(torch.float64).contiguous()
+    q = query64.to(torch.float64).contiguous()
+    n, m = x.shape[0], q.shape[0]
+    out = torch.empty(m, dtype=torch.float64, device=dev)
+    nbytes = int(nat.lib().fsk_ref64_workspace_bytes(n, m))
+    ws = torch.empty(max(nbytes, 256), dtype=torch.uint8, device=dev)
+    st = nat.lib().fsk_ref64_logdensity16(x.data_ptr(), n, q.data_ptr(), m, h, ws.data_ptr(),
+                                          nbytes, out.data_ptr(), nat.stream_handle(dev))
+    nat.check(st, "fsk_ref64_logdensity16")
+    return out

@@ -1,0 +1,75 @@
+"""Precision forms of the SD-KDE pipeline (no reference counterpart).
+
+The plain FP32 form (the drop-in, flash_sd_kde.kde) evaluates the cross term
+q.x and phi.X with FP32 MFMA. The forms here evaluate them with 16-bit MFMA
+operands and FP32 accumulation:
+  "bf16x3"  BF16 hi/lo split, three products (hi.hi + lo.hi + hi.lo)
+  "fp16"    single FP16 operands
+  "bf16"    single BF16 operands
+  "fp32", "fp32_valu"  FP32 MFMA / FP32 VALU cross term (log-density only)
+with points centred on the data mean, log-densities by log-sum-exp
+(per-split running maximum, FP64 combine) and FP64 debias.
+
+`ref64_*` are FP64 direct-difference GPU kernels used to measure the error of
+every form at sizes the CPU oracle cannot reach.
+"""
+
+from __future__ import annotations
+
+import torch
+
+from flash_sd_kde import _ops
+
+SCORE_PRECISIONS = ("fp32", "bf16x3", "fp16", "bf16")
+LOGDENSITY_PRECISIONS = ("fp32", "fp32_valu", "bf16x3", "fp16", "bf16")
+
+
+def kde_logdensity(data, queries, bandwidth: float, *, precision: str = "fp32",
+                   device="cuda") -> torch.Tensor:
+    """log p(q) = log(1/(n (2 pi)^8 h^16) sum_j exp(-|q - x_j|^2 / 2h^2)), float64."""
+    device = torch.device(device)
+    return _ops.kde16_logdensity(_ops.as_points16(data, device), _ops.as_points16(queries, device),
+                                 bandwidth, precision=precision)
+
+
+def emp_score(data, bandwidth: float, *, precision: str = "fp32", device="cuda"):
+    """(pdf_sum (n,), weighted (n, 16)) by the deterministic ordered kernel."""
+    device = torch.device(device)
+    return _ops.score16(_ops.as_points16(data, device), bandwidth, variant="ordered",
+                        precision=precision)
+
+
+def sdkde_fit_transform(data, bandwidth: float, *, precision: str = "fp32", eps: float = 1e-12,
+                        device="cuda") -> torch.Tensor:
+    """One-step empirical SD-KDE debiasing with the chosen score precision."""
+    device = torch.device(device)
+    train = _ops.as_points16(data, device)
+    pdf, w = _ops.score16(train, bandwidth, variant="ordered", precision=precision)
+    return _ops.debias(train, pdf, w, bandwidth, eps)
+
+
+def sdkde_logdensity(data, queries, bandwidth: float, *, precision: str = "fp32",
+                     device="cuda") -> torch.Tensor:
+    """SD-KDE log-density of `queries`: debias `data` (score in `precision`), then
+    the log-sum-exp KDE (cross term in `precision`)."""
+    device = torch.device(device)
+    deb = sdkde_fit_transform(data, bandwidth, precision=precision, device=device)
+    return _ops.kde16_logdensity(deb, _ops.as_points16(queries, device), bandwidth,
+                                 precision=precision)
+
+
+def ref64_sdkde_logdensity(data, queries, bandwidth: float, *, eps: float = 1e-12,
+                           device="cuda") -> torch.Tensor:
+    """FP64 reference of sdkde_logdensity (FP64 score, debias and log-sum-exp)."""
+    device = torch.device(device)
+    x = torch.as_tensor(data, device=device).to(torch.float64)
+    q = torch.as_tensor(queries, device=device).to(torch.float64)
+    pdf, w = _ops.ref64_score(x, bandwidth)
+    deb = x + 0.5 * (w / (pdf[:, None] + eps) - x)
+    return _ops.ref64_logdensity(deb, q, bandwidth)
+
+
+def rel_log_error(logp: torch.Tensor, ref: torch.Tensor) -> tuple[float, float]:
+    """(max, median) of |logp - ref| / |ref|."""
+    r = (logp.double() - ref.double()).abs() / ref.double().abs().clamp_min(1e-300)
+    return float(r.max()), float(r.median())

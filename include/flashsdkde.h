@@ -148,6 +148,9 @@ int fsk_score16_sym_atomic(const float *data, int64_t ld, int64_t n,
 
 /* Same accumulators, deterministic: bounded split-K over the data with
  * FP32 partials and an FP64 combine; bitwise reproducible for fixed n.
+ * flags may select a 16-bit cross term and phi.X (FSK_PREC_BF16X3 / _BF16 /
+ * _F16, points centred on their mean; the precision form); FSK_PREC_F32 is
+ * the plain FP32 form.
  * Replaces emp_score_16d_ordered_splitk,
  * kernels/emp_score_16d_ordered_splitk.py:162-263 (pass A :14-92, reduce
  * kernels/reduce_partials.py:82-108). */
@@ -178,6 +181,23 @@ int fsk_sdkde_debias_f64(const float *x, int64_t ldx, int64_t n,
                          const float *pdf, const float *w, int64_t ldw_row,
                          int64_t ldw_col, double bandwidth, double eps,
                          float *out, int64_t ldo, float *score_out, void *stream);
+
+/* ---- FP64 reference kernels (measurement; SURVEY row "error-suite oracle on
+ * the GPU"). Direct differences in FP64, FP64 exp and accumulation; inputs are
+ * contiguous (rows, 16) float64. */
+
+/* log p(q_i) by log-sum-exp (the FP64 semantics of
+ * experiments/error_suite_16d/reference.py:32-82, reference_log_density).
+ * Workspace: fsk_ref64_workspace_bytes(n, m). */
+size_t fsk_ref64_workspace_bytes(int64_t n, int64_t m);
+int fsk_ref64_logdensity16(const double *data, int64_t n, const double *query, int64_t m,
+                           double bandwidth, void *ws, size_t ws_bytes, double *out,
+                           void *stream);
+
+/* pdf[i] = sum_j phi_ij, w[i,:] = sum_j phi_ij x_j (FP64; the quantities of
+ * flash_sd_kde/reference.py:263-284 without rounding). */
+int fsk_ref64_score16(const double *data, int64_t n, double bandwidth, double *pdf, double *w,
+                      void *stream);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,78 @@
+"""GPU: FP64 reference kernels against the CPU FP64 oracle, and the
+reduced-precision forms (score accumulators and full SD-KDE log-densities)
+against FP64. Every case records its error (FSK_METRICS_OUT)."""
+
+import json
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from flash_sd_kde import _ops, precision
+from oracle import fp64
+
+pytestmark = pytest.mark.gpu
+
+
+def record(name, **vals):
+    path = os.environ.get("FSK_METRICS_OUT")
+    print(name, vals, flush=True)
+    if path:
+        with open(path, "a") as f:
+            f.write(json.dumps({"test": name, **vals}) + "\n")
+
+
+def test_ref64_kernels_match_cpu_fp64(gpu, golden):
+    c = "cfg_gmm_small"
+    x, q, h = golden(c, "x"), golden(c, "q"), golden(c, "h")
+    xd = torch.as_tensor(x, device=gpu, dtype=torch.float64)
+    qd = torch.as_tensor(q[:256], device=gpu, dtype=torch.float64)
+    lp = _ops.ref64_logdensity(xd, qd, h).cpu().numpy()
+    np.testing.assert_allclose(lp, fp64.log_density(q[:256], x, h), rtol=1e-12)
+    pdf, w = _ops.ref64_score(xd[:1500], h)
+    pe, we = fp64.score_sums(x[:1500], h)
+    np.testing.assert_allclose(pdf.cpu().numpy(), pe, rtol=1e-12)
+    np.testing.assert_allclose(w.cpu().numpy(), we, rtol=1e-11, atol=1e-11)
+
+
+# The cross term q.x enters exp2 unscaled by any normalisation: a relative
+# error eps of the 16-bit operands becomes ~eps * |2c q.x| in the exponent
+# (|2c q.x| ~ 100 on this mixture), hence the per-form tolerances.
+@pytest.mark.parametrize("prec,tol_pdf,tol_w", [("fp32", 2e-5, 2e-5), ("bf16x3", 1e-3, 5e-4),
+                                                 ("fp16", 5e-2, 2e-2), ("bf16", 0.5, 0.2)])
+def test_score_precision_forms(gpu, golden, prec, tol_pdf, tol_w):
+    c = "cfg_gmm_small"
+    x, h = golden(c, "x"), golden(c, "h")
+    pdf, w = precision.emp_score(x, h, precision=prec, device=gpu)
+    pe, we = fp64.score_sums(x, h)
+    e_pdf = float(np.max(np.abs(pdf.cpu().numpy() - pe) / pe))
+    e_w = float(np.max(np.abs(w.cpu().numpy() - we)) / np.max(np.abs(we)))
+    # score = (w/pdf - x)/h^2 is what the debias uses: error of w/pdf in units of h
+    ratio = w.cpu().numpy().astype(np.float64) / pdf.cpu().numpy()[:, None]
+    e_mean = float(np.max(np.abs(ratio - we / pe[:, None])) / h)
+    record(f"score_{prec}", pdf_max_rel=e_pdf, w_max_rel=e_w, mean_shift_err_over_h=e_mean)
+    assert e_pdf <= tol_pdf and e_w <= tol_w
+
+
+def test_score_lp_ragged_and_transposed_layout(gpu):
+    for n in (1, 17, 300, 1025):
+        x = np.random.default_rng(n).normal(size=(n, 16)).astype(np.float32) + 3.0
+        pdf, w = _ops.score16(torch.as_tensor(x, device=gpu), 1.1, variant="ordered",
+                              precision="bf16x3", transposed_w=True)
+        assert w.stride() == (1, n)
+        pe, we = fp64.score_sums(x, 1.1)
+        np.testing.assert_allclose(pdf.cpu().numpy(), pe, rtol=1e-3)
+        np.testing.assert_allclose(w.cpu().numpy(), we, rtol=1e-3, atol=1e-3 * np.abs(we).max())
+
+
+@pytest.mark.parametrize("prec,tol", [("fp32", 1e-5), ("bf16x3", 1e-4), ("fp16", 1e-3),
+                                      ("bf16", 1.0)])
+def test_sdkde_logdensity_precision_forms(gpu, golden, prec, tol):
+    c = "cfg_gmm_small"
+    x, q, h = golden(c, "x"), golden(c, "q"), golden(c, "h")
+    ref = precision.ref64_sdkde_logdensity(x, q, h, device=gpu)
+    lp = precision.sdkde_logdensity(x, q, h, precision=prec, device=gpu)
+    mx, med = precision.rel_log_error(lp, ref)
+    record(f"sdkde_logdensity_{prec}", max_rel=mx, median_rel=med)
+    assert mx <= tol
