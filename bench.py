@@ -45,6 +45,7 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-precision-forms", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--error-queries", type=int, default=4096)
     return ap.parse_args()
 
@@ -88,6 +89,69 @@ def cpu_baseline(x, q, h, budget_s):
         "sample": (f"{pairs / (blk * x.shape[0]):.0f} blocks of {blk} rows (score rows and KDE "
                    f"queries alternating) x all n={x.shape[0]} points, {pairs:.3g} pairs in "
                    f"{dt:.1f} s; numpy FP32 oracle, {threads} threads, BLAS 1 thread each"),
+    }
+
+
+def measure_rates():
+    """Issue rates measured on this GPU by the library's microbenchmarks."""
+    import ctypes
+
+    import torch
+
+    from flash_sd_kde import _native
+
+    lib = _native.lib()
+    out = torch.empty(2048 * 256, device="cuda")
+    r = ctypes.c_double()
+    rates = {}
+    for which, name, iters in ((0, "fp32_mfma_flops", 4000), (2, "exp_per_s", 4000),
+                               (3, "fma_per_s", 4000)):
+        lib.fsk_microbench(which, 2048, 50, out.data_ptr(), ctypes.addressof(r), None)
+        st = lib.fsk_microbench(which, 2048, iters, out.data_ptr(), ctypes.addressof(r), None)
+        rates[name] = r.value if st == 0 else None
+    return rates
+
+
+def batched_loo(rates, steps):
+    """Batched LOO bandwidth grid: 16 datasets, n_i log-uniform in [2048, 262144]
+    (seed 0), 4-component mixtures (means ~ N(0, 4 I), sigma 1, seed 100 + i),
+    32 bandwidths log-spaced in [0.05, 2] x the mean-std rule; one persistent
+    launch replayed from a HIP graph, launch shape from the autotuner."""
+    import numpy as np
+    import torch
+
+    from flash_sd_kde.batched import LooBandwidthGrid
+    from flash_sd_kde.datasets import gmm_isotropic, uneven_sizes
+
+    sizes = uneven_sizes(16, 2048, 262144, seed=0)
+    sets = [gmm_isotropic(n, k=4, mean_scale=2.0, seed=100 + i) for i, n in enumerate(sizes)]
+    t0 = time.perf_counter()
+    grid = LooBandwidthGrid(sets, factors=np.geomspace(0.05, 2.0, 32), autotune=True, graph=True)
+    setup_s = time.perf_counter() - t0
+    grid.run()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        grid.run()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) / steps * 1e3
+    pairs = grid.pair_count()
+    exps = pairs * 32
+    exp_rate = rates.get("exp_per_s") or 17e12
+    fma_rate = rates.get("fma_per_s") or 54e12
+    # per pair and bandwidth: one exp plus one multiply and one add on the VALU
+    t_roof = exps / exp_rate + 2 * exps / fma_rate
+    ll = grid.run().cpu().numpy()
+    return {
+        "workload": f"16 datasets, sum n = {sum(sizes)}, 32 bandwidths, LOO",
+        "ms_per_launch": round(ms, 3),
+        "gpair_evals_per_s": round(pairs / ms / 1e6, 3),
+        "gexp_per_s": round(exps / ms / 1e6, 3),
+        "exp_valu_frac": round(t_roof / (ms * 1e-3), 4),
+        "tune": list(grid.tune),
+        "tune_ms": {f"{w}x{g}": round(t, 3) for (w, g), t in getattr(grid, "tune_times", {}).items()},
+        "setup_s": round(setup_s, 2),
+        "best_factor_median": float(np.median(np.argmax(ll, axis=1))),
     }
 
 
@@ -250,6 +314,18 @@ def main():
         },
         "outputs_finite": finite,
     }
+    rates = measure_rates()
+    line["measured_rates"] = rates
+    if rates.get("exp_per_s") and rates.get("fma_per_s") and score_ms > 0:
+        # exp/VALU roofline with the measured issue rates (per ordered pair of
+        # the ordered kernel: 1 exp + 3 VALU ops; symmetric: per unique pair 1 + 5)
+        v_ops = 5.0 if args.variant == "symmetric" else 3.0
+        t = units / rates["exp_per_s"] + units * v_ops / rates["fma_per_s"]
+        line["roofline"]["exp_valu_frac_measured_rates"] = round(t / (score_ms * 1e-3), 4)
+        line["roofline"]["mfma_frac_of_measured_peak"] = round(
+            flops / (score_ms * 1e-3) / rates["fp32_mfma_flops"], 4)
+    if not args.no_batched:
+        line["batched_loo"] = batched_loo(rates, args.steps)
     if not args.no_precision_forms:
         line["precision_forms"] = precision_forms(xd, qd, h, ms_per_step, dens, args.steps,
                                                   args.warmup, args.error_queries)

@@ -39,6 +39,7 @@ OP_SCORE16_SYM = 5
 OP_SCORE16_ORDERED = 6
 OP_KDE16_LOGDENSITY = 7
 OP_KDE1_SCORE = 8
+OP_LOO_GRID = 9
 
 _p = ctypes.c_void_p
 _i64 = ctypes.c_int64
@@ -66,6 +67,9 @@ SIGNATURES = {
     "fsk_reduce_weighted_partials_f64": [_p, _i64, _i64, _i64, _p, _i64, _i64, _p],
     "fsk_sdkde_debias_f64": [_p, _i64, _i64, _p, _p, _i64, _i64, _d, _d, _p, _i64, _p, _p],
     "fsk_ref64_workspace_bytes": [_i64, _i64],
+    "fsk_loo_workspace_bytes": [_p, ctypes.c_int, ctypes.c_int],
+    "fsk_loo_bandwidth_grid": [_p, _i64, _p, ctypes.c_int, _p, ctypes.c_int, ctypes.c_int,
+                               ctypes.c_int, _p, _sz, _p, _p],
     "fsk_ref64_logdensity16": [_p, _i64, _p, _i64, _d, _p, _sz, _p, _p],
     "fsk_ref64_score16": [_p, _i64, _d, _p, _p, _p],
 }
@@ -75,6 +79,7 @@ _RESTYPES = {
     "fsk_workspace_bytes": _sz,
     "fsk_profile_last_ms": _d,
     "fsk_ref64_workspace_bytes": _sz,
+    "fsk_loo_workspace_bytes": _sz,
 }
 
 _lib: Optional[ctypes.CDLL] = None

@@ -55,6 +55,7 @@ extern "C" {
 #define FSK_OP_SCORE16_ORDERED 6
 #define FSK_OP_KDE16_LOGDENSITY 7
 #define FSK_OP_KDE1_SCORE 8
+#define FSK_OP_LOO_GRID 9 /* profiling id only; workspace: fsk_loo_workspace_bytes */
 
 const char *fsk_last_error(void);
 int fsk_version(void);
@@ -181,6 +182,21 @@ int fsk_sdkde_debias_f64(const float *x, int64_t ldx, int64_t n,
                          const float *pdf, const float *w, int64_t ldw_row,
                          int64_t ldw_col, double bandwidth, double eps,
                          float *out, int64_t ldo, float *score_out, void *stream);
+
+/* Batched leave-one-out KDE log-likelihood over a bandwidth grid (no reference
+ * counterpart; the batched pattern). Datasets are concatenated row-major in
+ * `data` (row stride ld); `offsets` (HOST array, n_sets + 1 entries, offsets[0]
+ * = 0) delimit them; `bandwidths` (DEVICE, float64, n_sets x n_bw, row-major)
+ * holds each dataset's grid. out (DEVICE, float64, n_sets x n_bw):
+ *   out[d, b] = 1/n_d sum_i log( 1/(n_d - 1) sum_{j != i} N(x_i; x_j, h_b^2 I) ).
+ * One persistent launch over all datasets (work queue, largest first);
+ * waves_per_wg in {4, 8} and wg_per_cu in 1..8 are the tunables (chosen by the
+ * Python autotuner). Limits: n_sets <= 64, n_bw <= 32, every n_d >= 2.
+ * Capturable: no allocation, no host synchronization, fills by kernel. */
+size_t fsk_loo_workspace_bytes(const int64_t *offsets, int n_sets, int n_bw);
+int fsk_loo_bandwidth_grid(const float *data, int64_t ld, const int64_t *offsets, int n_sets,
+                           const double *bandwidths, int n_bw, int waves_per_wg, int wg_per_cu,
+                           void *ws, size_t ws_bytes, double *out, void *stream);
 
 /* ---- FP64 reference kernels (measurement; SURVEY row "error-suite oracle on
  * the GPU"). Direct differences in FP64, FP64 exp and accumulation; inputs are
