@@ -104,8 +104,8 @@ def measure_rates():
     out = torch.empty(2048 * 256, device="cuda")
     r = ctypes.c_double()
     rates = {}
-    for which, name, iters in ((0, "fp32_mfma_flops", 4000), (2, "exp_per_s", 4000),
-                               (3, "fma_per_s", 4000)):
+    for which, name, iters in ((0, "fp32_mfma_flops", 20000), (2, "exp_per_s", 20000),
+                               (3, "fma_per_s", 20000)):
         lib.fsk_microbench(which, 2048, 50, out.data_ptr(), ctypes.addressof(r), None)
         st = lib.fsk_microbench(which, 2048, iters, out.data_ptr(), ctypes.addressof(r), None)
         rates[name] = r.value if st == 0 else None

@@ -116,16 +116,20 @@ def debias(data, pdf, weighted, bandwidth: float, eps: float = 1e-12):
 
 def loo_loglik(data, bandwidths) -> np.ndarray:
     """Leave-one-out mean log-likelihood per bandwidth:
-    mean_i log( 1/(n-1) sum_{j != i} N(x_i; x_j, h^2 I) ), FP64, brute force."""
+    mean_i log( 1/(n-1) sum_{j != i} N(x_i; x_j, h^2 I) ), FP64, brute force
+    over all pairs with a per-row log-sum-exp (the direct sum underflows double
+    precision for small h)."""
     x = np.asarray(data, dtype=np.float64)
     n, d = x.shape
     d2 = _sq_dist(x, x)
+    np.fill_diagonal(d2, np.inf)
     out = []
     for h in bandwidths:
-        k = np.exp(-0.5 * d2 / (h * h))
-        np.fill_diagonal(k, 0.0)
-        dens = k.sum(axis=1) / ((n - 1) * (2.0 * math.pi) ** (d / 2.0) * h ** d)
-        out.append(float(np.mean(np.log(dens))))
+        e = -0.5 * d2 / (h * h)
+        mx = e.max(axis=1, keepdims=True)
+        lse = mx[:, 0] + np.log(np.exp(e - mx).sum(axis=1))
+        const = -math.log(n - 1) - 0.5 * d * math.log(2.0 * math.pi) - d * math.log(h)
+        out.append(float(np.mean(lse + const)))
     return np.asarray(out)
 
 

@@ -149,3 +149,17 @@ def test_planted_mistake_is_caught(golden):
     x, q, h = golden("kde16", "x"), golden("kde16", "q"), golden("kde16", "h")
     wrong = R.kde_eval_nd_numpy(q, x, h) * h  # h^-15 instead of h^-16
     assert rel(wrong, golden("kde16", "density")) > 1e-3
+
+
+def test_fp64_loo_small_bandwidth_is_finite():
+    """For h far below the nearest-neighbour distance the direct sum underflows;
+    the log-sum-exp form must still give the nearest-neighbour term."""
+    x = np.random.default_rng(4).normal(size=(6, 16)) * 3
+    h = 0.05
+    got = fp64.loo_loglik(x, [h])[0]
+    d2 = ((x[:, None, :] - x[None, :, :]) ** 2).sum(-1)
+    np.fill_diagonal(d2, np.inf)
+    nn = d2.min(axis=1)
+    # only the nearest neighbour matters at this h (others are e^-hundreds smaller)
+    expect = np.mean(-nn / (2 * h * h) - math.log(5) - 8 * math.log(2 * math.pi) - 16 * math.log(h))
+    assert np.isfinite(got) and abs(got - expect) / abs(expect) < 1e-12
