@@ -46,6 +46,7 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-precision-forms", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--error-queries", type=int, default=4096)
     return ap.parse_args()
 
@@ -152,6 +153,64 @@ def batched_loo(rates, steps):
         "tune_ms": {f"{w}x{g}": round(t, 3) for (w, g), t in getattr(grid, "tune_times", {}).items()},
         "setup_s": round(setup_s, 2),
         "best_factor_median": float(np.median(np.argmax(ll, axis=1))),
+    }
+
+
+def surrogate_form(fp32_pair_rate):
+    """Learned-score form: n = 4,194,304 points of a 32-component
+    full-covariance mixture (eigenvalues log-U[0.1, 3], random rotations,
+    means ~ N(0, 16 I), seed 0), 65,536 held-out queries (seed 1). The MLP is
+    trained on exact FP32 shift labels of 262,144 rows; its debiasing is
+    compared with exact SD-KDE debiasing of all n points (log-densities of the
+    held-out queries, FP32 log-sum-exp KDE)."""
+    import numpy as np
+    import torch
+
+    from flash_sd_kde import _ops
+    from flash_sd_kde.bandwidth import scott_mean_std_bandwidth
+    from flash_sd_kde.datasets import gmm_full_cov, gmm_full_cov_logpdf, gmm_full_cov_params
+    from flash_sd_kde.surrogate import surrogate_fit_transform
+
+    params = gmm_full_cov_params(32, mean_scale=4.0, seed=0)
+    x = gmm_full_cov(1 << 22, params=params, seed=0)
+    q = gmm_full_cov(65536, params=params, seed=1)
+    h = scott_mean_std_bandwidth(x)
+    xd = torch.as_tensor(x, device="cuda")
+    qd = torch.as_tensor(q, device="cuda")
+    n = xd.shape[0]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    deb_s, info = surrogate_fit_transform(xd, h, tol=1e-3)
+    torch.cuda.synchronize()
+    t_sur = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    pdf, w = _ops.score16(xd, h, variant="ordered")
+    deb_e = _ops.debias(xd, pdf, w, h, 1e-12)
+    torch.cuda.synchronize()
+    t_exact = time.perf_counter() - t0
+    lp_e = _ops.kde16_logdensity(deb_e, qd, h)
+    lp_s = _ops.kde16_logdensity(deb_s, qd, h)
+    lp_k = _ops.kde16_logdensity(xd, qd, h)
+    r = ((lp_s - lp_e).abs() / lp_e.abs()).cpu().numpy()
+    rk = ((lp_k - lp_e).abs() / lp_e.abs()).cpu().numpy()
+    truth = gmm_full_cov_logpdf(q, params)
+    mae = lambda a: float(np.mean(np.abs(a.cpu().numpy() - truth)))
+    return {
+        "workload": "n=4194304 full-cov GMM32, 65536 queries, MLP 16-256-256-256-16 SiLU bf16",
+        "bandwidth": h,
+        "exact_debias_s": round(t_exact, 3),
+        "surrogate_debias_s": round(t_sur, 3),
+        "speedup_vs_exact": round(t_exact / t_sur, 3),
+        "label_s": round(info["label_s"], 3), "train_s": round(info["train_s"], 3),
+        "check_s": round(info["check_s"], 3), "transform_s": round(info["transform_s"], 3),
+        "final_train_loss": info["loss"][-1],
+        "holdout_max_rel_logp_err": info["holdout_max_rel_logp_err"],
+        "fallback_to_exact": info["fallback"],
+        "max_rel_logp_err_vs_exact_sdkde": float(r.max()),
+        "median_rel_logp_err_vs_exact_sdkde": float(np.median(r)),
+        "plain_kde_max_rel_logp_err_vs_exact_sdkde": float(rk.max()),
+        "mean_abs_logp_err_vs_truth": {"exact_sdkde": mae(lp_e), "surrogate_sdkde": mae(lp_s),
+                                       "plain_kde": mae(lp_k)},
     }
 
 
@@ -329,6 +388,10 @@ def main():
     if not args.no_precision_forms:
         line["precision_forms"] = precision_forms(xd, qd, h, ms_per_step, dens, args.steps,
                                                   args.warmup, args.error_queries)
+    if not args.no_surrogate:
+        del xd, qd
+        torch.cuda.empty_cache()
+        line["surrogate_form"] = surrogate_form(value)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(x, q, h, args.cpu_seconds)
     print(json.dumps(line), flush=True)

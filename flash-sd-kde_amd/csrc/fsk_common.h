@@ -90,6 +90,7 @@ size_t kde16_ws_bytes(int64_t n, int64_t m, bool with_partials);
 size_t kde16_lse_ws_bytes(int64_t n, int64_t m, unsigned flags);
 size_t score16_ws_bytes(int64_t n, bool ordered);
 size_t score16_lp_ws_bytes(int64_t n);
+size_t score16_rows_ws_bytes(int64_t n, int64_t m);
 int score16_lp_run(const float *data, int64_t ld, int64_t n, double h, unsigned prec, void *ws,
                    size_t ws_bytes, float *pdf, float *w, int64_t ldw_row, int64_t ldw_col,
                    void *stream);

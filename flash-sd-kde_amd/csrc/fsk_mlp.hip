@@ -1,0 +1,335 @@
+// fsk_mlp.hip - forward / backward / optimizer kernels of the learned score
+// surrogate (a small SiLU MLP with BF16 weights and FP32 accumulation).
+//
+//   linear_fwd : Y = act(X . Wt^T + b)   X (M x K) FP32, Wt (N x K) BF16,
+//                v_mfma_f32_16x16x32_bf16, FP32 accumulate, bias + SiLU fused,
+//                pre-activation Z optionally kept for the backward pass.
+//                The same kernel gives the data gradient dX = dZ . W by passing
+//                the transposed BF16 copy of W.
+//   act_bwd    : dZ = dY * silu'(Z)
+//   wgrad      : dW = dZ^T X and db = sum_m dZ over split batch ranges into
+//                partials (deterministic; the optimizer sums them)
+//   adam       : Adam on FP32 master weights; writes the BF16 copy and the
+//                transposed BF16 copy used by the data-gradient GEMM
+//   mse        : dY = scale (Y - T), loss += sum (Y - T)^2 (FP64)
+// Requirements: K and N multiples of 8 (row slices are 16-byte loads).
+#include "fsk_common.h"
+#include "fsk_lowp.h"
+
+namespace fsk {
+
+__device__ __forceinline__ float silu(float z) { return z / (1.f + __expf(-z)); }
+__device__ __forceinline__ float silu_grad(float z) {
+  const float s = 1.f / (1.f + __expf(-z));
+  return s * (1.f + z * (1.f - s));
+}
+
+__device__ __forceinline__ u16x8 bf16x8_from(const float *p) {
+  const f4 a = ld4(p), b = ld4(p + 4);
+  u16x8 r;
+  r[0] = to16<FSK_PREC_BF16>(a[0]);
+  r[1] = to16<FSK_PREC_BF16>(a[1]);
+  r[2] = to16<FSK_PREC_BF16>(a[2]);
+  r[3] = to16<FSK_PREC_BF16>(a[3]);
+  r[4] = to16<FSK_PREC_BF16>(b[0]);
+  r[5] = to16<FSK_PREC_BF16>(b[1]);
+  r[6] = to16<FSK_PREC_BF16>(b[2]);
+  r[7] = to16<FSK_PREC_BF16>(b[3]);
+  return r;
+}
+
+// Workgroup tile 64 x 64, wave tile 32 x 32 (2 x 2 MFMA blocks), K step 32.
+template <int ACT, bool STORE_Z>
+__global__ __launch_bounds__(256) void mlp_gemm_kernel(const float *__restrict__ X, int64_t ldx,
+                                                       int64_t M, int K,
+                                                       const unsigned short *__restrict__ Wt,
+                                                       const float *__restrict__ bias, int N,
+                                                       float *__restrict__ Z,
+                                                       float *__restrict__ Y, int64_t ldy) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int g = lane >> 4, c = lane & 15;
+  const int64_t m0 = int64_t(blockIdx.x) * 64 + (wave & 1) * 32;
+  const int n0 = blockIdx.y * 64 + (wave >> 1) * 32;
+  f4 acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) acc[a][b] = f4{0.f, 0.f, 0.f, 0.f};
+  const u16x8 zero8 = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int k0 = 0; k0 < K; k0 += 32) {
+    const int k = k0 + 8 * g;
+    u16x8 A[2], B[2];
+#pragma unroll
+    for (int rb = 0; rb < 2; ++rb) {
+      const int64_t row = m0 + 16 * rb + c;
+      A[rb] = (row < M && k < K) ? bf16x8_from(X + row * ldx + k) : zero8;
+    }
+#pragma unroll
+    for (int cb = 0; cb < 2; ++cb) {
+      const int n = n0 + 16 * cb + c;
+      B[cb] = (n < N && k < K) ? *reinterpret_cast<const u16x8 *>(Wt + int64_t(n) * K + k)
+                               : zero8;
+    }
+#pragma unroll
+    for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+      for (int cb = 0; cb < 2; ++cb)
+        acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+            __builtin_bit_cast(bf16x8, A[rb]), __builtin_bit_cast(bf16x8, B[cb]), acc[rb][cb], 0,
+            0, 0);
+  }
+#pragma unroll
+  for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+    for (int cb = 0; cb < 2; ++cb) {
+      const int n = n0 + 16 * cb + c;
+      const float bv = (bias && n < N) ? bias[n] : 0.f;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t row = m0 + 16 * rb + 4 * g + r;
+        if (row < M && n < N) {
+          const float z = acc[rb][cb][r] + bv;
+          if (STORE_Z) Z[row * N + n] = z;
+          Y[row * ldy + n] = ACT ? silu(z) : z;
+        }
+      }
+    }
+}
+
+__global__ void mlp_act_bwd_kernel(const float *__restrict__ dy, const float *__restrict__ z,
+                                   int64_t count, float *__restrict__ dz) {
+  const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e < count) dz[e] = dy[e] * silu_grad(z[e]);
+}
+
+// dW_part[s] (N x K) = sum_{m in split s} dZ[m, n] X[m, k]
+__global__ __launch_bounds__(256) void mlp_wgrad_kernel(const float *__restrict__ dZ, int64_t M,
+                                                        int N, const float *__restrict__ X,
+                                                        int64_t ldx, int K, int64_t split_len,
+                                                        float *__restrict__ dW_part) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int g = lane >> 4, c = lane & 15;
+  const int n0 = blockIdx.x * 64 + (wave & 1) * 32;
+  const int k0 = blockIdx.y * 64 + (wave >> 1) * 32;
+  const int64_t s = blockIdx.z;
+  const int64_t mb = s * split_len;
+  const int64_t me = (mb + split_len < M) ? mb + split_len : M;
+  f4 acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) acc[a][b] = f4{0.f, 0.f, 0.f, 0.f};
+  for (int64_t m0 = mb; m0 < me; m0 += 32) {
+    u16x8 A[2], B[2];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int64_t m = m0 + 8 * g + u;
+      const bool ok = m < me;
+#pragma unroll
+      for (int rb = 0; rb < 2; ++rb) {
+        const int n = n0 + 16 * rb + c;
+        A[rb][u] = to16<FSK_PREC_BF16>((ok && n < N) ? dZ[m * N + n] : 0.f);
+      }
+#pragma unroll
+      for (int cb = 0; cb < 2; ++cb) {
+        const int k = k0 + 16 * cb + c;
+        B[cb][u] = to16<FSK_PREC_BF16>((ok && k < K) ? X[m * ldx + k] : 0.f);
+      }
+    }
+#pragma unroll
+    for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+      for (int cb = 0; cb < 2; ++cb)
+        acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+            __builtin_bit_cast(bf16x8, A[rb]), __builtin_bit_cast(bf16x8, B[cb]), acc[rb][cb], 0,
+            0, 0);
+  }
+#pragma unroll
+  for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+    for (int cb = 0; cb < 2; ++cb) {
+      const int k = k0 + 16 * cb + c;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int n = n0 + 16 * rb + 4 * g + r;
+        if (n < N && k < K) dW_part[(s * N + n) * K + k] = acc[rb][cb][r];
+      }
+    }
+}
+
+// db_part[s][n] = sum_{m in split s} dZ[m, n]
+__global__ void mlp_bgrad_kernel(const float *__restrict__ dZ, int64_t M, int N, int64_t split_len,
+                                 float *__restrict__ db_part) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t s = blockIdx.y;
+  if (n >= N) return;
+  const int64_t mb = s * split_len;
+  const int64_t me = (mb + split_len < M) ? mb + split_len : M;
+  float acc = 0.f;
+  for (int64_t m = mb; m < me; ++m) acc += dZ[m * N + n];
+  db_part[s * N + n] = acc;
+}
+
+__global__ void mlp_adam_kernel(float *__restrict__ w, float *__restrict__ mom,
+                                float *__restrict__ vel, const float *__restrict__ part, int S,
+                                int64_t count, int cols, float lr, float b1, float b2, float eps,
+                                float bc1, float bc2, unsigned short *__restrict__ wb,
+                                unsigned short *__restrict__ wtb, int rows) {
+  const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= count) return;
+  float gsum = 0.f;
+  for (int s = 0; s < S; ++s) gsum += part[int64_t(s) * count + e];
+  const float m = b1 * mom[e] + (1.f - b1) * gsum;
+  const float v = b2 * vel[e] + (1.f - b2) * gsum * gsum;
+  mom[e] = m;
+  vel[e] = v;
+  const float upd = lr * (m / bc1) / (sqrtf(v / bc2) + eps);
+  const float nw = w[e] - upd;
+  w[e] = nw;
+  if (wb) wb[e] = to16<FSK_PREC_BF16>(nw);
+  if (wtb) {
+    const int64_t r = e / cols, cc = e % cols;
+    wtb[cc * rows + r] = to16<FSK_PREC_BF16>(nw);
+  }
+}
+
+__global__ void mlp_cast_kernel(const float *__restrict__ w, int64_t count, int rows, int cols,
+                                unsigned short *__restrict__ wb, unsigned short *__restrict__ wtb) {
+  const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= count) return;
+  const unsigned short b = to16<FSK_PREC_BF16>(w[e]);
+  if (wb) wb[e] = b;
+  if (wtb) wtb[(e % cols) * rows + e / cols] = b;
+}
+
+__global__ void mlp_mse_kernel(const float *__restrict__ y, const float *__restrict__ t,
+                               int64_t count, float scale, float *__restrict__ dy,
+                               double *__restrict__ loss) {
+  __shared__ double red[256];
+  const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  float d = 0.f;
+  if (e < count) {
+    d = y[e] - t[e];
+    dy[e] = scale * d;
+  }
+  red[threadIdx.x] = double(d) * d;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && loss) atomicAdd(loss, red[0]);
+}
+
+}  // namespace fsk
+
+using namespace fsk;
+
+static int mlp_dims_ok(const char *what, int64_t M, int K, int N) {
+  if (M <= 0 || K <= 0 || N <= 0 || (K & 7) || (N & 7)) {
+    set_error("%s: need M > 0 and K, N positive multiples of 8 (M=%lld K=%d N=%d)", what,
+              (long long)M, K, N);
+    return FSK_EINVAL;
+  }
+  return FSK_OK;
+}
+
+extern "C" int fsk_mlp_linear_fwd(const float *x, int64_t ldx, int64_t M, int K,
+                                  const unsigned short *wt_bf16, const float *bias, int N, int act,
+                                  float *z, float *y, int64_t ldy, void *stream) {
+  const char *what = "fsk_mlp_linear_fwd";
+  int st = mlp_dims_ok(what, M, K, N);
+  if (st != FSK_OK) return st;
+  if (!x || !wt_bf16 || !y || ldx < K || (ldx & 3) || ldy < N || (act != 0 && act != 1) ||
+      (reinterpret_cast<uintptr_t>(x) & 15) || (reinterpret_cast<uintptr_t>(wt_bf16) & 15)) {
+    set_error("%s: bad pointers, strides or activation", what);
+    return FSK_EINVAL;
+  }
+  hipStream_t s = as_stream(stream);
+  dim3 grid(unsigned(cdiv(M, 64)), unsigned(cdiv(N, 64)));
+  if (act) {
+    if (z)
+      hipLaunchKernelGGL((mlp_gemm_kernel<1, true>), grid, dim3(256), 0, s, x, ldx, M, K, wt_bf16,
+                         bias, N, z, y, ldy);
+    else
+      hipLaunchKernelGGL((mlp_gemm_kernel<1, false>), grid, dim3(256), 0, s, x, ldx, M, K, wt_bf16,
+                         bias, N, z, y, ldy);
+  } else {
+    if (z)
+      hipLaunchKernelGGL((mlp_gemm_kernel<0, true>), grid, dim3(256), 0, s, x, ldx, M, K, wt_bf16,
+                         bias, N, z, y, ldy);
+    else
+      hipLaunchKernelGGL((mlp_gemm_kernel<0, false>), grid, dim3(256), 0, s, x, ldx, M, K, wt_bf16,
+                         bias, N, z, y, ldy);
+  }
+  return launch_status(what);
+}
+
+extern "C" int fsk_mlp_act_bwd(const float *dy, const float *z, int64_t count, float *dz,
+                               void *stream) {
+  if (!dy || !z || !dz || count <= 0) {
+    set_error("fsk_mlp_act_bwd: bad arguments");
+    return FSK_EINVAL;
+  }
+  hipLaunchKernelGGL(mlp_act_bwd_kernel, dim3(cdiv(count, 256)), dim3(256), 0, as_stream(stream),
+                     dy, z, count, dz);
+  return launch_status("fsk_mlp_act_bwd");
+}
+
+extern "C" int fsk_mlp_linear_bwd_weight(const float *dz, int64_t M, int N, const float *x,
+                                         int64_t ldx, int K, int splits, float *dw_part,
+                                         float *db_part, void *stream) {
+  const char *what = "fsk_mlp_linear_bwd_weight";
+  int st = mlp_dims_ok(what, M, K, N);
+  if (st != FSK_OK) return st;
+  if (!dz || !x || !dw_part || !db_part || splits <= 0 || splits > 1024 || ldx < K) {
+    set_error("%s: bad arguments", what);
+    return FSK_EINVAL;
+  }
+  hipStream_t s = as_stream(stream);
+  const int64_t split_len = round_up(cdiv(M, splits), 32);
+  hipLaunchKernelGGL(mlp_wgrad_kernel, dim3(unsigned(cdiv(N, 64)), unsigned(cdiv(K, 64)), splits),
+                     dim3(256), 0, s, dz, M, N, x, ldx, K, split_len, dw_part);
+  hipLaunchKernelGGL(mlp_bgrad_kernel, dim3(unsigned(cdiv(N, 256)), splits), dim3(256), 0, s, dz,
+                     M, N, split_len, db_part);
+  return launch_status(what);
+}
+
+extern "C" int fsk_mlp_adam(float *w, float *m, float *v, const float *grad_part, int splits,
+                            int64_t count, int rows, int cols, double lr, double beta1,
+                            double beta2, double eps, int step, unsigned short *w_bf16,
+                            unsigned short *wt_bf16, void *stream) {
+  if (!w || !m || !v || !grad_part || splits <= 0 || count <= 0 || step <= 0 ||
+      int64_t(rows) * cols != count) {
+    set_error("fsk_mlp_adam: bad arguments");
+    return FSK_EINVAL;
+  }
+  const float bc1 = float(1.0 - pow(beta1, step)), bc2 = float(1.0 - pow(beta2, step));
+  hipLaunchKernelGGL(mlp_adam_kernel, dim3(cdiv(count, 256)), dim3(256), 0, as_stream(stream), w, m,
+                     v, grad_part, splits, count, cols, float(lr), float(beta1), float(beta2),
+                     float(eps), bc1, bc2, w_bf16, wt_bf16, rows);
+  return launch_status("fsk_mlp_adam");
+}
+
+extern "C" int fsk_mlp_cast_bf16(const float *w, int rows, int cols, unsigned short *w_bf16,
+                                 unsigned short *wt_bf16, void *stream) {
+  if (!w || rows <= 0 || cols <= 0) {
+    set_error("fsk_mlp_cast_bf16: bad arguments");
+    return FSK_EINVAL;
+  }
+  const int64_t count = int64_t(rows) * cols;
+  hipLaunchKernelGGL(mlp_cast_kernel, dim3(cdiv(count, 256)), dim3(256), 0, as_stream(stream), w,
+                     count, rows, cols, w_bf16, wt_bf16);
+  return launch_status("fsk_mlp_cast_bf16");
+}
+
+extern "C" int fsk_mlp_mse(const float *y, const float *t, int64_t count, double scale, float *dy,
+                           double *loss, void *stream) {
+  if (!y || !t || !dy || count <= 0) {
+    set_error("fsk_mlp_mse: bad arguments");
+    return FSK_EINVAL;
+  }
+  hipLaunchKernelGGL(mlp_mse_kernel, dim3(cdiv(count, 256)), dim3(256), 0, as_stream(stream), y, t,
+                     count, float(scale), dy, loss);
+  return launch_status("fsk_mlp_mse");
+}

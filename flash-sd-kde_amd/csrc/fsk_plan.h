@@ -63,6 +63,12 @@ inline SplitPlan score16_ordered_plan(int64_t n) {
   return make_split_plan(n, n, kScoreRowsPerWG, 16, 2048, cap);
 }
 
+inline SplitPlan score16_rows_plan(int64_t n, int64_t m) {
+  int64_t cap = kOrderedPartialCap / (m * 17 * 4 > 0 ? m * 17 * 4 : 1);
+  if (cap < 1) cap = 1;
+  return make_split_plan(n, m, kScoreRowsPerWG, 16, 2048, cap);
+}
+
 inline SplitPlan kde1_plan(int64_t n, int64_t m) {
   return make_split_plan(n, m, k1dRowsPerWG, k1dChunk, 4096, 1 << 20);
 }

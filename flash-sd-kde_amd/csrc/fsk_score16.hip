@@ -31,11 +31,15 @@ constexpr int kScoreFlush = 32;
 #define FSK_SYM_BAND 1
 #endif
 
+// Rows (queries) and data may differ: rows = data gives the empirical score of
+// the data set itself; a row subset gives the score of those rows under the
+// whole data set's KDE. qa = c|row|^2 padded to a multiple of 256 rows.
 template <int QT>
 __global__ __launch_bounds__(kWG) void score16_ordered_kernel(
     const float *__restrict__ data, int64_t ld, int64_t n, const float *__restrict__ bn,
-    int64_t n_pad, float qscale, int64_t split_len, int64_t qblocks, int64_t ntasks,
-    int64_t per_xcd, float *__restrict__ pdf_part, float *__restrict__ w_part) {
+    int64_t n_pad, const float *__restrict__ rows, int64_t ldr, int64_t m,
+    const float *__restrict__ qa, float qscale, int64_t split_len, int64_t qblocks,
+    int64_t ntasks, int64_t per_xcd, float *__restrict__ pdf_part, float *__restrict__ w_part) {
   const int64_t task = xcd_task(blockIdx.x, per_xcd);
   if (task >= ntasks) return;
   const int64_t split = task / qblocks;
@@ -52,9 +56,9 @@ __global__ __launch_bounds__(kWG) void score16_ordered_kernel(
 #pragma unroll
   for (int R = 0; R < QT; ++R) {
     int64_t i = row0 + 16 * R + c;
-    int64_t ic = i > n - 1 ? n - 1 : i;
-    qv[R] = ld4(data + ic * ld + 4 * g) * qscale;
-    qav[R] = bn[i];  // bn is padded to a multiple of 256 rows
+    int64_t ic = i > m - 1 ? m - 1 : i;
+    qv[R] = ld4(rows + ic * ldr + 4 * g) * qscale;
+    qav[R] = qa[i];  // padded to a multiple of 256 rows
     w[R] = f4{0.f, 0.f, 0.f, 0.f};
     w64[R] = d4{0.0, 0.0, 0.0, 0.0};
     pdf[R] = 0.f;
@@ -87,11 +91,11 @@ __global__ __launch_bounds__(kWG) void score16_ordered_kernel(
   for (int R = 0; R < QT; ++R) {
     double p = sum_groups(pdf64[R] + pdf[R]);
     int64_t i = row0 + 16 * R + c;
-    if (g == 0 && i < n) pdf_part[split * n + i] = float(p);
+    if (g == 0 && i < m) pdf_part[split * m + i] = float(p);
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       int64_t iw = row0 + 16 * R + 4 * g + r;
-      if (iw < n) w_part[(split * n + iw) * kD + c] = float(w64[R][r] + w[R][r]);
+      if (iw < m) w_part[(split * m + iw) * kD + c] = float(w64[R][r] + w[R][r]);
     }
   }
 }
@@ -268,18 +272,25 @@ constexpr int64_t kSymJC = FSK_SYM_JC;  // J blocks per task
 
 struct ScoreLayout {
   float *bn;
+  float *qa;
   float *pdf_part;
   float *w_part;
   double *pdf64;
   double *w64;
 };
 
-static size_t score_layout(void *ws, int64_t n, bool ordered, ScoreLayout *L) {
+static size_t score_layout(void *ws, int64_t n, bool ordered, ScoreLayout *L, int64_t m = -1) {
   Carve cv(ws);
   L->bn = cv.take<float>(round_up(n, kNormPad));
+  L->qa = nullptr;
   L->pdf_part = L->w_part = nullptr;
   L->pdf64 = L->w64 = nullptr;
-  if (ordered) {
+  if (m >= 0) {  // separate rows
+    L->qa = cv.take<float>(round_up(m, kNormPad));
+    SplitPlan p = score16_rows_plan(n, m);
+    L->pdf_part = cv.take<float>(p.splits * m);
+    L->w_part = cv.take<float>(p.splits * m * kD);
+  } else if (ordered) {
     SplitPlan p = score16_ordered_plan(n);
     L->pdf_part = cv.take<float>(p.splits * n);
     L->w_part = cv.take<float>(p.splits * n * kD);
@@ -293,6 +304,11 @@ static size_t score_layout(void *ws, int64_t n, bool ordered, ScoreLayout *L) {
 size_t score16_ws_bytes(int64_t n, bool ordered) {
   ScoreLayout L;
   return score_layout(nullptr, n, ordered, &L);
+}
+
+size_t score16_rows_ws_bytes(int64_t n, int64_t m) {
+  ScoreLayout L;
+  return score_layout(nullptr, n, true, &L, m);
 }
 
 static int score_check(const char *what, const float *data, int64_t ld, int64_t n, double h,
@@ -346,8 +362,8 @@ extern "C" int fsk_score16_ordered(const float *data, int64_t ld, int64_t n, dou
   const int64_t grid = xcd_grid(ntasks, &per_xcd);
   prof_begin(FSK_OP_SCORE16_ORDERED, s);
   hipLaunchKernelGGL((score16_ordered_kernel<kScoreQT>), dim3(grid), dim3(kWG), 0, s, data, ld,
-                     n, L.bn, round_up(n, kTile), float(-2.0 * coef), p.split_len, p.qblocks,
-                     ntasks, per_xcd, L.pdf_part, L.w_part);
+                     n, L.bn, round_up(n, kTile), data, ld, n, L.bn, float(-2.0 * coef),
+                     p.split_len, p.qblocks, ntasks, per_xcd, L.pdf_part, L.w_part);
   prof_end(FSK_OP_SCORE16_ORDERED, s);
   if ((st = launch_status(what)) != FSK_OK) return st;
   if ((st = fsk_reduce_partials_f64(L.pdf_part, p.splits, n, n, pdf, stream)) != FSK_OK)
@@ -392,4 +408,41 @@ extern "C" int fsk_score16_sym_atomic(const float *data, int64_t ld, int64_t n,
   hipLaunchKernelGGL(score_finish_kernel, dim3(cdiv(n * kD, 256)), dim3(256), 0, s, L.pdf64,
                      L.w64, n, pdf, w, ldw_row, ldw_col);
   return launch_status(what);
+}
+
+extern "C" int fsk_score16_rows(const float *rows, int64_t ldr, int64_t m, const float *data,
+                                int64_t ld, int64_t n, double bandwidth, unsigned flags, void *ws,
+                                size_t ws_bytes, float *pdf, float *w, int64_t ldw_row,
+                                int64_t ldw_col, void *stream) {
+  const char *what = "fsk_score16_rows";
+  int st = score_check(what, data, ld, n, bandwidth, flags, pdf, w, false);
+  if (st != FSK_OK) return st;
+  if ((st = check_points16(what, rows, ldr, m)) != FSK_OK) return st;
+  ScoreLayout L;
+  size_t need = score_layout(ws, n, true, &L, m);
+  if (!ws || ws_bytes < need) {
+    set_error("%s: workspace of %zu bytes needed, got %zu", what, need, ws_bytes);
+    return FSK_EWORKSPACE;
+  }
+  hipStream_t s = as_stream(stream);
+  const double coef = exp2_coef(bandwidth);
+  const int64_t n_norm = round_up(n, kNormPad), m_norm = round_up(m, kNormPad);
+  hipLaunchKernelGGL(prep_norms16, dim3(cdiv(n_norm, 256)), dim3(256), 0, s, data, ld, n, n_norm,
+                     coef, L.bn);
+  hipLaunchKernelGGL(prep_norms16, dim3(cdiv(m_norm, 256)), dim3(256), 0, s, rows, ldr, m, m_norm,
+                     coef, L.qa);
+  SplitPlan p = score16_rows_plan(n, m);
+  const int64_t ntasks = p.qblocks * p.splits;
+  int64_t per_xcd;
+  const int64_t grid = xcd_grid(ntasks, &per_xcd);
+  prof_begin(FSK_OP_SCORE16_ORDERED, s);
+  hipLaunchKernelGGL((score16_ordered_kernel<kScoreQT>), dim3(grid), dim3(kWG), 0, s, data, ld,
+                     n, L.bn, round_up(n, kTile), rows, ldr, m, L.qa, float(-2.0 * coef),
+                     p.split_len, p.qblocks, ntasks, per_xcd, L.pdf_part, L.w_part);
+  prof_end(FSK_OP_SCORE16_ORDERED, s);
+  if ((st = launch_status(what)) != FSK_OK) return st;
+  if ((st = fsk_reduce_partials_f64(L.pdf_part, p.splits, m, m, pdf, stream)) != FSK_OK)
+    return st;
+  return fsk_reduce_weighted_partials_f64(L.w_part, p.splits, m, m * kD, w, ldw_row, ldw_col,
+                                          stream);
 }

@@ -40,6 +40,7 @@ OP_SCORE16_ORDERED = 6
 OP_KDE16_LOGDENSITY = 7
 OP_KDE1_SCORE = 8
 OP_LOO_GRID = 9
+OP_SCORE16_ROWS = 10
 
 _p = ctypes.c_void_p
 _i64 = ctypes.c_int64
@@ -63,10 +64,20 @@ SIGNATURES = {
     "fsk_kde1_score": [_p, _i64, _p, _i64, _d, _p, _sz, _p, _p, _p],
     "fsk_score16_sym_atomic": [_p, _i64, _i64, _d, _u, _p, _sz, _p, _p, _i64, _i64, _p],
     "fsk_score16_ordered": [_p, _i64, _i64, _d, _u, _p, _sz, _p, _p, _i64, _i64, _p],
+    "fsk_score16_rows": [_p, _i64, _i64, _p, _i64, _i64, _d, _u, _p, _sz, _p, _p, _i64, _i64, _p],
     "fsk_reduce_partials_f64": [_p, _i64, _i64, _i64, _p, _p],
     "fsk_reduce_weighted_partials_f64": [_p, _i64, _i64, _i64, _p, _i64, _i64, _p],
     "fsk_sdkde_debias_f64": [_p, _i64, _i64, _p, _p, _i64, _i64, _d, _d, _p, _i64, _p, _p],
     "fsk_ref64_workspace_bytes": [_i64, _i64],
+    "fsk_mlp_linear_fwd": [_p, _i64, _i64, ctypes.c_int, _p, _p, ctypes.c_int, ctypes.c_int, _p,
+                           _p, _i64, _p],
+    "fsk_mlp_act_bwd": [_p, _p, _i64, _p, _p],
+    "fsk_mlp_linear_bwd_weight": [_p, _i64, ctypes.c_int, _p, _i64, ctypes.c_int, ctypes.c_int,
+                                  _p, _p, _p],
+    "fsk_mlp_adam": [_p, _p, _p, _p, ctypes.c_int, _i64, ctypes.c_int, ctypes.c_int, _d, _d, _d,
+                     _d, ctypes.c_int, _p, _p, _p],
+    "fsk_mlp_cast_bf16": [_p, ctypes.c_int, ctypes.c_int, _p, _p, _p],
+    "fsk_mlp_mse": [_p, _p, _i64, _d, _p, _p, _p],
     "fsk_loo_workspace_bytes": [_p, ctypes.c_int, ctypes.c_int],
     "fsk_loo_bandwidth_grid": [_p, _i64, _p, ctypes.c_int, _p, ctypes.c_int, ctypes.c_int,
                                ctypes.c_int, _p, _sz, _p, _p],

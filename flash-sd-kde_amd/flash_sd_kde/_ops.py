@@ -186,6 +186,24 @@ def score16(train: torch.Tensor, bandwidth: float, *, variant: str = "symmetric"
     return pdf, (wbuf.t() if transposed_w else wbuf)
 
 
+def score16_rows(rows: torch.Tensor, train: torch.Tensor, bandwidth: float):
+    """(pdf (m,), weighted (m,16)) of `rows` under the KDE of `train` (FP32 form)."""
+    h = _check_h(bandwidth)
+    dev = train.device
+    nat.require_gpu(dev, "score16_rows")
+    n, m = train.shape[0], rows.shape[0]
+    if n == 0 or m == 0:
+        raise ValueError("rows and data must be non-empty.")
+    pdf = torch.empty(m, dtype=torch.float32, device=dev)
+    w = torch.empty((m, ND), dtype=torch.float32, device=dev)
+    ws, nbytes = nat.workspace(nat.OP_SCORE16_ROWS, n, m, 0, dev)
+    st = nat.lib().fsk_score16_rows(rows.data_ptr(), rows.stride(0), m, train.data_ptr(),
+                                    train.stride(0), n, h, 0, ws.data_ptr(), nbytes,
+                                    pdf.data_ptr(), w.data_ptr(), ND, 1, nat.stream_handle(dev))
+    nat.check(st, "fsk_score16_rows")
+    return pdf, w
+
+
 def debias(train: torch.Tensor, pdf: torch.Tensor, weighted: torch.Tensor, bandwidth: float,
            eps: float, *, return_score: bool = False):
     """x + (h^2/2) * ((w/(pdf+eps) - x)/h^2), evaluated in FP64 on the GPU."""

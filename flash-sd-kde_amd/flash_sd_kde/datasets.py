@@ -72,18 +72,32 @@ def gmm_full_cov_params(k: int = 32, *, mean_scale: float = 4.0, seed: int = 0,
     return means, chol
 
 
-def gmm_full_cov(n: int, *, params, seed: int = 0, chunk: int = 1 << 20) -> np.ndarray:
+def gmm_full_cov(n: int, *, params, seed: int = 0) -> np.ndarray:
+    """n points of the equal-weight full-covariance mixture `params`."""
     means, chol = params
     k = means.shape[0]
     rng = np.random.default_rng(seed)
     comp = rng.choice(k, size=n, p=np.full(k, 1.0 / k))
+    z = rng.normal(size=(n, D))
     out = np.empty((n, D), dtype=np.float32)
-    for a in range(0, n, chunk):
-        b = min(n, a + chunk)
-        z = rng.normal(size=(b - a, D))
-        cc = comp[a:b]
-        out[a:b] = (np.einsum("nij,nj->ni", chol[cc], z) + means[cc]).astype(np.float32)
+    for c in range(k):
+        sel = np.nonzero(comp == c)[0]
+        out[sel] = (z[sel] @ chol[c].T + means[c]).astype(np.float32)
     return out
+
+
+def gmm_full_cov_logpdf(x: np.ndarray, params) -> np.ndarray:
+    """Exact log-density of the equal-weight full-covariance mixture (FP64)."""
+    means, chol = params
+    k = means.shape[0]
+    x = np.asarray(x, dtype=np.float64)
+    logs = np.empty((k, x.shape[0]))
+    for c in range(k):
+        y = np.linalg.solve(chol[c], (x - means[c]).T)  # L^-1 (x - mu)
+        logdet = 2.0 * np.sum(np.log(np.abs(np.diag(np.linalg.qr(chol[c])[1]))))
+        logs[c] = -0.5 * np.sum(y * y, axis=0) - 0.5 * logdet - 0.5 * D * np.log(2 * np.pi)
+    mx = logs.max(axis=0)
+    return mx + np.log(np.exp(logs - mx).sum(axis=0)) - np.log(k)
 
 
 def uneven_sizes(count: int = 16, lo: int = 2048, hi: int = 262144, seed: int = 0) -> list[int]:

@@ -56,6 +56,7 @@ extern "C" {
 #define FSK_OP_KDE16_LOGDENSITY 7
 #define FSK_OP_KDE1_SCORE 8
 #define FSK_OP_LOO_GRID 9 /* profiling id only; workspace: fsk_loo_workspace_bytes */
+#define FSK_OP_SCORE16_ROWS 10
 
 const char *fsk_last_error(void);
 int fsk_version(void);
@@ -161,6 +162,17 @@ int fsk_score16_ordered(const float *data, int64_t ld, int64_t n,
                         float *pdf, float *w, int64_t ldw_row, int64_t ldw_col,
                         void *stream);
 
+/* Empirical-score accumulators of m query rows under the KDE of n data points:
+ *   pdf[i] = sum_j exp(-|r_i - x_j|^2/(2h^2)),  w[i,:] = sum_j exp(...) x_j,
+ * FP32 form, deterministic split-K with FP64 combine (workspace
+ * FSK_OP_SCORE16_ROWS with (n, m)). With rows = data this is
+ * fsk_score16_ordered; a row subset gives exact score labels for the learned
+ * surrogate. The reference evaluates the score at separate points only in 1-D
+ * (kde_score_eval, flash_sd_kde/reference.py:83-120). */
+int fsk_score16_rows(const float *rows, int64_t ldr, int64_t m, const float *data, int64_t ld,
+                     int64_t n, double bandwidth, unsigned flags, void *ws, size_t ws_bytes,
+                     float *pdf, float *w, int64_t ldw_row, int64_t ldw_col, void *stream);
+
 /* FP64 split-K combine: out[i] = (float) sum_s (double) partials[s*ld_split + i].
  * Replaces reduce_pdf_partials, kernels/reduce_partials.py:61-79. */
 int fsk_reduce_partials_f64(const float *partials, int64_t n_splits, int64_t m,
@@ -197,6 +209,35 @@ size_t fsk_loo_workspace_bytes(const int64_t *offsets, int n_sets, int n_bw);
 int fsk_loo_bandwidth_grid(const float *data, int64_t ld, const int64_t *offsets, int n_sets,
                            const double *bandwidths, int n_bw, int waves_per_wg, int wg_per_cu,
                            void *ws, size_t ws_bytes, double *out, void *stream);
+
+/* ---- Learned score surrogate: MLP building blocks (no reference counterpart;
+ * the learned pattern). BF16 weights (uint16 bit patterns), FP32 activations
+ * and accumulation; K and N multiples of 8; row-major everything. ---- */
+
+/* y[m, n] = act(sum_k x[m, k] wt[n, k] + bias[n]); act 0 = identity, 1 = SiLU;
+ * z (nullable) keeps the pre-activation. With x = dZ and wt = W^T this is the
+ * data gradient dX = dZ W. */
+int fsk_mlp_linear_fwd(const float *x, int64_t ldx, int64_t M, int K, const uint16_t *wt_bf16,
+                       const float *bias, int N, int act, float *z, float *y, int64_t ldy,
+                       void *stream);
+/* dz = dy * silu'(z), elementwise. */
+int fsk_mlp_act_bwd(const float *dy, const float *z, int64_t count, float *dz, void *stream);
+/* dw_part[s] (N x K) = sum_{m in split s} dz[m, :]^T x[m, :],
+ * db_part[s] (N) = sum_{m in split s} dz[m, :], splits >= 1 (deterministic). */
+int fsk_mlp_linear_bwd_weight(const float *dz, int64_t M, int N, const float *x, int64_t ldx,
+                              int K, int splits, float *dw_part, float *db_part, void *stream);
+/* Adam on FP32 master weights w (rows x cols) with moments m, v; the gradient
+ * is the sum of `splits` partials; also writes w_bf16 (rows x cols) and
+ * wt_bf16 (cols x rows) when non-null. step counts from 1. */
+int fsk_mlp_adam(float *w, float *m, float *v, const float *grad_part, int splits, int64_t count,
+                 int rows, int cols, double lr, double beta1, double beta2, double eps, int step,
+                 uint16_t *w_bf16, uint16_t *wt_bf16, void *stream);
+/* BF16 copies of w (rows x cols) and of its transpose. */
+int fsk_mlp_cast_bf16(const float *w, int rows, int cols, uint16_t *w_bf16, uint16_t *wt_bf16,
+                      void *stream);
+/* dy = scale (y - t); loss (nullable, FP64) += sum (y - t)^2. */
+int fsk_mlp_mse(const float *y, const float *t, int64_t count, double scale, float *dy,
+                double *loss, void *stream);
 
 /* ---- FP64 reference kernels (measurement; SURVEY row "error-suite oracle on
  * the GPU"). Direct differences in FP64, FP64 exp and accumulation; inputs are

@@ -1,0 +1,116 @@
+"""GPU: MLP kernels against the CPU oracle (forward, gradients, Adam),
+training, checkpoint round trip, and the surrogate SD-KDE pipeline including
+what it gains over plain KDE (no debiasing)."""
+
+import json
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from flash_sd_kde import _ops
+from flash_sd_kde.bandwidth import scott_mean_std_bandwidth
+from flash_sd_kde.datasets import gmm_full_cov, gmm_full_cov_logpdf, gmm_full_cov_params
+from flash_sd_kde.surrogate import ScoreMLP, exact_shift, surrogate_fit_transform
+from oracle import mlp
+
+pytestmark = pytest.mark.gpu
+
+
+def record(name, **vals):
+    path = os.environ.get("FSK_METRICS_OUT")
+    print(name, vals, flush=True)
+    if path:
+        with open(path, "a") as f:
+            f.write(json.dumps({"test": name, **vals}) + "\n")
+
+
+def rel(a, b):
+    a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def _oracle_weights(net):
+    Ws = [L["W"].cpu().double() for L in net.layers]
+    bs = [L["b"].cpu().double() for L in net.layers]
+    return Ws, bs
+
+
+@pytest.mark.parametrize("M", [1, 37, 1000, 4096])
+def test_mlp_forward_and_backward_match_oracle(gpu, M):
+    net = ScoreMLP(seed=3, device=gpu)
+    g = torch.Generator().manual_seed(M)
+    x = torch.randn(M, 16, generator=g)
+    t = torch.randn(M, 16, generator=g)
+    cache = []
+    y = net.forward(x.to(gpu), cache)
+    Ws, bs = _oracle_weights(net)
+    y_ref, cache_ref = mlp.forward(Ws, bs, x)
+    e_fwd = rel(y.cpu(), y_ref)
+    dy = net.mse(y, t.to(gpu), None)
+    dy_ref, _ = mlp.mse_grad(y_ref, t)
+    splits = max(1, M // 1024)
+    grads = net.backward(dy, cache, splits)
+    grads_ref = mlp.backward(Ws, cache_ref, dy_ref)
+    e_w = max(rel(gw.sum(0).cpu(), gr[0]) for (gw, _), gr in zip(grads, grads_ref))
+    e_b = max(rel(gb.sum(0).cpu(), gr[1]) for (_, gb), gr in zip(grads, grads_ref))
+    record(f"mlp_M{M}", fwd_rel=e_fwd, dW_rel=e_w, db_rel=e_b)
+    assert e_fwd < 1e-4 and e_w < 2e-3 and e_b < 2e-3
+
+
+def test_mlp_adam_matches_oracle(gpu):
+    net = ScoreMLP(seed=4, device=gpu)
+    W0 = net.layers[0]["W"].cpu().double()
+    grads = [(torch.randn(2, L["N"], L["K"], device=gpu), torch.randn(2, L["N"], device=gpu))
+             for L in net.layers]
+    net.adam_step(grads, 1e-2)
+    g0 = grads[0][0].sum(0).cpu().double()
+    w_ref, _, _ = mlp.adam(W0, torch.zeros_like(W0), torch.zeros_like(W0), g0, 1, 1e-2)
+    assert rel(net.layers[0]["W"].cpu(), w_ref) < 1e-6
+    # BF16 copies follow the master weights (and the transposed copy its transpose)
+    Wb = net.layers[0]["Wb"].cpu().view(torch.bfloat16).double()
+    Wt = net.layers[0]["Wt"].cpu().view(torch.bfloat16).double()
+    assert torch.equal(Wb, mlp.bf16(net.layers[0]["W"].cpu().double()))
+    assert torch.equal(Wt, Wb.T)
+
+
+def test_mlp_trains_and_checkpoints(gpu, tmp_path):
+    g = torch.Generator(device=gpu).manual_seed(0)
+    x = torch.randn(20000, 16, device=gpu, generator=g)
+    t = torch.stack([torch.sin(x[:, k]) * x[:, (k + 1) % 16] for k in range(16)], dim=1)
+    net = ScoreMLP(seed=5, device=gpu)
+    hist = net.fit(x, t, steps=600, batch=4096, lr=3e-3, log_every=100)
+    record("mlp_train", first=hist[0], last=hist[-1])
+    assert hist[-1] < 0.1 * hist[0]
+    path = str(tmp_path / "net.npz")
+    net.save(path)
+    net2 = ScoreMLP.load(path, device=gpu)
+    assert torch.equal(net.predict(x[:100]), net2.predict(x[:100]))
+
+
+def test_surrogate_sdkde_gain_over_plain_kde(gpu):
+    params = gmm_full_cov_params(8, mean_scale=4.0, seed=7)
+    x = gmm_full_cov(32768, params=params, seed=7)
+    q = gmm_full_cov(2048, params=params, seed=8)
+    h = scott_mean_std_bandwidth(x)
+    xd = torch.as_tensor(x, device=gpu)
+    qd = torch.as_tensor(q, device=gpu)
+    deb_s, info = surrogate_fit_transform(xd, h, n_label=16384, n_holdout=4096, n_probe=1024,
+                                          steps=1500, batch=4096, device=gpu, tol=1e-3)
+    deb_e = xd + 0.5 * exact_shift(xd, xd, h)
+    lp_e = _ops.kde16_logdensity(deb_e, qd, h).cpu().numpy()
+    lp_s = _ops.kde16_logdensity(deb_s, qd, h).cpu().numpy()
+    lp_k = _ops.kde16_logdensity(xd, qd, h).cpu().numpy()
+    truth = gmm_full_cov_logpdf(q, params)
+    err_s = float(np.max(np.abs(lp_s - lp_e) / np.abs(lp_e)))
+    err_k = float(np.max(np.abs(lp_k - lp_e) / np.abs(lp_e)))
+    mae = lambda a: float(np.mean(np.abs(a - truth)))
+    record("surrogate_small", fallback=info["fallback"],
+           holdout_err=info["holdout_max_rel_logp_err"], surrogate_vs_exact=err_s,
+           plain_kde_vs_exact=err_k, mae_truth_exact=mae(lp_e), mae_truth_surrogate=mae(lp_s),
+           mae_truth_plain=mae(lp_k), loss_last=info["loss"][-1])
+    # the learned shift recovers most of what SD-KDE changes, and the KDE
+    # built from it is closer to the true density than plain KDE
+    assert err_s < 0.25 * err_k
+    assert mae(lp_s) < mae(lp_k)
