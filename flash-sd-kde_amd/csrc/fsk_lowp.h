@@ -17,6 +17,7 @@ namespace fsk {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef _Float16 h16x8 __attribute__((ext_vector_type(8)));
 typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned short u16x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kRec = 48;            // 16-bit values per packed record
 

@@ -5,10 +5,13 @@
 //                v_mfma_f32_16x16x32_bf16, FP32 accumulate, bias + SiLU fused,
 //                pre-activation Z optionally kept for the backward pass.
 //                The same kernel gives the data gradient dX = dZ . W by passing
-//                the transposed BF16 copy of W.
-//   act_bwd    : dZ = dY * silu'(Z)
+//                the transposed BF16 copy of W; act 2 multiplies that by
+//                silu'(Z_prev) so it yields the previous layer's dZ directly.
 //   wgrad      : dW = dZ^T X and db = sum_m dZ over split batch ranges into
-//                partials (deterministic; the optimizer sums them)
+//                partials (deterministic; the optimizer sums them). Chunks of
+//                64 batch rows are loaded coalesced (float4 per lane), rounded
+//                to BF16 and written transposed into LDS so the MFMA operands
+//                (batch index contiguous) are 16-byte LDS reads.
 //   adam       : Adam on FP32 master weights; writes the BF16 copy and the
 //                transposed BF16 copy used by the data-gradient GEMM
 //   mse        : dY = scale (Y - T), loss += sum (Y - T)^2 (FP64)
@@ -39,6 +42,8 @@ __device__ __forceinline__ u16x8 bf16x8_from(const float *p) {
 }
 
 // Workgroup tile 64 x 64, wave tile 32 x 32 (2 x 2 MFMA blocks), K step 32.
+// ACT 0: Y = Z; 1: Y = silu(Z), Z optionally stored; 2: Y = Z * silu'(Zin)
+// with Zin read from the Z pointer (row stride N).
 template <int ACT, bool STORE_Z>
 __global__ __launch_bounds__(256) void mlp_gemm_kernel(const float *__restrict__ X, int64_t ldx,
                                                        int64_t M, int K,
@@ -89,85 +94,124 @@ __global__ __launch_bounds__(256) void mlp_gemm_kernel(const float *__restrict__
         const int64_t row = m0 + 16 * rb + 4 * g + r;
         if (row < M && n < N) {
           const float z = acc[rb][cb][r] + bv;
-          if (STORE_Z) Z[row * N + n] = z;
-          Y[row * ldy + n] = ACT ? silu(z) : z;
+          if (ACT == 2) {
+            Y[row * ldy + n] = z * silu_grad(Z[row * N + n]);
+          } else {
+            if (STORE_Z) Z[row * N + n] = z;
+            Y[row * ldy + n] = ACT ? silu(z) : z;
+          }
         }
       }
     }
 }
 
-__global__ void mlp_act_bwd_kernel(const float *__restrict__ dy, const float *__restrict__ z,
-                                   int64_t count, float *__restrict__ dz) {
-  const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (e < count) dz[e] = dy[e] * silu_grad(z[e]);
+// dW_part[s] (N x K) = sum_{m in split s} bf16(dZ[m, n]) bf16(X[m, k]) and,
+// from the workgroups of the first K block, db_part[s][n] = sum dZ[m, n] in
+// FP32. Workgroup tile 64 (n) x 64 (k), wave tile 32 x 32, batch chunk 64.
+// Thread (q, b) = (tid >> 4, tid & 15) loads rows 4q..4q+3, columns 4b..4b+3
+// of the chunk (dZ and X), so it holds a 4 x 4 block that it writes
+// transposed as four 8-byte LDS stores. The next chunk's global loads are
+// issued before the current chunk's MFMAs. split_len is a multiple of 64.
+constexpr int WG_LDS_STRIDE = 72;  // u16 per LDS row: 64 batch rows + 8 pad (144 B, 16 B aligned)
+
+__device__ __forceinline__ void wgrad_load(const float *__restrict__ p, int64_t ld, int64_t m,
+                                           int64_t me, bool col_ok, f4 (&v)[4]) {
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+    v[j] = (col_ok && m + j < me) ? ld4(p + (m + j) * ld) : f4{0.f, 0.f, 0.f, 0.f};
 }
 
-// dW_part[s] (N x K) = sum_{m in split s} dZ[m, n] X[m, k]
+__device__ __forceinline__ void wgrad_stage(unsigned short *sT, int b, int q, const f4 (&v)[4]) {
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    u16x4 w;
+    w[0] = to16<FSK_PREC_BF16>(v[0][i]);
+    w[1] = to16<FSK_PREC_BF16>(v[1][i]);
+    w[2] = to16<FSK_PREC_BF16>(v[2][i]);
+    w[3] = to16<FSK_PREC_BF16>(v[3][i]);
+    *reinterpret_cast<u16x4 *>(sT + (4 * b + i) * WG_LDS_STRIDE + 4 * q) = w;
+  }
+}
+
 __global__ __launch_bounds__(256) void mlp_wgrad_kernel(const float *__restrict__ dZ, int64_t M,
                                                         int N, const float *__restrict__ X,
                                                         int64_t ldx, int K, int64_t split_len,
-                                                        float *__restrict__ dW_part) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+                                                        float *__restrict__ dW_part,
+                                                        float *__restrict__ db_part) {
+  __shared__ __align__(16) unsigned short sA[64 * WG_LDS_STRIDE];  // dZ^T chunk [n][m]
+  __shared__ __align__(16) unsigned short sB[64 * WG_LDS_STRIDE];  // X^T chunk [k][m]
+  __shared__ float sbias[16][64];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int g = lane >> 4, c = lane & 15;
-  const int n0 = blockIdx.x * 64 + (wave & 1) * 32;
-  const int k0 = blockIdx.y * 64 + (wave >> 1) * 32;
+  const int q = tid >> 4, b = tid & 15;
+  const int n0 = blockIdx.x * 64, k0 = blockIdx.y * 64;
+  const int wn = (wave & 1) * 32, wk = (wave >> 1) * 32;
   const int64_t s = blockIdx.z;
   const int64_t mb = s * split_len;
   const int64_t me = (mb + split_len < M) ? mb + split_len : M;
+  const bool a_ok = n0 + 4 * b < N, b_ok = k0 + 4 * b < K;  // N, K multiples of 8
+  const float *pa = dZ + n0 + 4 * b;
+  const float *pb = X + k0 + 4 * b;
   f4 acc[2][2];
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
+  for (int x = 0; x < 2; ++x)
 #pragma unroll
-    for (int b = 0; b < 2; ++b) acc[a][b] = f4{0.f, 0.f, 0.f, 0.f};
-  for (int64_t m0 = mb; m0 < me; m0 += 32) {
-    u16x8 A[2], B[2];
+    for (int y = 0; y < 2; ++y) acc[x][y] = f4{0.f, 0.f, 0.f, 0.f};
+  f4 bsum = {0.f, 0.f, 0.f, 0.f};
+  f4 va[4], vb[4];
+  wgrad_load(pa, N, mb + 4 * q, me, a_ok, va);
+  wgrad_load(pb, ldx, mb + 4 * q, me, b_ok, vb);
+  for (int64_t m0 = mb; m0 < me; m0 += 64) {
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      const int64_t m = m0 + 8 * g + u;
-      const bool ok = m < me;
-#pragma unroll
-      for (int rb = 0; rb < 2; ++rb) {
-        const int n = n0 + 16 * rb + c;
-        A[rb][u] = to16<FSK_PREC_BF16>((ok && n < N) ? dZ[m * N + n] : 0.f);
-      }
-#pragma unroll
-      for (int cb = 0; cb < 2; ++cb) {
-        const int k = k0 + 16 * cb + c;
-        B[cb][u] = to16<FSK_PREC_BF16>((ok && k < K) ? X[m * ldx + k] : 0.f);
-      }
+    for (int j = 0; j < 4; ++j) bsum += va[j];
+    __syncthreads();  // previous chunk's LDS reads are done
+    wgrad_stage(sA, b, q, va);
+    wgrad_stage(sB, b, q, vb);
+    __syncthreads();
+    if (m0 + 64 < me) {
+      wgrad_load(pa, N, m0 + 64 + 4 * q, me, a_ok, va);
+      wgrad_load(pb, ldx, m0 + 64 + 4 * q, me, b_ok, vb);
     }
 #pragma unroll
-    for (int rb = 0; rb < 2; ++rb)
+    for (int kk = 0; kk < 64; kk += 32) {
+      u16x8 A[2], B[2];
 #pragma unroll
-      for (int cb = 0; cb < 2; ++cb)
-        acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-            __builtin_bit_cast(bf16x8, A[rb]), __builtin_bit_cast(bf16x8, B[cb]), acc[rb][cb], 0,
-            0, 0);
+      for (int x = 0; x < 2; ++x)
+        A[x] = *reinterpret_cast<const u16x8 *>(sA + (wn + 16 * x + c) * WG_LDS_STRIDE + kk + 8 * g);
+#pragma unroll
+      for (int y = 0; y < 2; ++y)
+        B[y] = *reinterpret_cast<const u16x8 *>(sB + (wk + 16 * y + c) * WG_LDS_STRIDE + kk + 8 * g);
+#pragma unroll
+      for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+          acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+              __builtin_bit_cast(bf16x8, A[x]), __builtin_bit_cast(bf16x8, B[y]), acc[x][y], 0, 0,
+              0);
+    }
   }
 #pragma unroll
-  for (int rb = 0; rb < 2; ++rb)
+  for (int x = 0; x < 2; ++x)
 #pragma unroll
-    for (int cb = 0; cb < 2; ++cb) {
-      const int k = k0 + 16 * cb + c;
+    for (int y = 0; y < 2; ++y) {
+      const int k = k0 + wk + 16 * y + c;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int n = n0 + 16 * rb + 4 * g + r;
-        if (n < N && k < K) dW_part[(s * N + n) * K + k] = acc[rb][cb][r];
+        const int n = n0 + wn + 16 * x + 4 * g + r;
+        if (n < N && k < K) dW_part[(s * N + n) * K + k] = acc[x][y][r];
       }
     }
-}
-
-// db_part[s][n] = sum_{m in split s} dZ[m, n]
-__global__ void mlp_bgrad_kernel(const float *__restrict__ dZ, int64_t M, int N, int64_t split_len,
-                                 float *__restrict__ db_part) {
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t s = blockIdx.y;
-  if (n >= N) return;
-  const int64_t mb = s * split_len;
-  const int64_t me = (mb + split_len < M) ? mb + split_len : M;
-  float acc = 0.f;
-  for (int64_t m = mb; m < me; ++m) acc += dZ[m * N + n];
-  db_part[s * N + n] = acc;
+  if (blockIdx.y == 0) {  // bias gradient: fixed-order sum over the 16 row groups
+#pragma unroll
+    for (int i = 0; i < 4; ++i) sbias[q][4 * b + i] = bsum[i];
+    __syncthreads();
+    if (tid < 64 && n0 + tid < N) {
+      float t = 0.f;
+#pragma unroll
+      for (int j = 0; j < 16; ++j) t += sbias[j][tid];
+      db_part[s * N + n0 + tid] = t;
+    }
+  }
 }
 
 __global__ void mlp_adam_kernel(float *__restrict__ w, float *__restrict__ mom,
@@ -184,6 +228,7 @@ __global__ void mlp_adam_kernel(float *__restrict__ w, float *__restrict__ mom,
     bc2 = sched[2];
   }
   float gsum = 0.f;
+#pragma unroll 8
   for (int s = 0; s < S; ++s) gsum += part[int64_t(s) * count + e];
   const float m = b1 * mom[e] + (1.f - b1) * gsum;
   const float v = b2 * vel[e] + (1.f - b2) * gsum * gsum;
@@ -246,14 +291,18 @@ extern "C" int fsk_mlp_linear_fwd(const float *x, int64_t ldx, int64_t M, int K,
   const char *what = "fsk_mlp_linear_fwd";
   int st = mlp_dims_ok(what, M, K, N);
   if (st != FSK_OK) return st;
-  if (!x || !wt_bf16 || !y || ldx < K || (ldx & 3) || ldy < N || (act != 0 && act != 1) ||
+  if (!x || !wt_bf16 || !y || ldx < K || (ldx & 3) || ldy < N || act < 0 || act > 2 ||
+      (act == 2 && (!z || bias)) ||
       (reinterpret_cast<uintptr_t>(x) & 15) || (reinterpret_cast<uintptr_t>(wt_bf16) & 15)) {
     set_error("%s: bad pointers, strides or activation", what);
     return FSK_EINVAL;
   }
   hipStream_t s = as_stream(stream);
   dim3 grid(unsigned(cdiv(M, 64)), unsigned(cdiv(N, 64)));
-  if (act) {
+  if (act == 2) {
+    hipLaunchKernelGGL((mlp_gemm_kernel<2, false>), grid, dim3(256), 0, s, x, ldx, M, K, wt_bf16,
+                       bias, N, z, y, ldy);
+  } else if (act) {
     if (z)
       hipLaunchKernelGGL((mlp_gemm_kernel<1, true>), grid, dim3(256), 0, s, x, ldx, M, K, wt_bf16,
                          bias, N, z, y, ldy);
@@ -271,33 +320,21 @@ extern "C" int fsk_mlp_linear_fwd(const float *x, int64_t ldx, int64_t M, int K,
   return launch_status(what);
 }
 
-extern "C" int fsk_mlp_act_bwd(const float *dy, const float *z, int64_t count, float *dz,
-                               void *stream) {
-  if (!dy || !z || !dz || count <= 0) {
-    set_error("fsk_mlp_act_bwd: bad arguments");
-    return FSK_EINVAL;
-  }
-  hipLaunchKernelGGL(mlp_act_bwd_kernel, dim3(cdiv(count, 256)), dim3(256), 0, as_stream(stream),
-                     dy, z, count, dz);
-  return launch_status("fsk_mlp_act_bwd");
-}
-
 extern "C" int fsk_mlp_linear_bwd_weight(const float *dz, int64_t M, int N, const float *x,
                                          int64_t ldx, int K, int splits, float *dw_part,
                                          float *db_part, void *stream) {
   const char *what = "fsk_mlp_linear_bwd_weight";
   int st = mlp_dims_ok(what, M, K, N);
   if (st != FSK_OK) return st;
-  if (!dz || !x || !dw_part || !db_part || splits <= 0 || splits > 1024 || ldx < K) {
-    set_error("%s: bad arguments", what);
+  if (!dz || !x || !dw_part || !db_part || splits <= 0 || splits > 1024 || ldx < K || (ldx & 3) ||
+      (reinterpret_cast<uintptr_t>(x) & 15) || (reinterpret_cast<uintptr_t>(dz) & 15)) {
+    set_error("%s: bad arguments (pointers 16-byte aligned, ldx a multiple of 4)", what);
     return FSK_EINVAL;
   }
   hipStream_t s = as_stream(stream);
-  const int64_t split_len = round_up(cdiv(M, splits), 32);
+  const int64_t split_len = round_up(cdiv(M, splits), 64);
   hipLaunchKernelGGL(mlp_wgrad_kernel, dim3(unsigned(cdiv(N, 64)), unsigned(cdiv(K, 64)), splits),
-                     dim3(256), 0, s, dz, M, N, x, ldx, K, split_len, dw_part);
-  hipLaunchKernelGGL(mlp_bgrad_kernel, dim3(unsigned(cdiv(N, 256)), splits), dim3(256), 0, s, dz,
-                     M, N, split_len, db_part);
+                     dim3(256), 0, s, dz, M, N, x, ldx, K, split_len, dw_part, db_part);
   return launch_status(what);
 }
 

@@ -71,7 +71,6 @@ SIGNATURES = {
     "fsk_ref64_workspace_bytes": [_i64, _i64],
     "fsk_mlp_linear_fwd": [_p, _i64, _i64, ctypes.c_int, _p, _p, ctypes.c_int, ctypes.c_int, _p,
                            _p, _i64, _p],
-    "fsk_mlp_act_bwd": [_p, _p, _i64, _p, _p],
     "fsk_mlp_linear_bwd_weight": [_p, _i64, ctypes.c_int, _p, _i64, ctypes.c_int, ctypes.c_int,
                                   _p, _p, _p],
     "fsk_mlp_adam": [_p, _p, _p, _p, ctypes.c_int, _i64, ctypes.c_int, ctypes.c_int, _d, _d, _d,

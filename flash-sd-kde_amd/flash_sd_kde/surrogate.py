@@ -2,8 +2,8 @@
 
 The exact one-step SD-KDE shifts every point by half its mean shift,
     x_i -> x_i + (h^2 / 2) score_i = x_i + delta_i / 2,   delta_i = w_i / pdf_i - x_i,
-which costs n^2 kernel evaluations. Here a small MLP, 16 -> 256 -> 256 ->
-256 -> 16 with SiLU (BF16 weights, FP32 accumulation), learns x -> delta from
+which costs n^2 kernel evaluations. Here a small MLP, 16 -> 384 -> 384 ->
+384 -> 16 with SiLU (BF16 weights, FP32 accumulation), learns x -> delta from
 exact labels on a subset (the FP32 score of the subset rows under the KDE of
 ALL points, fsk_score16_rows) and then shifts all n points at O(n) cost.
 
@@ -31,7 +31,7 @@ import torch
 from flash_sd_kde import _native as nat
 from flash_sd_kde import _ops
 
-DIMS = (16, 256, 256, 256, 16)
+DIMS = (16, 384, 384, 384, 16)
 
 
 def _bf16_buffer(shape, device):
@@ -94,21 +94,15 @@ class ScoreMLP:
         return h
 
     def backward(self, dy: torch.Tensor, cache: list, splits: int):
-        """Gradient partials per layer: list of (dW_part, db_part)."""
+        """Gradient partials per layer: list of (dW_part, db_part). The data
+        gradient GEMM of layer l also applies silu'(z_{l-1}), so it hands the
+        previous layer its dZ directly."""
         M = dy.shape[0]
         grads = [None] * len(self.layers)
-        last = len(self.layers) - 1
-        g = dy.contiguous()
-        for l in range(last, -1, -1):
+        dz = dy.contiguous()
+        for l in range(len(self.layers) - 1, -1, -1):
             L = self.layers[l]
-            h, z = cache[l]
-            if l != last:
-                dz = torch.empty_like(g)
-                st = nat.lib().fsk_mlp_act_bwd(g.data_ptr(), z.data_ptr(), g.numel(),
-                                               dz.data_ptr(), self._stream())
-                nat.check(st, "fsk_mlp_act_bwd")
-            else:
-                dz = g
+            h, _ = cache[l]
             dWp = torch.empty((splits, L["N"], L["K"]), dtype=torch.float32, device=self.device)
             dbp = torch.empty((splits, L["N"]), dtype=torch.float32, device=self.device)
             st = nat.lib().fsk_mlp_linear_bwd_weight(dz.data_ptr(), M, L["N"], h.data_ptr(),
@@ -117,12 +111,14 @@ class ScoreMLP:
             nat.check(st, "fsk_mlp_linear_bwd_weight")
             grads[l] = (dWp, dbp)
             if l > 0:
-                dh = torch.empty((M, L["K"]), dtype=torch.float32, device=self.device)
+                z_prev = cache[l - 1][1]
+                dz_prev = torch.empty((M, L["K"]), dtype=torch.float32, device=self.device)
                 st = nat.lib().fsk_mlp_linear_fwd(dz.data_ptr(), L["N"], M, L["N"],
-                                                  L["Wt"].data_ptr(), None, L["K"], 0, None,
-                                                  dh.data_ptr(), L["K"], self._stream())
-                nat.check(st, "fsk_mlp_linear_fwd (data gradient)")
-                g = dh
+                                                  L["Wt"].data_ptr(), None, L["K"], 2,
+                                                  z_prev.data_ptr(), dz_prev.data_ptr(), L["K"],
+                                                  self._stream())
+                nat.check(st, "fsk_mlp_linear_fwd (data gradient x silu')")
+                dz = dz_prev
         return grads
 
     def adam_step(self, grads, lr: float, beta1=0.9, beta2=0.999, eps=1e-8,
@@ -155,10 +151,12 @@ class ScoreMLP:
     # ---- training / inference in data units ----
     def fit(self, x: torch.Tensor, target: torch.Tensor, *, steps: int = 3000, batch: int = 8192,
             lr: float = 2e-3, seed: int = 0, log_every: int = 250, graph: bool = True,
-            beta1: float = 0.9, beta2: float = 0.999):
+            beta1: float = 0.9, beta2: float = 0.999, splits: int | None = None,
+            warmup: int = 0):
         """Fit x (n,16) -> target (n,16); inputs standardised per feature and
         targets by their overall scale; cosine learning-rate decay; minibatches
-        drawn uniformly with replacement. With graph=True one training step
+        drawn uniformly with replacement; `warmup` steps of linear learning-rate
+        ramp before the decay. With graph=True one training step
         (gather, forward, loss, backward, Adam) is captured once as a HIP graph
         and replayed; only the batch indices and the schedule entry change
         between replays (both written on the device)."""
@@ -171,13 +169,14 @@ class ScoreMLP:
         tn = (target / self.y_scale).contiguous()
         n = xn.shape[0]
         batch = min(batch, n)
-        splits = max(1, batch // 1024)
+        splits = splits or max(1, batch // 1024)
         dev = self.device
         gen = torch.Generator(device=dev)
         gen.manual_seed(seed)
         t0 = self.step_count
         k = torch.arange(1, steps + 1, dtype=torch.float64)
-        table = torch.stack([lr * 0.5 * (1.0 + torch.cos(math.pi * (k - 1) / steps)),
+        ramp = torch.clamp(k / max(warmup, 1), max=1.0) if warmup > 0 else torch.ones_like(k)
+        table = torch.stack([lr * ramp * 0.5 * (1.0 + torch.cos(math.pi * (k - 1) / steps)),
                              1.0 - beta1 ** (k + t0), 1.0 - beta2 ** (k + t0)], dim=1)
         table = table.to(torch.float32).to(dev)
         sched = torch.empty(3, dtype=torch.float32, device=dev)
@@ -262,8 +261,8 @@ def exact_shift(rows: torch.Tensor, data: torch.Tensor, h: float, eps: float = 1
 
 def surrogate_fit_transform(data, bandwidth: float, *, n_label: int = 262144,
                             n_holdout: int = 16384, n_probe: int = 16384, tol: float = 1e-3,
-                            check_margin: float = 0.5, steps: int = 6000, batch: int = 16384,
-                            lr: float = 2e-3, seed: int = 0, device="cuda",
+                            check_margin: float = 0.5, steps: int = 8000, batch: int = 8192,
+                            lr: float = 6e-3, seed: int = 0, device="cuda",
                             net: ScoreMLP | None = None):
     """One-step SD-KDE debiasing with the learned shift field.
 

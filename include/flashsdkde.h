@@ -216,14 +216,16 @@ int fsk_loo_bandwidth_grid(const float *data, int64_t ld, const int64_t *offsets
 
 /* y[m, n] = act(sum_k x[m, k] wt[n, k] + bias[n]); act 0 = identity, 1 = SiLU;
  * z (nullable) keeps the pre-activation. With x = dZ and wt = W^T this is the
- * data gradient dX = dZ W. */
+ * data gradient dX = dZ W. act 2 (bias must be null, z required and READ):
+ * y[m, n] = (sum_k x[m, k] wt[n, k]) * silu'(z[m, n]), z with row stride N,
+ * i.e. the previous layer's dZ in one pass. */
 int fsk_mlp_linear_fwd(const float *x, int64_t ldx, int64_t M, int K, const uint16_t *wt_bf16,
                        const float *bias, int N, int act, float *z, float *y, int64_t ldy,
                        void *stream);
-/* dz = dy * silu'(z), elementwise. */
-int fsk_mlp_act_bwd(const float *dy, const float *z, int64_t count, float *dz, void *stream);
-/* dw_part[s] (N x K) = sum_{m in split s} dz[m, :]^T x[m, :],
- * db_part[s] (N) = sum_{m in split s} dz[m, :], splits >= 1 (deterministic). */
+/* dw_part[s] (N x K) = sum_{m in split s} bf16(dz[m, :])^T bf16(x[m, :]),
+ * db_part[s] (N) = sum_{m in split s} dz[m, :] (FP32), splits >= 1, split s
+ * covering rows [s L, (s+1) L) with L = ceil(M / splits) rounded up to 64
+ * (deterministic). dz and x 16-byte aligned, ldx a multiple of 4. */
 int fsk_mlp_linear_bwd_weight(const float *dz, int64_t M, int N, const float *x, int64_t ldx,
                               int K, int splits, float *dw_part, float *db_part, void *stream);
 /* Adam on FP32 master weights w (rows x cols) with moments m, v; the gradient
