@@ -93,6 +93,26 @@ def cpu_baseline(x, q, h, budget_s):
     }
 
 
+# HBM bytes per launch of the dominant kernel, from two rocprofv3 --pmc runs of
+# `bench.py --steps 1 --warmup 0 --no-cpu-baseline --no-batched
+# --no-precision-forms --no-surrogate` (FETCH_SIZE, then WRITE_SIZE, KiB per
+# dispatch), each scaled by what the same counter reads for the library's copy
+# microbenchmark in the same run (512 MiB read and 512 MiB written per
+# dispatch: FETCH_SIZE reads 262169 KiB -> scale 2.0, WRITE_SIZE reads
+# 524288 KiB -> scale 1.0). Keyed by (score kernel, n); other sizes report null.
+PMC_TRAFFIC = {
+    ("ordered", 1 << 20): {"fetch_kib": 4507360.0, "write_kib": 139264.0,
+                           "fetch_scale": 524288.0 / 262169.0, "write_scale": 1.0},
+}
+
+
+def pmc_traffic_bytes(variant, n):
+    t = PMC_TRAFFIC.get((variant, n))
+    if t is None:
+        return None
+    return round((t["fetch_kib"] * t["fetch_scale"] + t["write_kib"] * t["write_scale"]) * 1024.0)
+
+
 def measure_rates():
     """Issue rates measured on this GPU by the library's microbenchmarks."""
     import ctypes
@@ -110,6 +130,12 @@ def measure_rates():
         lib.fsk_microbench(which, 2048, 50, out.data_ptr(), ctypes.addressof(r), None)
         st = lib.fsk_microbench(which, 2048, iters, out.data_ptr(), ctypes.addressof(r), None)
         rates[name] = r.value if st == 0 else None
+    # HBM copy: 512 MiB read + 512 MiB written per launch (2^20 blocks x 1 KiB)
+    buf = torch.empty(1 << 28, device="cuda")
+    lib.fsk_microbench(5, 1 << 20, 1, buf.data_ptr(), ctypes.addressof(r), None)
+    st = lib.fsk_microbench(5, 1 << 20, 10, buf.data_ptr(), ctypes.addressof(r), None)
+    rates["hbm_copy_bytes_per_s"] = r.value if st == 0 else None
+    del buf
     return rates
 
 
@@ -361,7 +387,7 @@ def main():
             "peak": PEAK_FP32_MFMA_TFLOPS,
             "unit": "TFLOP/s",
             "frac": round(achieved / PEAK_FP32_MFMA_TFLOPS, 4) if achieved else None,
-            "traffic": None,
+            "traffic": pmc_traffic_bytes(args.variant, n),
             "kernel_ms": round(score_ms, 3),
             "exp_valu_frac": round(exp_valu_s / (score_ms * 1e-3), 4) if score_ms > 0 else None,
         },

@@ -1,6 +1,8 @@
 // fsk_microbench.hip - issue-rate microbenchmarks for the roofline of the pair
-// kernels: FP32 MFMA 16x16x4, BF16 MFMA 16x16x32, v_exp_f32, v_fma_f32, and
-// an MFMA + exp mix that shows whether the two pipes overlap.
+// kernels: FP32 MFMA 16x16x4, BF16 MFMA 16x16x32, v_exp_f32, v_fma_f32, an
+// MFMA + exp mix that shows whether the two pipes overlap, and a plain HBM
+// copy (known bytes per launch: the calibration point for the FETCH_SIZE /
+// WRITE_SIZE counters and the measured HBM bandwidth).
 //
 // Every kernel runs independent dependency chains per lane in a loop and
 // writes one value per thread to the caller's buffer so nothing is removed
@@ -92,15 +94,26 @@ __global__ __launch_bounds__(256) void mb_mix(float *out, int iters) {
   out[blockIdx.x * blockDim.x + threadIdx.x] = s;
 }
 
+// dst[i] = src[i] over n4 float4s; grid-stride, 16-byte loads and stores.
+__global__ __launch_bounds__(256) void mb_copy(const f4 *__restrict__ src, f4 *__restrict__ dst,
+                                               int64_t n4) {
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n4; i += stride)
+    dst[i] = src[i];
+}
+
 }  // namespace fsk
 
 using namespace fsk;
 
 // which: 0 = FP32 MFMA (flop/s), 1 = BF16 MFMA (flop/s), 2 = v_exp_f32 (exp/s),
-// 3 = v_fma_f32 (fma/s), 4 = mix (returns seconds per iteration-block, see header).
+// 3 = v_fma_f32 (fma/s), 4 = mix (returns seconds per iteration-block, see header),
+// 5 = copy of the first half of `out` to its second half, `iters` launches
+// (bytes read + written per second).
 extern "C" int fsk_microbench(int which, int blocks, int iters, float *out, double *result,
                               void *stream) {
-  if (which < 0 || which > 4 || blocks <= 0 || iters <= 0 || !out || !result) {
+  if (which < 0 || which > 5 || blocks <= 0 || iters <= 0 || !out || !result ||
+      (which == 5 && (reinterpret_cast<uintptr_t>(out) & 15))) {
     set_error("fsk_microbench: bad arguments");
     return FSK_EINVAL;
   }
@@ -116,7 +129,15 @@ extern "C" int fsk_microbench(int which, int blocks, int iters, float *out, doub
     case 1: hipLaunchKernelGGL(mb_mfma_bf16, dim3(blocks), dim3(256), 0, s, out, iters); break;
     case 2: hipLaunchKernelGGL(mb_exp, dim3(blocks), dim3(256), 0, s, out, iters); break;
     case 3: hipLaunchKernelGGL(mb_fma, dim3(blocks), dim3(256), 0, s, out, iters); break;
-    default: hipLaunchKernelGGL(mb_mix, dim3(blocks), dim3(256), 0, s, out, iters); break;
+    case 4: hipLaunchKernelGGL(mb_mix, dim3(blocks), dim3(256), 0, s, out, iters); break;
+    default: {
+      // blocks * 256 floats = blocks * 64 float4; each half is blocks * 32 float4
+      const int64_t n4 = int64_t(blocks) * 32;
+      const f4 *src = reinterpret_cast<const f4 *>(out);
+      f4 *dst = reinterpret_cast<f4 *>(out) + n4;
+      for (int i = 0; i < iters; ++i)
+        hipLaunchKernelGGL(mb_copy, dim3(4096), dim3(256), 0, s, src, dst, n4);
+    } break;
   }
   (void)hipEventRecord(e1, s);
   int st = launch_status("fsk_microbench");
@@ -133,6 +154,7 @@ extern "C" int fsk_microbench(int which, int blocks, int iters, float *out, doub
     case 1: *result = waves * iters * 4.0 * 16384.0 / sec; break;
     case 2: *result = waves * 64.0 * iters * 8.0 / sec; break;
     case 3: *result = waves * 64.0 * iters * 8.0 / sec; break;
+    case 5: *result = double(blocks) * 1024.0 * iters / sec; break;  // 2 x half-buffer bytes
     default: *result = sec; break;
   }
   return FSK_OK;

@@ -73,8 +73,11 @@ double fsk_profile_last_ms(int op);
 /* Issue-rate microbenchmark (no reference counterpart; used to measure the
  * roofline the pair kernels are judged against). which: 0 FP32 MFMA flop/s,
  * 1 BF16 MFMA flop/s, 2 v_exp_f32 per s, 3 v_fma_f32 per s, 4 seconds of the
- * MFMA+exp mix kernel. `blocks` workgroups of 256 threads, `iters` loop trips;
- * `out` is a device buffer of blocks*256 floats. Synchronizes the stream. */
+ * MFMA+exp mix kernel, 5 HBM copy bytes/s (read + write): the first half of
+ * `out` copied to its second half by `iters` launches of a 4096-workgroup
+ * copy kernel (blocks*512 bytes read and written per launch). `blocks`
+ * workgroups of 256 threads, `iters` loop trips; `out` is a 16-byte aligned
+ * device buffer of blocks*256 floats. Synchronizes the stream. */
 int fsk_microbench(int which, int blocks, int iters, float *out, double *result, void *stream);
 
 /* Bytes of scratch the operation needs for the given sizes (0 if none). */
