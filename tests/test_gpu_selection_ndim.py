@@ -1,0 +1,76 @@
+"""GPU: held-out-likelihood bandwidth selection (reference
+benchmarks/mnist_fashion_pca16_ood.py:63-92) and KDE / SD-KDE for d <= 16 by
+zero padding, against the FP64 oracle."""
+
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from flash_sd_kde.bandwidth import silverman_bandwidth_nd
+from flash_sd_kde.datasets import gmm_isotropic_split
+from flash_sd_kde.ndim import kde_eval_nd, kde_logdensity_nd, sd_kde_fit_transform_nd
+from flash_sd_kde.selection import select_bandwidth, select_bandwidth_lse, validation_loglik_grid
+from oracle import fp64
+
+pytestmark = pytest.mark.gpu
+
+MULTS = [0.5, 0.7, 1.0, 1.4, 2.0]
+
+
+def test_select_bandwidth_matches_oracle(gpu):
+    train, val, _ = gmm_isotropic_split(4096, 1024, k=8, mean_scale=3.0, seed=0)
+    base = silverman_bandwidth_nd(train)
+    best_h, best_ll, scores = select_bandwidth(train, val, base, MULTS, device=gpu,
+                                               return_scores=True)
+    ref = []
+    for mult in MULTS:
+        dens = fp64.kde_density(val, train, base * mult)
+        ref.append(float(np.mean(np.log(dens + 1e-12))))
+    got = [ll for _, ll in scores]
+    assert [h for h, _ in scores] == [base * m for m in MULTS]
+    # float32 host log/mean of float32 densities vs FP64 throughout
+    np.testing.assert_allclose(got, ref, rtol=1e-5)
+    k = int(np.argmax(ref))
+    assert best_h == base * MULTS[k] and best_ll == got[k]
+
+
+def test_validation_loglik_grid_matches_fp64(gpu):
+    train, val, _ = gmm_isotropic_split(4096, 1024, k=8, mean_scale=3.0, seed=2)
+    base = silverman_bandwidth_nd(train)
+    hs = [base * m for m in MULTS]
+    got = validation_loglik_grid(train, val, hs, device=gpu)
+    ref = np.array([fp64.log_density(val, train, h).mean() for h in hs])
+    np.testing.assert_allclose(got, ref, rtol=2e-6)
+    h, ll = select_bandwidth_lse(train, val, base, MULTS, device=gpu)
+    assert h == hs[int(np.argmax(ref))] and ll == pytest.approx(ref.max(), rel=2e-6)
+
+
+@pytest.mark.parametrize("d", [1, 2, 5, 11, 16])
+def test_kde_nd_by_padding_matches_fp64(gpu, d):
+    rng = np.random.default_rng(d)
+    x = rng.standard_normal((3000, d)).astype(np.float32)
+    q = rng.standard_normal((700, d)).astype(np.float32)
+    h = 0.9 * float(np.std(x)) * 3000 ** (-1.0 / (d + 4))
+    dens = kde_eval_nd(x, q, h, device=gpu).cpu().numpy()
+    ref = fp64.kde_density(q, x, h)
+    assert np.max(np.abs(dens - ref) / ref) < 5e-5
+    # log-density: the FP32 cross term rounds at the scale of |q|^2 / (2 h^2)
+    # (about 140 for d = 1, h = 0.18, |q| = 3), an absolute error in log p
+    # that a relative bound cannot express where |log p| is near 0
+    lp = kde_logdensity_nd(x, q, h, device=gpu).cpu().numpy()
+    lref = fp64.log_density(q, x, h)
+    assert np.all(np.abs(lp - lref) < 3e-5 + 2e-6 * np.abs(lref))
+
+
+@pytest.mark.parametrize("d", [2, 7])
+def test_sd_kde_nd_by_padding_matches_fp64(gpu, d):
+    rng = np.random.default_rng(10 + d)
+    x = rng.standard_normal((2500, d)).astype(np.float32)
+    h = 0.9 * float(np.std(x)) * 2500 ** (-1.0 / (d + 4))
+    deb = sd_kde_fit_transform_nd(x, h, device=gpu).cpu().numpy()
+    assert deb.shape == (2500, d)
+    pdf, w = fp64.score_sums(x, h)
+    ref, _ = fp64.debias(x, pdf, w, h)
+    assert np.max(np.abs(deb - ref)) < 2e-5 * max(1.0, float(np.abs(ref).max()))

@@ -83,3 +83,18 @@ def test_cpu_tensors_are_rejected_not_computed():
 
     with pytest.raises((ValueError, RuntimeError)):
         reduce_pdf_partials(torch.ones(3, 5))
+
+
+def test_ndim_rejects_bad_dimensions():
+    from flash_sd_kde.ndim import pad16
+
+    with pytest.raises(ValueError, match="dimension"):
+        pad16(np.zeros((4, 17), np.float32), "cpu")
+    with pytest.raises(ValueError, match="2D"):
+        pad16(np.zeros((2, 2, 2), np.float32), "cpu")
+    with pytest.raises(ValueError, match="non-empty"):
+        pad16(np.zeros((0, 3), np.float32), "cpu")
+    with pytest.raises(ValueError, match="positive"):
+        from flash_sd_kde.ndim import kde_eval_nd
+
+        kde_eval_nd(np.zeros((4, 3)), np.zeros((2, 3)), 0.0)
