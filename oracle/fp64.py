@@ -92,18 +92,26 @@ def log_density_expansion(queries, data, bandwidth: float, *, query_chunk: int =
     return out
 
 
-def score_sums(data, bandwidth: float):
-    """(pdf (n,), weighted (n,d)) over all ordered pairs incl. j = i, FP64 direct."""
+def score_rows(rows, data, bandwidth: float):
+    """(pdf (m,), weighted (m,d)) of `rows` against all of `data`: sum_j phi(r_i, x_j)
+    and sum_j phi(r_i, x_j) x_j, FP64 direct differences."""
+    r = np.asarray(rows, dtype=np.float64)
     x = np.asarray(data, dtype=np.float64)
-    n = x.shape[0]
-    pdf = np.zeros(n)
-    w = np.zeros_like(x)
-    per = max(1, 4_000_000 // max(1, n * x.shape[1]))
-    for a, b in _blocks(n, per):
-        phi = np.exp(-0.5 * _sq_dist(x[a:b], x) / (bandwidth * bandwidth))
+    m = r.shape[0]
+    pdf = np.zeros(m)
+    w = np.zeros((m, x.shape[1]))
+    per = max(1, 4_000_000 // max(1, x.shape[0] * x.shape[1]))
+    for a, b in _blocks(m, per):
+        phi = np.exp(-0.5 * _sq_dist(r[a:b], x) / (bandwidth * bandwidth))
         pdf[a:b] = phi.sum(axis=1)
         w[a:b] = phi @ x
     return pdf, w
+
+
+def score_sums(data, bandwidth: float):
+    """(pdf (n,), weighted (n,d)) over all ordered pairs incl. j = i, FP64 direct:
+    score_rows with the data as its own rows."""
+    return score_rows(data, data, bandwidth)
 
 
 def debias(data, pdf, weighted, bandwidth: float, eps: float = 1e-12):

@@ -233,6 +233,20 @@ def test_score_ragged_and_symmetric_equals_ordered(gpu, n):
         assert abs(cpu(po)[0] - 1.0) < 3e-5 and np.allclose(cpu(wo), x, rtol=3e-5)
 
 
+@pytest.mark.parametrize("m,n", [(1, 5), (37, 1000), (1000, 4097), (300, 1)])
+def test_score_rows_matches_fp64(gpu, m, n):
+    """fsk_score16_rows: rows that are not data points, ragged sizes."""
+    rng = np.random.default_rng(m * 7 + n)
+    rows = rng.normal(size=(m, 16)).astype(np.float32)
+    x = rng.normal(size=(n, 16)).astype(np.float32)
+    h = 1.05
+    pdf, w = _ops.score16_rows(torch.as_tensor(rows, device=gpu), torch.as_tensor(x, device=gpu), h)
+    pe, we = fp64.score_rows(rows, x, h)
+    np.testing.assert_allclose(cpu(pdf), pe, rtol=3e-5, atol=1e-30)
+    scale = max(1e-30, float(np.max(np.abs(we))))
+    assert np.max(np.abs(cpu(w) - we)) / scale < 3e-5
+
+
 def test_score_permutation_equivariance(gpu):
     x = np.random.default_rng(9).normal(size=(1500, 16)).astype(np.float32)
     perm = np.random.default_rng(10).permutation(1500)

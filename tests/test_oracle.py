@@ -49,6 +49,20 @@ def test_empirical_score_matches_reference(golden, case):
     np.testing.assert_allclose(score, golden(case, "score"), rtol=1e-4, atol=1e-4)
 
 
+@pytest.mark.parametrize("case", ["score_a", "score_large"])
+def test_score_rows_matches_reference_rows(golden, case):
+    """A subset of rows against all points = those rows of the reference's
+    full empirical-score accumulators (reference.py:276-282)."""
+    x, h = golden(case, "x"), golden(case, "h")
+    idx = np.random.default_rng(3).choice(x.shape[0], size=min(37, x.shape[0]), replace=False)
+    pdf, w = R.score_rows(x[idx], x, h)
+    assert rel(pdf, golden(case, "pdf")[idx]) < 2e-6
+    np.testing.assert_allclose(w, golden(case, "weighted")[idx], rtol=2e-5, atol=2e-5)
+    pdf64, w64 = fp64.score_rows(x[idx], x, h)
+    assert rel(pdf64, golden(case, "pdf")[idx]) < 1e-4
+    np.testing.assert_allclose(w64, golden(case, "weighted")[idx], rtol=1e-4, atol=2e-5)
+
+
 def test_transform_matches_reference(golden):
     x, h = golden("transform", "x"), golden("transform", "h")
     np.testing.assert_allclose(R.empirical_sd_kde_transform_nd_numpy(x, h),
