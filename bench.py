@@ -101,7 +101,7 @@ def cpu_baseline(x, q, h, budget_s):
 # dispatch: FETCH_SIZE reads 262169 KiB -> scale 2.0, WRITE_SIZE reads
 # 524288 KiB -> scale 1.0). Keyed by (score kernel, n); other sizes report null.
 PMC_TRAFFIC = {
-    ("ordered", 1 << 20): {"fetch_kib": 4507360.0, "write_kib": 139264.0,
+    ("ordered", 1 << 20): {"fetch_kib": 2348100.0, "write_kib": 278528.0,
                            "fetch_scale": 524288.0 / 262169.0, "write_scale": 1.0},
 }
 

@@ -32,8 +32,10 @@ constexpr double kLn2 = 0.6931471805599453;
 // c|x|^2 of padded tail points: exp2 of anything this negative is exactly 0,
 // and 0 * (finite) stays finite (no inf - inf or 0 * inf).
 constexpr float kPadNeg = -1.0e30f;
-// Norm arrays are padded to this many rows so any tile / block read is in bounds.
-constexpr int64_t kNormPad = 256;
+// Norm arrays (and 16-bit records) are padded to a multiple of this many rows,
+// a multiple of the rows any workgroup owns, so every tile / block read is in
+// bounds.
+constexpr int64_t kNormPad = 512;
 
 // ---- error plumbing (host) ----
 void set_error(const char *fmt, ...);

@@ -12,12 +12,16 @@ namespace fsk {
 // 16-D KDE split-K: a workgroup owns 4 waves x QT16 row tiles of 16 queries.
 constexpr int kKdeQT = 4;
 constexpr int kKdeRowsPerWG = 4 * 16 * kKdeQT;  // 256 queries
-// 16-D score (ordered): rows per workgroup.
+// 16-D score (ordered, FP32): rows per workgroup. Eight row tiles per wave
+// reuse each loaded data tile for 128 rows (229 VGPRs, 2 waves per SIMD).
 #ifndef FSK_SCORE_QT
-#define FSK_SCORE_QT 4
+#define FSK_SCORE_QT 8
 #endif
 constexpr int kScoreQT = FSK_SCORE_QT;
-constexpr int kScoreRowsPerWG = 4 * 16 * kScoreQT;  // 256 rows
+constexpr int kScoreRowsPerWG = 4 * 16 * kScoreQT;  // 512 rows
+// 16-bit score forms: row tiles per wave and rows per workgroup.
+constexpr int kScoreLpQT = 4;
+constexpr int kScoreLpRowsPerWG = 4 * 16 * kScoreLpQT;  // 256 rows
 // Symmetric score: I block = rows of one workgroup (4 waves x kSymQT tiles).
 constexpr int kSymQT = 4;
 // Target number of (query block, split) tasks per launch: a few per CU slot.
@@ -61,6 +65,12 @@ inline SplitPlan score16_ordered_plan(int64_t n) {
   int64_t cap = kOrderedPartialCap / (n * 17 * 4 > 0 ? n * 17 * 4 : 1);
   if (cap < 1) cap = 1;
   return make_split_plan(n, n, kScoreRowsPerWG, 16, 2048, cap);
+}
+
+inline SplitPlan score16_lp_plan(int64_t n) {
+  int64_t cap = kOrderedPartialCap / (n * 17 * 4 > 0 ? n * 17 * 4 : 1);
+  if (cap < 1) cap = 1;
+  return make_split_plan(n, n, kScoreLpRowsPerWG, 16, 2048, cap);
 }
 
 inline SplitPlan score16_rows_plan(int64_t n, int64_t m) {

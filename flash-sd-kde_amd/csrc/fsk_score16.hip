@@ -23,6 +23,7 @@
 namespace fsk {
 
 constexpr int kScoreFlush = 32;
+static_assert(kNormPad % kScoreRowsPerWG == 0, "row norms must cover whole row blocks");
 
 #ifndef FSK_SYM_JC
 #define FSK_SYM_JC 8

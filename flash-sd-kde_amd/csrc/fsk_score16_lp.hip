@@ -21,6 +21,7 @@
 namespace fsk {
 
 constexpr int kLpFlush = 32;
+static_assert(kNormPad % kScoreLpRowsPerWG == 0, "records must cover whole row blocks");
 
 template <int P16>
 __global__ void prep_xt16_kernel(const float *__restrict__ x, int64_t ld, int64_t n,
@@ -69,7 +70,7 @@ __global__ __launch_bounds__(kWG) void score16_lp_kernel(
   double pdf64[QT];
 #pragma unroll
   for (int R = 0; R < QT; ++R) {
-    const unsigned short *rq = qrec + (row0 + 16 * R + c) * kRec;  // records padded to 256 rows
+    const unsigned short *rq = qrec + (row0 + 16 * R + c) * kRec;  // records padded to whole row blocks
     if (X3) {
       B1[R] = *reinterpret_cast<const u16x8 *>(rq + 8 * (g & 1));
       B2[R] = *reinterpret_cast<const u16x8 *>(rq + 16 + 8 * g);
@@ -188,7 +189,7 @@ struct ScoreLpLayout {
 
 static size_t score_lp_layout(void *ws, int64_t n, ScoreLpLayout *L) {
   Carve cv(ws);
-  SplitPlan p = score16_ordered_plan(n);
+  SplitPlan p = score16_lp_plan(n);
   const int64_t rows = round_up(n, kNormPad);
   L->mu = cv.take<double>(kD);
   L->xrec = cv.take<unsigned short>(rows * kRec);
@@ -225,7 +226,7 @@ int score16_lp_run(const float *data, int64_t ld, int64_t n, double h, unsigned 
     if (blocks > 1024) blocks = 1024;
     hipLaunchKernelGGL(mean16_kernel, dim3(blocks), dim3(256), 0, s, data, ld, n, L.mu);
   }
-  SplitPlan p = score16_ordered_plan(n);
+  SplitPlan p = score16_lp_plan(n);
   const int64_t ntasks = p.qblocks * p.splits;
   int64_t per_xcd;
   const int64_t grid = xcd_grid(ntasks, &per_xcd);
@@ -238,7 +239,7 @@ int score16_lp_run(const float *data, int64_t ld, int64_t n, double h, unsigned 
     hipLaunchKernelGGL(prep_xt16_kernel<P16>, dim3(cdiv(n16 * kD, 256)), dim3(256), 0, s, data, \
                        ld, n, n16, L.mu, inv_n, L.xt);                                          \
     prof_begin(FSK_OP_SCORE16_ORDERED, s);                                                      \
-    hipLaunchKernelGGL((score16_lp_kernel<P, kScoreQT>), dim3(grid), dim3(kWG), 0, s, L.xrec,   \
+    hipLaunchKernelGGL((score16_lp_kernel<P, kScoreLpQT>), dim3(grid), dim3(kWG), 0, s, L.xrec,   \
                        L.qrec, L.xt, n, n16, p.split_len, p.qblocks, ntasks, per_xcd,           \
                        L.pdf_part, L.w_part);                                                   \
     prof_end(FSK_OP_SCORE16_ORDERED, s);                                                        \

@@ -84,7 +84,9 @@ __device__ __forceinline__ void phi_x_tile(const f4 (&d)[QT], const TileData &t,
 
 // Walk the data range [j_begin, j_end) of one split tile by tile: full tiles
 // (all 16 points < n) without index clamps, then at most one clamped tail
-// tile. body(TileData&) does the compute for one tile.
+// tile. body(TileData&) does the compute for one tile. (Issuing the next
+// tile's loads before the current tile's compute measured no faster: with
+// 2-3 waves per SIMD the MFMA pipe already covers the load latency.)
 template <bool WITH_XT, class Body>
 __device__ __forceinline__ void for_each_tile(const float *__restrict__ data, int64_t ld, int64_t n,
                                               const float *__restrict__ bn, int64_t j_begin,
