@@ -240,12 +240,14 @@ def surrogate_form(fp32_pair_rate):
     }
 
 
-def precision_forms(xd, qd, h, fp32_ms, fp32_dens, steps, warmup, n_err):
+def precision_forms(xd, qd, h, fp32_ms, fp32_dens, steps, warmup, n_err, rates=None):
     """Reduced-precision SD-KDE forms on the same workload: score with 16-bit
     MFMA operands (FP32 accumulate), FP64 debias, log-sum-exp KDE with the same
     cross-term precision. Errors: max / median relative log-density error on
     the first n_err queries against the FP64 GPU reference pipeline (FP64
-    direct-difference score, debias and log-sum-exp)."""
+    direct-difference score, debias and log-sum-exp). The score kernel's
+    exp/VALU fraction uses the algorithmic minimum per ordered pair (1 exp +
+    3 VALU ops) and the issue rates measured in this run."""
     import math
 
     import torch
@@ -267,7 +269,7 @@ def precision_forms(xd, qd, h, fp32_ms, fp32_dens, steps, warmup, n_err):
                     "within_1e-3": mx <= 1e-3}}
     lib = _native.lib()
     pairs = float(m) * n + float(n) * n
-    for prec in ("bf16x3", "fp16", "bf16"):
+    for prec in ("bf16x6", "bf16x3", "fp16", "bf16"):
         def step():
             pdf, w = _ops.score16(xd, h, variant="ordered", precision=prec)
             deb = _ops.debias(xd, pdf, w, h, 1e-12)
@@ -289,6 +291,10 @@ def precision_forms(xd, qd, h, fp32_ms, fp32_dens, steps, warmup, n_err):
                      "speedup_vs_fp32": round(fp32_ms / ms, 3), "score_kernel_ms": round(score_ms, 3),
                      "kde_kernel_ms": round(kde_ms, 3), "max_rel_logp_err": mx,
                      "median_rel_logp_err": med, "within_1e-3": mx <= 1e-3}
+        if rates and rates.get("exp_per_s") and rates.get("fma_per_s") and score_ms > 0:
+            units = float(n) * n
+            t = units / rates["exp_per_s"] + 3.0 * units / rates["fma_per_s"]
+            out[prec]["score_exp_valu_frac_measured_rates"] = round(t / (score_ms * 1e-3), 4)
     return out
 
 
@@ -427,7 +433,7 @@ def main():
         line["batched_loo"] = batched_loo(rates, args.steps)
     if not args.no_precision_forms:
         line["precision_forms"] = precision_forms(xd, qd, h, ms_per_step, dens, args.steps,
-                                                  args.warmup, args.error_queries)
+                                                  args.warmup, args.error_queries, rates)
     if not args.no_surrogate:
         del xd, qd
         torch.cuda.empty_cache()

@@ -65,6 +65,12 @@ __device__ __forceinline__ f4 mfma_f32(float a, float b, f4 c) {
 }
 
 __device__ __forceinline__ float fexp2(float x) { return __builtin_amdgcn_exp2f(x); }
+// exp2(min(x, 0)) in one instruction: exp2 is >= 0 and exceeds 1 exactly when
+// x > 0, so clamping its result to [0, 1] (the clamp bit of v_exp_f32, which
+// the compiler folds fmed3(., 0, 1) into) equals clamping its argument to 0.
+__device__ __forceinline__ float fexp2_le1(float x) {
+  return __builtin_amdgcn_fmed3f(__builtin_amdgcn_exp2f(x), 0.f, 1.f);
+}
 
 __device__ __forceinline__ f4 ld4(const float *p) {
   return *reinterpret_cast<const f4 *>(p);
@@ -91,7 +97,7 @@ void prof_end(int op, hipStream_t s);
 size_t kde16_ws_bytes(int64_t n, int64_t m, bool with_partials);
 size_t kde16_lse_ws_bytes(int64_t n, int64_t m, unsigned flags);
 size_t score16_ws_bytes(int64_t n, bool ordered);
-size_t score16_lp_ws_bytes(int64_t n);
+size_t score16_lp_ws_bytes(int64_t n, unsigned prec);
 size_t score16_rows_ws_bytes(int64_t n, int64_t m);
 int score16_lp_run(const float *data, int64_t ld, int64_t n, double h, unsigned prec, void *ws,
                    size_t ws_bytes, float *pdf, float *w, int64_t ldw_row, int64_t ldw_col,

@@ -180,7 +180,7 @@ extern "C" size_t fsk_workspace_bytes(int op, int64_t n, int64_t m, unsigned fla
     case FSK_OP_SCORE16_ROWS: return m > 0 ? score16_rows_ws_bytes(n, m) : 0;
     case FSK_OP_SCORE16_ORDERED:
       return (flags & FSK_PREC_MASK) == FSK_PREC_F32 ? score16_ws_bytes(n, true)
-                                                     : score16_lp_ws_bytes(n);
+                                                     : score16_lp_ws_bytes(n, flags & FSK_PREC_MASK);
     default: return 0;
   }
 }

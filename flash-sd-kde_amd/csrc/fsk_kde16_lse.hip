@@ -11,17 +11,20 @@
 // raised afterwards (rare, wave-uniform branch). Partials (m, s) are combined
 // in FP64: log p_i = ln2 (M + log2 sum_s s_s 2^(m_s - M)) + log_const.
 //
-// 16-bit forms use v_mfma_f32_16x16x32_{bf16,f16}. K = 32 holds the 16
-// features plus a "fold" block that adds the norm terms and -m_i inside the
-// MFMA, so no per-pair VALU add is needed:
-//   data  record (48 x 16-bit): [ xh[16] | xl[16] | 1 1 1 bh bm bl 1 1 1 0... ]
-//   query record (48 x 16-bit): [ qh[16] | ql[16] | ah am al 1 1 1 mh mm ml 0... ]
-// (x = xh + xl, q' = -2c q = qh + ql, a = c|q|^2, b = c|x|^2 each split into
-// three 16-bit pieces; -m_i is split into three pieces and patched in place).
-//   BF16 / F16 : one MFMA,  A = [xh | fold_b],  B = [qh | fold_a]
-//   BF16X3     : two MFMAs, A1 = [xh | xl] x B1 = [qh | qh]  and
-//                           A2 = [xh | fold_b] x B2 = [ql | fold_a]
-//                i.e. xh.qh + xl.qh + xh.ql + norms - m  (the lo.lo term dropped)
+// 16-bit forms use v_mfma_f32_16x16x32_{bf16,f16} on 64-value records
+// (fsk_lowp.h): x = x1 + x2 + x3, q' = -2c q = q1 + q2 + q3 in 16-bit pieces,
+// plus a "fold" block that adds the norm terms and -m_i inside the MFMA, so no
+// per-pair VALU add is needed ([lo half | hi half] of K = 32 per operand):
+//   BF16 / F16 : one MFMA,    [x1 | fold] . [q1 | fold]
+//   BF16X3     : two MFMAs,   [x1 | x2] . [q1 | q1]  +  [x1 | fold] . [q2 | fold]
+//                (x1q1 + x2q1 + x1q2 + norms - m; the 2^-16 terms dropped)
+//   BF16X6     : four MFMAs,  [x1 | x1] . [q1 | q2]  +  [x2 | x1] . [q1 | q3]
+//                           + [x2 | x3] . [q2 | q1]  +  [x2 | fold] . [q3 | fold]
+//                (all products down to 2^-16 relative plus x2q3; the dropped
+//                x3q2 and x3q3 are at or below FP32 rounding: an FP32-class
+//                cross term at BF16 MFMA rate)
+// -m_i is split into three pieces and patched into the B operand whose high
+// half is the fold block.
 // FP32 forms: MFMA f32 16x16x4 with the norm terms (and -m) in the
 // accumulator initialiser, or plain FP32 FMAs on the VALU.
 #include "fsk_common.h"
@@ -75,7 +78,10 @@ __global__ __launch_bounds__(kWG) void kde16_lse_mfma_kernel(
     int64_t split_len, int64_t qblocks, int64_t ntasks, int64_t per_xcd,
     float2 *__restrict__ part) {
   constexpr bool F32 = (PREC == FSK_PREC_F32);
+  constexpr bool X3 = (PREC == FSK_PREC_BF16X3);
+  constexpr bool X6 = (PREC == FSK_PREC_BF16X6);
   constexpr int P16 = (PREC == FSK_PREC_F16) ? FSK_PREC_F16 : FSK_PREC_BF16;
+  using RL = RecLayout<PREC>;
   const int64_t task = xcd_task(blockIdx.x, per_xcd);
   if (task >= ntasks) return;
   const int64_t split = task / qblocks;
@@ -86,7 +92,7 @@ __global__ __launch_bounds__(kWG) void kde16_lse_mfma_kernel(
   const int64_t row0 = qb * (64 * QT) + int64_t(wave) * (16 * QT);
 
   f4 qv[QT];
-  u16x8 B1[QT], B2[QT];
+  u16x8 B1[QT], B2[QT], B3[QT], B4[QT];
   float qav[QT], negm[QT], s[QT];
   double s64[QT];
   double mrec[QT];  // exactly the m subtracted inside the MFMA
@@ -98,11 +104,18 @@ __global__ __launch_bounds__(kWG) void kde16_lse_mfma_kernel(
       qv[R] = ld4(qf + i * kD + 4 * g);
       qav[R] = qa[i];
     } else {
-      const unsigned short *rq = qrec + i * kRec;
-      B1[R] = *reinterpret_cast<const u16x8 *>(rq + 8 * (g & 1));
-      B2[R] = *reinterpret_cast<const u16x8 *>(rq + 16 + 8 * g);
-      if (PREC != FSK_PREC_BF16X3)
-        B1[R] = *reinterpret_cast<const u16x8 *>(rq + (g < 2 ? 8 * g : 16 + 8 * g));
+      const unsigned short *rq = qrec + i * RL::kSize;
+      if (X6) {
+        B1[R] = ld_u16x8(rq + rec_off(g, RL::kP1, RL::kP2));
+        B2[R] = ld_u16x8(rq + rec_off(g, RL::kP1, RL::kP3));
+        B3[R] = ld_u16x8(rq + rec_off(g, RL::kP2, RL::kP1));
+        B4[R] = ld_u16x8(rq + rec_off(g, RL::kP3, RL::kFold));
+      } else if (X3) {
+        B1[R] = ld_u16x8(rq + rec_off(g, RL::kP1, RL::kP1));
+        B2[R] = ld_u16x8(rq + rec_off(g, RL::kP2, RL::kFold));
+      } else {
+        B1[R] = ld_u16x8(rq + rec_off(g, RL::kP1, RL::kFold));
+      }
     }
     negm[R] = 0.f;
     mrec[R] = 0.0;
@@ -127,16 +140,29 @@ __global__ __launch_bounds__(kWG) void kde16_lse_mfma_kernel(
         for (int R = 0; R < QT; ++R) d[R] = mfma_f32(xa[k], qv[R][k], d[R]);
       }
     } else {
-      const unsigned short *rx = xrec + (j0 + c) * kRec;
-      const u16x8 A2 = *reinterpret_cast<const u16x8 *>(rx + (g < 2 ? 8 * g : 16 + 8 * g));
-      if (PREC == FSK_PREC_BF16X3) {
-        const u16x8 A1 = *reinterpret_cast<const u16x8 *>(rx + 8 * g);
+      const unsigned short *rx = xrec + (j0 + c) * RL::kSize;
+      if (X6) {
+        const u16x8 A1 = ld_u16x8(rx + rec_off(g, RL::kP1, RL::kP1));
+        const u16x8 A2 = ld_u16x8(rx + rec_off(g, RL::kP2, RL::kP1));
+        const u16x8 A3 = ld_u16x8(rx + rec_off(g, RL::kP2, RL::kP3));
+        const u16x8 A4 = ld_u16x8(rx + rec_off(g, RL::kP2, RL::kFold));
+#pragma unroll
+        for (int R = 0; R < QT; ++R) {
+          d[R] = mfma16(A1, B1[R], f4{0.f, 0.f, 0.f, 0.f}, P16);
+          d[R] = mfma16(A2, B2[R], d[R], P16);
+          d[R] = mfma16(A3, B3[R], d[R], P16);
+          d[R] = mfma16(A4, B4[R], d[R], P16);
+        }
+      } else if (X3) {
+        const u16x8 A1 = ld_u16x8(rx + rec_off(g, RL::kP1, RL::kP2));
+        const u16x8 A2 = ld_u16x8(rx + rec_off(g, RL::kP1, RL::kFold));
 #pragma unroll
         for (int R = 0; R < QT; ++R) {
           d[R] = mfma16(A1, B1[R], f4{0.f, 0.f, 0.f, 0.f}, P16);
           d[R] = mfma16(A2, B2[R], d[R], P16);
         }
       } else {
+        const u16x8 A2 = ld_u16x8(rx + rec_off(g, RL::kP1, RL::kFold));
 #pragma unroll
         for (int R = 0; R < QT; ++R) d[R] = mfma16(A2, B1[R], f4{0.f, 0.f, 0.f, 0.f}, P16);
       }
@@ -156,20 +182,14 @@ __global__ __launch_bounds__(kWG) void kde16_lse_mfma_kernel(
       split3<P16>(-double(mnew), pc);
       double mr = -(double(from16<P16>(pc[0])) + double(from16<P16>(pc[1])) +
                     double(from16<P16>(pc[2])));
-      // record slots 38, 39 live in lane group 2 (elements 6, 7), slot 40 in group 3 (element 0)
-      if (PREC == FSK_PREC_BF16X3) {
-        if (g == 2) {
-          B2[R][6] = pc[0];
-          B2[R][7] = pc[1];
-        }
-        if (g == 3) B2[R][0] = pc[2];
-      } else {
-        if (g == 2) {
-          B1[R][6] = pc[0];
-          B1[R][7] = pc[1];
-        }
-        if (g == 3) B1[R][0] = pc[2];
+      // fold slots 6, 7 live in lane group 2 (elements 6, 7), slot 8 in group 3
+      // (element 0) of the operand whose high K half is the fold block
+      u16x8 &Bf = X6 ? B4[R] : (X3 ? B2[R] : B1[R]);
+      if (g == 2) {
+        Bf[6] = pc[0];
+        Bf[7] = pc[1];
       }
+      if (g == 3) Bf[0] = pc[2];
       mrec[R] = mr;
       negm[R] = float(-mr);
     }
@@ -321,7 +341,8 @@ struct LseLayout {
 };
 
 static bool lse_is16(unsigned prec) {
-  return prec == FSK_PREC_BF16 || prec == FSK_PREC_F16 || prec == FSK_PREC_BF16X3;
+  return prec == FSK_PREC_BF16 || prec == FSK_PREC_F16 || prec == FSK_PREC_BF16X3 ||
+         prec == FSK_PREC_BF16X6;
 }
 
 static size_t lse_layout(void *ws, int64_t n, int64_t m, unsigned flags, LseLayout *L) {
@@ -333,8 +354,8 @@ static size_t lse_layout(void *ws, int64_t n, int64_t m, unsigned flags, LseLayo
   L->xf = L->bn = L->qf = L->qa = nullptr;
   L->xrec = L->qrec = nullptr;
   if (lse_is16(prec)) {
-    L->xrec = cv.take<unsigned short>(n_pad * kRec);
-    L->qrec = cv.take<unsigned short>(m * kRec);
+    L->xrec = cv.take<unsigned short>(n_pad * rec_size(prec));
+    L->qrec = cv.take<unsigned short>(m * rec_size(prec));
   } else {
     L->xf = cv.take<float>(n_pad * kD);
     L->bn = cv.take<float>(n_pad);
@@ -371,7 +392,7 @@ extern "C" int fsk_kde16_logdensity(const float *data, int64_t ld, int64_t n,
     return FSK_EINVAL;
   }
   const unsigned prec = flags & FSK_PREC_MASK;
-  if (prec > FSK_PREC_BF16X3 || (flags & ~FSK_PREC_MASK)) {
+  if (prec > FSK_PREC_BF16X6 || (flags & ~FSK_PREC_MASK)) {
     set_error("%s: flags 0x%x not available (Laplacian has no log form)", what, flags);
     return FSK_EUNSUPPORTED;
   }
@@ -403,6 +424,7 @@ extern "C" int fsk_kde16_logdensity(const float *data, int64_t ld, int64_t n,
     case FSK_PREC_F32_VALU: FSK_PREP(FSK_PREC_F32_VALU); break;
     case FSK_PREC_BF16: FSK_PREP(FSK_PREC_BF16); break;
     case FSK_PREC_F16: FSK_PREP(FSK_PREC_F16); break;
+    case FSK_PREC_BF16X6: FSK_PREP(FSK_PREC_BF16X6); break;
     default: FSK_PREP(FSK_PREC_BF16X3); break;
   }
 #undef FSK_PREP
@@ -427,6 +449,7 @@ extern "C" int fsk_kde16_logdensity(const float *data, int64_t ld, int64_t n,
       case FSK_PREC_F32: FSK_LSE(FSK_PREC_F32); break;
       case FSK_PREC_BF16: FSK_LSE(FSK_PREC_BF16); break;
       case FSK_PREC_F16: FSK_LSE(FSK_PREC_F16); break;
+      case FSK_PREC_BF16X6: FSK_LSE(FSK_PREC_BF16X6); break;
       default: FSK_LSE(FSK_PREC_BF16X3); break;
     }
 #undef FSK_LSE

@@ -1,13 +1,20 @@
 // fsk_lowp.h - 16-bit (BF16 / FP16) operand helpers shared by the
 // reduced-precision pair kernels.
 //
-// Packed 48-value records per point (K = 32 of v_mfma_f32_16x16x32_*):
-//   data  (A side): [ xh[16] | xl[16] | 1 1 1 bh bm bl 1 1 1 0 ... ]
-//   query (B side): [ qh[16] | ql[16] | ah am al 1 1 1 mh mm ml 0 ... ]
-// with x = xh + xl the centred point, q' = -2c (q - mu) = qh + ql, b = c|x - mu|^2,
-// a = c|q - mu|^2 and -m (log-sum-exp shift, 0 outside the LSE kernel) each
-// split into three 16-bit pieces, so one MFMA over the fold block adds the
-// norm terms (and -m) with no per-pair VALU work.
+// Packed records per point, 16-value blocks. BF16X6 uses four blocks
+//   data  (A side): [ x1[16] | x2[16] | x3[16] | 1 1 1 bh bm bl 1 1 1 0 ... ]
+//   query (B side): [ q1[16] | q2[16] | q3[16] | ah am al 1 1 1 mh mm ml 0 ... ]
+// and the other 16-bit forms three ([ x1 | x2 | fold ], 96 bytes: the smaller
+// stride measured ~10 % faster for them than the 128-byte one), with
+// x = x1 + x2 + x3 the centred point split into 16-bit pieces
+// (x1 = rn(x), x2 = rn(x - x1), x3 = rn(x - x1 - x2); exact for BF16 and an
+// FP32 x), q' = -2c (q - mu) = q1 + q2 + q3, b = c|x - mu|^2, a = c|q - mu|^2
+// and -m (log-sum-exp shift, 0 outside the LSE kernel) each split into three
+// 16-bit pieces, so one MFMA over the fold block adds the norm terms (and -m)
+// with no per-pair VALU work. A lane of v_mfma_f32_16x16x32_* reads 8
+// consecutive K values, so every operand below is one 16-byte load at a
+// per-lane offset: the low K half (lane groups 0, 1) and the high K half
+// (groups 2, 3) may come from different blocks.
 #pragma once
 
 #include "fsk_common.h"
@@ -19,7 +26,26 @@ typedef _Float16 h16x8 __attribute__((ext_vector_type(8)));
 typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
 typedef unsigned short u16x4 __attribute__((ext_vector_type(4)));
 
-constexpr int kRec = 48;            // 16-bit values per packed record
+template <int PREC>
+struct RecLayout {
+  static constexpr bool kX6 = PREC == FSK_PREC_BF16X6;
+  static constexpr int kSize = kX6 ? 64 : 48;  // 16-bit values per record
+  static constexpr int kP1 = 0;                // first piece
+  static constexpr int kP2 = 16;               // second piece
+  static constexpr int kP3 = 32;               // third piece (BF16X6 only)
+  static constexpr int kFold = kX6 ? 48 : 32;  // norm terms and -m
+};
+inline int64_t rec_size(unsigned prec) { return prec == FSK_PREC_BF16X6 ? 64 : 48; }
+
+__device__ __forceinline__ u16x8 ld_u16x8(const unsigned short *p) {
+  return *reinterpret_cast<const u16x8 *>(p);
+}
+
+// Offset of lane group g's 8 values for an operand whose low K half is block
+// `lo` and high K half is block `hi` (each 16 values).
+__device__ __forceinline__ int rec_off(int g, int lo, int hi) {
+  return g < 2 ? lo + 8 * g : hi + 8 * (g - 2);
+}
 
 // ---- 16-bit helpers (round to nearest even) ----
 template <int PREC>
@@ -85,16 +111,20 @@ __global__ void prep_lse_kernel(const float *__restrict__ x, int64_t ld, int64_t
     return;
   }
   constexpr int P16 = (PREC == FSK_PREC_F16) ? FSK_PREC_F16 : FSK_PREC_BF16;
-  unsigned short r[kRec];
+  using RL = RecLayout<PREC>;
+  unsigned short r[RL::kSize];
 #pragma unroll
   for (int f = 0; f < kD; ++f) {
-    float v = float(sc * xc[f]);
-    r[f] = to16<P16>(v);
-    r[kD + f] = to16<P16>(v - from16<P16>(r[f]));
+    const float v = float(sc * xc[f]);
+    r[RL::kP1 + f] = to16<P16>(v);
+    const float r1 = v - from16<P16>(r[RL::kP1 + f]);
+    r[RL::kP2 + f] = to16<P16>(r1);
+    if (RL::kX6) r[RL::kP3 + f] = to16<P16>(r1 - from16<P16>(r[RL::kP2 + f]));
   }
   const unsigned short one = to16<P16>(1.0f), zero = to16<P16>(0.0f);
+  unsigned short *fold = r + RL::kFold;
 #pragma unroll
-  for (int k = 0; k < kD; ++k) r[2 * kD + k] = zero;
+  for (int k = 0; k < kD; ++k) fold[k] = zero;
   unsigned short piece[3];
   if (side == 0) {
     if (pad) {
@@ -103,21 +133,21 @@ __global__ void prep_lse_kernel(const float *__restrict__ x, int64_t ld, int64_t
     } else {
       split3<P16>(nt, piece);
     }
-    r[32] = r[33] = r[34] = one;
-    r[35] = piece[0];
-    r[36] = piece[1];
-    r[37] = piece[2];
-    r[38] = r[39] = r[40] = one;
+    fold[0] = fold[1] = fold[2] = one;
+    fold[3] = piece[0];
+    fold[4] = piece[1];
+    fold[5] = piece[2];
+    fold[6] = fold[7] = fold[8] = one;
   } else {
     split3<P16>(nt, piece);
-    r[32] = piece[0];
-    r[33] = piece[1];
-    r[34] = piece[2];
-    r[35] = r[36] = r[37] = one;
-    // r[38..40] = pieces of -m, patched inside the kernel
+    fold[0] = piece[0];
+    fold[1] = piece[1];
+    fold[2] = piece[2];
+    fold[3] = fold[4] = fold[5] = one;
+    // fold[6 .. 8] = pieces of -m, patched inside the kernel
   }
 #pragma unroll
-  for (int k = 0; k < kRec; ++k) rec_out[j * kRec + k] = r[k];
+  for (int k = 0; k < RL::kSize; ++k) rec_out[j * RL::kSize + k] = r[k];
 }
 
 __device__ __forceinline__ f4 mfma16(u16x8 a, u16x8 b, f4 c, int prec) {

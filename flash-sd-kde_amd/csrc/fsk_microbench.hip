@@ -94,6 +94,28 @@ __global__ __launch_bounds__(256) void mb_mix(float *out, int iters) {
   out[blockIdx.x * blockDim.x + threadIdx.x] = s;
 }
 
+// As mb_mix with 16 FMAs instead of 16 exps per lane per iteration: whether
+// plain VALU work overlaps the MFMA pipe where transcendentals may not.
+__global__ __launch_bounds__(256) void mb_mix_fma(float *out, int iters) {
+  const float a = 1.0f + threadIdx.x * 1e-7f, b = 0.999f;
+  f4 c0 = {0, 0, 0, 0}, c1 = c0, c2 = c0, c3 = c0;
+  float v[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) v[k] = (threadIdx.x + k) * 1e-3f;
+  for (int i = 0; i < iters; ++i) {
+    c0 = mfma_f32(a, b, c0);
+    c1 = mfma_f32(a, b, c1);
+    c2 = mfma_f32(a, b, c2);
+    c3 = mfma_f32(a, b, c3);
+#pragma unroll
+    for (int k = 0; k < 16; ++k) v[k] = fmaf(v[k], 0.999f, 1e-3f);
+  }
+  float s = c0[0] + c1[1] + c2[2] + c3[3];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) s += v[k];
+  out[blockIdx.x * blockDim.x + threadIdx.x] = s;
+}
+
 // dst[i] = src[i] over n4 float4s; grid-stride, 16-byte loads and stores.
 __global__ __launch_bounds__(256) void mb_copy(const f4 *__restrict__ src, f4 *__restrict__ dst,
                                                int64_t n4) {
@@ -109,10 +131,10 @@ using namespace fsk;
 // which: 0 = FP32 MFMA (flop/s), 1 = BF16 MFMA (flop/s), 2 = v_exp_f32 (exp/s),
 // 3 = v_fma_f32 (fma/s), 4 = mix (returns seconds per iteration-block, see header),
 // 5 = copy of the first half of `out` to its second half, `iters` launches
-// (bytes read + written per second).
+// (bytes read + written per second), 6 = mix with FMAs (seconds, as 4).
 extern "C" int fsk_microbench(int which, int blocks, int iters, float *out, double *result,
                               void *stream) {
-  if (which < 0 || which > 5 || blocks <= 0 || iters <= 0 || !out || !result ||
+  if (which < 0 || which > 6 || blocks <= 0 || iters <= 0 || !out || !result ||
       (which == 5 && (reinterpret_cast<uintptr_t>(out) & 15))) {
     set_error("fsk_microbench: bad arguments");
     return FSK_EINVAL;
@@ -130,6 +152,7 @@ extern "C" int fsk_microbench(int which, int blocks, int iters, float *out, doub
     case 2: hipLaunchKernelGGL(mb_exp, dim3(blocks), dim3(256), 0, s, out, iters); break;
     case 3: hipLaunchKernelGGL(mb_fma, dim3(blocks), dim3(256), 0, s, out, iters); break;
     case 4: hipLaunchKernelGGL(mb_mix, dim3(blocks), dim3(256), 0, s, out, iters); break;
+    case 6: hipLaunchKernelGGL(mb_mix_fma, dim3(blocks), dim3(256), 0, s, out, iters); break;
     default: {
       // blocks * 256 floats = blocks * 64 float4; each half is blocks * 32 float4
       const int64_t n4 = int64_t(blocks) * 32;

@@ -7,6 +7,8 @@
 #include <stdint.h>
 #include <stddef.h>
 
+#include "flashsdkde.h"
+
 namespace fsk {
 
 // 16-D KDE split-K: a workgroup owns 4 waves x QT16 row tiles of 16 queries.
@@ -19,9 +21,14 @@ constexpr int kKdeRowsPerWG = 4 * 16 * kKdeQT;  // 256 queries
 #endif
 constexpr int kScoreQT = FSK_SCORE_QT;
 constexpr int kScoreRowsPerWG = 4 * 16 * kScoreQT;  // 512 rows
-// 16-bit score forms: row tiles per wave and rows per workgroup.
+// 16-bit score forms: row tiles per wave and rows per workgroup (BF16X6
+// holds four query operands per row tile, so it may use fewer tiles).
 constexpr int kScoreLpQT = 4;
-constexpr int kScoreLpRowsPerWG = 4 * 16 * kScoreLpQT;  // 256 rows
+#ifndef FSK_LP_QT_X6
+#define FSK_LP_QT_X6 4
+#endif
+constexpr int kScoreLpQTx6 = FSK_LP_QT_X6;
+inline int score_lp_qt(unsigned prec) { return prec == FSK_PREC_BF16X6 ? kScoreLpQTx6 : kScoreLpQT; }
 // Symmetric score: I block = rows of one workgroup (4 waves x kSymQT tiles).
 constexpr int kSymQT = 4;
 // Target number of (query block, split) tasks per launch: a few per CU slot.
@@ -67,10 +74,10 @@ inline SplitPlan score16_ordered_plan(int64_t n) {
   return make_split_plan(n, n, kScoreRowsPerWG, 16, 2048, cap);
 }
 
-inline SplitPlan score16_lp_plan(int64_t n) {
+inline SplitPlan score16_lp_plan(int64_t n, unsigned prec) {
   int64_t cap = kOrderedPartialCap / (n * 17 * 4 > 0 ? n * 17 * 4 : 1);
   if (cap < 1) cap = 1;
-  return make_split_plan(n, n, kScoreLpRowsPerWG, 16, 2048, cap);
+  return make_split_plan(n, n, 4 * 16 * score_lp_qt(prec), 16, 2048, cap);
 }
 
 inline SplitPlan score16_rows_plan(int64_t n, int64_t m) {

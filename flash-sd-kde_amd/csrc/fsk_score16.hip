@@ -195,7 +195,7 @@ __global__ __launch_bounds__(64 * WAVES) void score16_sym_kernel(
       for (int R = 0; R < QT; ++R) {
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          float e = fexp2(fminf(d[R][r], 0.f));
+          float e = fexp2_le1(d[R][r]);
           d[R][r] = e;
           pI[R] += e;
         }
@@ -325,7 +325,8 @@ static int score_check(const char *what, const float *data, int64_t ld, int64_t 
     return FSK_EINVAL;
   }
   const unsigned prec = flags & FSK_PREC_MASK;
-  const bool lowp = prec == FSK_PREC_BF16 || prec == FSK_PREC_F16 || prec == FSK_PREC_BF16X3;
+  const bool lowp = prec == FSK_PREC_BF16 || prec == FSK_PREC_F16 || prec == FSK_PREC_BF16X3 ||
+                    prec == FSK_PREC_BF16X6;
   if ((prec != FSK_PREC_F32 && !(lowp && lowp_ok)) || (flags & ~FSK_PREC_MASK)) {
     set_error("%s: flags 0x%x not available on this path", what, flags);
     return FSK_EUNSUPPORTED;

@@ -1,14 +1,18 @@
 // fsk_score16_lp.hip - empirical-score accumulators with 16-bit MFMA operands
 // (BF16 x3 split, single BF16, single FP16), ordered split-K, deterministic.
 //
-// Cross term: data / query 48-value records (fsk_lowp.h); BF16x3 uses
-//   A1 = [xh | xl] . B1 = [qh | qh]  and  A2 = [xh | fold_b] . B2 = [ql | fold_a]
-// (xh.qh + xl.qh + xh.ql + c|x|^2 + c|q|^2); single forms use A2 . [qh | fold_a].
+// Cross term: data / query 64-value records (fsk_lowp.h), the same MFMA
+// forms as the log-density kernel:
+//   BF16 / F16 : [x1 | fold] . [q1 | fold]
+//   BF16X3     : [x1 | x2] . [q1 | q1]  +  [x1 | fold] . [q2 | fold]
+//   BF16X6     : [x1 | x1] . [q1 | q2]  +  [x2 | x1] . [q1 | q3]
+//              + [x2 | x3] . [q2 | q1]  +  [x2 | fold] . [q3 | fold]
+//                (FP32-class cross term; phi.X as in BF16X3)
 // phi = exp2(min(t, 0)) comes out with lane l holding phi(i = l%16, j = 4(l/16)+u),
 // u = 0..3, which is exactly four consecutive K slots of the 16x16x32 A operand,
 // so phi.X needs no data movement:
 //   A = [phi_h(u=0..3) | phi_l(u=0..3)] per lane (phi = phi_h + phi_l),
-//   W += A . [Xh | Xh]  +  A . [Xl | 0]      (BF16x3: drops phi_l.Xl)
+//   W += A . [Xh | Xh]  +  A . [Xl | 0]      (BF16x3, BF16x6: drops phi_l.Xl)
 //   W += [phi | 0] . [Xh | 0]                (single forms)
 // X for phi.X comes from a tile-transposed copy: per 16-point tile, per feature
 // f and lane group g, [Xh(4g+u, f) u=0..3 | Xl(4g+u, f) u=0..3] (16 bytes).
@@ -21,7 +25,8 @@
 namespace fsk {
 
 constexpr int kLpFlush = 32;
-static_assert(kNormPad % kScoreLpRowsPerWG == 0, "records must cover whole row blocks");
+static_assert(kNormPad % (64 * kScoreLpQT) == 0 && kNormPad % (64 * kScoreLpQTx6) == 0,
+              "records must cover whole row blocks");
 
 template <int P16>
 __global__ void prep_xt16_kernel(const float *__restrict__ x, int64_t ld, int64_t n,
@@ -53,7 +58,10 @@ __global__ __launch_bounds__(kWG) void score16_lp_kernel(
     int64_t qblocks, int64_t ntasks, int64_t per_xcd, float *__restrict__ pdf_part,
     float *__restrict__ w_part) {
   constexpr int P16 = (PREC == FSK_PREC_F16) ? FSK_PREC_F16 : FSK_PREC_BF16;
+  constexpr bool X6 = (PREC == FSK_PREC_BF16X6);
   constexpr bool X3 = (PREC == FSK_PREC_BF16X3);
+  constexpr bool SPLIT_PHI = X3 || X6;  // phi.X with phi = phi_h + phi_l
+  using RL = RecLayout<PREC>;
   const int64_t task = xcd_task(blockIdx.x, per_xcd);
   if (task >= ntasks) return;
   const int64_t split = task / qblocks;
@@ -63,19 +71,24 @@ __global__ __launch_bounds__(kWG) void score16_lp_kernel(
   const int g = lane >> 4, c = lane & 15;
   const int64_t row0 = qb * (64 * QT) + int64_t(wave) * (16 * QT);
 
-  u16x8 B1[QT], B2[QT];
+  u16x8 B1[QT], B2[QT], B3[QT], B4[QT];
   f4 w[QT];
   d4 w64[QT];
   float pdf[QT];
   double pdf64[QT];
 #pragma unroll
   for (int R = 0; R < QT; ++R) {
-    const unsigned short *rq = qrec + (row0 + 16 * R + c) * kRec;  // records padded to whole row blocks
-    if (X3) {
-      B1[R] = *reinterpret_cast<const u16x8 *>(rq + 8 * (g & 1));
-      B2[R] = *reinterpret_cast<const u16x8 *>(rq + 16 + 8 * g);
+    const unsigned short *rq = qrec + (row0 + 16 * R + c) * RL::kSize;  // records padded to whole row blocks
+    if (X6) {
+      B1[R] = ld_u16x8(rq + rec_off(g, RL::kP1, RL::kP2));
+      B2[R] = ld_u16x8(rq + rec_off(g, RL::kP1, RL::kP3));
+      B3[R] = ld_u16x8(rq + rec_off(g, RL::kP2, RL::kP1));
+      B4[R] = ld_u16x8(rq + rec_off(g, RL::kP3, RL::kFold));
+    } else if (X3) {
+      B1[R] = ld_u16x8(rq + rec_off(g, RL::kP1, RL::kP1));
+      B2[R] = ld_u16x8(rq + rec_off(g, RL::kP2, RL::kFold));
     } else {
-      B1[R] = *reinterpret_cast<const u16x8 *>(rq + (g < 2 ? 8 * g : 16 + 8 * g));
+      B1[R] = ld_u16x8(rq + rec_off(g, RL::kP1, RL::kFold));
     }
     w[R] = f4{0.f, 0.f, 0.f, 0.f};
     w64[R] = d4{0.0, 0.0, 0.0, 0.0};
@@ -88,23 +101,35 @@ __global__ __launch_bounds__(kWG) void score16_lp_kernel(
   if (j_end > n_pad16) j_end = n_pad16;
   int tc = 0;
   for (int64_t j0 = j_begin; j0 < j_end; j0 += kTile) {
-    const unsigned short *rx = xrec + (j0 + c) * kRec;
-    const u16x8 A2 = *reinterpret_cast<const u16x8 *>(rx + (g < 2 ? 8 * g : 16 + 8 * g));
-    u16x8 A1;
-    if (X3) A1 = *reinterpret_cast<const u16x8 *>(rx + 8 * g);
+    const unsigned short *rx = xrec + (j0 + c) * RL::kSize;
+    u16x8 A1, A2, A3, A4;
+    if (X6) {
+      A1 = ld_u16x8(rx + rec_off(g, RL::kP1, RL::kP1));
+      A2 = ld_u16x8(rx + rec_off(g, RL::kP2, RL::kP1));
+      A3 = ld_u16x8(rx + rec_off(g, RL::kP2, RL::kP3));
+      A4 = ld_u16x8(rx + rec_off(g, RL::kP2, RL::kFold));
+    } else {
+      A2 = ld_u16x8(rx + rec_off(g, RL::kP1, RL::kFold));
+      if (X3) A1 = ld_u16x8(rx + rec_off(g, RL::kP1, RL::kP2));
+    }
     const u16x8 XT = *reinterpret_cast<const u16x8 *>(xt + (((j0 / kTile) * kD + c) * 4 + g) * 8);
     u16x8 BX1, BX2;
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       BX1[u] = XT[u];
-      BX1[4 + u] = X3 ? XT[u] : (unsigned short)0;
+      BX1[4 + u] = SPLIT_PHI ? XT[u] : (unsigned short)0;
       BX2[u] = XT[4 + u];
       BX2[4 + u] = 0;
     }
 #pragma unroll
     for (int R = 0; R < QT; ++R) {
       f4 d;
-      if (X3) {
+      if (X6) {
+        d = mfma16(A1, B1[R], f4{0.f, 0.f, 0.f, 0.f}, P16);
+        d = mfma16(A2, B2[R], d, P16);
+        d = mfma16(A3, B3[R], d, P16);
+        d = mfma16(A4, B4[R], d, P16);
+      } else if (X3) {
         d = mfma16(A1, B1[R], f4{0.f, 0.f, 0.f, 0.f}, P16);
         d = mfma16(A2, B2[R], d, P16);
       } else {
@@ -113,7 +138,7 @@ __global__ __launch_bounds__(kWG) void score16_lp_kernel(
       float e[4];
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
-        e[u] = fexp2(fminf(d[u], 0.f));
+        e[u] = fexp2_le1(d[u]);
         pdf[R] += e[u];
       }
       u16x8 A;
@@ -123,7 +148,7 @@ __global__ __launch_bounds__(kWG) void score16_lp_kernel(
       A[1] = (unsigned short)(h01 >> 16);
       A[2] = (unsigned short)(h23 & 0xffffu);
       A[3] = (unsigned short)(h23 >> 16);
-      if (X3) {
+      if (SPLIT_PHI) {
         const unsigned l01 = pack2<P16>(e[0] - from16<P16>(A[0]), e[1] - from16<P16>(A[1]));
         const unsigned l23 = pack2<P16>(e[2] - from16<P16>(A[2]), e[3] - from16<P16>(A[3]));
         A[4] = (unsigned short)(l01 & 0xffffu);
@@ -187,22 +212,22 @@ struct ScoreLpLayout {
   float *pdf_part, *w_part;
 };
 
-static size_t score_lp_layout(void *ws, int64_t n, ScoreLpLayout *L) {
+static size_t score_lp_layout(void *ws, int64_t n, unsigned prec, ScoreLpLayout *L) {
   Carve cv(ws);
-  SplitPlan p = score16_lp_plan(n);
+  SplitPlan p = score16_lp_plan(n, prec);
   const int64_t rows = round_up(n, kNormPad);
   L->mu = cv.take<double>(kD);
-  L->xrec = cv.take<unsigned short>(rows * kRec);
-  L->qrec = cv.take<unsigned short>(rows * kRec);
+  L->xrec = cv.take<unsigned short>(rows * rec_size(prec));
+  L->qrec = cv.take<unsigned short>(rows * rec_size(prec));
   L->xt = cv.take<unsigned short>(round_up(n, kTile) * kD * 2);
   L->pdf_part = cv.take<float>(p.splits * n);
   L->w_part = cv.take<float>(p.splits * n * kD);
   return cv.used;
 }
 
-size_t score16_lp_ws_bytes(int64_t n) {
+size_t score16_lp_ws_bytes(int64_t n, unsigned prec) {
   ScoreLpLayout L;
-  return score_lp_layout(nullptr, n, &L);
+  return score_lp_layout(nullptr, n, prec, &L);
 }
 
 int score16_lp_run(const float *data, int64_t ld, int64_t n, double h, unsigned prec, void *ws,
@@ -210,7 +235,7 @@ int score16_lp_run(const float *data, int64_t ld, int64_t n, double h, unsigned 
                    void *stream) {
   const char *what = "fsk_score16_ordered";
   ScoreLpLayout L;
-  size_t need = score_lp_layout(ws, n, &L);
+  size_t need = score_lp_layout(ws, n, prec, &L);
   if (!ws || ws_bytes < need) {
     set_error("%s: workspace of %zu bytes needed, got %zu", what, need, ws_bytes);
     return FSK_EWORKSPACE;
@@ -226,11 +251,11 @@ int score16_lp_run(const float *data, int64_t ld, int64_t n, double h, unsigned 
     if (blocks > 1024) blocks = 1024;
     hipLaunchKernelGGL(mean16_kernel, dim3(blocks), dim3(256), 0, s, data, ld, n, L.mu);
   }
-  SplitPlan p = score16_lp_plan(n);
+  SplitPlan p = score16_lp_plan(n, prec);
   const int64_t ntasks = p.qblocks * p.splits;
   int64_t per_xcd;
   const int64_t grid = xcd_grid(ntasks, &per_xcd);
-#define FSK_LP(P, P16)                                                                          \
+#define FSK_LP(P, P16, QT)                                                                      \
   do {                                                                                          \
     hipLaunchKernelGGL(prep_lse_kernel<P>, dim3(cdiv(rows, 256)), dim3(256), 0, s, data, ld, n, \
                        rows, L.mu, inv_n, coef, 0, (float *)nullptr, (float *)nullptr, L.xrec);  \
@@ -239,15 +264,16 @@ int score16_lp_run(const float *data, int64_t ld, int64_t n, double h, unsigned 
     hipLaunchKernelGGL(prep_xt16_kernel<P16>, dim3(cdiv(n16 * kD, 256)), dim3(256), 0, s, data, \
                        ld, n, n16, L.mu, inv_n, L.xt);                                          \
     prof_begin(FSK_OP_SCORE16_ORDERED, s);                                                      \
-    hipLaunchKernelGGL((score16_lp_kernel<P, kScoreLpQT>), dim3(grid), dim3(kWG), 0, s, L.xrec,   \
+    hipLaunchKernelGGL((score16_lp_kernel<P, QT>), dim3(grid), dim3(kWG), 0, s, L.xrec,           \
                        L.qrec, L.xt, n, n16, p.split_len, p.qblocks, ntasks, per_xcd,           \
                        L.pdf_part, L.w_part);                                                   \
     prof_end(FSK_OP_SCORE16_ORDERED, s);                                                        \
   } while (0)
   switch (prec) {
-    case FSK_PREC_BF16: FSK_LP(FSK_PREC_BF16, FSK_PREC_BF16); break;
-    case FSK_PREC_F16: FSK_LP(FSK_PREC_F16, FSK_PREC_F16); break;
-    default: FSK_LP(FSK_PREC_BF16X3, FSK_PREC_BF16); break;
+    case FSK_PREC_BF16: FSK_LP(FSK_PREC_BF16, FSK_PREC_BF16, kScoreLpQT); break;
+    case FSK_PREC_F16: FSK_LP(FSK_PREC_F16, FSK_PREC_F16, kScoreLpQT); break;
+    case FSK_PREC_BF16X6: FSK_LP(FSK_PREC_BF16X6, FSK_PREC_BF16, kScoreLpQTx6); break;
+    default: FSK_LP(FSK_PREC_BF16X3, FSK_PREC_BF16, kScoreLpQT); break;
   }
 #undef FSK_LP
   int st = launch_status(what);

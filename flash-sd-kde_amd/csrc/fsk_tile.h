@@ -64,10 +64,15 @@ __device__ __forceinline__ void kernel_tile(f4 (&d)[QT], float (&acc)[QT], float
   for (int R = 0; R < QT; ++R) {
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const float t = fminf(d[R][r], 0.f);
-      const float e = fexp2(t);
+      float e;
+      if (LAPL) {
+        const float t = fminf(d[R][r], 0.f);
+        e = fexp2(t);
+        accT[R] = fmaf(e, t, accT[R]);
+      } else {
+        e = fexp2_le1(d[R][r]);
+      }
       acc[R] += e;
-      if (LAPL) accT[R] = fmaf(e, t, accT[R]);
       d[R][r] = e;
     }
   }

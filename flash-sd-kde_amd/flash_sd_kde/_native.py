@@ -29,6 +29,7 @@ PREC_F32_VALU = 1
 PREC_BF16 = 2
 PREC_F16 = 3
 PREC_BF16X3 = 4
+PREC_BF16X6 = 5
 FLAG_LAPLACIAN = 1 << 8
 
 OP_KDE16_SPLITK = 1

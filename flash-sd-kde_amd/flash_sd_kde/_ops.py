@@ -28,6 +28,7 @@ PRECISION_FLAGS = {
     "bf16": nat.PREC_BF16,
     "fp16": nat.PREC_F16,
     "bf16x3": nat.PREC_BF16X3,
+    "bf16x6": nat.PREC_BF16X6,
 }
 
 
@@ -175,7 +176,7 @@ def score16(train: torch.Tensor, bandwidth: float, *, variant: str = "symmetric"
     else:
         raise ValueError(f"unknown score variant {variant}")
     flags = precision_flag(precision)
-    if flags not in (nat.PREC_F32, nat.PREC_BF16, nat.PREC_F16, nat.PREC_BF16X3):
+    if flags not in (nat.PREC_F32, nat.PREC_BF16, nat.PREC_F16, nat.PREC_BF16X3, nat.PREC_BF16X6):
         raise ValueError(f"precision {precision} is not available for the score pass")
     if flags != nat.PREC_F32 and variant != "ordered":
         raise ValueError("reduced-precision score accumulators use the ordered kernel")

@@ -3,6 +3,9 @@
 The plain FP32 form (the drop-in, flash_sd_kde.kde) evaluates the cross term
 q.x and phi.X with FP32 MFMA. The forms here evaluate them with 16-bit MFMA
 operands and FP32 accumulation:
+  "bf16x6"  BF16 three-piece split of both operands, seven cross-term
+            products (every term above FP32 rounding): an FP32-class cross
+            term at BF16 MFMA rate; phi.X as in bf16x3
   "bf16x3"  BF16 hi/lo split, three products (hi.hi + lo.hi + hi.lo)
   "fp16"    single FP16 operands
   "bf16"    single BF16 operands
@@ -20,8 +23,8 @@ import torch
 
 from flash_sd_kde import _ops
 
-SCORE_PRECISIONS = ("fp32", "bf16x3", "fp16", "bf16")
-LOGDENSITY_PRECISIONS = ("fp32", "fp32_valu", "bf16x3", "fp16", "bf16")
+SCORE_PRECISIONS = ("fp32", "bf16x6", "bf16x3", "fp16", "bf16")
+LOGDENSITY_PRECISIONS = ("fp32", "fp32_valu", "bf16x6", "bf16x3", "fp16", "bf16")
 
 
 def kde_logdensity(data, queries, bandwidth: float, *, precision: str = "fp32",

@@ -15,6 +15,7 @@ PRECISION_FP32_VALU = "fp32_valu"
 PRECISION_BF16 = "bf16"
 PRECISION_FP16 = "fp16"
 PRECISION_BF16X3 = "bf16x3"
+PRECISION_BF16X6 = "bf16x6"
 
 KDE_BACKEND_SPLITK_STREAM = "splitk_stream"
 KDE_BACKEND_ATOMIC = "atomic"

@@ -43,6 +43,8 @@ extern "C" {
 #define FSK_PREC_BF16 2u     /* BF16 MFMA, FP32 accumulate */
 #define FSK_PREC_F16 3u      /* FP16 MFMA, FP32 accumulate */
 #define FSK_PREC_BF16X3 4u   /* BF16 hi/lo split, 3 products, FP32 accumulate */
+#define FSK_PREC_BF16X6 5u   /* BF16 three-piece split, 7 products (all terms above
+                              * FP32 rounding), FP32 accumulate: FP32-class cross term */
 #define FSK_PREC_MASK 0xffu
 #define FSK_FLAG_LAPLACIAN (1u << 8) /* phi * (1 + d/2 - |q-x|^2/(2h^2)) */
 
@@ -73,7 +75,8 @@ double fsk_profile_last_ms(int op);
 /* Issue-rate microbenchmark (no reference counterpart; used to measure the
  * roofline the pair kernels are judged against). which: 0 FP32 MFMA flop/s,
  * 1 BF16 MFMA flop/s, 2 v_exp_f32 per s, 3 v_fma_f32 per s, 4 seconds of the
- * MFMA+exp mix kernel, 5 HBM copy bytes/s (read + write): the first half of
+ * MFMA+exp mix kernel, 6 seconds of the same mix with FMAs in place of the
+ * exps, 5 HBM copy bytes/s (read + write): the first half of
  * `out` copied to its second half by `iters` launches of a 4096-workgroup
  * copy kernel (blocks*512 bytes read and written per launch). `blocks`
  * workgroups of 256 threads, `iters` loop trips; `out` is a 16-byte aligned
@@ -153,9 +156,9 @@ int fsk_score16_sym_atomic(const float *data, int64_t ld, int64_t n,
 
 /* Same accumulators, deterministic: bounded split-K over the data with
  * FP32 partials and an FP64 combine; bitwise reproducible for fixed n.
- * flags may select a 16-bit cross term and phi.X (FSK_PREC_BF16X3 / _BF16 /
- * _F16, points centred on their mean; the precision form); FSK_PREC_F32 is
- * the plain FP32 form.
+ * flags may select a 16-bit cross term and phi.X (FSK_PREC_BF16X6 / _BF16X3 /
+ * _BF16 / _F16, points centred on their mean; the precision form);
+ * FSK_PREC_F32 is the plain FP32 form.
  * Replaces emp_score_16d_ordered_splitk,
  * kernels/emp_score_16d_ordered_splitk.py:162-263 (pass A :14-92, reduce
  * kernels/reduce_partials.py:82-108). */
