@@ -269,7 +269,7 @@ def precision_forms(xd, qd, h, fp32_ms, fp32_dens, steps, warmup, n_err, rates=N
                     "within_1e-3": mx <= 1e-3}}
     lib = _native.lib()
     pairs = float(m) * n + float(n) * n
-    for prec in ("bf16x6", "bf16x3", "fp16", "bf16"):
+    for prec in ("fp16x3", "bf16x6", "bf16x3", "fp16", "bf16"):
         def step():
             pdf, w = _ops.score16(xd, h, variant="ordered", precision=prec)
             deb = _ops.debias(xd, pdf, w, h, 1e-12)
@@ -295,6 +295,12 @@ def precision_forms(xd, qd, h, fp32_ms, fp32_dens, steps, warmup, n_err, rates=N
             units = float(n) * n
             t = units / rates["exp_per_s"] + 3.0 * units / rates["fma_per_s"]
             out[prec]["score_exp_valu_frac_measured_rates"] = round(t / (score_ms * 1e-3), 4)
+    ok = [p for p in out if p != "reference" and out[p]["within_1e-3"]]
+    best = max(ok, key=lambda p: out[p]["speedup_vs_fp32"])
+    out["best_within_1e-3"] = {"form": best, "speedup_vs_fp32": out[best]["speedup_vs_fp32"],
+                               "gpair_evals_per_s": out[best].get("gpair_evals_per_s",
+                                                                  round(pairs / fp32_ms / 1e6, 3)),
+                               "max_rel_logp_err": out[best]["max_rel_logp_err"]}
     return out
 
 

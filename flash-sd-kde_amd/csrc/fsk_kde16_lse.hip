@@ -18,6 +18,9 @@
 //   BF16 / F16 : one MFMA,    [x1 | fold] . [q1 | fold]
 //   BF16X3     : two MFMAs,   [x1 | x2] . [q1 | q1]  +  [x1 | fold] . [q2 | fold]
 //                (x1q1 + x2q1 + x1q2 + norms - m; the 2^-16 terms dropped)
+//   F16X3      : the same with FP16 pieces (11-bit significands: the dropped
+//                x2q2 term is ~2^-22 relative; values are centred and scaled so
+//                they stay inside FP16's range)
 //   BF16X6     : four MFMAs,  [x1 | x1] . [q1 | q2]  +  [x2 | x1] . [q1 | q3]
 //                           + [x2 | x3] . [q2 | q1]  +  [x2 | fold] . [q3 | fold]
 //                (all products down to 2^-16 relative plus x2q3; the dropped
@@ -78,9 +81,9 @@ __global__ __launch_bounds__(kWG) void kde16_lse_mfma_kernel(
     int64_t split_len, int64_t qblocks, int64_t ntasks, int64_t per_xcd,
     float2 *__restrict__ part) {
   constexpr bool F32 = (PREC == FSK_PREC_F32);
-  constexpr bool X3 = (PREC == FSK_PREC_BF16X3);
+  constexpr bool X3 = (PREC == FSK_PREC_BF16X3 || PREC == FSK_PREC_F16X3);
   constexpr bool X6 = (PREC == FSK_PREC_BF16X6);
-  constexpr int P16 = (PREC == FSK_PREC_F16) ? FSK_PREC_F16 : FSK_PREC_BF16;
+  constexpr int P16 = CrossType<PREC>::kP16;
   using RL = RecLayout<PREC>;
   const int64_t task = xcd_task(blockIdx.x, per_xcd);
   if (task >= ntasks) return;
@@ -342,7 +345,7 @@ struct LseLayout {
 
 static bool lse_is16(unsigned prec) {
   return prec == FSK_PREC_BF16 || prec == FSK_PREC_F16 || prec == FSK_PREC_BF16X3 ||
-         prec == FSK_PREC_BF16X6;
+         prec == FSK_PREC_BF16X6 || prec == FSK_PREC_F16X3;
 }
 
 static size_t lse_layout(void *ws, int64_t n, int64_t m, unsigned flags, LseLayout *L) {
@@ -392,7 +395,7 @@ extern "C" int fsk_kde16_logdensity(const float *data, int64_t ld, int64_t n,
     return FSK_EINVAL;
   }
   const unsigned prec = flags & FSK_PREC_MASK;
-  if (prec > FSK_PREC_BF16X6 || (flags & ~FSK_PREC_MASK)) {
+  if (prec > FSK_PREC_F16X3 || (flags & ~FSK_PREC_MASK)) {
     set_error("%s: flags 0x%x not available (Laplacian has no log form)", what, flags);
     return FSK_EUNSUPPORTED;
   }
@@ -425,6 +428,7 @@ extern "C" int fsk_kde16_logdensity(const float *data, int64_t ld, int64_t n,
     case FSK_PREC_BF16: FSK_PREP(FSK_PREC_BF16); break;
     case FSK_PREC_F16: FSK_PREP(FSK_PREC_F16); break;
     case FSK_PREC_BF16X6: FSK_PREP(FSK_PREC_BF16X6); break;
+    case FSK_PREC_F16X3: FSK_PREP(FSK_PREC_F16X3); break;
     default: FSK_PREP(FSK_PREC_BF16X3); break;
   }
 #undef FSK_PREP
@@ -450,6 +454,7 @@ extern "C" int fsk_kde16_logdensity(const float *data, int64_t ld, int64_t n,
       case FSK_PREC_BF16: FSK_LSE(FSK_PREC_BF16); break;
       case FSK_PREC_F16: FSK_LSE(FSK_PREC_F16); break;
       case FSK_PREC_BF16X6: FSK_LSE(FSK_PREC_BF16X6); break;
+      case FSK_PREC_F16X3: FSK_LSE(FSK_PREC_F16X3); break;
       default: FSK_LSE(FSK_PREC_BF16X3); break;
     }
 #undef FSK_LSE

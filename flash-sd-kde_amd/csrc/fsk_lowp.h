@@ -37,6 +37,13 @@ struct RecLayout {
 };
 inline int64_t rec_size(unsigned prec) { return prec == FSK_PREC_BF16X6 ? 64 : 48; }
 
+// 16-bit type of the cross-term operands of a form.
+template <int PREC>
+struct CrossType {
+  static constexpr int kP16 =
+      (PREC == FSK_PREC_F16 || PREC == FSK_PREC_F16X3) ? FSK_PREC_F16 : FSK_PREC_BF16;
+};
+
 __device__ __forceinline__ u16x8 ld_u16x8(const unsigned short *p) {
   return *reinterpret_cast<const u16x8 *>(p);
 }
@@ -110,7 +117,7 @@ __global__ void prep_lse_kernel(const float *__restrict__ x, int64_t ld, int64_t
     norm_out[j] = pad ? kPadNeg : float(nt);
     return;
   }
-  constexpr int P16 = (PREC == FSK_PREC_F16) ? FSK_PREC_F16 : FSK_PREC_BF16;
+  constexpr int P16 = CrossType<PREC>::kP16;
   using RL = RecLayout<PREC>;
   unsigned short r[RL::kSize];
 #pragma unroll

@@ -5,13 +5,15 @@
 // forms as the log-density kernel:
 //   BF16 / F16 : [x1 | fold] . [q1 | fold]
 //   BF16X3     : [x1 | x2] . [q1 | q1]  +  [x1 | fold] . [q2 | fold]
+//   F16X3      : the same with FP16 pieces (phi.X as in BF16X3)
 //   BF16X6     : [x1 | x1] . [q1 | q2]  +  [x2 | x1] . [q1 | q3]
 //              + [x2 | x3] . [q2 | q1]  +  [x2 | fold] . [q3 | fold]
 //                (FP32-class cross term; phi.X as in BF16X3)
 // phi = exp2(min(t, 0)) comes out with lane l holding phi(i = l%16, j = 4(l/16)+u),
 // u = 0..3, which is exactly four consecutive K slots of the 16x16x32 A operand,
 // so phi.X needs no data movement:
-//   A = [phi_h(u=0..3) | phi_l(u=0..3)] per lane (phi = phi_h + phi_l),
+//   A = [phi_h(u=0..3) | phi_l(u=0..3)] per lane (phi = phi_h + phi_l,
+//   phi_h = phi truncated to BF16, phi_l = rn_bf16(phi - phi_h)),
 //   W += A . [Xh | Xh]  +  A . [Xl | 0]      (BF16x3, BF16x6: drops phi_l.Xl)
 //   W += [phi | 0] . [Xh | 0]                (single forms)
 // X for phi.X comes from a tile-transposed copy: per 16-point tile, per feature
@@ -46,9 +48,46 @@ __global__ void prep_xt16_kernel(const float *__restrict__ x, int64_t ld, int64_
   xt[base + 4 + u] = lo;
 }
 
+typedef float f2v __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2v __attribute__((ext_vector_type(2)));
+typedef _Float16 h16x2v __attribute__((ext_vector_type(2)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+
+// Two floats rounded (nearest even) to 16 bits and packed [a | b << 16]: one
+// v_cvt_pk_bf16_f32 for BF16.
 template <int P16>
 __device__ __forceinline__ unsigned pack2(float a, float b) {
-  return unsigned(to16<P16>(a)) | (unsigned(to16<P16>(b)) << 16);
+  const f2v v = {a, b};
+  if (P16 == FSK_PREC_F16) return __builtin_bit_cast(unsigned, __builtin_convertvector(v, h16x2v));
+  return __builtin_bit_cast(unsigned, __builtin_convertvector(v, bf16x2v));
+}
+
+// The 16-bit A operand of phi.X from a lane's four phi values (K slots
+// 0..3) and, for the split forms, their low parts (K slots 4..7). Split: the
+// high part is phi truncated to its upper 16 bits (a BF16 value, exact as
+// FP32), the low part phi - high (exact in FP32) rounded to BF16; a byte
+// permute packs two high parts, one conversion packs two low parts.
+template <int P16, bool SPLIT>
+__device__ __forceinline__ u16x8 phi_operand(const float (&e)[4]) {
+  u32x4 a;
+  if (SPLIT) {
+    unsigned b[4];
+    float lo[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      b[u] = __builtin_bit_cast(unsigned, e[u]);
+      lo[u] = e[u] - __builtin_bit_cast(float, b[u] & 0xffff0000u);
+    }
+    a[0] = __builtin_amdgcn_perm(b[1], b[0], 0x07060302u);
+    a[1] = __builtin_amdgcn_perm(b[3], b[2], 0x07060302u);
+    a[2] = pack2<FSK_PREC_BF16>(lo[0], lo[1]);
+    a[3] = pack2<FSK_PREC_BF16>(lo[2], lo[3]);
+  } else {
+    a[0] = pack2<P16>(e[0], e[1]);
+    a[1] = pack2<P16>(e[2], e[3]);
+    a[2] = a[3] = 0u;
+  }
+  return __builtin_bit_cast(u16x8, a);
 }
 
 template <int PREC, int QT>
@@ -57,10 +96,13 @@ __global__ __launch_bounds__(kWG) void score16_lp_kernel(
     const unsigned short *__restrict__ xt, int64_t n, int64_t n_pad16, int64_t split_len,
     int64_t qblocks, int64_t ntasks, int64_t per_xcd, float *__restrict__ pdf_part,
     float *__restrict__ w_part) {
-  constexpr int P16 = (PREC == FSK_PREC_F16) ? FSK_PREC_F16 : FSK_PREC_BF16;
+  constexpr int P16 = CrossType<PREC>::kP16;  // cross-term operands
   constexpr bool X6 = (PREC == FSK_PREC_BF16X6);
-  constexpr bool X3 = (PREC == FSK_PREC_BF16X3);
+  constexpr bool X3 = (PREC == FSK_PREC_BF16X3 || PREC == FSK_PREC_F16X3);
   constexpr bool SPLIT_PHI = X3 || X6;  // phi.X with phi = phi_h + phi_l
+  // phi.X operands: BF16 for the split forms (phi in [0, 1] keeps full
+  // relative precision down to tiny values), else the form's own type
+  constexpr int PX = SPLIT_PHI ? FSK_PREC_BF16 : P16;
   using RL = RecLayout<PREC>;
   const int64_t task = xcd_task(blockIdx.x, per_xcd);
   if (task >= ntasks) return;
@@ -141,26 +183,9 @@ __global__ __launch_bounds__(kWG) void score16_lp_kernel(
         e[u] = fexp2_le1(d[u]);
         pdf[R] += e[u];
       }
-      u16x8 A;
-      const unsigned h01 = pack2<P16>(e[0], e[1]);
-      const unsigned h23 = pack2<P16>(e[2], e[3]);
-      A[0] = (unsigned short)(h01 & 0xffffu);
-      A[1] = (unsigned short)(h01 >> 16);
-      A[2] = (unsigned short)(h23 & 0xffffu);
-      A[3] = (unsigned short)(h23 >> 16);
-      if (SPLIT_PHI) {
-        const unsigned l01 = pack2<P16>(e[0] - from16<P16>(A[0]), e[1] - from16<P16>(A[1]));
-        const unsigned l23 = pack2<P16>(e[2] - from16<P16>(A[2]), e[3] - from16<P16>(A[3]));
-        A[4] = (unsigned short)(l01 & 0xffffu);
-        A[5] = (unsigned short)(l01 >> 16);
-        A[6] = (unsigned short)(l23 & 0xffffu);
-        A[7] = (unsigned short)(l23 >> 16);
-        w[R] = mfma16(A, BX1, w[R], P16);
-        w[R] = mfma16(A, BX2, w[R], P16);
-      } else {
-        A[4] = A[5] = A[6] = A[7] = 0;
-        w[R] = mfma16(A, BX1, w[R], P16);
-      }
+      const u16x8 A = phi_operand<PX, SPLIT_PHI>(e);
+      w[R] = mfma16(A, BX1, w[R], PX);
+      if (SPLIT_PHI) w[R] = mfma16(A, BX2, w[R], PX);
     }
     if (++tc == kLpFlush) {
       tc = 0;
@@ -273,6 +298,7 @@ int score16_lp_run(const float *data, int64_t ld, int64_t n, double h, unsigned 
     case FSK_PREC_BF16: FSK_LP(FSK_PREC_BF16, FSK_PREC_BF16, kScoreLpQT); break;
     case FSK_PREC_F16: FSK_LP(FSK_PREC_F16, FSK_PREC_F16, kScoreLpQT); break;
     case FSK_PREC_BF16X6: FSK_LP(FSK_PREC_BF16X6, FSK_PREC_BF16, kScoreLpQTx6); break;
+    case FSK_PREC_F16X3: FSK_LP(FSK_PREC_F16X3, FSK_PREC_BF16, kScoreLpQT); break;
     default: FSK_LP(FSK_PREC_BF16X3, FSK_PREC_BF16, kScoreLpQT); break;
   }
 #undef FSK_LP

@@ -29,6 +29,7 @@ PRECISION_FLAGS = {
     "fp16": nat.PREC_F16,
     "bf16x3": nat.PREC_BF16X3,
     "bf16x6": nat.PREC_BF16X6,
+    "fp16x3": nat.PREC_F16X3,
 }
 
 
@@ -93,6 +94,27 @@ def kde16_sums(train: torch.Tensor, query: torch.Tensor, bandwidth: float, *,
     return out
 
 
+FP16_SAFE = 3.0e4  # largest magnitude an FP16 record value may take (FP16 max 65504)
+
+
+def check_fp16_range(train: torch.Tensor, query: torch.Tensor, h: float, flags: int) -> None:
+    """The FP16 forms store the centred, scaled coordinates -2c (q - mu) and the
+    norm terms c |x - mu|^2 (c = log2(e) / (2 h^2) in magnitude) as FP16
+    pieces; refuse inputs whose values would leave FP16's range instead of
+    returning inf / NaN."""
+    if flags not in (nat.PREC_F16, nat.PREC_F16X3):
+        return
+    mu = train.mean(0)
+    r2 = float(((train - mu) ** 2).sum(1).max())
+    if query is not train:
+        r2 = max(r2, float(((query - mu) ** 2).sum(1).max()))
+    c = 1.4426950408889634 / (2.0 * h * h)
+    if c * r2 > FP16_SAFE or 2.0 * c * r2 ** 0.5 > FP16_SAFE:
+        raise ValueError(
+            f"FP16 forms need c*max|x-mu|^2 <= {FP16_SAFE:g} (got {c * r2:.3g}): the points are too "
+            "far from their mean for this bandwidth; use bf16x3 / fp32.")
+
+
 def kde16_logdensity(train: torch.Tensor, query: torch.Tensor, bandwidth: float, *,
                      precision: str = "fp32") -> torch.Tensor:
     """Normalised log-density by log-sum-exp, float64 (m,)."""
@@ -103,6 +125,7 @@ def kde16_logdensity(train: torch.Tensor, query: torch.Tensor, bandwidth: float,
     if n == 0 or m == 0:
         raise ValueError("data and queries must be non-empty.")
     flags = precision_flag(precision)
+    check_fp16_range(train, query, h, flags)
     out = torch.empty(m, dtype=torch.float64, device=dev)
     ws, nbytes = nat.workspace(nat.OP_KDE16_LOGDENSITY, n, m, flags, dev)
     st = nat.lib().fsk_kde16_logdensity(train.data_ptr(), train.stride(0), n, query.data_ptr(),
@@ -176,10 +199,12 @@ def score16(train: torch.Tensor, bandwidth: float, *, variant: str = "symmetric"
     else:
         raise ValueError(f"unknown score variant {variant}")
     flags = precision_flag(precision)
-    if flags not in (nat.PREC_F32, nat.PREC_BF16, nat.PREC_F16, nat.PREC_BF16X3, nat.PREC_BF16X6):
+    if flags not in (nat.PREC_F32, nat.PREC_BF16, nat.PREC_F16, nat.PREC_BF16X3, nat.PREC_BF16X6,
+                     nat.PREC_F16X3):
         raise ValueError(f"precision {precision} is not available for the score pass")
     if flags != nat.PREC_F32 and variant != "ordered":
         raise ValueError("reduced-precision score accumulators use the ordered kernel")
+    check_fp16_range(train, train, h, flags)
     ws, nbytes = nat.workspace(op, n, n, flags, dev)
     st = fn(train.data_ptr(), train.stride(0), n, h, flags, ws.data_ptr(), nbytes, pdf.data_ptr(),
             wbuf.data_ptr(), ld_row, ld_col, nat.stream_handle(dev))

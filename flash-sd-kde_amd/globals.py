@@ -16,6 +16,7 @@ PRECISION_BF16 = "bf16"
 PRECISION_FP16 = "fp16"
 PRECISION_BF16X3 = "bf16x3"
 PRECISION_BF16X6 = "bf16x6"
+PRECISION_FP16X3 = "fp16x3"
 
 KDE_BACKEND_SPLITK_STREAM = "splitk_stream"
 KDE_BACKEND_ATOMIC = "atomic"

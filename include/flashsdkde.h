@@ -45,6 +45,12 @@ extern "C" {
 #define FSK_PREC_BF16X3 4u   /* BF16 hi/lo split, 3 products, FP32 accumulate */
 #define FSK_PREC_BF16X6 5u   /* BF16 three-piece split, 7 products (all terms above
                               * FP32 rounding), FP32 accumulate: FP32-class cross term */
+#define FSK_PREC_F16X3 6u    /* FP16 hi/lo split cross term, 3 products (2^-22 terms
+                              * dropped), FP32 accumulate; phi.X as in BF16X3 */
+/* The FP16 forms (_F16, _F16X3) hold -2c (q - mu) and c |x - mu|^2, with
+ * c = log2(e) / (2 h^2) and mu the data mean, as FP16 pieces: the caller
+ * keeps c * max |x - mu|^2 within FP16's range (the Python layer refuses
+ * inputs above 3e4). */
 #define FSK_PREC_MASK 0xffu
 #define FSK_FLAG_LAPLACIAN (1u << 8) /* phi * (1 + d/2 - |q-x|^2/(2h^2)) */
 
@@ -156,8 +162,8 @@ int fsk_score16_sym_atomic(const float *data, int64_t ld, int64_t n,
 
 /* Same accumulators, deterministic: bounded split-K over the data with
  * FP32 partials and an FP64 combine; bitwise reproducible for fixed n.
- * flags may select a 16-bit cross term and phi.X (FSK_PREC_BF16X6 / _BF16X3 /
- * _BF16 / _F16, points centred on their mean; the precision form);
+ * flags may select a 16-bit cross term and phi.X (FSK_PREC_F16X3 / _BF16X6 /
+ * _BF16X3 / _BF16 / _F16, points centred on their mean; the precision form);
  * FSK_PREC_F32 is the plain FP32 form.
  * Replaces emp_score_16d_ordered_splitk,
  * kernels/emp_score_16d_ordered_splitk.py:162-263 (pass A :14-92, reduce

@@ -98,3 +98,15 @@ def test_ndim_rejects_bad_dimensions():
         from flash_sd_kde.ndim import kde_eval_nd
 
         kde_eval_nd(np.zeros((4, 3)), np.zeros((2, 3)), 0.0)
+
+
+def test_fp16_forms_refuse_out_of_range_inputs():
+    from flash_sd_kde import _native as nat
+    from flash_sd_kde import _ops
+
+    x = torch.zeros((4, 16))
+    x[0, 0] = 100.0  # c |x - mu|^2 ~ 0.72 * 5600 / h^2
+    _ops.check_fp16_range(x, x, 1.0, nat.PREC_BF16X3)  # BF16 forms have no such limit
+    with pytest.raises(ValueError, match="FP16 forms"):
+        _ops.check_fp16_range(x, x, 0.1, nat.PREC_F16X3)
+    _ops.check_fp16_range(x, x, 10.0, nat.PREC_F16)  # in range
