@@ -130,6 +130,23 @@ def measure_rates():
         lib.fsk_microbench(which, 2048, 50, out.data_ptr(), ctypes.addressof(r), None)
         st = lib.fsk_microbench(which, 2048, iters, out.data_ptr(), ctypes.addressof(r), None)
         rates[name] = r.value if st == 0 else None
+    # MFMA / VALU co-issue: 4 FP32 MFMAs + 16 exps (or 16 FMAs) per lane per
+    # iteration against the two parts timed alone (2048 workgroups, 20000
+    # iterations). A ratio near 1 means the two issue times add up; near
+    # max/sum they overlap.
+    t = {}
+    for which in (0, 2, 3, 4, 6):
+        lib.fsk_microbench(which, 2048, 50, out.data_ptr(), ctypes.addressof(r), None)
+        if lib.fsk_microbench(which, 2048, 20000, out.data_ptr(), ctypes.addressof(r), None) == 0:
+            t[which] = r.value
+    if len(t) == 5:
+        waves = 2048 * 4
+        t_mfma = waves * 20000 * 4 * 2048 / t[0]
+        t_exp16 = 2 * waves * 64 * 20000 * 8 / t[2]
+        t_fma16 = 2 * waves * 64 * 20000 * 8 / t[3]
+        rates["mfma_exp_mix_time_over_sum"] = round(t[4] / (t_mfma + t_exp16), 4)
+        rates["mfma_exp_mix_time_over_max"] = round(t[4] / max(t_mfma, t_exp16), 4)
+        rates["mfma_fma_mix_time_over_sum"] = round(t[6] / (t_mfma + t_fma16), 4)
     # HBM copy: 512 MiB read + 512 MiB written per launch (2^20 blocks x 1 KiB)
     buf = torch.empty(1 << 28, device="cuda")
     lib.fsk_microbench(5, 1 << 20, 1, buf.data_ptr(), ctypes.addressof(r), None)
