@@ -47,6 +47,7 @@ def parse():
     ap.add_argument("--no-precision-forms", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true")
+    ap.add_argument("--no-one-d", action="store_true")
     ap.add_argument("--error-queries", type=int, default=4096)
     return ap.parse_args()
 
@@ -257,6 +258,63 @@ def surrogate_form(fp32_pair_rate):
     }
 
 
+def one_d_sdkde(rates, steps):
+    """1-D SD-KDE as in the reference's Triton scaling sweep
+    (experiments/configs/runtime/run_triton_scaling.sh, n = 2^21, m = n/8):
+    empirical score of every point against all points, debias x + (h^2/2)
+    score, KDE of m queries; data N(0, 1) (seed 0), queries seed 1, bandwidth
+    0.4 min(std, IQR/1.34) n^(-1/9) (the reference's 1-D SD-KDE rule)."""
+    import numpy as np
+    import torch
+
+    from flash_sd_kde import _native, _ops
+
+    n, m = 1 << 21, 1 << 18
+    rng = np.random.default_rng(0)
+    x = rng.standard_normal(n).astype(np.float32)
+    q = np.random.default_rng(1).standard_normal(m).astype(np.float32)
+    iqr = float(np.percentile(x, 75) - np.percentile(x, 25))
+    h = 0.4 * min(float(x.std()), iqr / 1.34) * n ** (-1.0 / 9.0)
+    xd = torch.as_tensor(x, device="cuda")
+    qd = torch.as_tensor(q, device="cuda")
+
+    def step():
+        pdf, der = _ops.kde1_score_sums(xd, xd, h)
+        deb = xd + (0.5 * h) * (der / pdf)  # (h^2/2) * score, score = der / (h pdf)
+        return _ops.kde1_sums(deb, qd, h, atomic=True)
+
+    step()
+    torch.cuda.synchronize()
+    lib = _native.lib()
+    lib.fsk_profile_enable(1)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        out = step()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) / steps * 1e3
+    score_ms = lib.fsk_profile_last_ms(_native.OP_KDE1_SCORE)
+    kde_ms = lib.fsk_profile_last_ms(_native.OP_KDE1_ATOMIC)
+    lib.fsk_profile_enable(0)
+    pairs_s, pairs_k = float(n) * n, float(n) * m
+    res = {"workload": f"1-D SD-KDE, n = {n}, m = {m}, N(0,1)", "bandwidth": h,
+           "ms_per_step": round(ms, 3), "gpair_evals_per_s": round((pairs_s + pairs_k) / ms / 1e6, 3),
+           "score_kernel_ms": round(score_ms, 3), "kde_kernel_ms": round(kde_ms, 3),
+           "outputs_finite": bool(torch.isfinite(out).all().item())}
+    if rates.get("exp_per_s") and rates.get("fma_per_s"):
+        # per pair: score 1 exp + 4 VALU (sub, mul, add, fma); KDE 1 exp + 3 VALU.
+        # The transcendental issue overlaps the other VALU work in these kernels
+        # (the additive exp + VALU time exceeds the measured time), so each
+        # resource is reported as its own fraction; the larger one binds.
+        for name, pairs, v_ops, kms in (("score", pairs_s, 4, score_ms), ("kde", pairs_k, 3, kde_ms)):
+            t_exp = pairs / rates["exp_per_s"]
+            t_valu = v_ops * pairs / rates["fma_per_s"]
+            res[f"{name}_roofline"] = {
+                "exp_frac": round(t_exp / (kms * 1e-3), 4),
+                "valu_frac": round(t_valu / (kms * 1e-3), 4),
+                "bound": "valu" if t_valu >= t_exp else "exp"}
+    return res
+
+
 def precision_forms(xd, qd, h, fp32_ms, fp32_dens, steps, warmup, n_err, rates=None):
     """Reduced-precision SD-KDE forms on the same workload: score with 16-bit
     MFMA operands (FP32 accumulate), FP64 debias, log-sum-exp KDE with the same
@@ -457,6 +515,8 @@ def main():
     if not args.no_precision_forms:
         line["precision_forms"] = precision_forms(xd, qd, h, ms_per_step, dens, args.steps,
                                                   args.warmup, args.error_queries, rates)
+    if not args.no_one_d:
+        line["one_d_sdkde"] = one_d_sdkde(rates, args.steps)
     if not args.no_surrogate:
         del xd, qd
         torch.cuda.empty_cache()
