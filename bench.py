@@ -83,7 +83,7 @@ def cpu_baseline(x, q, h, budget_s):
             for p in ex.map(work, range(nblocks)):
                 pairs += p
     dt = time.perf_counter() - t0
-    return {
+    out = {
         "value": pairs / dt / 1e9,
         "unit": "Gpair-evals/s",
         "cores": threads,
@@ -92,6 +92,31 @@ def cpu_baseline(x, q, h, budget_s):
                    f"queries alternating) x all n={x.shape[0]} points, {pairs:.3g} pairs in "
                    f"{dt:.1f} s; numpy FP32 oracle, {threads} threads, BLAS 1 thread each"),
     }
+    out["sklearn"] = sklearn_baseline(x, q, h, budget_s / 2)
+    return out
+
+
+def sklearn_baseline(x, q, h, budget_s):
+    """scikit-learn KernelDensity (gaussian, exact: rtol = atol = 0) fitted on
+    all n points and scored on a bounded sample of the queries, as the
+    reference's runtime benchmark times it
+    (experiments/runtime/benchmark_triton_kde.py:63-74); one thread."""
+    try:
+        from sklearn.neighbors import KernelDensity
+    except ImportError:
+        return None
+    t0 = time.perf_counter()
+    kd = KernelDensity(kernel="gaussian", bandwidth=h, rtol=0.0, atol=0.0).fit(x)
+    fit_s = time.perf_counter() - t0
+    done, k = 0, 4
+    t1 = time.perf_counter()
+    while done < q.shape[0] and time.perf_counter() - t1 < budget_s:
+        kd.score_samples(q[done:done + k])
+        done += k
+        k = min(2 * k, 256)
+    dt = time.perf_counter() - t1
+    return {"value": done * x.shape[0] / dt / 1e9, "unit": "Gpair-evals/s (KDE queries)",
+            "cores": 1, "sample": f"{done} queries x n={x.shape[0]} in {dt:.1f} s (fit {fit_s:.1f} s)"}
 
 
 # HBM bytes per launch of the dominant kernel, from two rocprofv3 --pmc runs of
