@@ -535,6 +535,15 @@ def main():
         line["roofline"]["exp_valu_frac_measured_rates"] = round(t / (score_ms * 1e-3), 4)
         line["roofline"]["mfma_frac_of_measured_peak"] = round(
             flops / (score_ms * 1e-3) / rates["fp32_mfma_flops"], 4)
+        if args.variant == "ordered" and rates.get("fp32_mfma_flops"):
+            # FP32 MFMA never co-executes with VALU work on this GPU (counter
+            # SQ_VALU_MFMA_COEXEC_CYCLES = 0), so the issue bound of the
+            # ordered kernel is the SUM of its MFMA time and its VALU time:
+            # per ordered pair 64 flop MFMA, 1 exp and 2 other VALU ops (norm
+            # add, row-sum add; the clamp rides on the exp).
+            t_add = (flops / rates["fp32_mfma_flops"] + units / rates["exp_per_s"]
+                     + 2.0 * units / rates["fma_per_s"])
+            line["roofline"]["additive_issue_bound_frac"] = round(t_add / (score_ms * 1e-3), 4)
     if not args.no_batched:
         line["batched_loo"] = batched_loo(rates, args.steps)
     if not args.no_precision_forms:

@@ -101,9 +101,12 @@ def check_fp16_range(train: torch.Tensor, query: torch.Tensor, h: float, flags: 
     """The FP16 forms store the centred, scaled coordinates -2c (q - mu) and the
     norm terms c |x - mu|^2 (c = log2(e) / (2 h^2) in magnitude) as FP16
     pieces; refuse inputs whose values would leave FP16's range instead of
-    returning inf / NaN."""
+    returning inf / NaN. Skipped while a HIP graph is being captured (it needs
+    a device-to-host read)."""
     if flags not in (nat.PREC_F16, nat.PREC_F16X3):
         return
+    if train.is_cuda and torch.cuda.is_current_stream_capturing():
+        return  # the check reads a device value; under graph capture the caller vouches
     mu = train.mean(0)
     r2 = float(((train - mu) ** 2).sum(1).max())
     if query is not train:

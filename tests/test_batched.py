@@ -64,3 +64,29 @@ def test_batched_graph_replays_and_autotune(gpu):
     for k, x in enumerate(sets):
         ref = fp64.loo_loglik(x, grid.bandwidths[k])
         assert np.argmax(ref) == np.argmax(eager[k].cpu().numpy())
+
+
+@pytest.mark.gpu
+def test_batched_agrees_with_score_kernel_identity(gpu):
+    """Cross-kernel check: the leave-one-out density is the score kernel's
+    diagonal-inclusive sum minus the self term, LOO_i = (pdf_sum_i - 1) / (n - 1)
+    times the Gaussian normaliser, so the LOO grid and the ordered score kernel
+    (two different kernels) must give the same mean log-likelihood."""
+    from flash_sd_kde import _ops
+
+    x = _sets([1500], seed=5)[0]
+    h0 = scott_mean_std_bandwidth(x)
+    # bandwidths wide enough that every point has neighbours: pdf_sum - 1 is
+    # then far above the FP32 resolution of pdf_sum (at small h the LOO grid's
+    # shifted form is exact where this difference is not)
+    factors = np.array([2.0, 4.0])
+    ll = loo_loglik_grid([x], factors=factors, device=gpu)[0]
+    n = x.shape[0]
+    for f, got in zip(factors, ll):
+        h = h0 * f
+        pdf, _ = _ops.score16(torch.as_tensor(x, device=gpu), h, variant="ordered")
+        s = pdf.double().cpu().numpy() - 1.0
+        assert s.min() > 0.05
+        log_norm = -np.log(n - 1) - 16 * np.log(h) - 8 * np.log(2 * np.pi)
+        ref = float(np.mean(np.log(s) + log_norm))
+        assert abs(got - ref) <= 1e-5 * abs(ref), (f, got, ref)
