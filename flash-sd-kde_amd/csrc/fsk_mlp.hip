@@ -21,9 +21,13 @@
 
 namespace fsk {
 
-__device__ __forceinline__ float silu(float z) { return z / (1.f + __expf(-z)); }
+// sigmoid with the hardware reciprocal (1 ulp) instead of a full division
+__device__ __forceinline__ float sigmoid(float z) {
+  return __builtin_amdgcn_rcpf(1.f + __expf(-z));
+}
+__device__ __forceinline__ float silu(float z) { return z * sigmoid(z); }
 __device__ __forceinline__ float silu_grad(float z) {
-  const float s = 1.f / (1.f + __expf(-z));
+  const float s = sigmoid(z);
   return s * (1.f + z * (1.f - s));
 }
 
@@ -44,6 +48,11 @@ __device__ __forceinline__ u16x8 bf16x8_from(const float *p) {
 // Workgroup tile 64 x 64, wave tile 32 x 32 (2 x 2 MFMA blocks), K step 32.
 // ACT 0: Y = Z; 1: Y = silu(Z), Z optionally stored; 2: Y = Z * silu'(Zin)
 // with Zin read from the Z pointer (row stride N).
+// Loads are unconditional with clamped row / column indices (rows >= M and
+// columns >= N only feed outputs that are not stored; K is a multiple of 8,
+// so a lane's 8 K values are all valid or all past K and zeroed by a select),
+// and the next K step's operands are loaded before the current step's MFMAs:
+// no branch or wait sits between a load and the work that hides it.
 template <int ACT, bool STORE_Z>
 __global__ __launch_bounds__(256) void mlp_gemm_kernel(const float *__restrict__ X, int64_t ldx,
                                                        int64_t M, int K,
@@ -55,26 +64,55 @@ __global__ __launch_bounds__(256) void mlp_gemm_kernel(const float *__restrict__
   const int g = lane >> 4, c = lane & 15;
   const int64_t m0 = int64_t(blockIdx.x) * 64 + (wave & 1) * 32;
   const int n0 = blockIdx.y * 64 + (wave >> 1) * 32;
+  const float *xr[2];
+  const unsigned short *wr[2];
+#pragma unroll
+  for (int rb = 0; rb < 2; ++rb) {
+    const int64_t row = m0 + 16 * rb + c;
+    xr[rb] = X + (row < M ? row : M - 1) * ldx;
+  }
+#pragma unroll
+  for (int cb = 0; cb < 2; ++cb) {
+    const int n = n0 + 16 * cb + c;
+    wr[cb] = Wt + int64_t(n < N ? n : N - 1) * K;
+  }
   f4 acc[2][2];
 #pragma unroll
   for (int a = 0; a < 2; ++a)
 #pragma unroll
     for (int b = 0; b < 2; ++b) acc[a][b] = f4{0.f, 0.f, 0.f, 0.f};
+  // raw operands of one K step: A as FP32 (converted to BF16 when consumed,
+  // so a prefetched load is not waited on before the current MFMAs), B as BF16
+  struct Raw {
+    f4 a[2][2];
+    u16x8 b[2];
+  };
+  auto load = [&](int k0, Raw &r) {
+    const int kc = (k0 + 8 * g < K) ? k0 + 8 * g : 0;
+#pragma unroll
+    for (int rb = 0; rb < 2; ++rb) {
+      r.a[rb][0] = ld4(xr[rb] + kc);
+      r.a[rb][1] = ld4(xr[rb] + kc + 4);
+    }
+#pragma unroll
+    for (int cb = 0; cb < 2; ++cb) r.b[cb] = *reinterpret_cast<const u16x8 *>(wr[cb] + kc);
+  };
   const u16x8 zero8 = {0, 0, 0, 0, 0, 0, 0, 0};
-  for (int k0 = 0; k0 < K; k0 += 32) {
-    const int k = k0 + 8 * g;
+  auto consume = [&](int k0, const Raw &r) {
+    const bool kok = k0 + 8 * g < K;
     u16x8 A[2], B[2];
 #pragma unroll
     for (int rb = 0; rb < 2; ++rb) {
-      const int64_t row = m0 + 16 * rb + c;
-      A[rb] = (row < M && k < K) ? bf16x8_from(X + row * ldx + k) : zero8;
+      u16x8 v;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        v[e] = to16<FSK_PREC_BF16>(r.a[rb][0][e]);
+        v[4 + e] = to16<FSK_PREC_BF16>(r.a[rb][1][e]);
+      }
+      A[rb] = kok ? v : zero8;
     }
 #pragma unroll
-    for (int cb = 0; cb < 2; ++cb) {
-      const int n = n0 + 16 * cb + c;
-      B[cb] = (n < N && k < K) ? *reinterpret_cast<const u16x8 *>(Wt + int64_t(n) * K + k)
-                               : zero8;
-    }
+    for (int cb = 0; cb < 2; ++cb) B[cb] = kok ? r.b[cb] : zero8;
 #pragma unroll
     for (int rb = 0; rb < 2; ++rb)
 #pragma unroll
@@ -82,7 +120,19 @@ __global__ __launch_bounds__(256) void mlp_gemm_kernel(const float *__restrict__
         acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
             __builtin_bit_cast(bf16x8, A[rb]), __builtin_bit_cast(bf16x8, B[cb]), acc[rb][cb], 0,
             0, 0);
+  };
+  // two register buffers used in turn (no copies between them, so a
+  // prefetched load is only waited on where it is consumed)
+  Raw r0, r1;
+  load(0, r0);
+  for (int k0 = 0; k0 < K; k0 += 64) {
+    if (k0 + 32 < K) load(k0 + 32, r1);
+    consume(k0, r0);
+    if (k0 + 32 >= K) break;
+    if (k0 + 64 < K) load(k0 + 64, r0);
+    consume(k0 + 32, r1);
   }
+  const bool full = (m0 + 32 <= M) && (n0 + 32 <= N);  // wave-uniform
 #pragma unroll
   for (int rb = 0; rb < 2; ++rb)
 #pragma unroll
@@ -92,7 +142,7 @@ __global__ __launch_bounds__(256) void mlp_gemm_kernel(const float *__restrict__
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int64_t row = m0 + 16 * rb + 4 * g + r;
-        if (row < M && n < N) {
+        if (full || (row < M && n < N)) {
           const float z = acc[rb][cb][r] + bv;
           if (ACT == 2) {
             Y[row * ldy + n] = z * silu_grad(Z[row * N + n]);
@@ -114,11 +164,18 @@ __global__ __launch_bounds__(256) void mlp_gemm_kernel(const float *__restrict__
 // issued before the current chunk's MFMAs. split_len is a multiple of 64.
 constexpr int WG_LDS_STRIDE = 72;  // u16 per LDS row: 64 batch rows + 8 pad (144 B, 16 B aligned)
 
+// Four rows m..m+3 of one 4-column slice; rows past `me` (loaded from row
+// me - 1, valid since me >= 1) and columns past the matrix (the caller points
+// p at column 0 for them) are zeroed by selects, so the loads themselves are
+// unconditional and in bounds.
 __device__ __forceinline__ void wgrad_load(const float *__restrict__ p, int64_t ld, int64_t m,
                                            int64_t me, bool col_ok, f4 (&v)[4]) {
 #pragma unroll
-  for (int j = 0; j < 4; ++j)
-    v[j] = (col_ok && m + j < me) ? ld4(p + (m + j) * ld) : f4{0.f, 0.f, 0.f, 0.f};
+  for (int j = 0; j < 4; ++j) {
+    const bool ok = col_ok && m + j < me;
+    const f4 x = ld4(p + (m + j < me ? m + j : me - 1) * ld);
+    v[j] = ok ? x : f4{0.f, 0.f, 0.f, 0.f};
+  }
 }
 
 __device__ __forceinline__ void wgrad_stage(unsigned short *sT, int b, int q, const f4 (&v)[4]) {
@@ -150,8 +207,8 @@ __global__ __launch_bounds__(256) void mlp_wgrad_kernel(const float *__restrict_
   const int64_t mb = s * split_len;
   const int64_t me = (mb + split_len < M) ? mb + split_len : M;
   const bool a_ok = n0 + 4 * b < N, b_ok = k0 + 4 * b < K;  // N, K multiples of 8
-  const float *pa = dZ + n0 + 4 * b;
-  const float *pb = X + k0 + 4 * b;
+  const float *pa = dZ + (a_ok ? n0 + 4 * b : 0);
+  const float *pb = X + (b_ok ? k0 + 4 * b : 0);
   f4 acc[2][2];
 #pragma unroll
   for (int x = 0; x < 2; ++x)

@@ -255,9 +255,9 @@ int fsk_mlp_cast_bf16(const float *w, int rows, int cols, uint16_t *w_bf16, uint
 int fsk_mlp_mse(const float *y, const float *t, int64_t count, double scale, float *dy,
                 double *loss, void *stream);
 
-/* ---- FP64 reference kernels (measurement; SURVEY row "error-suite oracle on
- * the GPU"). Direct differences in FP64, FP64 exp and accumulation; inputs are
- * contiguous (rows, 16) float64. */
+/* ---- FP64 reference kernels (measurement: the error of the reduced-precision
+ * forms at sizes a CPU FP64 oracle cannot reach). Direct differences in FP64,
+ * FP64 exp and accumulation; inputs are contiguous (rows, 16) float64. */
 
 /* log p(q_i) by log-sum-exp (the FP64 semantics of
  * experiments/error_suite_16d/reference.py:32-82, reference_log_density).

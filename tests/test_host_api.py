@@ -1,6 +1,6 @@
 """Host-side API behaviour that needs no GPU: validation, error paths,
-bandwidth rules, backend selection (reference error behaviour, SURVEY-listed
-in flash_sd_kde/kde.py:64-71, estimator.py:93-157, config.py:46-75)."""
+bandwidth rules, backend selection (the reference's error behaviour in
+flash_sd_kde/kde.py:64-71, estimator.py:93-157, config.py:46-75)."""
 
 import numpy as np
 import pytest
