@@ -301,6 +301,39 @@ __global__ void mlp_adam_kernel(float *__restrict__ w, float *__restrict__ mom,
   }
 }
 
+// Batched form: block b of the grid belongs to tensor k when
+// block_start[k] <= b < block_start[k + 1] (blocks never straddle tensors).
+struct AdamBatch {
+  fsk_adam_tensor t[FSK_ADAM_MAX_TENSORS];
+  int64_t block_start[FSK_ADAM_MAX_TENSORS + 1];
+  int n;
+};
+
+__global__ void mlp_adam_multi_kernel(AdamBatch batch, float lr, float b1, float b2, float eps,
+                                      float bc1, float bc2, const float *__restrict__ sched) {
+  int k = 0;
+  while (k + 1 < batch.n && int64_t(blockIdx.x) >= batch.block_start[k + 1]) ++k;
+  const fsk_adam_tensor &T = batch.t[k];
+  const int64_t e = (int64_t(blockIdx.x) - batch.block_start[k]) * blockDim.x + threadIdx.x;
+  if (e >= T.count) return;
+  if (sched) {
+    lr = sched[0];
+    bc1 = sched[1];
+    bc2 = sched[2];
+  }
+  float gsum = 0.f;
+#pragma unroll 8
+  for (int s = 0; s < T.splits; ++s) gsum += T.grad_part[int64_t(s) * T.count + e];
+  const float m = b1 * T.m[e] + (1.f - b1) * gsum;
+  const float v = b2 * T.v[e] + (1.f - b2) * gsum * gsum;
+  T.m[e] = m;
+  T.v[e] = v;
+  const float nw = T.w[e] - lr * (m / bc1) / (sqrtf(v / bc2) + eps);
+  T.w[e] = nw;
+  if (T.w_bf16) T.w_bf16[e] = to16<FSK_PREC_BF16>(nw);
+  if (T.wt_bf16) T.wt_bf16[(e % T.cols) * T.rows + e / T.cols] = to16<FSK_PREC_BF16>(nw);
+}
+
 __global__ void mlp_cast_kernel(const float *__restrict__ w, int64_t count, int rows, int cols,
                                 unsigned short *__restrict__ wb, unsigned short *__restrict__ wtb) {
   const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -410,6 +443,35 @@ extern "C" int fsk_mlp_adam(float *w, float *m, float *v, const float *grad_part
                      v, grad_part, splits, count, cols, float(lr), float(beta1), float(beta2),
                      float(eps), bc1, bc2, w_bf16, wt_bf16, rows, sched);
   return launch_status("fsk_mlp_adam");
+}
+
+extern "C" int fsk_mlp_adam_multi(const fsk_adam_tensor *tensors, int n_tensors, double lr,
+                                  double beta1, double beta2, double eps, int step,
+                                  const float *sched, void *stream) {
+  const char *what = "fsk_mlp_adam_multi";
+  if (!tensors || n_tensors <= 0 || n_tensors > FSK_ADAM_MAX_TENSORS || (step <= 0 && !sched)) {
+    set_error("%s: need 1..%d tensors and step > 0 or a schedule", what, FSK_ADAM_MAX_TENSORS);
+    return FSK_EINVAL;
+  }
+  AdamBatch b{};
+  b.n = n_tensors;
+  b.block_start[0] = 0;
+  for (int k = 0; k < n_tensors; ++k) {
+    const fsk_adam_tensor &T = tensors[k];
+    if (!T.w || !T.m || !T.v || !T.grad_part || T.splits <= 0 || T.count <= 0 ||
+        int64_t(T.rows) * T.cols != T.count) {
+      set_error("%s: bad tensor %d", what, k);
+      return FSK_EINVAL;
+    }
+    b.t[k] = T;
+    b.block_start[k + 1] = b.block_start[k] + cdiv(T.count, 256);
+  }
+  const float bc1 = step > 0 ? float(1.0 - pow(beta1, step)) : 1.f;
+  const float bc2 = step > 0 ? float(1.0 - pow(beta2, step)) : 1.f;
+  hipLaunchKernelGGL(mlp_adam_multi_kernel, dim3(unsigned(b.block_start[n_tensors])), dim3(256), 0,
+                     as_stream(stream), b, float(lr), float(beta1), float(beta2), float(eps), bc1,
+                     bc2, sched);
+  return launch_status(what);
 }
 
 extern "C" int fsk_mlp_cast_bf16(const float *w, int rows, int cols, unsigned short *w_bf16,

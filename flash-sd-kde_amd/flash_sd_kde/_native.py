@@ -77,6 +77,7 @@ SIGNATURES = {
                                   _p, _p, _p],
     "fsk_mlp_adam": [_p, _p, _p, _p, ctypes.c_int, _i64, ctypes.c_int, ctypes.c_int, _d, _d, _d,
                      _d, ctypes.c_int, _p, _p, _p, _p],
+    "fsk_mlp_adam_multi": [_p, ctypes.c_int, _d, _d, _d, _d, ctypes.c_int, _p, _p],
     "fsk_mlp_cast_bf16": [_p, ctypes.c_int, ctypes.c_int, _p, _p, _p],
     "fsk_mlp_mse": [_p, _p, _i64, _d, _p, _p, _p],
     "fsk_loo_workspace_bytes": [_p, ctypes.c_int, ctypes.c_int],
@@ -93,6 +94,16 @@ _RESTYPES = {
     "fsk_ref64_workspace_bytes": _sz,
     "fsk_loo_workspace_bytes": _sz,
 }
+
+class AdamTensor(ctypes.Structure):
+    """fsk_adam_tensor of include/flashsdkde.h."""
+    _fields_ = [("w", ctypes.c_void_p), ("m", ctypes.c_void_p), ("v", ctypes.c_void_p),
+                ("grad_part", ctypes.c_void_p), ("w_bf16", ctypes.c_void_p),
+                ("wt_bf16", ctypes.c_void_p), ("count", ctypes.c_int64), ("rows", ctypes.c_int),
+                ("cols", ctypes.c_int), ("splits", ctypes.c_int), ("reserved", ctypes.c_int)]
+
+
+ADAM_MAX_TENSORS = 16
 
 _lib: Optional[ctypes.CDLL] = None
 _lock = threading.Lock()

@@ -22,6 +22,7 @@ back to the exact score kernel for all points.
 
 from __future__ import annotations
 
+import ctypes
 import math
 import time
 
@@ -123,23 +124,27 @@ class ScoreMLP:
 
     def adam_step(self, grads, lr: float, beta1=0.9, beta2=0.999, eps=1e-8,
                   sched: torch.Tensor | None = None):
-        """One Adam update. With `sched` (device [lr, 1-b1^t, 1-b2^t]) the
-        learning rate and bias corrections are read on the device, so the
-        call can live inside a captured graph."""
+        """One Adam update of every weight and bias in a single launch. With
+        `sched` (device [lr, 1-b1^t, 1-b2^t]) the learning rate and bias
+        corrections are read on the device, so the call can live inside a
+        captured graph."""
         self.step_count += 1
         step = 0 if sched is not None else self.step_count
+        arr = (nat.AdamTensor * (2 * len(self.layers)))()
+        k = 0
         for L, (dWp, dbp) in zip(self.layers, grads):
-            S = dWp.shape[0]
-            st = nat.lib().fsk_mlp_adam(L["W"].data_ptr(), L["mW"].data_ptr(), L["vW"].data_ptr(),
-                                        dWp.data_ptr(), S, L["W"].numel(), L["N"], L["K"], lr,
-                                        beta1, beta2, eps, step, L["Wb"].data_ptr(),
-                                        L["Wt"].data_ptr(), nat.ptr(sched), self._stream())
-            nat.check(st, "fsk_mlp_adam")
-            st = nat.lib().fsk_mlp_adam(L["b"].data_ptr(), L["mb"].data_ptr(), L["vb"].data_ptr(),
-                                        dbp.data_ptr(), S, L["b"].numel(), 1, L["N"], lr, beta1,
-                                        beta2, eps, step, None, None, nat.ptr(sched),
-                                        self._stream())
-            nat.check(st, "fsk_mlp_adam")
+            arr[k] = nat.AdamTensor(L["W"].data_ptr(), L["mW"].data_ptr(), L["vW"].data_ptr(),
+                                    dWp.data_ptr(), L["Wb"].data_ptr(), L["Wt"].data_ptr(),
+                                    L["W"].numel(), L["N"], L["K"], dWp.shape[0], 0)
+            arr[k + 1] = nat.AdamTensor(L["b"].data_ptr(), L["mb"].data_ptr(), L["vb"].data_ptr(),
+                                        dbp.data_ptr(), None, None, L["b"].numel(), 1, L["N"],
+                                        dbp.shape[0], 0)
+            k += 2
+        for a in range(0, k, nat.ADAM_MAX_TENSORS):
+            cnt = min(nat.ADAM_MAX_TENSORS, k - a)
+            st = nat.lib().fsk_mlp_adam_multi(ctypes.addressof(arr[a]), cnt, lr, beta1, beta2,
+                                              eps, step, nat.ptr(sched), self._stream())
+            nat.check(st, "fsk_mlp_adam_multi")
 
     def mse(self, y: torch.Tensor, t: torch.Tensor, loss_acc: torch.Tensor | None):
         dy = torch.empty_like(y)

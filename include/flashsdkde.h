@@ -248,6 +248,19 @@ int fsk_mlp_linear_bwd_weight(const float *dz, int64_t M, int N, const float *x,
 int fsk_mlp_adam(float *w, float *m, float *v, const float *grad_part, int splits, int64_t count,
                  int rows, int cols, double lr, double beta1, double beta2, double eps, int step,
                  uint16_t *w_bf16, uint16_t *wt_bf16, const float *sched, void *stream);
+/* One tensor of a batched Adam update (fsk_mlp_adam_multi). */
+typedef struct {
+  float *w, *m, *v;           /* master weights and moments, rows x cols */
+  const float *grad_part;     /* splits partials of count values each */
+  uint16_t *w_bf16, *wt_bf16; /* nullable BF16 copies (w, transposed w) */
+  int64_t count;
+  int rows, cols, splits, reserved;
+} fsk_adam_tensor;
+#define FSK_ADAM_MAX_TENSORS 16
+/* The Adam update of fsk_mlp_adam for up to FSK_ADAM_MAX_TENSORS tensors in
+ * one launch (the host array is read at the call; capturable). */
+int fsk_mlp_adam_multi(const fsk_adam_tensor *tensors, int n_tensors, double lr, double beta1,
+                       double beta2, double eps, int step, const float *sched, void *stream);
 /* BF16 copies of w (rows x cols) and of its transpose. */
 int fsk_mlp_cast_bf16(const float *w, int rows, int cols, uint16_t *w_bf16, uint16_t *wt_bf16,
                       void *stream);

@@ -80,6 +80,33 @@ def test_mlp_adam_matches_oracle(gpu):
     assert torch.equal(Wt, Wb.T)
 
 
+def test_adam_multi_equals_per_tensor_updates(gpu):
+    """The one-launch optimizer step updates every tensor exactly as the
+    per-tensor entry point does (same arithmetic, bitwise)."""
+    from flash_sd_kde import _native as nat
+
+    a = ScoreMLP(seed=6, device=gpu)
+    b = ScoreMLP(seed=6, device=gpu)
+    g = torch.Generator(device=gpu).manual_seed(3)
+    grads = [(torch.randn(3, L["N"], L["K"], device=gpu, generator=g),
+              torch.randn(3, L["N"], device=gpu, generator=g)) for L in a.layers]
+    for step in (1, 2):
+        a.adam_step(grads, 1e-2)
+        for L, (dW, db) in zip(b.layers, grads):
+            for t, mm, vv, part, wb, wt, rows, cols in (
+                    (L["W"], L["mW"], L["vW"], dW, L["Wb"], L["Wt"], L["N"], L["K"]),
+                    (L["b"], L["mb"], L["vb"], db, None, None, 1, L["N"])):
+                st = nat.lib().fsk_mlp_adam(t.data_ptr(), mm.data_ptr(), vv.data_ptr(),
+                                            part.data_ptr(), part.shape[0], t.numel(), rows, cols,
+                                            1e-2, 0.9, 0.999, 1e-8, step, nat.ptr(wb), nat.ptr(wt),
+                                            None, None)
+                nat.check(st, "fsk_mlp_adam")
+        b.step_count += 1
+    for La, Lb in zip(a.layers, b.layers):
+        for key in ("W", "b", "mW", "vW", "Wb", "Wt"):
+            assert torch.equal(La[key], Lb[key]), key
+
+
 def test_mlp_trains_and_checkpoints(gpu, tmp_path):
     g = torch.Generator(device=gpu).manual_seed(0)
     x = torch.randn(20000, 16, device=gpu, generator=g)
