@@ -45,14 +45,19 @@ __device__ __forceinline__ u16x8 bf16x8_from(const float *p) {
   return r;
 }
 
-// Workgroup tile 64 x 64, wave tile 32 x 32 (2 x 2 MFMA blocks), K step 32.
+// Workgroup tile 64 rows x 128 columns, 4 waves of 32 x 64 (2 x 4 MFMA
+// blocks), K step 32. Each K step's operands are loaded once per workgroup
+// with coalesced 16-byte loads (X as FP32, rounded to BF16 on the way into
+// LDS; Wt as BF16) into one of two LDS buffers, while the MFMAs read the
+// other: the next step's global loads are issued before the current step's
+// LDS reads and MFMAs and only waited on when stored. Rows >= M and columns
+// >= N load clamped (in-bounds) rows and feed outputs that are not stored;
+// K is a multiple of 8 and K slots past K are zero.
 // ACT 0: Y = Z; 1: Y = silu(Z), Z optionally stored; 2: Y = Z * silu'(Zin)
 // with Zin read from the Z pointer (row stride N).
-// Loads are unconditional with clamped row / column indices (rows >= M and
-// columns >= N only feed outputs that are not stored; K is a multiple of 8,
-// so a lane's 8 K values are all valid or all past K and zeroed by a select),
-// and the next K step's operands are loaded before the current step's MFMAs:
-// no branch or wait sits between a load and the work that hides it.
+constexpr int GM_BM = 64, GM_BN = 128, GM_BK = 32;
+constexpr int GM_LDS = GM_BK + 8;  // u16 per LDS row (80 B: 16-byte aligned, staggered banks)
+
 template <int ACT, bool STORE_Z>
 __global__ __launch_bounds__(256) void mlp_gemm_kernel(const float *__restrict__ X, int64_t ldx,
                                                        int64_t M, int K,
@@ -60,90 +65,95 @@ __global__ __launch_bounds__(256) void mlp_gemm_kernel(const float *__restrict__
                                                        const float *__restrict__ bias, int N,
                                                        float *__restrict__ Z,
                                                        float *__restrict__ Y, int64_t ldy) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  __shared__ __align__(16) unsigned short sA[2][GM_BM * GM_LDS];
+  __shared__ __align__(16) unsigned short sB[2][GM_BN * GM_LDS];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int g = lane >> 4, c = lane & 15;
-  const int64_t m0 = int64_t(blockIdx.x) * 64 + (wave & 1) * 32;
-  const int n0 = blockIdx.y * 64 + (wave >> 1) * 32;
-  const float *xr[2];
-  const unsigned short *wr[2];
+  const int64_t bm = int64_t(blockIdx.x) * GM_BM;
+  const int bn = blockIdx.y * GM_BN;
+  // global -> LDS mapping: A 64 rows x 8 float4 (2 per thread), B 128 rows x 4 u16x8 (2 per thread)
+  const int ar = tid >> 3, ac = (tid & 7) * 4;  // A: rows ar and ar + 32, k offset ac
+  const int br = tid >> 2, bc = (tid & 3) * 8;  // B: rows br and br + 64, k offset bc
+  const float *xa[2];
+  const unsigned short *wb[2];
 #pragma unroll
-  for (int rb = 0; rb < 2; ++rb) {
-    const int64_t row = m0 + 16 * rb + c;
-    xr[rb] = X + (row < M ? row : M - 1) * ldx;
+  for (int i = 0; i < 2; ++i) {
+    const int64_t row = bm + ar + 32 * i;
+    xa[i] = X + (row < M ? row : M - 1) * ldx;
+    const int n = bn + br + 64 * i;
+    wb[i] = Wt + int64_t(n < N ? n : N - 1) * K;
   }
-#pragma unroll
-  for (int cb = 0; cb < 2; ++cb) {
-    const int n = n0 + 16 * cb + c;
-    wr[cb] = Wt + int64_t(n < N ? n : N - 1) * K;
-  }
-  f4 acc[2][2];
-#pragma unroll
-  for (int a = 0; a < 2; ++a)
-#pragma unroll
-    for (int b = 0; b < 2; ++b) acc[a][b] = f4{0.f, 0.f, 0.f, 0.f};
-  // raw operands of one K step: A as FP32 (converted to BF16 when consumed,
-  // so a prefetched load is not waited on before the current MFMAs), B as BF16
   struct Raw {
-    f4 a[2][2];
+    f4 a[2];
     u16x8 b[2];
   };
-  auto load = [&](int k0, Raw &r) {
-    const int kc = (k0 + 8 * g < K) ? k0 + 8 * g : 0;
-#pragma unroll
-    for (int rb = 0; rb < 2; ++rb) {
-      r.a[rb][0] = ld4(xr[rb] + kc);
-      r.a[rb][1] = ld4(xr[rb] + kc + 4);
-    }
-#pragma unroll
-    for (int cb = 0; cb < 2; ++cb) r.b[cb] = *reinterpret_cast<const u16x8 *>(wr[cb] + kc);
-  };
   const u16x8 zero8 = {0, 0, 0, 0, 0, 0, 0, 0};
-  auto consume = [&](int k0, const Raw &r) {
-    const bool kok = k0 + 8 * g < K;
-    u16x8 A[2], B[2];
+  auto load = [&](int k0, Raw &r) {
+    const bool aok = k0 + ac < K, bok = k0 + bc < K;
 #pragma unroll
-    for (int rb = 0; rb < 2; ++rb) {
-      u16x8 v;
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        v[e] = to16<FSK_PREC_BF16>(r.a[rb][0][e]);
-        v[4 + e] = to16<FSK_PREC_BF16>(r.a[rb][1][e]);
-      }
-      A[rb] = kok ? v : zero8;
+    for (int i = 0; i < 2; ++i) {
+      r.a[i] = ld4(xa[i] + (aok ? k0 + ac : 0));
+      r.b[i] = *reinterpret_cast<const u16x8 *>(wb[i] + (bok ? k0 + bc : 0));
+      if (!aok) r.a[i] = f4{0.f, 0.f, 0.f, 0.f};
+      if (!bok) r.b[i] = zero8;
     }
-#pragma unroll
-    for (int cb = 0; cb < 2; ++cb) B[cb] = kok ? r.b[cb] : zero8;
-#pragma unroll
-    for (int rb = 0; rb < 2; ++rb)
-#pragma unroll
-      for (int cb = 0; cb < 2; ++cb)
-        acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-            __builtin_bit_cast(bf16x8, A[rb]), __builtin_bit_cast(bf16x8, B[cb]), acc[rb][cb], 0,
-            0, 0);
   };
-  // two register buffers used in turn (no copies between them, so a
-  // prefetched load is only waited on where it is consumed)
-  Raw r0, r1;
-  load(0, r0);
-  for (int k0 = 0; k0 < K; k0 += 64) {
-    if (k0 + 32 < K) load(k0 + 32, r1);
-    consume(k0, r0);
-    if (k0 + 32 >= K) break;
-    if (k0 + 64 < K) load(k0 + 64, r0);
-    consume(k0 + 32, r1);
+  auto store = [&](int buf, const Raw &r) {
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      u16x4 w;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) w[e] = to16<FSK_PREC_BF16>(r.a[i][e]);
+      *reinterpret_cast<u16x4 *>(sA[buf] + (ar + 32 * i) * GM_LDS + ac) = w;
+      *reinterpret_cast<u16x8 *>(sB[buf] + (br + 64 * i) * GM_LDS + bc) = r.b[i];
+    }
+  };
+  const int wr = (wave & 1) * 32, wc = (wave >> 1) * 64;  // this wave's 32 x 64 sub-tile
+  f4 acc[2][4];
+#pragma unroll
+  for (int x = 0; x < 2; ++x)
+#pragma unroll
+    for (int y = 0; y < 4; ++y) acc[x][y] = f4{0.f, 0.f, 0.f, 0.f};
+  Raw r;
+  load(0, r);
+  store(0, r);
+  __syncthreads();
+  int buf = 0;
+  for (int k0 = 0; k0 < K; k0 += GM_BK) {
+    const bool more = k0 + GM_BK < K;  // uniform
+    if (more) load(k0 + GM_BK, r);
+    u16x8 A[2], B[4];
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+      A[x] = *reinterpret_cast<const u16x8 *>(sA[buf] + (wr + 16 * x + c) * GM_LDS + 8 * g);
+#pragma unroll
+    for (int y = 0; y < 4; ++y)
+      B[y] = *reinterpret_cast<const u16x8 *>(sB[buf] + (wc + 16 * y + c) * GM_LDS + 8 * g);
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+      for (int y = 0; y < 4; ++y)
+        acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+            __builtin_bit_cast(bf16x8, A[x]), __builtin_bit_cast(bf16x8, B[y]), acc[x][y], 0, 0,
+            0);
+    if (more) store(buf ^ 1, r);
+    __syncthreads();
+    buf ^= 1;
   }
-  const bool full = (m0 + 32 <= M) && (n0 + 32 <= N);  // wave-uniform
+  const int64_t m0 = bm + wr;
+  const int n0 = bn + wc;
+  const bool full = (m0 + 32 <= M) && (n0 + 64 <= N);  // wave-uniform
 #pragma unroll
-  for (int rb = 0; rb < 2; ++rb)
+  for (int x = 0; x < 2; ++x)
 #pragma unroll
-    for (int cb = 0; cb < 2; ++cb) {
-      const int n = n0 + 16 * cb + c;
+    for (int y = 0; y < 4; ++y) {
+      const int n = n0 + 16 * y + c;
       const float bv = (bias && n < N) ? bias[n] : 0.f;
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int64_t row = m0 + 16 * rb + 4 * g + r;
+      for (int rr = 0; rr < 4; ++rr) {
+        const int64_t row = m0 + 16 * x + 4 * g + rr;
         if (full || (row < M && n < N)) {
-          const float z = acc[rb][cb][r] + bv;
+          const float z = acc[x][y][rr] + bv;
           if (ACT == 2) {
             Y[row * ldy + n] = z * silu_grad(Z[row * N + n]);
           } else {
@@ -334,6 +344,37 @@ __global__ void mlp_adam_multi_kernel(AdamBatch batch, float lr, float b1, float
   if (T.wt_bf16) T.wt_bf16[(e % T.cols) * T.rows + e / T.cols] = to16<FSK_PREC_BF16>(nw);
 }
 
+// Minibatch of training step c = min(*counter, steps - 1): rows idx[c][r] of
+// x (n x dx) and t (n x dt) into xb, tb, and the step's schedule entry
+// table[c] (3 floats) into sched. One thread per (row, float4 of x or t).
+__global__ void mlp_gather_kernel(const float *__restrict__ x, const float *__restrict__ t, int dx,
+                                  int dt, const int *__restrict__ idx, int64_t batch,
+                                  const float *__restrict__ table, int64_t steps,
+                                  const int64_t *__restrict__ counter, float *__restrict__ xb,
+                                  float *__restrict__ tb, float *__restrict__ sched) {
+  int64_t c = *counter;
+  c = c < steps ? c : steps - 1;
+  const int per = (dx + dt) / 4;
+  const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e == 0) {
+    sched[0] = table[c * 3];
+    sched[1] = table[c * 3 + 1];
+    sched[2] = table[c * 3 + 2];
+  }
+  if (e >= batch * per) return;
+  const int64_t r = e / per;
+  const int p = int(e % per) * 4;
+  const int64_t j = idx[c * batch + r];
+  if (p < dx)
+    *reinterpret_cast<f4 *>(xb + r * dx + p) = ld4(x + j * dx + p);
+  else
+    *reinterpret_cast<f4 *>(tb + r * dt + (p - dx)) = ld4(t + j * dt + (p - dx));
+}
+
+__global__ void mlp_step_advance_kernel(int64_t *counter) {
+  if (threadIdx.x == 0) *counter += 1;
+}
+
 __global__ void mlp_cast_kernel(const float *__restrict__ w, int64_t count, int rows, int cols,
                                 unsigned short *__restrict__ wb, unsigned short *__restrict__ wtb) {
   const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -388,7 +429,7 @@ extern "C" int fsk_mlp_linear_fwd(const float *x, int64_t ldx, int64_t M, int K,
     return FSK_EINVAL;
   }
   hipStream_t s = as_stream(stream);
-  dim3 grid(unsigned(cdiv(M, 64)), unsigned(cdiv(N, 64)));
+  dim3 grid(unsigned(cdiv(M, GM_BM)), unsigned(cdiv(N, GM_BN)));
   if (act == 2) {
     hipLaunchKernelGGL((mlp_gemm_kernel<2, false>), grid, dim3(256), 0, s, x, ldx, M, K, wt_bf16,
                        bias, N, z, y, ldy);
@@ -472,6 +513,34 @@ extern "C" int fsk_mlp_adam_multi(const fsk_adam_tensor *tensors, int n_tensors,
                      as_stream(stream), b, float(lr), float(beta1), float(beta2), float(eps), bc1,
                      bc2, sched);
   return launch_status(what);
+}
+
+extern "C" int fsk_mlp_gather_batch(const float *x, const float *t, int64_t n, int dx, int dt,
+                                    const int *idx, int64_t batch, const float *table,
+                                    int64_t steps, const int64_t *counter, float *xb, float *tb,
+                                    float *sched, void *stream) {
+  const char *what = "fsk_mlp_gather_batch";
+  if (!x || !t || !idx || !table || !counter || !xb || !tb || !sched || n <= 0 || batch <= 0 ||
+      steps <= 0 || dx <= 0 || dt <= 0 || (dx & 3) || (dt & 3) ||
+      (reinterpret_cast<uintptr_t>(x) & 15) || (reinterpret_cast<uintptr_t>(t) & 15) ||
+      (reinterpret_cast<uintptr_t>(xb) & 15) || (reinterpret_cast<uintptr_t>(tb) & 15)) {
+    set_error("%s: bad arguments (widths multiples of 4, 16-byte aligned rows)", what);
+    return FSK_EINVAL;
+  }
+  const int64_t threads = batch * ((dx + dt) / 4);
+  hipLaunchKernelGGL(mlp_gather_kernel, dim3(unsigned(cdiv(threads, 256))), dim3(256), 0,
+                     as_stream(stream), x, t, dx, dt, idx, batch, table, steps, counter, xb, tb,
+                     sched);
+  return launch_status(what);
+}
+
+extern "C" int fsk_mlp_step_advance(int64_t *counter, void *stream) {
+  if (!counter) {
+    set_error("fsk_mlp_step_advance: null counter");
+    return FSK_EINVAL;
+  }
+  hipLaunchKernelGGL(mlp_step_advance_kernel, dim3(1), dim3(64), 0, as_stream(stream), counter);
+  return launch_status("fsk_mlp_step_advance");
 }
 
 extern "C" int fsk_mlp_cast_bf16(const float *w, int rows, int cols, unsigned short *w_bf16,

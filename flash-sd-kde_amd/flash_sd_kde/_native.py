@@ -78,6 +78,9 @@ SIGNATURES = {
     "fsk_mlp_adam": [_p, _p, _p, _p, ctypes.c_int, _i64, ctypes.c_int, ctypes.c_int, _d, _d, _d,
                      _d, ctypes.c_int, _p, _p, _p, _p],
     "fsk_mlp_adam_multi": [_p, ctypes.c_int, _d, _d, _d, _d, ctypes.c_int, _p, _p],
+    "fsk_mlp_gather_batch": [_p, _p, _i64, ctypes.c_int, ctypes.c_int, _p, _i64, _p, _i64, _p,
+                             _p, _p, _p, _p],
+    "fsk_mlp_step_advance": [_p, _p],
     "fsk_mlp_cast_bf16": [_p, ctypes.c_int, ctypes.c_int, _p, _p, _p],
     "fsk_mlp_mse": [_p, _p, _i64, _d, _p, _p, _p],
     "fsk_loo_workspace_bytes": [_p, ctypes.c_int, ctypes.c_int],

@@ -94,20 +94,30 @@ class ScoreMLP:
             h = y
         return h
 
-    def backward(self, dy: torch.Tensor, cache: list, splits: int):
+    @staticmethod
+    def auto_splits(M: int, N: int, K: int, target_groups: int = 512) -> int:
+        """Batch splits of a weight-gradient launch: enough (64 x 64 tile,
+        split) workgroups to fill the GPU (~512), at most one split per 64
+        batch rows."""
+        tiles = -(-N // 64) * -(-K // 64)
+        return max(1, min(-(-target_groups // tiles), max(1, M // 64)))
+
+    def backward(self, dy: torch.Tensor, cache: list, splits: int | None = None):
         """Gradient partials per layer: list of (dW_part, db_part). The data
         gradient GEMM of layer l also applies silu'(z_{l-1}), so it hands the
-        previous layer its dZ directly."""
+        previous layer its dZ directly. `splits` fixes the batch split count of
+        every layer; None picks it per layer (auto_splits)."""
         M = dy.shape[0]
         grads = [None] * len(self.layers)
         dz = dy.contiguous()
         for l in range(len(self.layers) - 1, -1, -1):
             L = self.layers[l]
             h, _ = cache[l]
-            dWp = torch.empty((splits, L["N"], L["K"]), dtype=torch.float32, device=self.device)
-            dbp = torch.empty((splits, L["N"]), dtype=torch.float32, device=self.device)
+            S = splits or self.auto_splits(M, L["N"], L["K"])
+            dWp = torch.empty((S, L["N"], L["K"]), dtype=torch.float32, device=self.device)
+            dbp = torch.empty((S, L["N"]), dtype=torch.float32, device=self.device)
             st = nat.lib().fsk_mlp_linear_bwd_weight(dz.data_ptr(), M, L["N"], h.data_ptr(),
-                                                     h.stride(0), L["K"], splits, dWp.data_ptr(),
+                                                     h.stride(0), L["K"], S, dWp.data_ptr(),
                                                      dbp.data_ptr(), self._stream())
             nat.check(st, "fsk_mlp_linear_bwd_weight")
             grads[l] = (dWp, dbp)
@@ -162,9 +172,10 @@ class ScoreMLP:
         targets by their overall scale; cosine learning-rate decay; minibatches
         drawn uniformly with replacement; `warmup` steps of linear learning-rate
         ramp before the decay. With graph=True one training step
-        (gather, forward, loss, backward, Adam) is captured once as a HIP graph
-        and replayed; only the batch indices and the schedule entry change
-        between replays (both written on the device)."""
+        (gather, forward, loss, backward, Adam, step counter) is captured once
+        as a HIP graph and replayed: the step's batch indices and schedule
+        entry are selected on the device by a step counter, so nothing is
+        launched from the host between replays."""
         x = x.float().contiguous()
         target = target.float().contiguous()
         self.x_mean = x.mean(0)
@@ -174,7 +185,7 @@ class ScoreMLP:
         tn = (target / self.y_scale).contiguous()
         n = xn.shape[0]
         batch = min(batch, n)
-        splits = splits or max(1, batch // 1024)
+        # splits: None = per-layer choice (auto_splits)
         dev = self.device
         gen = torch.Generator(device=dev)
         gen.manual_seed(seed)
@@ -185,23 +196,32 @@ class ScoreMLP:
                              1.0 - beta1 ** (k + t0), 1.0 - beta2 ** (k + t0)], dim=1)
         table = table.to(torch.float32).to(dev)
         sched = torch.empty(3, dtype=torch.float32, device=dev)
-        idx = torch.empty(batch, dtype=torch.int64, device=dev)
+        # every step's batch indices up front (int32, steps x batch) and a device
+        # step counter: the gather reads both, so one captured step replays as is
+        idx = torch.randint(0, n, (steps, batch), device=dev, generator=gen, dtype=torch.int32)
+        counter = torch.zeros(1, dtype=torch.int64, device=dev)
+        xb = torch.empty((batch, xn.shape[1]), dtype=torch.float32, device=dev)
+        tb = torch.empty((batch, tn.shape[1]), dtype=torch.float32, device=dev)
         loss_acc = torch.zeros(1, dtype=torch.float64, device=dev)
+        lib = nat.lib()
 
         def train_step():
-            xb = xn.index_select(0, idx)
-            tb = tn.index_select(0, idx)
+            st = lib.fsk_mlp_gather_batch(xn.data_ptr(), tn.data_ptr(), n, xn.shape[1],
+                                          tn.shape[1], idx.data_ptr(), batch, table.data_ptr(),
+                                          steps, counter.data_ptr(), xb.data_ptr(), tb.data_ptr(),
+                                          sched.data_ptr(), self._stream())
+            nat.check(st, "fsk_mlp_gather_batch")
             cache = []
             y = self.forward(xb, cache)
             dy = self.mse(y, tb, loss_acc)
             grads = self.backward(dy, cache, splits)
             self.adam_step(grads, lr, beta1, beta2, sched=sched)
+            nat.check(lib.fsk_mlp_step_advance(counter.data_ptr(), self._stream()),
+                      "fsk_mlp_step_advance")
 
         g = None
         history = []
         for s in range(steps):
-            torch.randint(0, n, (batch,), device=dev, generator=gen, out=idx)
-            sched.copy_(table[s])
             if not graph or s < 2:
                 train_step()  # eager (also the warm-up before capture)
             else:
@@ -218,7 +238,8 @@ class ScoreMLP:
                 g.replay()
                 self.step_count += 1
             if (s + 1) % log_every == 0 or s == steps - 1:
-                history.append(float(loss_acc.item()) / (log_every * batch * tn.shape[1]))
+                done = (s % log_every) + 1
+                history.append(float(loss_acc.item()) / (done * batch * tn.shape[1]))
                 loss_acc.zero_()
         torch.cuda.synchronize(dev)
         return history

@@ -261,6 +261,16 @@ typedef struct {
  * one launch (the host array is read at the call; capturable). */
 int fsk_mlp_adam_multi(const fsk_adam_tensor *tensors, int n_tensors, double lr, double beta1,
                        double beta2, double eps, int step, const float *sched, void *stream);
+/* Training-step inputs read on the device, so a captured step replays
+ * unchanged: with c = min(*counter, steps - 1), xb[r] = x[idx[c*batch + r]],
+ * tb[r] = t[idx[c*batch + r]] (rows of dx and dt floats, multiples of 4,
+ * 16-byte aligned) and sched[0..2] = table[3c .. 3c+2]; idx holds values in
+ * [0, n). fsk_mlp_step_advance adds 1 to *counter (call it last in a step). */
+int fsk_mlp_gather_batch(const float *x, const float *t, int64_t n, int dx, int dt,
+                         const int32_t *idx, int64_t batch, const float *table, int64_t steps,
+                         const int64_t *counter, float *xb, float *tb, float *sched,
+                         void *stream);
+int fsk_mlp_step_advance(int64_t *counter, void *stream);
 /* BF16 copies of w (rows x cols) and of its transpose. */
 int fsk_mlp_cast_bf16(const float *w, int rows, int cols, uint16_t *w_bf16, uint16_t *wt_bf16,
                       void *stream);
