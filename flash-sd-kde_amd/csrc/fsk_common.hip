@@ -177,10 +177,15 @@ extern "C" size_t fsk_workspace_bytes(int op, int64_t n, int64_t m, unsigned fla
     case FSK_OP_KDE1_ATOMIC: return m > 0 ? kde1_ws_bytes(n, m, false, false) : 0;
     case FSK_OP_KDE1_SCORE: return m > 0 ? kde1_ws_bytes(n, m, true, true) : 0;
     case FSK_OP_SCORE16_SYM: return score16_ws_bytes(n, false);
-    case FSK_OP_SCORE16_ROWS: return m > 0 ? score16_rows_ws_bytes(n, m) : 0;
+    case FSK_OP_SCORE16_ROWS:
+      if (m <= 0) return 0;
+      return (flags & FSK_PREC_MASK) == FSK_PREC_F32
+                 ? score16_rows_ws_bytes(n, m)
+                 : score16_lp_ws_bytes(n, m, flags & FSK_PREC_MASK);
     case FSK_OP_SCORE16_ORDERED:
-      return (flags & FSK_PREC_MASK) == FSK_PREC_F32 ? score16_ws_bytes(n, true)
-                                                     : score16_lp_ws_bytes(n, flags & FSK_PREC_MASK);
+      return (flags & FSK_PREC_MASK) == FSK_PREC_F32
+                 ? score16_ws_bytes(n, true)
+                 : score16_lp_ws_bytes(n, n, flags & FSK_PREC_MASK);
     default: return 0;
   }
 }

@@ -74,10 +74,11 @@ inline SplitPlan score16_ordered_plan(int64_t n) {
   return make_split_plan(n, n, kScoreRowsPerWG, 16, 2048, cap);
 }
 
-inline SplitPlan score16_lp_plan(int64_t n, unsigned prec) {
-  int64_t cap = kOrderedPartialCap / (n * 17 * 4 > 0 ? n * 17 * 4 : 1);
+// 16-bit score of m rows against n data points (m = n for the data set itself).
+inline SplitPlan score16_lp_plan(int64_t n, int64_t m, unsigned prec) {
+  int64_t cap = kOrderedPartialCap / (m * 17 * 4 > 0 ? m * 17 * 4 : 1);
   if (cap < 1) cap = 1;
-  return make_split_plan(n, n, 4 * 16 * score_lp_qt(prec), 16, 2048, cap);
+  return make_split_plan(n, m, 4 * 16 * score_lp_qt(prec), 16, 2048, cap);
 }
 
 inline SplitPlan score16_rows_plan(int64_t n, int64_t m) {

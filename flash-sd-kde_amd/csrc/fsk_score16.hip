@@ -345,8 +345,8 @@ extern "C" int fsk_score16_ordered(const float *data, int64_t ld, int64_t n, dou
   int st = score_check(what, data, ld, n, bandwidth, flags, pdf, w, true);
   if (st != FSK_OK) return st;
   if ((flags & FSK_PREC_MASK) != FSK_PREC_F32)
-    return score16_lp_run(data, ld, n, bandwidth, flags & FSK_PREC_MASK, ws, ws_bytes, pdf, w,
-                          ldw_row, ldw_col, stream);
+    return score16_lp_run(what, data, ld, n, data, ld, n, bandwidth, flags & FSK_PREC_MASK, ws,
+                          ws_bytes, pdf, w, ldw_row, ldw_col, stream);
   ScoreLayout L;
   size_t need = score_layout(ws, n, true, &L);
   if (!ws || ws_bytes < need) {
@@ -417,9 +417,12 @@ extern "C" int fsk_score16_rows(const float *rows, int64_t ldr, int64_t m, const
                                 size_t ws_bytes, float *pdf, float *w, int64_t ldw_row,
                                 int64_t ldw_col, void *stream) {
   const char *what = "fsk_score16_rows";
-  int st = score_check(what, data, ld, n, bandwidth, flags, pdf, w, false);
+  int st = score_check(what, data, ld, n, bandwidth, flags, pdf, w, true);
   if (st != FSK_OK) return st;
   if ((st = check_points16(what, rows, ldr, m)) != FSK_OK) return st;
+  if ((flags & FSK_PREC_MASK) != FSK_PREC_F32)
+    return score16_lp_run(what, rows, ldr, m, data, ld, n, bandwidth, flags & FSK_PREC_MASK, ws,
+                          ws_bytes, pdf, w, ldw_row, ldw_col, stream);
   ScoreLayout L;
   size_t need = score_layout(ws, n, true, &L, m);
   if (!ws || ws_bytes < need) {

@@ -93,7 +93,7 @@ __device__ __forceinline__ u16x8 phi_operand(const float (&e)[4]) {
 template <int PREC, int QT>
 __global__ __launch_bounds__(kWG) void score16_lp_kernel(
     const unsigned short *__restrict__ xrec, const unsigned short *__restrict__ qrec,
-    const unsigned short *__restrict__ xt, int64_t n, int64_t n_pad16, int64_t split_len,
+    const unsigned short *__restrict__ xt, int64_t m, int64_t n_pad16, int64_t split_len,
     int64_t qblocks, int64_t ntasks, int64_t per_xcd, float *__restrict__ pdf_part,
     float *__restrict__ w_part) {
   constexpr int P16 = CrossType<PREC>::kP16;  // cross-term operands
@@ -203,16 +203,16 @@ __global__ __launch_bounds__(kWG) void score16_lp_kernel(
   for (int R = 0; R < QT; ++R) {
     double p = sum_groups(pdf64[R] + pdf[R]);
     int64_t i = row0 + 16 * R + c;
-    if (g == 0 && i < n) pdf_part[split * n + i] = float(p);
+    if (g == 0 && i < m) pdf_part[split * m + i] = float(p);
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       int64_t iw = row0 + 16 * R + 4 * g + r;
-      if (iw < n) w_part[(split * n + iw) * kD + c] = float(w64[R][r] + w[R][r]);
+      if (iw < m) w_part[(split * m + iw) * kD + c] = float(w64[R][r] + w[R][r]);
     }
   }
 }
 
-// pdf[i] = sum_s pdf_part; w[i,f] = sum_s w_part + mu_f pdf[i]  (FP64).
+// pdf[i] = sum_s pdf_part; w[i,f] = sum_s w_part + mu_f pdf[i]  (FP64), n rows.
 __global__ void score_lp_finish_kernel(const float *__restrict__ pdf_part,
                                        const float *__restrict__ w_part, int64_t S, int64_t n,
                                        const double *__restrict__ mu_sum, double inv_n,
@@ -237,37 +237,38 @@ struct ScoreLpLayout {
   float *pdf_part, *w_part;
 };
 
-static size_t score_lp_layout(void *ws, int64_t n, unsigned prec, ScoreLpLayout *L) {
+static size_t score_lp_layout(void *ws, int64_t n, int64_t m, unsigned prec, ScoreLpLayout *L) {
   Carve cv(ws);
-  SplitPlan p = score16_lp_plan(n, prec);
-  const int64_t rows = round_up(n, kNormPad);
+  SplitPlan p = score16_lp_plan(n, m, prec);
   L->mu = cv.take<double>(kD);
-  L->xrec = cv.take<unsigned short>(rows * rec_size(prec));
-  L->qrec = cv.take<unsigned short>(rows * rec_size(prec));
+  L->xrec = cv.take<unsigned short>(round_up(n, kNormPad) * rec_size(prec));
+  L->qrec = cv.take<unsigned short>(round_up(m, kNormPad) * rec_size(prec));
   L->xt = cv.take<unsigned short>(round_up(n, kTile) * kD * 2);
-  L->pdf_part = cv.take<float>(p.splits * n);
-  L->w_part = cv.take<float>(p.splits * n * kD);
+  L->pdf_part = cv.take<float>(p.splits * m);
+  L->w_part = cv.take<float>(p.splits * m * kD);
   return cv.used;
 }
 
-size_t score16_lp_ws_bytes(int64_t n, unsigned prec) {
+size_t score16_lp_ws_bytes(int64_t n, int64_t m, unsigned prec) {
   ScoreLpLayout L;
-  return score_lp_layout(nullptr, n, prec, &L);
+  return score_lp_layout(nullptr, n, m, prec, &L);
 }
 
-int score16_lp_run(const float *data, int64_t ld, int64_t n, double h, unsigned prec, void *ws,
+// Score accumulators of m rows against the KDE of n data points with 16-bit
+// MFMA operands (rows == data for the empirical score of the data set).
+int score16_lp_run(const char *what, const float *rows_in, int64_t ldr, int64_t m,
+                   const float *data, int64_t ld, int64_t n, double h, unsigned prec, void *ws,
                    size_t ws_bytes, float *pdf, float *w, int64_t ldw_row, int64_t ldw_col,
                    void *stream) {
-  const char *what = "fsk_score16_ordered";
   ScoreLpLayout L;
-  size_t need = score_lp_layout(ws, n, prec, &L);
+  size_t need = score_lp_layout(ws, n, m, prec, &L);
   if (!ws || ws_bytes < need) {
     set_error("%s: workspace of %zu bytes needed, got %zu", what, need, ws_bytes);
     return FSK_EWORKSPACE;
   }
   hipStream_t s = as_stream(stream);
   const double coef = exp2_coef(h);
-  const int64_t rows = round_up(n, kNormPad);
+  const int64_t xrows = round_up(n, kNormPad), qrows = round_up(m, kNormPad);
   const int64_t n16 = round_up(n, kTile);
   const double inv_n = 1.0 / double(n);
   hipLaunchKernelGGL(fill_f64, dim3(1), dim3(64), 0, s, L.mu, int64_t(kD), 0.0);
@@ -276,23 +277,24 @@ int score16_lp_run(const float *data, int64_t ld, int64_t n, double h, unsigned 
     if (blocks > 1024) blocks = 1024;
     hipLaunchKernelGGL(mean16_kernel, dim3(blocks), dim3(256), 0, s, data, ld, n, L.mu);
   }
-  SplitPlan p = score16_lp_plan(n, prec);
+  SplitPlan p = score16_lp_plan(n, m, prec);
   const int64_t ntasks = p.qblocks * p.splits;
   int64_t per_xcd;
   const int64_t grid = xcd_grid(ntasks, &per_xcd);
-#define FSK_LP(P, P16, QT)                                                                      \
-  do {                                                                                          \
-    hipLaunchKernelGGL(prep_lse_kernel<P>, dim3(cdiv(rows, 256)), dim3(256), 0, s, data, ld, n, \
-                       rows, L.mu, inv_n, coef, 0, (float *)nullptr, (float *)nullptr, L.xrec);  \
-    hipLaunchKernelGGL(prep_lse_kernel<P>, dim3(cdiv(rows, 256)), dim3(256), 0, s, data, ld, n, \
-                       rows, L.mu, inv_n, coef, 1, (float *)nullptr, (float *)nullptr, L.qrec);  \
-    hipLaunchKernelGGL(prep_xt16_kernel<P16>, dim3(cdiv(n16 * kD, 256)), dim3(256), 0, s, data, \
-                       ld, n, n16, L.mu, inv_n, L.xt);                                          \
-    prof_begin(FSK_OP_SCORE16_ORDERED, s);                                                      \
-    hipLaunchKernelGGL((score16_lp_kernel<P, QT>), dim3(grid), dim3(kWG), 0, s, L.xrec,           \
-                       L.qrec, L.xt, n, n16, p.split_len, p.qblocks, ntasks, per_xcd,           \
-                       L.pdf_part, L.w_part);                                                   \
-    prof_end(FSK_OP_SCORE16_ORDERED, s);                                                        \
+#define FSK_LP(P, P16, QT)                                                                       \
+  do {                                                                                           \
+    hipLaunchKernelGGL(prep_lse_kernel<P>, dim3(cdiv(xrows, 256)), dim3(256), 0, s, data, ld, n, \
+                       xrows, L.mu, inv_n, coef, 0, (float *)nullptr, (float *)nullptr, L.xrec);  \
+    hipLaunchKernelGGL(prep_lse_kernel<P>, dim3(cdiv(qrows, 256)), dim3(256), 0, s, rows_in,     \
+                       ldr, m, qrows, L.mu, inv_n, coef, 1, (float *)nullptr, (float *)nullptr,   \
+                       L.qrec);                                                                  \
+    hipLaunchKernelGGL(prep_xt16_kernel<P16>, dim3(cdiv(n16 * kD, 256)), dim3(256), 0, s, data,  \
+                       ld, n, n16, L.mu, inv_n, L.xt);                                           \
+    prof_begin(FSK_OP_SCORE16_ORDERED, s);                                                       \
+    hipLaunchKernelGGL((score16_lp_kernel<P, QT>), dim3(grid), dim3(kWG), 0, s, L.xrec, L.qrec,  \
+                       L.xt, m, n16, p.split_len, p.qblocks, ntasks, per_xcd, L.pdf_part,        \
+                       L.w_part);                                                                \
+    prof_end(FSK_OP_SCORE16_ORDERED, s);                                                         \
   } while (0)
   switch (prec) {
     case FSK_PREC_BF16: FSK_LP(FSK_PREC_BF16, FSK_PREC_BF16, kScoreLpQT); break;
@@ -304,8 +306,8 @@ int score16_lp_run(const float *data, int64_t ld, int64_t n, double h, unsigned 
 #undef FSK_LP
   int st = launch_status(what);
   if (st != FSK_OK) return st;
-  hipLaunchKernelGGL(score_lp_finish_kernel, dim3(cdiv(n * kD, 256)), dim3(256), 0, s,
-                     L.pdf_part, L.w_part, p.splits, n, L.mu, inv_n, pdf, w, ldw_row, ldw_col);
+  hipLaunchKernelGGL(score_lp_finish_kernel, dim3(cdiv(m * kD, 256)), dim3(256), 0, s,
+                     L.pdf_part, L.w_part, p.splits, m, L.mu, inv_n, pdf, w, ldw_row, ldw_col);
   return launch_status(what);
 }
 

@@ -215,19 +215,26 @@ def score16(train: torch.Tensor, bandwidth: float, *, variant: str = "symmetric"
     return pdf, (wbuf.t() if transposed_w else wbuf)
 
 
-def score16_rows(rows: torch.Tensor, train: torch.Tensor, bandwidth: float):
-    """(pdf (m,), weighted (m,16)) of `rows` under the KDE of `train` (FP32 form)."""
+def score16_rows(rows: torch.Tensor, train: torch.Tensor, bandwidth: float, *,
+                 precision: str = "fp32"):
+    """(pdf (m,), weighted (m,16)) of `rows` under the KDE of `train`, in the
+    FP32 form or one of the 16-bit MFMA forms of the ordered score."""
     h = _check_h(bandwidth)
     dev = train.device
     nat.require_gpu(dev, "score16_rows")
     n, m = train.shape[0], rows.shape[0]
     if n == 0 or m == 0:
         raise ValueError("rows and data must be non-empty.")
+    flags = precision_flag(precision)
+    if flags not in (nat.PREC_F32, nat.PREC_BF16, nat.PREC_F16, nat.PREC_BF16X3, nat.PREC_BF16X6,
+                     nat.PREC_F16X3):
+        raise ValueError(f"precision {precision} is not available for the score pass")
+    check_fp16_range(train, rows, h, flags)
     pdf = torch.empty(m, dtype=torch.float32, device=dev)
     w = torch.empty((m, ND), dtype=torch.float32, device=dev)
-    ws, nbytes = nat.workspace(nat.OP_SCORE16_ROWS, n, m, 0, dev)
+    ws, nbytes = nat.workspace(nat.OP_SCORE16_ROWS, n, m, flags, dev)
     st = nat.lib().fsk_score16_rows(rows.data_ptr(), rows.stride(0), m, train.data_ptr(),
-                                    train.stride(0), n, h, 0, ws.data_ptr(), nbytes,
+                                    train.stride(0), n, h, flags, ws.data_ptr(), nbytes,
                                     pdf.data_ptr(), w.data_ptr(), ND, 1, nat.stream_handle(dev))
     nat.check(st, "fsk_score16_rows")
     return pdf, w

@@ -4,8 +4,8 @@ The exact one-step SD-KDE shifts every point by half its mean shift,
     x_i -> x_i + (h^2 / 2) score_i = x_i + delta_i / 2,   delta_i = w_i / pdf_i - x_i,
 which costs n^2 kernel evaluations. Here a small MLP, 16 -> 384 -> 384 ->
 384 -> 16 with SiLU (BF16 weights, FP32 accumulation), learns x -> delta from
-exact labels on a subset (the FP32 score of the subset rows under the KDE of
-ALL points, fsk_score16_rows) and then shifts all n points at O(n) cost.
+exact labels on a subset (the score of the subset rows under the KDE of ALL
+points, fsk_score16_rows) and then shifts all n points at O(n) cost.
 
 Training runs entirely in this library's HIP kernels: forward GEMMs with
 fused bias + SiLU, SiLU backward, split-batch weight gradients, data
@@ -277,10 +277,18 @@ class ScoreMLP:
         return net
 
 
-def exact_shift(rows: torch.Tensor, data: torch.Tensor, h: float, eps: float = 1e-12):
-    """delta = w/(pdf + eps) - x of `rows` under the KDE of `data` (FP32 kernel,
-    FP64 debias): x + delta/2 is the exact one-step SD-KDE point."""
-    pdf, w = _ops.score16_rows(rows, data, h)
+def exact_shift(rows: torch.Tensor, data: torch.Tensor, h: float, eps: float = 1e-12, *,
+                precision: str = "fp32"):
+    """delta = w/(pdf + eps) - x of `rows` under the KDE of `data` (score kernel
+    in `precision`, FP64 debias): x + delta/2 is the exact one-step SD-KDE
+    point. A 16-bit form whose range the inputs exceed (ValueError from the
+    FP16 range check) falls back to the FP32 form."""
+    try:
+        pdf, w = _ops.score16_rows(rows, data, h, precision=precision)
+    except ValueError:
+        if precision == "fp32":
+            raise
+        pdf, w = _ops.score16_rows(rows, data, h)
     deb = _ops.debias(rows, pdf, w, h, eps)
     return 2.0 * (deb - rows)
 
@@ -289,8 +297,13 @@ def surrogate_fit_transform(data, bandwidth: float, *, n_label: int = 262144,
                             n_holdout: int = 16384, n_probe: int = 16384, tol: float = 1e-3,
                             check_margin: float = 0.5, steps: int = 8000, batch: int = 8192,
                             lr: float = 6e-3, seed: int = 0, device="cuda",
-                            net: ScoreMLP | None = None):
+                            net: ScoreMLP | None = None, label_precision: str = "fp16x3"):
     """One-step SD-KDE debiasing with the learned shift field.
+
+    The training labels come from the score kernel in `label_precision`
+    (default fp16x3: FP32-level accuracy, about 1e-6 relative, at twice the
+    FP32 form's rate; FP32 when the inputs leave FP16's range). The held-out
+    check and the fallback use the FP32 form.
 
     Returns (debiased (n,16) tensor, info dict). info["fallback"] is True when
     the held-out log-density error exceeded check_margin * tol and the exact
@@ -310,7 +323,7 @@ def surrogate_fit_transform(data, bandwidth: float, *, n_label: int = 262144,
     probe = x[perm[n_label + n_holdout:n_label + n_holdout + n_probe]].contiguous()
 
     t0 = time.perf_counter()
-    target = exact_shift(lab, x, h)
+    target = exact_shift(lab, x, h, precision=label_precision)
     torch.cuda.synchronize(device)
     info["label_s"] = time.perf_counter() - t0
     t0 = time.perf_counter()

@@ -176,8 +176,10 @@ int fsk_score16_ordered(const float *data, int64_t ld, int64_t n,
 
 /* Empirical-score accumulators of m query rows under the KDE of n data points:
  *   pdf[i] = sum_j exp(-|r_i - x_j|^2/(2h^2)),  w[i,:] = sum_j exp(...) x_j,
- * FP32 form, deterministic split-K with FP64 combine (workspace
- * FSK_OP_SCORE16_ROWS with (n, m)). With rows = data this is
+ * deterministic split-K with FP64 combine (workspace FSK_OP_SCORE16_ROWS with
+ * (n, m, flags)). flags takes the precisions of fsk_score16_ordered: FSK_PREC_F32
+ * or a 16-bit MFMA form (BF16, F16, BF16X3, BF16X6, F16X3; the FP16 forms carry
+ * the range limit stated there, here over rows and data alike). With rows = data this is
  * fsk_score16_ordered; a row subset gives exact score labels for the learned
  * surrogate. The reference evaluates the score at separate points only in 1-D
  * (kde_score_eval, flash_sd_kde/reference.py:83-120). */

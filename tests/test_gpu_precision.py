@@ -69,6 +69,34 @@ def test_score_lp_ragged_and_transposed_layout(gpu, prec, tol):
         np.testing.assert_allclose(w.cpu().numpy(), we, rtol=tol, atol=tol * np.abs(we).max())
 
 
+@pytest.mark.parametrize("prec,tol", [("bf16x3", 1e-3), ("bf16x6", 5e-5), ("fp16x3", 5e-5)])
+@pytest.mark.parametrize("m,n", [(1, 5), (37, 1000), (1000, 4097), (700, 1)])
+def test_score_rows_lp_forms_match_fp64(gpu, prec, tol, m, n):
+    """fsk_score16_rows in the 16-bit forms: rows that are not data points,
+    m != n, ragged sizes on both sides."""
+    rng = np.random.default_rng(m * 11 + n)
+    rows = (rng.normal(size=(m, 16)) + 2.0).astype(np.float32)
+    x = (rng.normal(size=(n, 16)) + 2.0).astype(np.float32)
+    h = 1.05
+    pdf, w = _ops.score16_rows(torch.as_tensor(rows, device=gpu), torch.as_tensor(x, device=gpu),
+                               h, precision=prec)
+    pe, we = fp64.score_rows(rows, x, h)
+    keep = pe > 1e-20  # rows far from every data point have pdf ~ 0 in every form
+    np.testing.assert_allclose(pdf.cpu().numpy()[keep], pe[keep], rtol=tol)
+    np.testing.assert_allclose(w.cpu().numpy(), we, rtol=tol, atol=tol * np.abs(we).max())
+
+
+@pytest.mark.parametrize("prec", ["bf16x3", "fp16x3", "bf16x6"])
+def test_score_rows_on_the_data_equal_ordered(gpu, prec):
+    """rows = data runs the same plan and kernel as the ordered score: the
+    results are bitwise equal."""
+    x = torch.as_tensor(np.random.default_rng(4).normal(size=(3000, 16)).astype(np.float32),
+                        device=gpu)
+    p1, w1 = _ops.score16(x, 0.9, variant="ordered", precision=prec)
+    p2, w2 = _ops.score16_rows(x, x, 0.9, precision=prec)
+    assert torch.equal(p1, p2) and torch.equal(w1, w2)
+
+
 @pytest.mark.parametrize("prec,tol", [("fp32", 1e-5), ("bf16x6", 1e-5), ("fp16x3", 1e-5),
                                       ("bf16x3", 1e-4),
                                       ("fp16", 1e-3),
