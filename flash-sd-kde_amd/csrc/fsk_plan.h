@@ -23,7 +23,10 @@ constexpr int kScoreQT = FSK_SCORE_QT;
 constexpr int kScoreRowsPerWG = 4 * 16 * kScoreQT;  // 512 rows
 // 16-bit score forms: row tiles per wave and rows per workgroup (BF16X6
 // holds four query operands per row tile, so it may use fewer tiles).
-constexpr int kScoreLpQT = 4;
+#ifndef FSK_LP_QT
+#define FSK_LP_QT 4
+#endif
+constexpr int kScoreLpQT = FSK_LP_QT;
 #ifndef FSK_LP_QT_X6
 #define FSK_LP_QT_X6 4
 #endif
