@@ -39,8 +39,10 @@ namespace fsk {
 constexpr float kLseSlack = 8.0f;  // raise m only when a term exceeds 2^8 * 2^m
 constexpr int kLseFlush = 32;
 
+// Column sums of the data in FP64, deterministic: block b writes its 16 sums
+// to part[16 b ..], one thread per feature then adds the blocks in order.
 __global__ void mean16_kernel(const float *__restrict__ x, int64_t ld, int64_t n,
-                              double *__restrict__ sum) {
+                              double *__restrict__ part) {
   __shared__ double red[256][kD + 1];
   double acc[kD];
 #pragma unroll
@@ -62,8 +64,23 @@ __global__ void mean16_kernel(const float *__restrict__ x, int64_t ld, int64_t n
   if (threadIdx.x < kD) {
     double s = 0.0;
     for (int t = 0; t < 256; ++t) s += red[t][threadIdx.x];
-    atomicAdd(&sum[threadIdx.x], s);
+    part[int64_t(blockIdx.x) * kD + threadIdx.x] = s;
   }
+}
+
+__global__ void mean16_finish_kernel(const double *__restrict__ part, int nblocks,
+                                     double *__restrict__ sum) {
+  if (threadIdx.x >= kD) return;
+  double s = 0.0;
+  for (int b = 0; b < nblocks; ++b) s += part[b * kD + threadIdx.x];
+  sum[threadIdx.x] = s;
+}
+
+void launch_mean16(const float *x, int64_t ld, int64_t n, double *mu_ws, hipStream_t s) {
+  int64_t blocks = cdiv(n, 256);
+  if (blocks > kMeanBlocks) blocks = kMeanBlocks;
+  hipLaunchKernelGGL(mean16_kernel, dim3(blocks), dim3(256), 0, s, x, ld, n, mu_ws + kD);
+  hipLaunchKernelGGL(mean16_finish_kernel, dim3(1), dim3(64), 0, s, mu_ws + kD, int(blocks), mu_ws);
 }
 
 __device__ __forceinline__ float max4(f4 v) { return fmaxf(fmaxf(v[0], v[1]), fmaxf(v[2], v[3])); }
@@ -353,7 +370,7 @@ static size_t lse_layout(void *ws, int64_t n, int64_t m, unsigned flags, LseLayo
   const unsigned prec = flags & FSK_PREC_MASK;
   SplitPlan p = kde16_plan(n, m);
   const int64_t n_pad = round_up(n, kValuChunk);
-  L->mu = cv.take<double>(kD);
+  L->mu = cv.take<double>(kMeanWs);
   L->xf = L->bn = L->qf = L->qa = nullptr;
   L->xrec = L->qrec = nullptr;
   if (lse_is16(prec)) {
@@ -408,12 +425,7 @@ extern "C" int fsk_kde16_logdensity(const float *data, int64_t ld, int64_t n,
   hipStream_t s = as_stream(stream);
   const double coef = exp2_coef(bandwidth);
   const int64_t n_pad = round_up(n, kValuChunk);
-  hipLaunchKernelGGL(fill_f64, dim3(1), dim3(64), 0, s, L.mu, int64_t(kD), 0.0);
-  {
-    int64_t blocks = cdiv(n, 256);
-    if (blocks > 1024) blocks = 1024;
-    hipLaunchKernelGGL(mean16_kernel, dim3(blocks), dim3(256), 0, s, data, ld, n, L.mu);
-  }
+  launch_mean16(data, ld, n, L.mu, s);
   const double inv_n = 1.0 / double(n);
 #define FSK_PREP(P)                                                                             \
   do {                                                                                          \

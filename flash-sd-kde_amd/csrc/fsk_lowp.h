@@ -165,8 +165,18 @@ __device__ __forceinline__ f4 mfma16(u16x8 a, u16x8 b, f4 c, int prec) {
                                                  __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
 }
 
-// Sum of the data rows (FP64) into sum[16] (pre-zeroed); mean = sum / n.
-__global__ void mean16_kernel(const float *__restrict__ x, int64_t ld, int64_t n,
-                              double *__restrict__ sum);
+// Column sums of the data rows (FP64, deterministic order) into mu_ws[0..15];
+// mean = sum / n. mu_ws holds kMeanWs doubles (the sums, then per-block parts).
+constexpr int kMeanBlocks = 256;
+constexpr int64_t kMeanWs = int64_t(kD) * (1 + kMeanBlocks);
+void launch_mean16(const float *x, int64_t ld, int64_t n, double *mu_ws, hipStream_t s);
+
+// pdf[i] = sum_s pdf_part; w[i,f] = sum_s w_part + mu_f pdf[i] (FP64) for
+// partials of points centred on mu = mu_sum * inv_n.
+__global__ void score_lp_finish_kernel(const float *__restrict__ pdf_part,
+                                       const float *__restrict__ w_part, int64_t S, int64_t n,
+                                       const double *__restrict__ mu_sum, double inv_n,
+                                       float *__restrict__ pdf, float *__restrict__ w,
+                                       int64_t ldw_row, int64_t ldw_col);
 
 }  // namespace fsk

@@ -16,7 +16,10 @@
 // (W_J += phi^T X_I) needs phi transposed, done through a per-wave LDS tile;
 // column sums are gathered in LDS per J block and scattered with FP64 global
 // atomics, then converted to the FP32 outputs.
+#include <type_traits>
+
 #include "fsk_common.h"
+#include "fsk_lowp.h"
 #include "fsk_plan.h"
 #include "fsk_tile.h"
 
@@ -32,10 +35,24 @@ static_assert(kNormPad % kScoreRowsPerWG == 0, "row norms must cover whole row b
 #define FSK_SYM_BAND 1
 #endif
 
+// Largest spread of the query terms A_i = -c|q_i|^2 (c < 0) inside one
+// workgroup for which the factored form below is used.
+constexpr float kFactSpread = 160.f;
+
 // Rows (queries) and data may differ: rows = data gives the empirical score of
 // the data set itself; a row subset gives the score of those rows under the
 // whole data set's KDE. qa = c|row|^2 padded to a multiple of 256 rows.
-template <int QT>
+//
+// FACT (rows = data only): phi_ij = exp2(t_ij) with t_ij = qa_i + c|x_j|^2 -
+// 2c q_i.x_j is evaluated as exp2(d_ij) * exp2(qa_i + K), d_ij = (c|x_j|^2 - K)
+// - 2c q_i.x_j: the MFMA accumulator starts from the data-side term (shared by
+// every row tile of a data tile), so no per-pair add is needed, and the
+// per-row factor multiplies the finished FP64 sums once. K is the midpoint of
+// the workgroup's A_i range; with the self pair t_ii = 0 in every row, the
+// largest d of row i is A_i - K, within +-kFactSpread/2, so exp2 neither
+// overflows nor leaves the normal range for any pair that matters. A
+// workgroup whose rows spread wider takes the direct form.
+template <int QT, bool FACT>
 __global__ __launch_bounds__(kWG) void score16_ordered_kernel(
     const float *__restrict__ data, int64_t ld, int64_t n, const float *__restrict__ bn,
     int64_t n_pad, const float *__restrict__ rows, int64_t ldr, int64_t m,
@@ -66,37 +83,96 @@ __global__ __launch_bounds__(kWG) void score16_ordered_kernel(
     pdf64[R] = 0.0;
   }
 
+  // workgroup range of A_i = -qa_i over its real rows (FACT only)
+  float kq = 0.f;
+  bool fact = false;
+  if (FACT) {
+    __shared__ float red[8];
+    float amin = INFINITY, amax = -INFINITY;
+#pragma unroll
+    for (int R = 0; R < QT; ++R) {
+      const float a = R == 0 || row0 + 16 * R + c < m ? -qav[R] : -qav[0];
+      amin = fminf(amin, a);
+      amax = fmaxf(amax, a);
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+      amin = fminf(amin, __shfl_xor(amin, o));
+      amax = fmaxf(amax, __shfl_xor(amax, o));
+    }
+    if (lane == 0) {
+      red[wave] = amin;
+      red[4 + wave] = amax;
+    }
+    __syncthreads();
+    amin = fminf(fminf(red[0], red[1]), fminf(red[2], red[3]));
+    amax = fmaxf(fmaxf(red[4], red[5]), fmaxf(red[6], red[7]));
+    fact = amax - amin <= kFactSpread;  // finite and narrow (padded-only rows give -inf)
+    kq = 0.5f * (amax + amin);
+  }
+
   const int64_t j_begin = split * split_len;
   int64_t j_end = j_begin + split_len;
   if (j_end > n_pad) j_end = n_pad;
-  float unused[QT];
-  int tc = 0;
-  for_each_tile<true>(data, ld, n, bn, j_begin, j_end, g, c, [&](const TileData &t) {
-    f4 d[QT];
-    cross_tile<QT>(qv, qav, t, d);
-    kernel_tile<QT, false>(d, pdf, unused);
-    phi_x_tile<QT>(d, t, w);
-    if (++tc == kScoreFlush) {
-      tc = 0;
+  auto run = [&](auto factored) {
+    constexpr bool F = decltype(factored)::value;
+    float unused[QT];
+    int tc = 0;
+    for_each_tile<true>(data, ld, n, bn, j_begin, j_end, g, c, [&](const TileData &t) {
+      f4 d[QT];
+      if constexpr (F) {
+        const f4 ci = t.bb - f4{kq, kq, kq, kq};
 #pragma unroll
-      for (int R = 0; R < QT; ++R) {
-        pdf64[R] += pdf[R];
-        pdf[R] = 0.f;
+        for (int R = 0; R < QT; ++R) d[R] = mfma_f32(t.xa[0], qv[R][0], ci);
 #pragma unroll
-        for (int r = 0; r < 4; ++r) w64[R][r] += w[R][r];
-        w[R] = f4{0.f, 0.f, 0.f, 0.f};
+        for (int s = 1; s < 4; ++s) {
+#pragma unroll
+          for (int R = 0; R < QT; ++R) d[R] = mfma_f32(t.xa[s], qv[R][s], d[R]);
+        }
+#pragma unroll
+        for (int R = 0; R < QT; ++R) {
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const float e = fexp2(d[R][r]);
+            pdf[R] += e;
+            d[R][r] = e;
+          }
+        }
+      } else {
+        cross_tile<QT>(qv, qav, t, d);
+        kernel_tile<QT, false>(d, pdf, unused);
       }
-    }
-  });
+      phi_x_tile<QT>(d, t, w);
+      if (++tc == kScoreFlush) {
+        tc = 0;
+#pragma unroll
+        for (int R = 0; R < QT; ++R) {
+          pdf64[R] += pdf[R];
+          pdf[R] = 0.f;
+#pragma unroll
+          for (int r = 0; r < 4; ++r) w64[R][r] += w[R][r];
+          w[R] = f4{0.f, 0.f, 0.f, 0.f};
+        }
+      }
+    });
+  };
+  if (FACT && fact)
+    run(std::true_type{});
+  else
+    run(std::false_type{});
+  // per-row factor exp2(qa_i + K) of the factored form
+  auto row_scale = [&](int64_t i) -> double {
+    return FACT && fact ? exp2(double(qa[i]) + double(kq)) : 1.0;
+  };
 #pragma unroll
   for (int R = 0; R < QT; ++R) {
     double p = sum_groups(pdf64[R] + pdf[R]);
     int64_t i = row0 + 16 * R + c;
-    if (g == 0 && i < m) pdf_part[split * m + i] = float(p);
+    if (g == 0 && i < m) pdf_part[split * m + i] = float(p * row_scale(i));
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       int64_t iw = row0 + 16 * R + 4 * g + r;
-      if (iw < m) w_part[(split * m + iw) * kD + c] = float(w64[R][r] + w[R][r]);
+      if (iw < m) w_part[(split * m + iw) * kD + c] = float((w64[R][r] + w[R][r]) * row_scale(iw));
     }
   }
 }
@@ -271,6 +347,18 @@ constexpr int kSymWaves = 4;
 constexpr int kSymB = kSymWaves * 16 * kSymQT;
 constexpr int64_t kSymJC = FSK_SYM_JC;  // J blocks per task
 
+// Points centred on the data mean: x - mu (FP64 difference, FP32 result),
+// rows [n, n_pad) zero.
+__global__ void center16_kernel(const float *__restrict__ x, int64_t ld, int64_t n, int64_t n_pad,
+                                const double *__restrict__ mu_sum, double inv_n,
+                                float *__restrict__ xc) {
+  const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= n_pad * kD) return;
+  const int64_t j = e / kD;
+  const int f = int(e % kD);
+  xc[e] = j < n ? float(double(x[j * ld + f]) - mu_sum[f] * inv_n) : 0.f;
+}
+
 struct ScoreLayout {
   float *bn;
   float *qa;
@@ -278,6 +366,8 @@ struct ScoreLayout {
   float *w_part;
   double *pdf64;
   double *w64;
+  double *mu;       // ordered / rows: data mean workspace
+  float *xc, *rc;   // ordered / rows: centred data and rows (16 floats per point)
 };
 
 static size_t score_layout(void *ws, int64_t n, bool ordered, ScoreLayout *L, int64_t m = -1) {
@@ -286,8 +376,15 @@ static size_t score_layout(void *ws, int64_t n, bool ordered, ScoreLayout *L, in
   L->qa = nullptr;
   L->pdf_part = L->w_part = nullptr;
   L->pdf64 = L->w64 = nullptr;
+  L->mu = nullptr;
+  L->xc = L->rc = nullptr;
+  if (m >= 0 || ordered) {
+    L->mu = cv.take<double>(kMeanWs);
+    L->xc = cv.take<float>(round_up(n, kTile) * kD);
+  }
   if (m >= 0) {  // separate rows
     L->qa = cv.take<float>(round_up(m, kNormPad));
+    L->rc = cv.take<float>(m * kD);
     SplitPlan p = score16_rows_plan(n, m);
     L->pdf_part = cv.take<float>(p.splits * m);
     L->w_part = cv.take<float>(p.splits * m * kD);
@@ -355,23 +452,27 @@ extern "C" int fsk_score16_ordered(const float *data, int64_t ld, int64_t n, dou
   }
   hipStream_t s = as_stream(stream);
   const double coef = exp2_coef(bandwidth);
-  const int64_t n_norm = round_up(n, kNormPad);
-  hipLaunchKernelGGL(prep_norms16, dim3(cdiv(n_norm, 256)), dim3(256), 0, s, data, ld, n, n_norm,
-                     coef, L.bn);
+  const int64_t n_norm = round_up(n, kNormPad), n16 = round_up(n, kTile);
+  const double inv_n = 1.0 / double(n);
+  // centre on the data mean (smaller norm terms, less cancellation in t)
+  launch_mean16(data, ld, n, L.mu, s);
+  hipLaunchKernelGGL(center16_kernel, dim3(cdiv(n16 * kD, 256)), dim3(256), 0, s, data, ld, n,
+                     n16, L.mu, inv_n, L.xc);
+  hipLaunchKernelGGL(prep_norms16, dim3(cdiv(n_norm, 256)), dim3(256), 0, s, L.xc, int64_t(kD), n,
+                     n_norm, coef, L.bn);
   SplitPlan p = score16_ordered_plan(n);
   const int64_t ntasks = p.qblocks * p.splits;
   int64_t per_xcd;
   const int64_t grid = xcd_grid(ntasks, &per_xcd);
   prof_begin(FSK_OP_SCORE16_ORDERED, s);
-  hipLaunchKernelGGL((score16_ordered_kernel<kScoreQT>), dim3(grid), dim3(kWG), 0, s, data, ld,
-                     n, L.bn, round_up(n, kTile), data, ld, n, L.bn, float(-2.0 * coef),
+  hipLaunchKernelGGL((score16_ordered_kernel<kScoreQT, true>), dim3(grid), dim3(kWG), 0, s, L.xc,
+                     int64_t(kD), n, L.bn, n16, L.xc, int64_t(kD), n, L.bn, float(-2.0 * coef),
                      p.split_len, p.qblocks, ntasks, per_xcd, L.pdf_part, L.w_part);
   prof_end(FSK_OP_SCORE16_ORDERED, s);
   if ((st = launch_status(what)) != FSK_OK) return st;
-  if ((st = fsk_reduce_partials_f64(L.pdf_part, p.splits, n, n, pdf, stream)) != FSK_OK)
-    return st;
-  return fsk_reduce_weighted_partials_f64(L.w_part, p.splits, n, n * kD, w, ldw_row, ldw_col,
-                                          stream);
+  hipLaunchKernelGGL(score_lp_finish_kernel, dim3(cdiv(n * kD, 256)), dim3(256), 0, s,
+                     L.pdf_part, L.w_part, p.splits, n, L.mu, inv_n, pdf, w, ldw_row, ldw_col);
+  return launch_status(what);
 }
 
 extern "C" int fsk_score16_sym_atomic(const float *data, int64_t ld, int64_t n,
@@ -432,22 +533,29 @@ extern "C" int fsk_score16_rows(const float *rows, int64_t ldr, int64_t m, const
   hipStream_t s = as_stream(stream);
   const double coef = exp2_coef(bandwidth);
   const int64_t n_norm = round_up(n, kNormPad), m_norm = round_up(m, kNormPad);
-  hipLaunchKernelGGL(prep_norms16, dim3(cdiv(n_norm, 256)), dim3(256), 0, s, data, ld, n, n_norm,
-                     coef, L.bn);
-  hipLaunchKernelGGL(prep_norms16, dim3(cdiv(m_norm, 256)), dim3(256), 0, s, rows, ldr, m, m_norm,
-                     coef, L.qa);
+  const int64_t n16 = round_up(n, kTile);
+  const double inv_n = 1.0 / double(n);
+  // rows and data centred on the data mean
+  launch_mean16(data, ld, n, L.mu, s);
+  hipLaunchKernelGGL(center16_kernel, dim3(cdiv(n16 * kD, 256)), dim3(256), 0, s, data, ld, n,
+                     n16, L.mu, inv_n, L.xc);
+  hipLaunchKernelGGL(center16_kernel, dim3(cdiv(m * kD, 256)), dim3(256), 0, s, rows, ldr, m, m,
+                     L.mu, inv_n, L.rc);
+  hipLaunchKernelGGL(prep_norms16, dim3(cdiv(n_norm, 256)), dim3(256), 0, s, L.xc, int64_t(kD), n,
+                     n_norm, coef, L.bn);
+  hipLaunchKernelGGL(prep_norms16, dim3(cdiv(m_norm, 256)), dim3(256), 0, s, L.rc, int64_t(kD), m,
+                     m_norm, coef, L.qa);
   SplitPlan p = score16_rows_plan(n, m);
   const int64_t ntasks = p.qblocks * p.splits;
   int64_t per_xcd;
   const int64_t grid = xcd_grid(ntasks, &per_xcd);
   prof_begin(FSK_OP_SCORE16_ORDERED, s);
-  hipLaunchKernelGGL((score16_ordered_kernel<kScoreQT>), dim3(grid), dim3(kWG), 0, s, data, ld,
-                     n, L.bn, round_up(n, kTile), rows, ldr, m, L.qa, float(-2.0 * coef),
+  hipLaunchKernelGGL((score16_ordered_kernel<kScoreQT, false>), dim3(grid), dim3(kWG), 0, s, L.xc,
+                     int64_t(kD), n, L.bn, n16, L.rc, int64_t(kD), m, L.qa, float(-2.0 * coef),
                      p.split_len, p.qblocks, ntasks, per_xcd, L.pdf_part, L.w_part);
   prof_end(FSK_OP_SCORE16_ORDERED, s);
   if ((st = launch_status(what)) != FSK_OK) return st;
-  if ((st = fsk_reduce_partials_f64(L.pdf_part, p.splits, m, m, pdf, stream)) != FSK_OK)
-    return st;
-  return fsk_reduce_weighted_partials_f64(L.w_part, p.splits, m, m * kD, w, ldw_row, ldw_col,
-                                          stream);
+  hipLaunchKernelGGL(score_lp_finish_kernel, dim3(cdiv(m * kD, 256)), dim3(256), 0, s,
+                     L.pdf_part, L.w_part, p.splits, m, L.mu, inv_n, pdf, w, ldw_row, ldw_col);
+  return launch_status(what);
 }

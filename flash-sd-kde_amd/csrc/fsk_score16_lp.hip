@@ -371,7 +371,7 @@ static size_t score_lp_layout(void *ws, int64_t n, int64_t m, unsigned prec, Sco
   SplitPlan p = score16_lp_plan(n, m, prec);
   L->ntiles_data = cdiv(n, kTile);
   L->ntiles_alloc = L->ntiles_data + lp_chunk(prec);  // whole-chunk copies may read past the end
-  L->mu = cv.take<double>(kD);
+  L->mu = cv.take<double>(kMeanWs);
   L->xrec = cv.take<unsigned short>(round_up(n, kNormPad) * rec_size(prec));
   L->qrec = cv.take<unsigned short>(round_up(m, kNormPad) * rec_size(prec));
   L->xt = cv.take<unsigned short>(L->ntiles_data * kTile * kD * 2);
@@ -403,12 +403,7 @@ int score16_lp_run(const char *what, const float *rows_in, int64_t ldr, int64_t 
   const int64_t xrows = round_up(n, kNormPad), qrows = round_up(m, kNormPad);
   const int64_t n16 = L.ntiles_data * kTile;
   const double inv_n = 1.0 / double(n);
-  hipLaunchKernelGGL(fill_f64, dim3(1), dim3(64), 0, s, L.mu, int64_t(kD), 0.0);
-  {
-    int64_t blocks = cdiv(n, 256);
-    if (blocks > 1024) blocks = 1024;
-    hipLaunchKernelGGL(mean16_kernel, dim3(blocks), dim3(256), 0, s, data, ld, n, L.mu);
-  }
+  launch_mean16(data, ld, n, L.mu, s);
   SplitPlan p = score16_lp_plan(n, m, prec);
   const int64_t ntasks = p.qblocks * p.splits;
   const int64_t split_tiles = p.split_len / kTile;

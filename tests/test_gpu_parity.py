@@ -247,6 +247,33 @@ def test_score_rows_matches_fp64(gpu, m, n):
     assert np.max(np.abs(cpu(w) - we)) / scale < 3e-5
 
 
+@pytest.mark.parametrize("shift", [0.0, 6.0, 25.0])
+def test_score_ordered_factored_and_direct_rows_agree_with_fp64(gpu, shift):
+    """The ordered score evaluates a workgroup's rows in factored form when
+    their norm terms c|q - mu|^2 span a narrow range and directly otherwise; an
+    outlier (shifted by `shift` bandwidths along every axis) forces the direct
+    form in its workgroup while the others stay factored (shift = 0: all
+    factored). Every row agrees with the FP64 sums to 2e-5, except that the
+    outlier's own row carries the FP32 cancellation error of the norm
+    expansion t = c|q|^2 + c|x|^2 - 2c q.x, a few units of 2^-24 times its
+    norm terms (|c||x - mu|^2 ~ 7200 at shift 25)."""
+    rng = np.random.default_rng(21)
+    x = rng.normal(size=(2600, 16)).astype(np.float32)
+    x[700] += np.float32(shift)
+    h = 1.0
+    po, wo = _ops.score16(torch.as_tensor(x, device=gpu), h, variant="ordered")
+    pe, we = score_exact(x, h)
+    xd = x.astype(np.float64)
+    cnorm = np.log2(np.e) / (2 * h * h) * ((xd - xd.mean(0)) ** 2).sum(1)
+    tol = 2e-5 + 1.2e-7 * cnorm
+    tol[np.arange(len(x)) != 700] = 2e-5
+    rel = np.abs(cpu(po) - pe) / pe
+    assert np.all(rel <= tol), (rel.max(), int(rel.argmax()))
+    scale = np.maximum(np.abs(we).max(1), 1e-30)
+    relw = np.abs(cpu(wo) - we).max(1) / scale
+    assert np.all(relw <= tol), (relw.max(), int(relw.argmax()))
+
+
 def test_score_permutation_equivariance(gpu):
     x = np.random.default_rng(9).normal(size=(1500, 16)).astype(np.float32)
     perm = np.random.default_rng(10).permutation(1500)
