@@ -462,7 +462,8 @@ def main():
     else:
         units = float(n) * n
         flops = units * 64.0
-        exp_valu_s = units / R_EXP + units * 4.0 / R_VALU
+        # per ordered pair: 1 exp + 1 VALU op (row-sum add; factored norm terms)
+        exp_valu_s = units / R_EXP + units * 1.0 / R_VALU
     achieved = flops / (score_ms * 1e-3) / 1e12 if score_ms > 0 else None
     kde_flops = float(m) * n * 32.0
     kde_tflops = kde_flops / (kde_ms * 1e-3) / 1e12 if kde_ms > 0 else None
@@ -497,7 +498,8 @@ def main():
             "bandwidth": h,
             "bandwidth_rule": "mean_std*(4/(d+2))^(1/(d+4))*n^(-1/(d+4))",
             "score_kernel": args.variant,
-            "precision": "fp32 (MFMA f32 cross term and phi.X, FP64 combine/debias)",
+            "precision": "fp32 (MFMA f32 cross term and phi.X on mean-centred points, "
+                         "FP64 combine/debias)",
             "parallelism": "single_gpu",
         },
         "roofline": {
@@ -529,8 +531,9 @@ def main():
     line["measured_rates"] = rates
     if rates.get("exp_per_s") and rates.get("fma_per_s") and score_ms > 0:
         # exp/VALU roofline with the measured issue rates (per ordered pair of
-        # the ordered kernel: 1 exp + 3 VALU ops; symmetric: per unique pair 1 + 5)
-        v_ops = 5.0 if args.variant == "symmetric" else 3.0
+        # the ordered kernel in factored form: 1 exp + 1 VALU op, the row-sum
+        # add; symmetric: per unique pair 1 + 5)
+        v_ops = 5.0 if args.variant == "symmetric" else 1.0
         t = units / rates["exp_per_s"] + units * v_ops / rates["fma_per_s"]
         line["roofline"]["exp_valu_frac_measured_rates"] = round(t / (score_ms * 1e-3), 4)
         line["roofline"]["mfma_frac_of_measured_peak"] = round(
@@ -539,10 +542,11 @@ def main():
             # FP32 MFMA never co-executes with VALU work on this GPU (counter
             # SQ_VALU_MFMA_COEXEC_CYCLES = 0), so the issue bound of the
             # ordered kernel is the SUM of its MFMA time and its VALU time:
-            # per ordered pair 64 flop MFMA, 1 exp and 2 other VALU ops (norm
-            # add, row-sum add; the clamp rides on the exp).
+            # per ordered pair 64 flop MFMA, 1 exp and 1 other VALU op (the
+            # row-sum add; the factored form starts the accumulator from the
+            # data-side norm term, 4 subtractions per 2048 pairs).
             t_add = (flops / rates["fp32_mfma_flops"] + units / rates["exp_per_s"]
-                     + 2.0 * units / rates["fma_per_s"])
+                     + (1.0 + 4.0 / 2048) * units / rates["fma_per_s"])
             line["roofline"]["additive_issue_bound_frac"] = round(t_add / (score_ms * 1e-3), 4)
     if not args.no_batched:
         line["batched_loo"] = batched_loo(rates, args.steps)
