@@ -127,7 +127,7 @@ def sklearn_baseline(x, q, h, budget_s):
 # dispatch: FETCH_SIZE reads 262169 KiB -> scale 2.0, WRITE_SIZE reads
 # 524288 KiB -> scale 1.0). Keyed by (score kernel, n); other sizes report null.
 PMC_TRAFFIC = {
-    ("ordered", 1 << 20): {"fetch_kib": 2348100.0, "write_kib": 278528.0,
+    ("ordered", 1 << 20): {"fetch_kib": 2355570.0, "write_kib": 278528.0,
                            "fetch_scale": 524288.0 / 262169.0, "write_scale": 1.0},
 }
 
@@ -392,8 +392,13 @@ def precision_forms(xd, qd, h, fp32_ms, fp32_dens, steps, warmup, n_err, rates=N
                      "kde_kernel_ms": round(kde_ms, 3), "max_rel_logp_err": mx,
                      "median_rel_logp_err": med, "within_1e-3": mx <= 1e-3}
         if rates and rates.get("exp_per_s") and rates.get("fma_per_s") and score_ms > 0:
+            # exp/VALU issue time of the score kernel at the measured rates, per
+            # ordered pair from its instruction mix: 1 exp, the row-sum add, and
+            # the phi operand (single forms: half a v_cvt_pk; split forms: and,
+            # sub, half a v_perm, half a v_cvt_pk). The MFMA work co-issues.
             units = float(n) * n
-            t = units / rates["exp_per_s"] + 3.0 * units / rates["fma_per_s"]
+            v_ops = 4.0 if prec in ("bf16x3", "fp16x3", "bf16x6") else 1.5
+            t = units / rates["exp_per_s"] + v_ops * units / rates["fma_per_s"]
             out[prec]["score_exp_valu_frac_measured_rates"] = round(t / (score_ms * 1e-3), 4)
     ok = [p for p in out if p != "reference" and out[p]["within_1e-3"]]
     best = max(ok, key=lambda p: out[p]["speedup_vs_fp32"])
