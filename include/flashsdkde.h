@@ -162,9 +162,13 @@ int fsk_score16_sym_atomic(const float *data, int64_t ld, int64_t n,
 
 /* Same accumulators, deterministic: bounded split-K over the data with
  * FP32 partials and an FP64 combine; bitwise reproducible for fixed n.
+ * Points are centred on their mean (FP64, fixed summation order) before the
+ * pair terms are formed, and mu * pdf is added back in FP64.
  * flags may select a 16-bit cross term and phi.X (FSK_PREC_F16X3 / _BF16X6 /
- * _BF16X3 / _BF16 / _F16, points centred on their mean; the precision form);
- * FSK_PREC_F32 is the plain FP32 form.
+ * _BF16X3 / _BF16 / _F16; the precision form); FSK_PREC_F32 is the plain
+ * FP32 form (FP32 MFMA; rows whose norm terms span a narrow range are
+ * evaluated as exp2(d_ij) * exp2(c|x_i - mu|^2 + K), which needs no per-pair
+ * add, the others directly).
  * Replaces emp_score_16d_ordered_splitk,
  * kernels/emp_score_16d_ordered_splitk.py:162-263 (pass A :14-92, reduce
  * kernels/reduce_partials.py:82-108). */
