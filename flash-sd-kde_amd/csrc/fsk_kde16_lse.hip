@@ -362,7 +362,7 @@ struct LseLayout {
 
 static bool lse_is16(unsigned prec) {
   return prec == FSK_PREC_BF16 || prec == FSK_PREC_F16 || prec == FSK_PREC_BF16X3 ||
-         prec == FSK_PREC_BF16X6 || prec == FSK_PREC_F16X3;
+         prec == FSK_PREC_BF16X6 || prec == FSK_PREC_F16X3 || prec == FSK_PREC_F16X3S;
 }
 
 static size_t lse_layout(void *ws, int64_t n, int64_t m, unsigned flags, LseLayout *L) {
@@ -412,7 +412,7 @@ extern "C" int fsk_kde16_logdensity(const float *data, int64_t ld, int64_t n,
     return FSK_EINVAL;
   }
   const unsigned prec = flags & FSK_PREC_MASK;
-  if (prec > FSK_PREC_F16X3 || (flags & ~FSK_PREC_MASK)) {
+  if (prec > FSK_PREC_F16X3S || (flags & ~FSK_PREC_MASK)) {
     set_error("%s: flags 0x%x not available (Laplacian has no log form)", what, flags);
     return FSK_EUNSUPPORTED;
   }
@@ -440,7 +440,8 @@ extern "C" int fsk_kde16_logdensity(const float *data, int64_t ld, int64_t n,
     case FSK_PREC_BF16: FSK_PREP(FSK_PREC_BF16); break;
     case FSK_PREC_F16: FSK_PREP(FSK_PREC_F16); break;
     case FSK_PREC_BF16X6: FSK_PREP(FSK_PREC_BF16X6); break;
-    case FSK_PREC_F16X3: FSK_PREP(FSK_PREC_F16X3); break;
+    case FSK_PREC_F16X3:
+    case FSK_PREC_F16X3S: FSK_PREP(FSK_PREC_F16X3); break;
     default: FSK_PREP(FSK_PREC_BF16X3); break;
   }
 #undef FSK_PREP
@@ -466,7 +467,8 @@ extern "C" int fsk_kde16_logdensity(const float *data, int64_t ld, int64_t n,
       case FSK_PREC_BF16: FSK_LSE(FSK_PREC_BF16); break;
       case FSK_PREC_F16: FSK_LSE(FSK_PREC_F16); break;
       case FSK_PREC_BF16X6: FSK_LSE(FSK_PREC_BF16X6); break;
-      case FSK_PREC_F16X3: FSK_LSE(FSK_PREC_F16X3); break;
+      case FSK_PREC_F16X3:
+      case FSK_PREC_F16X3S: FSK_LSE(FSK_PREC_F16X3); break;
       default: FSK_LSE(FSK_PREC_BF16X3); break;
     }
 #undef FSK_LSE

@@ -41,7 +41,8 @@ inline int64_t rec_size(unsigned prec) { return prec == FSK_PREC_BF16X6 ? 64 : 4
 template <int PREC>
 struct CrossType {
   static constexpr int kP16 =
-      (PREC == FSK_PREC_F16 || PREC == FSK_PREC_F16X3) ? FSK_PREC_F16 : FSK_PREC_BF16;
+      (PREC == FSK_PREC_F16 || PREC == FSK_PREC_F16X3 || PREC == FSK_PREC_F16X3S) ? FSK_PREC_F16
+                                                                                  : FSK_PREC_BF16;
 };
 
 __device__ __forceinline__ u16x8 ld_u16x8(const unsigned short *p) {

@@ -7,6 +7,7 @@
 //   BF16 / F16 : [x1 | fold] . [q1 | fold]
 //   BF16X3     : [x1 | x2] . [q1 | q1]  +  [x1 | fold] . [q2 | fold]
 //   F16X3      : the same with FP16 pieces (phi.X as in BF16X3)
+//   F16X3S     : the F16X3 cross term; phi.X with one FP16 phi (below)
 //   BF16X6     : [x1 | x1] . [q1 | q2]  +  [x2 | x1] . [q1 | q3]
 //              + [x2 | x3] . [q2 | q1]  +  [x2 | fold] . [q3 | fold]
 //                (FP32-class cross term; phi.X as in BF16X3)
@@ -17,6 +18,8 @@
 //   phi_h = phi truncated to BF16, phi_l = rn_bf16(phi - phi_h)),
 //   W += A . [Xh | Xh]  +  A . [Xl | 0]      (BF16x3, BF16x6, FP16x3: drops phi_l.Xl)
 //   W += [phi | 0] . [Xh | Xl]               (single forms; the zero half drops Xl)
+//   W += [phi | phi] . [Xh | Xl]             (F16X3S: phi rounded to FP16 once,
+//                                             X to ~22 bits, one MFMA)
 // The row sums pdf_i = sum_j phi_ij are FP32 VALU adds of the unrounded phi
 // (an MFMA against a ones operand measured slower here).
 //
@@ -49,14 +52,19 @@ typedef __attribute__((address_space(3))) void lds_void;
 template <int PREC>
 struct LpBlock {
   static constexpr bool kX6 = PREC == FSK_PREC_BF16X6;
-  static constexpr bool kX3 = PREC == FSK_PREC_BF16X3 || PREC == FSK_PREC_F16X3;
+  static constexpr bool kX3 =
+      PREC == FSK_PREC_BF16X3 || PREC == FSK_PREC_F16X3 || PREC == FSK_PREC_F16X3S;
+  static constexpr bool kSplitPhi = kX6 || (kX3 && PREC != FSK_PREC_F16X3S);
   static constexpr int kCross = kX6 ? 4 : kX3 ? 2 : 1;  // cross-term A operands
-  static constexpr int kX = (kX6 || kX3) ? 2 : 1;       // phi.X B operands
+  static constexpr int kX = kSplitPhi ? 2 : 1;          // phi.X B operands
   static constexpr int kSlots = kCross + kX;
   static constexpr int kSlotU16 = 64 * 8;               // 16 bytes per lane
   static constexpr int kTileU16 = kSlots * kSlotU16;
-  // tiles per LDS chunk: 16 KiB chunks (12 KiB for BF16X6), two buffers
-  static constexpr int kChunk = 16384 / (kTileU16 * 2);
+  // tiles per LDS chunk: up to 16 KiB per buffer, two buffers, a whole number
+  // of slots per wave
+  static constexpr int kChunk0 = 16384 / (kTileU16 * 2);
+  static constexpr int kChunk =
+      kSlots % 4 == 0 ? kChunk0 : kSlots % 2 == 0 ? (kChunk0 & ~1) : (kChunk0 & ~3);
   static constexpr int kChunkU16 = kChunk * kTileU16;
   static_assert((kChunk * kSlots) % 4 == 0, "four waves copy whole slots");
 };
@@ -147,7 +155,7 @@ __device__ __forceinline__ unsigned pack2(float a, float b) {
 // high part is phi truncated to its upper 16 bits (a BF16 value, exact as
 // FP32), the low part phi - high (exact in FP32) rounded to BF16; a byte
 // permute packs two high parts, one conversion packs two low parts.
-template <int P16, bool SPLIT>
+template <int P16, bool SPLIT, bool DUP = false>
 __device__ __forceinline__ u16x8 phi_operand(const float (&e)[4]) {
   u32x4 a;
   if (SPLIT) {
@@ -165,7 +173,8 @@ __device__ __forceinline__ u16x8 phi_operand(const float (&e)[4]) {
   } else {
     a[0] = pack2<P16>(e[0], e[1]);
     a[1] = pack2<P16>(e[2], e[3]);
-    a[2] = a[3] = 0u;
+    a[2] = DUP ? a[0] : 0u;  // DUP: [phi | phi] against [Xh | Xl]
+    a[3] = DUP ? a[1] : 0u;
   }
   return __builtin_bit_cast(u16x8, a);
 }
@@ -183,7 +192,8 @@ __global__ __launch_bounds__(kWG) void score16_lp_kernel(
   constexpr int P16 = CrossType<PREC>::kP16;  // cross-term operands
   constexpr bool X6 = BL::kX6;
   constexpr bool X3 = BL::kX3;
-  constexpr bool SPLIT_PHI = X3 || X6;  // phi.X with phi = phi_h + phi_l
+  constexpr bool SPLIT_PHI = BL::kSplitPhi;  // phi.X with phi = phi_h + phi_l
+  constexpr bool DUP_PHI = PREC == FSK_PREC_F16X3S;
   // phi.X operands: BF16 for the split forms (phi in [0, 1] keeps full
   // relative precision down to tiny values), else the form's own type
   constexpr int PX = SPLIT_PHI ? FSK_PREC_BF16 : P16;
@@ -293,7 +303,7 @@ __global__ __launch_bounds__(kWG) void score16_lp_kernel(
           e[u] = fexp2_le1(d[u]);
           pdf[R] += e[u];
         }
-        const u16x8 A = phi_operand<PX, SPLIT_PHI>(e);
+        const u16x8 A = phi_operand<PX, SPLIT_PHI, DUP_PHI>(e);
         w[R] = mfma16(A, BX1, w[R], PX);
         if (SPLIT_PHI) w[R] = mfma16(A, BX2, w[R], PX);
       }
@@ -347,6 +357,7 @@ static int lp_tile_u16(unsigned prec) {
     case FSK_PREC_BF16X6: return LpBlock<FSK_PREC_BF16X6>::kTileU16;
     case FSK_PREC_BF16X3: return LpBlock<FSK_PREC_BF16X3>::kTileU16;
     case FSK_PREC_F16X3: return LpBlock<FSK_PREC_F16X3>::kTileU16;
+    case FSK_PREC_F16X3S: return LpBlock<FSK_PREC_F16X3S>::kTileU16;
     default: return LpBlock<FSK_PREC_BF16>::kTileU16;
   }
 }
@@ -355,6 +366,7 @@ static int lp_chunk(unsigned prec) {
     case FSK_PREC_BF16X6: return LpBlock<FSK_PREC_BF16X6>::kChunk;
     case FSK_PREC_BF16X3: return LpBlock<FSK_PREC_BF16X3>::kChunk;
     case FSK_PREC_F16X3: return LpBlock<FSK_PREC_F16X3>::kChunk;
+    case FSK_PREC_F16X3S: return LpBlock<FSK_PREC_F16X3S>::kChunk;
     default: return LpBlock<FSK_PREC_BF16>::kChunk;
   }
 }
@@ -432,6 +444,7 @@ int score16_lp_run(const char *what, const float *rows_in, int64_t ldr, int64_t 
     case FSK_PREC_F16: FSK_LP(FSK_PREC_F16, FSK_PREC_F16, kScoreLpQT); break;
     case FSK_PREC_BF16X6: FSK_LP(FSK_PREC_BF16X6, FSK_PREC_BF16, kScoreLpQTx6); break;
     case FSK_PREC_F16X3: FSK_LP(FSK_PREC_F16X3, FSK_PREC_BF16, kScoreLpQT); break;
+    case FSK_PREC_F16X3S: FSK_LP(FSK_PREC_F16X3S, FSK_PREC_F16, kScoreLpQT); break;
     default: FSK_LP(FSK_PREC_BF16X3, FSK_PREC_BF16, kScoreLpQT); break;
   }
 #undef FSK_LP

@@ -47,7 +47,10 @@ extern "C" {
                               * FP32 rounding), FP32 accumulate: FP32-class cross term */
 #define FSK_PREC_F16X3 6u    /* FP16 hi/lo split cross term, 3 products (2^-22 terms
                               * dropped), FP32 accumulate; phi.X as in BF16X3 */
-/* The FP16 forms (_F16, _F16X3) hold -2c (q - mu) and c |x - mu|^2, with
+#define FSK_PREC_F16X3S 7u   /* cross term as F16X3; phi.X with a single FP16 phi
+                              * against FP16 [Xh | Xl] (one MFMA, no phi split):
+                              * score forms only (log-densities as F16X3) */
+/* The FP16 forms (_F16, _F16X3, _F16X3S) hold -2c (q - mu) and c |x - mu|^2, with
  * c = log2(e) / (2 h^2) and mu the data mean, as FP16 pieces: the caller
  * keeps c * max |x - mu|^2 within FP16's range (the Python layer refuses
  * inputs above 3e4). */
