@@ -369,7 +369,7 @@ def precision_forms(xd, qd, h, fp32_ms, fp32_dens, steps, warmup, n_err, rates=N
                     "within_1e-3": mx <= 1e-3}}
     lib = _native.lib()
     pairs = float(m) * n + float(n) * n
-    for prec in ("fp16x3", "fp16x3s", "bf16x6", "bf16x3", "fp16", "bf16"):
+    for prec in ("fp16x3", "fp16x3s", "bf16x6", "bf16x3", "bf16x3s", "fp16", "bf16"):
         def step():
             pdf, w = _ops.score16(xd, h, variant="ordered", precision=prec)
             deb = _ops.debias(xd, pdf, w, h, 1e-12)
@@ -398,7 +398,8 @@ def precision_forms(xd, qd, h, fp32_ms, fp32_dens, steps, warmup, n_err, rates=N
             # sub, half a v_perm, half a v_cvt_pk; fp16x3s: half a v_cvt_pk and
             # half a copy). The MFMA work co-issues.
             units = float(n) * n
-            v_ops = {"bf16x3": 4.0, "fp16x3": 4.0, "bf16x6": 4.0, "fp16x3s": 2.0}.get(prec, 1.5)
+            v_ops = {"bf16x3": 4.0, "fp16x3": 4.0, "bf16x6": 4.0, "fp16x3s": 2.0,
+                     "bf16x3s": 2.0}.get(prec, 1.5)
             t = units / rates["exp_per_s"] + v_ops * units / rates["fma_per_s"]
             out[prec]["score_exp_valu_frac_measured_rates"] = round(t / (score_ms * 1e-3), 4)
     ok = [p for p in out if p != "reference" and out[p]["within_1e-3"]]

@@ -362,7 +362,8 @@ struct LseLayout {
 
 static bool lse_is16(unsigned prec) {
   return prec == FSK_PREC_BF16 || prec == FSK_PREC_F16 || prec == FSK_PREC_BF16X3 ||
-         prec == FSK_PREC_BF16X6 || prec == FSK_PREC_F16X3 || prec == FSK_PREC_F16X3S;
+         prec == FSK_PREC_BF16X6 || prec == FSK_PREC_F16X3 || prec == FSK_PREC_F16X3S ||
+         prec == FSK_PREC_BF16X3S;
 }
 
 static size_t lse_layout(void *ws, int64_t n, int64_t m, unsigned flags, LseLayout *L) {
@@ -412,7 +413,7 @@ extern "C" int fsk_kde16_logdensity(const float *data, int64_t ld, int64_t n,
     return FSK_EINVAL;
   }
   const unsigned prec = flags & FSK_PREC_MASK;
-  if (prec > FSK_PREC_F16X3S || (flags & ~FSK_PREC_MASK)) {
+  if (prec > FSK_PREC_BF16X3S || (flags & ~FSK_PREC_MASK)) {
     set_error("%s: flags 0x%x not available (Laplacian has no log form)", what, flags);
     return FSK_EUNSUPPORTED;
   }

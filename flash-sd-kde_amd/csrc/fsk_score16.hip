@@ -423,7 +423,8 @@ static int score_check(const char *what, const float *data, int64_t ld, int64_t 
   }
   const unsigned prec = flags & FSK_PREC_MASK;
   const bool lowp = prec == FSK_PREC_BF16 || prec == FSK_PREC_F16 || prec == FSK_PREC_BF16X3 ||
-                    prec == FSK_PREC_BF16X6 || prec == FSK_PREC_F16X3 || prec == FSK_PREC_F16X3S;
+                    prec == FSK_PREC_BF16X6 || prec == FSK_PREC_F16X3 || prec == FSK_PREC_F16X3S ||
+                    prec == FSK_PREC_BF16X3S;
   if ((prec != FSK_PREC_F32 && !(lowp && lowp_ok)) || (flags & ~FSK_PREC_MASK)) {
     set_error("%s: flags 0x%x not available on this path", what, flags);
     return FSK_EUNSUPPORTED;

@@ -8,6 +8,7 @@
 //   BF16X3     : [x1 | x2] . [q1 | q1]  +  [x1 | fold] . [q2 | fold]
 //   F16X3      : the same with FP16 pieces (phi.X as in BF16X3)
 //   F16X3S     : the F16X3 cross term; phi.X with one FP16 phi (below)
+//   BF16X3S    : the BF16X3 cross term; phi.X as F16X3S
 //   BF16X6     : [x1 | x1] . [q1 | q2]  +  [x2 | x1] . [q1 | q3]
 //              + [x2 | x3] . [q2 | q1]  +  [x2 | fold] . [q3 | fold]
 //                (FP32-class cross term; phi.X as in BF16X3)
@@ -18,8 +19,9 @@
 //   phi_h = phi truncated to BF16, phi_l = rn_bf16(phi - phi_h)),
 //   W += A . [Xh | Xh]  +  A . [Xl | 0]      (BF16x3, BF16x6, FP16x3: drops phi_l.Xl)
 //   W += [phi | 0] . [Xh | Xl]               (single forms; the zero half drops Xl)
-//   W += [phi | phi] . [Xh | Xl]             (F16X3S: phi rounded to FP16 once,
-//                                             X to ~22 bits, one MFMA)
+//   W += [phi | phi] . [Xh | Xl]             (F16X3S, BF16X3S: phi rounded to
+//                                             FP16 once, X in FP16 pieces to
+//                                             ~22 bits, one MFMA)
 // The row sums pdf_i = sum_j phi_ij are FP32 VALU adds of the unrounded phi
 // (an MFMA against a ones operand measured slower here).
 //
@@ -52,9 +54,9 @@ typedef __attribute__((address_space(3))) void lds_void;
 template <int PREC>
 struct LpBlock {
   static constexpr bool kX6 = PREC == FSK_PREC_BF16X6;
-  static constexpr bool kX3 =
-      PREC == FSK_PREC_BF16X3 || PREC == FSK_PREC_F16X3 || PREC == FSK_PREC_F16X3S;
-  static constexpr bool kSplitPhi = kX6 || (kX3 && PREC != FSK_PREC_F16X3S);
+  static constexpr bool kOnePhi = PREC == FSK_PREC_F16X3S || PREC == FSK_PREC_BF16X3S;
+  static constexpr bool kX3 = PREC == FSK_PREC_BF16X3 || PREC == FSK_PREC_F16X3 || kOnePhi;
+  static constexpr bool kSplitPhi = kX6 || (kX3 && !kOnePhi);
   static constexpr int kCross = kX6 ? 4 : kX3 ? 2 : 1;  // cross-term A operands
   static constexpr int kX = kSplitPhi ? 2 : 1;          // phi.X B operands
   static constexpr int kSlots = kCross + kX;
@@ -193,10 +195,11 @@ __global__ __launch_bounds__(kWG) void score16_lp_kernel(
   constexpr bool X6 = BL::kX6;
   constexpr bool X3 = BL::kX3;
   constexpr bool SPLIT_PHI = BL::kSplitPhi;  // phi.X with phi = phi_h + phi_l
-  constexpr bool DUP_PHI = PREC == FSK_PREC_F16X3S;
+  constexpr bool DUP_PHI = BL::kOnePhi;
   // phi.X operands: BF16 for the split forms (phi in [0, 1] keeps full
-  // relative precision down to tiny values), else the form's own type
-  constexpr int PX = SPLIT_PHI ? FSK_PREC_BF16 : P16;
+  // relative precision down to tiny values), FP16 for the one-phi forms (phi
+  // to 2^-12, X in FP16 pieces), else the form's own type
+  constexpr int PX = SPLIT_PHI ? FSK_PREC_BF16 : DUP_PHI ? FSK_PREC_F16 : P16;
   constexpr int CH = BL::kChunk;
   constexpr int S1 = BL::kSlotU16;
   using RL = RecLayout<PREC>;
@@ -358,6 +361,7 @@ static int lp_tile_u16(unsigned prec) {
     case FSK_PREC_BF16X3: return LpBlock<FSK_PREC_BF16X3>::kTileU16;
     case FSK_PREC_F16X3: return LpBlock<FSK_PREC_F16X3>::kTileU16;
     case FSK_PREC_F16X3S: return LpBlock<FSK_PREC_F16X3S>::kTileU16;
+    case FSK_PREC_BF16X3S: return LpBlock<FSK_PREC_BF16X3S>::kTileU16;
     default: return LpBlock<FSK_PREC_BF16>::kTileU16;
   }
 }
@@ -367,6 +371,7 @@ static int lp_chunk(unsigned prec) {
     case FSK_PREC_BF16X3: return LpBlock<FSK_PREC_BF16X3>::kChunk;
     case FSK_PREC_F16X3: return LpBlock<FSK_PREC_F16X3>::kChunk;
     case FSK_PREC_F16X3S: return LpBlock<FSK_PREC_F16X3S>::kChunk;
+    case FSK_PREC_BF16X3S: return LpBlock<FSK_PREC_BF16X3S>::kChunk;
     default: return LpBlock<FSK_PREC_BF16>::kChunk;
   }
 }
@@ -445,6 +450,7 @@ int score16_lp_run(const char *what, const float *rows_in, int64_t ldr, int64_t 
     case FSK_PREC_BF16X6: FSK_LP(FSK_PREC_BF16X6, FSK_PREC_BF16, kScoreLpQTx6); break;
     case FSK_PREC_F16X3: FSK_LP(FSK_PREC_F16X3, FSK_PREC_BF16, kScoreLpQT); break;
     case FSK_PREC_F16X3S: FSK_LP(FSK_PREC_F16X3S, FSK_PREC_F16, kScoreLpQT); break;
+    case FSK_PREC_BF16X3S: FSK_LP(FSK_PREC_BF16X3S, FSK_PREC_F16, kScoreLpQT); break;
     default: FSK_LP(FSK_PREC_BF16X3, FSK_PREC_BF16, kScoreLpQT); break;
   }
 #undef FSK_LP

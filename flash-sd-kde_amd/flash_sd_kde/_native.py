@@ -32,6 +32,7 @@ PREC_BF16X3 = 4
 PREC_BF16X6 = 5
 PREC_F16X3 = 6
 PREC_F16X3S = 7
+PREC_BF16X3S = 8
 FLAG_LAPLACIAN = 1 << 8
 
 OP_KDE16_SPLITK = 1

@@ -31,6 +31,7 @@ PRECISION_FLAGS = {
     "bf16x6": nat.PREC_BF16X6,
     "fp16x3": nat.PREC_F16X3,
     "fp16x3s": nat.PREC_F16X3S,
+    "bf16x3s": nat.PREC_BF16X3S,
 }
 
 
@@ -104,11 +105,18 @@ def check_fp16_range(train: torch.Tensor, query: torch.Tensor, h: float, flags: 
     pieces; refuse inputs whose values would leave FP16's range instead of
     returning inf / NaN. Skipped while a HIP graph is being captured (it needs
     a device-to-host read)."""
-    if flags not in (nat.PREC_F16, nat.PREC_F16X3, nat.PREC_F16X3S):
+    if flags not in (nat.PREC_F16, nat.PREC_F16X3, nat.PREC_F16X3S, nat.PREC_BF16X3S):
         return
     if train.is_cuda and torch.cuda.is_current_stream_capturing():
         return  # the check reads a device value; under graph capture the caller vouches
     mu = train.mean(0)
+    if flags == nat.PREC_BF16X3S:
+        # only the phi.X operand is FP16 there: the pieces of x - mu itself
+        amax = float((train - mu).abs().max())
+        if amax > FP16_SAFE:
+            raise ValueError(f"bf16x3s holds x - mu as FP16 pieces: max|x - mu| = {amax:.3g} "
+                             f"exceeds {FP16_SAFE:g}; use bf16x3 / fp32.")
+        return
     r2 = float(((train - mu) ** 2).sum(1).max())
     if query is not train:
         r2 = max(r2, float(((query - mu) ** 2).sum(1).max()))
@@ -204,7 +212,7 @@ def score16(train: torch.Tensor, bandwidth: float, *, variant: str = "symmetric"
         raise ValueError(f"unknown score variant {variant}")
     flags = precision_flag(precision)
     if flags not in (nat.PREC_F32, nat.PREC_BF16, nat.PREC_F16, nat.PREC_BF16X3, nat.PREC_BF16X6,
-                     nat.PREC_F16X3, nat.PREC_F16X3S):
+                     nat.PREC_F16X3, nat.PREC_F16X3S, nat.PREC_BF16X3S):
         raise ValueError(f"precision {precision} is not available for the score pass")
     if flags != nat.PREC_F32 and variant != "ordered":
         raise ValueError("reduced-precision score accumulators use the ordered kernel")
@@ -228,7 +236,7 @@ def score16_rows(rows: torch.Tensor, train: torch.Tensor, bandwidth: float, *,
         raise ValueError("rows and data must be non-empty.")
     flags = precision_flag(precision)
     if flags not in (nat.PREC_F32, nat.PREC_BF16, nat.PREC_F16, nat.PREC_BF16X3, nat.PREC_BF16X6,
-                     nat.PREC_F16X3, nat.PREC_F16X3S):
+                     nat.PREC_F16X3, nat.PREC_F16X3S, nat.PREC_BF16X3S):
         raise ValueError(f"precision {precision} is not available for the score pass")
     check_fp16_range(train, rows, h, flags)
     pdf = torch.empty(m, dtype=torch.float32, device=dev)

@@ -12,6 +12,9 @@ operands and FP32 accumulation:
             against X in FP16 hi/lo pieces (one MFMA instead of two and no
             phi split; phi to 2^-12 relative). Score forms; its
             log-densities are fp16x3's
+  "bf16x3s" the bf16x3 cross term (BF16 range) with fp16x3s's phi.X (the FP16
+            pieces there are x - mu itself: |x - mu| < 6e4); log-densities
+            are bf16x3's
   "bf16x3"  BF16 hi/lo split, three products (hi.hi + lo.hi + hi.lo)
   "fp16"    single FP16 operands
   "bf16"    single BF16 operands
@@ -29,9 +32,9 @@ import torch
 
 from flash_sd_kde import _ops
 
-SCORE_PRECISIONS = ("fp32", "fp16x3", "fp16x3s", "bf16x6", "bf16x3", "fp16", "bf16")
-LOGDENSITY_PRECISIONS = ("fp32", "fp32_valu", "fp16x3", "fp16x3s", "bf16x6", "bf16x3", "fp16",
-                         "bf16")
+SCORE_PRECISIONS = ("fp32", "fp16x3", "fp16x3s", "bf16x6", "bf16x3", "bf16x3s", "fp16", "bf16")
+LOGDENSITY_PRECISIONS = ("fp32", "fp32_valu", "fp16x3", "fp16x3s", "bf16x6", "bf16x3", "bf16x3s",
+                         "fp16", "bf16")
 
 
 def kde_logdensity(data, queries, bandwidth: float, *, precision: str = "fp32",

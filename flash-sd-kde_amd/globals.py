@@ -18,6 +18,7 @@ PRECISION_BF16X3 = "bf16x3"
 PRECISION_BF16X6 = "bf16x6"
 PRECISION_FP16X3 = "fp16x3"
 PRECISION_FP16X3S = "fp16x3s"
+PRECISION_BF16X3S = "bf16x3s"
 
 KDE_BACKEND_SPLITK_STREAM = "splitk_stream"
 KDE_BACKEND_ATOMIC = "atomic"

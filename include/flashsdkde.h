@@ -50,6 +50,9 @@ extern "C" {
 #define FSK_PREC_F16X3S 7u   /* cross term as F16X3; phi.X with a single FP16 phi
                               * against FP16 [Xh | Xl] (one MFMA, no phi split):
                               * score forms only (log-densities as F16X3) */
+#define FSK_PREC_BF16X3S 8u  /* cross term as BF16X3 (BF16 range); phi.X as F16X3S
+                              * (FP16 x - mu pieces: |x - mu| < 6e4); log-densities
+                              * as BF16X3 */
 /* The FP16 forms (_F16, _F16X3, _F16X3S) hold -2c (q - mu) and c |x - mu|^2, with
  * c = log2(e) / (2 h^2) and mu the data mean, as FP16 pieces: the caller
  * keeps c * max |x - mu|^2 within FP16's range (the Python layer refuses

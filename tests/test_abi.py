@@ -87,9 +87,9 @@ def test_unsupported_flags_and_small_workspace():
     assert lib.fsk_kde1_splitk(_fake(), 10, _fake(), 10, 1.0, nat.PREC_BF16, _fake(), 1 << 20,
                                _fake(), None) == nat.FSK_EUNSUPPORTED
     # precision values past the last defined form are rejected
-    assert lib.fsk_kde16_logdensity(_fake(), 16, 10, _fake(), 16, 10, 1.0, nat.PREC_F16X3S + 1,
+    assert lib.fsk_kde16_logdensity(_fake(), 16, 10, _fake(), 16, 10, 1.0, nat.PREC_BF16X3S + 1,
                                     _fake(), 1 << 20, _fake(), None) == nat.FSK_EUNSUPPORTED
-    assert lib.fsk_score16_ordered(_fake(), 16, 10, 1.0, nat.PREC_F16X3S + 1, _fake(), 1 << 20,
+    assert lib.fsk_score16_ordered(_fake(), 16, 10, 1.0, nat.PREC_BF16X3S + 1, _fake(), 1 << 20,
                                    _fake(), _fake(), 16, 1, None) == nat.FSK_EUNSUPPORTED
     assert lib.fsk_kde16_splitk(_fake(), 16, 1000, _fake(), 16, 1000, 1.0, 0, _fake(), 16,
                                 _fake(), None) == nat.FSK_EWORKSPACE

@@ -41,6 +41,7 @@ def test_ref64_kernels_match_cpu_fp64(gpu, golden):
 # (|2c q.x| ~ 100 on this mixture), hence the per-form tolerances.
 @pytest.mark.parametrize("prec,tol_pdf,tol_w", [("fp32", 2e-5, 2e-5), ("bf16x6", 2e-5, 5e-5),
                                                  ("fp16x3", 2e-5, 5e-5), ("fp16x3s", 5e-5, 1e-4),
+                                                 ("bf16x3s", 1e-3, 5e-4),
                                                  ("bf16x3", 1e-3, 5e-4),
                                                  ("fp16", 5e-2, 2e-2), ("bf16", 0.5, 0.2)])
 def test_score_precision_forms(gpu, golden, prec, tol_pdf, tol_w):
@@ -58,7 +59,7 @@ def test_score_precision_forms(gpu, golden, prec, tol_pdf, tol_w):
 
 
 @pytest.mark.parametrize("prec,tol", [("bf16x3", 1e-3), ("bf16x6", 5e-5), ("fp16x3", 5e-5),
-                                      ("fp16x3s", 1e-3)])
+                                      ("fp16x3s", 1e-3), ("bf16x3s", 1e-3)])
 def test_score_lp_ragged_and_transposed_layout(gpu, prec, tol):
     for n in (1, 17, 300, 1025):
         x = np.random.default_rng(n).normal(size=(n, 16)).astype(np.float32) + 3.0
@@ -87,7 +88,7 @@ def test_score_rows_lp_forms_match_fp64(gpu, prec, tol, m, n):
     np.testing.assert_allclose(w.cpu().numpy(), we, rtol=tol, atol=tol * np.abs(we).max())
 
 
-@pytest.mark.parametrize("prec", ["bf16x3", "fp16x3", "fp16x3s", "bf16x6"])
+@pytest.mark.parametrize("prec", ["bf16x3", "fp16x3", "fp16x3s", "bf16x3s", "bf16x6"])
 def test_score_rows_on_the_data_equal_ordered(gpu, prec):
     """rows = data runs the same plan and kernel as the ordered score: the
     results are bitwise equal."""
@@ -99,7 +100,7 @@ def test_score_rows_on_the_data_equal_ordered(gpu, prec):
 
 
 @pytest.mark.parametrize("prec,tol", [("fp32", 1e-5), ("bf16x6", 1e-5), ("fp16x3", 1e-5),
-                                      ("fp16x3s", 1e-5), ("bf16x3", 1e-4),
+                                      ("fp16x3s", 1e-5), ("bf16x3", 1e-4), ("bf16x3s", 1e-4),
                                       ("fp16", 1e-3),
                                       ("bf16", 1.0)])
 def test_sdkde_logdensity_precision_forms(gpu, golden, prec, tol):

@@ -110,3 +110,8 @@ def test_fp16_forms_refuse_out_of_range_inputs():
     with pytest.raises(ValueError, match="FP16 forms"):
         _ops.check_fp16_range(x, x, 0.1, nat.PREC_F16X3)
     _ops.check_fp16_range(x, x, 10.0, nat.PREC_F16)  # in range
+    _ops.check_fp16_range(x, x, 0.1, nat.PREC_BF16X3S)  # only x - mu is FP16 there
+    with pytest.raises(ValueError, match="bf16x3s"):
+        y = torch.zeros((4, 16))
+        y[0, 0] = 1e5
+        _ops.check_fp16_range(y, y, 1.0, nat.PREC_BF16X3S)
