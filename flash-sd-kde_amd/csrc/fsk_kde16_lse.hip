@@ -428,12 +428,14 @@ extern "C" int fsk_kde16_logdensity(const float *data, int64_t ld, int64_t n,
   const int64_t n_pad = round_up(n, kValuChunk);
   launch_mean16(data, ld, n, L.mu, s);
   const double inv_n = 1.0 / double(n);
+  const double scl = lse_is16(prec) ? lowp_prescale(bandwidth) : 1.0;
+  const double coef_s = coef / (scl * scl);
 #define FSK_PREP(P)                                                                             \
   do {                                                                                          \
     hipLaunchKernelGGL(prep_lse_kernel<P>, dim3(cdiv(n_pad, 256)), dim3(256), 0, s, data, ld, n, \
-                       n_pad, L.mu, inv_n, coef, 0, L.xf, L.bn, L.xrec);                        \
+                       n_pad, L.mu, inv_n, scl, coef_s, 0, L.xf, L.bn, L.xrec);                 \
     hipLaunchKernelGGL(prep_lse_kernel<P>, dim3(cdiv(m, 256)), dim3(256), 0, s, query, ldq, m,  \
-                       m, L.mu, inv_n, coef, 1, L.qf, L.qa, L.qrec);                            \
+                       m, L.mu, inv_n, scl, coef_s, 1, L.qf, L.qa, L.qrec);                     \
   } while (0)
   switch (prec) {
     case FSK_PREC_F32: FSK_PREP(FSK_PREC_F32); break;

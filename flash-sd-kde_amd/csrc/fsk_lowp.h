@@ -96,7 +96,7 @@ __device__ __forceinline__ void split3(double v, unsigned short *p) {
 template <int PREC>
 __global__ void prep_lse_kernel(const float *__restrict__ x, int64_t ld, int64_t rows,
                                 int64_t rows_pad, const double *__restrict__ mu_sum, double inv_n,
-                                double coef, int side, float *__restrict__ f32_out,
+                                double scale, double coef, int side, float *__restrict__ f32_out,
                                 float *__restrict__ norm_out,
                                 unsigned short *__restrict__ rec_out) {
   int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -106,7 +106,7 @@ __global__ void prep_lse_kernel(const float *__restrict__ x, int64_t ld, int64_t
   double nrm = 0.0;
 #pragma unroll
   for (int f = 0; f < kD; ++f) {
-    double v = pad ? 0.0 : double(x[j * ld + f]) - mu_sum[f] * inv_n;
+    double v = pad ? 0.0 : (double(x[j * ld + f]) - mu_sum[f] * inv_n) * scale;
     xc[f] = v;
     nrm += v * v;
   }
@@ -158,6 +158,13 @@ __global__ void prep_lse_kernel(const float *__restrict__ x, int64_t ld, int64_t
   for (int k = 0; k < RL::kSize; ++k) rec_out[j * RL::kSize + k] = r[k];
 }
 
+// Power-of-two prescale of the 16-bit forms: points enter as (x - mu) * s with
+// s = 2^round(-log2 h), so the bandwidth becomes h s in [0.71, 1.41] and the
+// 16-bit pieces resolve distances relative to h whatever the data's units
+// (FP16 subnormals would otherwise swallow the low pieces of small-scale
+// data). Exact in binary arithmetic; c is divided by s^2 to match.
+inline double lowp_prescale(double h) { return exp2(round(-log2(h))); }
+
 __device__ __forceinline__ f4 mfma16(u16x8 a, u16x8 b, f4 c, int prec) {
   if (prec == FSK_PREC_F16)
     return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(h16x8, a),
@@ -177,7 +184,7 @@ void launch_mean16(const float *x, int64_t ld, int64_t n, double *mu_ws, hipStre
 __global__ void score_lp_finish_kernel(const float *__restrict__ pdf_part,
                                        const float *__restrict__ w_part, int64_t S, int64_t n,
                                        const double *__restrict__ mu_sum, double inv_n,
-                                       float *__restrict__ pdf, float *__restrict__ w,
-                                       int64_t ldw_row, int64_t ldw_col);
+                                       double inv_scale, float *__restrict__ pdf,
+                                       float *__restrict__ w, int64_t ldw_row, int64_t ldw_col);
 
 }  // namespace fsk

@@ -472,7 +472,7 @@ extern "C" int fsk_score16_ordered(const float *data, int64_t ld, int64_t n, dou
   prof_end(FSK_OP_SCORE16_ORDERED, s);
   if ((st = launch_status(what)) != FSK_OK) return st;
   hipLaunchKernelGGL(score_lp_finish_kernel, dim3(cdiv(n * kD, 256)), dim3(256), 0, s,
-                     L.pdf_part, L.w_part, p.splits, n, L.mu, inv_n, pdf, w, ldw_row, ldw_col);
+                     L.pdf_part, L.w_part, p.splits, n, L.mu, inv_n, 1.0, pdf, w, ldw_row, ldw_col);
   return launch_status(what);
 }
 
@@ -557,6 +557,6 @@ extern "C" int fsk_score16_rows(const float *rows, int64_t ldr, int64_t m, const
   prof_end(FSK_OP_SCORE16_ORDERED, s);
   if ((st = launch_status(what)) != FSK_OK) return st;
   hipLaunchKernelGGL(score_lp_finish_kernel, dim3(cdiv(m * kD, 256)), dim3(256), 0, s,
-                     L.pdf_part, L.w_part, p.splits, m, L.mu, inv_n, pdf, w, ldw_row, ldw_col);
+                     L.pdf_part, L.w_part, p.splits, m, L.mu, inv_n, 1.0, pdf, w, ldw_row, ldw_col);
   return launch_status(what);
 }

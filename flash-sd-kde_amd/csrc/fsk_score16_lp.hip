@@ -85,16 +85,16 @@ __device__ __forceinline__ int cross_slot_off(int s, int g) {
 }
 
 // Tile-transposed X for phi.X: per 16-point tile T, feature f and lane group g,
-// [Xh(4g+u, f) | Xl(4g+u, f)], u = 0..3, as P16 hi/lo pieces of x - mu.
+// [Xh(4g+u, f) | Xl(4g+u, f)], u = 0..3, as P16 hi/lo pieces of (x - mu) * scale.
 template <int P16>
 __global__ void prep_xt16_kernel(const float *__restrict__ x, int64_t ld, int64_t n,
                                  int64_t n_pad, const double *__restrict__ mu_sum, double inv_n,
-                                 unsigned short *__restrict__ xt) {
+                                 double scale, unsigned short *__restrict__ xt) {
   const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (e >= n_pad * kD) return;
   const int64_t j = e / kD;
   const int f = int(e % kD);
-  const float v = j < n ? float(double(x[j * ld + f]) - mu_sum[f] * inv_n) : 0.f;
+  const float v = j < n ? float((double(x[j * ld + f]) - mu_sum[f] * inv_n) * scale) : 0.f;
   const unsigned short hi = to16<P16>(v);
   const unsigned short lo = to16<P16>(v - from16<P16>(hi));
   const int64_t T = j / kTile;
@@ -336,12 +336,13 @@ __global__ __launch_bounds__(kWG) void score16_lp_kernel(
   }
 }
 
-// pdf[i] = sum_s pdf_part; w[i,f] = sum_s w_part + mu_f pdf[i]  (FP64), n rows.
+// pdf[i] = sum_s pdf_part; w[i,f] = sum_s w_part / scale + mu_f pdf[i]
+// (FP64), n rows.
 __global__ void score_lp_finish_kernel(const float *__restrict__ pdf_part,
                                        const float *__restrict__ w_part, int64_t S, int64_t n,
                                        const double *__restrict__ mu_sum, double inv_n,
-                                       float *__restrict__ pdf, float *__restrict__ w,
-                                       int64_t ldw_row, int64_t ldw_col) {
+                                       double inv_scale, float *__restrict__ pdf,
+                                       float *__restrict__ w, int64_t ldw_row, int64_t ldw_col) {
   const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (e >= n * kD) return;
   const int64_t i = e / kD;
@@ -351,7 +352,7 @@ __global__ void score_lp_finish_kernel(const float *__restrict__ pdf_part,
     p += double(pdf_part[k * n + i]);
     s += double(w_part[(k * n + i) * kD + f]);
   }
-  w[i * ldw_row + f * ldw_col] = float(s + mu_sum[f] * inv_n * p);
+  w[i * ldw_row + f * ldw_col] = float(s * inv_scale + mu_sum[f] * inv_n * p);
   if (f == 0) pdf[i] = float(p);
 }
 
@@ -416,7 +417,8 @@ int score16_lp_run(const char *what, const float *rows_in, int64_t ldr, int64_t 
     return FSK_EWORKSPACE;
   }
   hipStream_t s = as_stream(stream);
-  const double coef = exp2_coef(h);
+  const double scl = lowp_prescale(h);
+  const double coef = exp2_coef(h) / (scl * scl);
   const int64_t xrows = round_up(n, kNormPad), qrows = round_up(m, kNormPad);
   const int64_t n16 = L.ntiles_data * kTile;
   const double inv_n = 1.0 / double(n);
@@ -429,12 +431,14 @@ int score16_lp_run(const char *what, const float *rows_in, int64_t ldr, int64_t 
 #define FSK_LP(P, PXT, QT)                                                                       \
   do {                                                                                           \
     hipLaunchKernelGGL(prep_lse_kernel<P>, dim3(cdiv(xrows, 256)), dim3(256), 0, s, data, ld, n, \
-                       xrows, L.mu, inv_n, coef, 0, (float *)nullptr, (float *)nullptr, L.xrec);  \
+                       xrows, L.mu, inv_n, scl, coef, 0, (float *)nullptr, (float *)nullptr,      \
+                       L.xrec);                                                                  \
     hipLaunchKernelGGL(prep_lse_kernel<P>, dim3(cdiv(qrows, 256)), dim3(256), 0, s, rows_in,     \
-                       ldr, m, qrows, L.mu, inv_n, coef, 1, (float *)nullptr, (float *)nullptr,   \
+                       ldr, m, qrows, L.mu, inv_n, scl, coef, 1, (float *)nullptr,                \
+                       (float *)nullptr,                                                         \
                        L.qrec);                                                                  \
     hipLaunchKernelGGL(prep_xt16_kernel<PXT>, dim3(cdiv(n16 * kD, 256)), dim3(256), 0, s, data,  \
-                       ld, n, n16, L.mu, inv_n, L.xt);                                           \
+                       ld, n, n16, L.mu, inv_n, scl, L.xt);                                      \
     hipLaunchKernelGGL(prep_lp_blocks_kernel<P>,                                                 \
                        dim3(cdiv(L.ntiles_alloc * LpBlock<P>::kSlots * 64, 256)), dim3(256), 0,  \
                        s, L.xrec, L.xt, L.ntiles_data, L.ntiles_alloc, L.blk);                    \
@@ -457,7 +461,8 @@ int score16_lp_run(const char *what, const float *rows_in, int64_t ldr, int64_t 
   int st = launch_status(what);
   if (st != FSK_OK) return st;
   hipLaunchKernelGGL(score_lp_finish_kernel, dim3(cdiv(m * kD, 256)), dim3(256), 0, s,
-                     L.pdf_part, L.w_part, p.splits, m, L.mu, inv_n, pdf, w, ldw_row, ldw_col);
+                     L.pdf_part, L.w_part, p.splits, m, L.mu, inv_n, 1.0 / scl, pdf, w, ldw_row,
+                     ldw_col);
   return launch_status(what);
 }
 

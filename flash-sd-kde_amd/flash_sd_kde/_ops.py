@@ -8,6 +8,7 @@ top of them.
 
 from __future__ import annotations
 
+import math
 from typing import Sequence
 
 import torch
@@ -100,28 +101,31 @@ FP16_SAFE = 3.0e4  # largest magnitude an FP16 record value may take (FP16 max 6
 
 
 def check_fp16_range(train: torch.Tensor, query: torch.Tensor, h: float, flags: int) -> None:
-    """The FP16 forms store the centred, scaled coordinates -2c (q - mu) and the
-    norm terms c |x - mu|^2 (c = log2(e) / (2 h^2) in magnitude) as FP16
-    pieces; refuse inputs whose values would leave FP16's range instead of
-    returning inf / NaN. Skipped while a HIP graph is being captured (it needs
-    a device-to-host read)."""
+    """The FP16 forms store the centred coordinates, prescaled by s =
+    2^round(-log2 h), as FP16 pieces: x' = (x - mu) s on the data side,
+    -2c' (q - mu) s on the query side and the norm terms c |x - mu|^2
+    (c = log2(e) / (2 h^2) in magnitude, c' = c / s^2); refuse inputs whose
+    values would leave FP16's range instead of returning inf / NaN. Skipped
+    while a HIP graph is being captured (it needs a device-to-host read)."""
     if flags not in (nat.PREC_F16, nat.PREC_F16X3, nat.PREC_F16X3S, nat.PREC_BF16X3S):
         return
     if train.is_cuda and torch.cuda.is_current_stream_capturing():
         return  # the check reads a device value; under graph capture the caller vouches
     mu = train.mean(0)
+    s = 2.0 ** round(-math.log2(h))
     if flags == nat.PREC_BF16X3S:
-        # only the phi.X operand is FP16 there: the pieces of x - mu itself
-        amax = float((train - mu).abs().max())
+        # only the phi.X operand is FP16 there: the pieces of (x - mu) s
+        amax = float((train - mu).abs().max()) * s
         if amax > FP16_SAFE:
-            raise ValueError(f"bf16x3s holds x - mu as FP16 pieces: max|x - mu| = {amax:.3g} "
-                             f"exceeds {FP16_SAFE:g}; use bf16x3 / fp32.")
+            raise ValueError(f"bf16x3s holds (x - mu) s as FP16 pieces: {amax:.3g} exceeds "
+                             f"{FP16_SAFE:g} (points more than {FP16_SAFE:g} bandwidths from their "
+                             "mean); use bf16x3 / fp32.")
         return
     r2 = float(((train - mu) ** 2).sum(1).max())
     if query is not train:
         r2 = max(r2, float(((query - mu) ** 2).sum(1).max()))
     c = 1.4426950408889634 / (2.0 * h * h)
-    if c * r2 > FP16_SAFE or 2.0 * c * r2 ** 0.5 > FP16_SAFE:
+    if c * r2 > FP16_SAFE or 2.0 * c * r2 ** 0.5 / s > FP16_SAFE:
         raise ValueError(
             f"FP16 forms need c*max|x-mu|^2 <= {FP16_SAFE:g} (got {c * r2:.3g}): the points are too "
             "far from their mean for this bandwidth; use bf16x3 / fp32.")

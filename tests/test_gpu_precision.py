@@ -88,6 +88,28 @@ def test_score_rows_lp_forms_match_fp64(gpu, prec, tol, m, n):
     np.testing.assert_allclose(w.cpu().numpy(), we, rtol=tol, atol=tol * np.abs(we).max())
 
 
+@pytest.mark.parametrize("prec", ["fp16x3", "fp16x3s", "bf16x3s", "bf16x3", "fp16"])
+@pytest.mark.parametrize("unit", [1e-6, 1.0, 1e5])
+def test_lowp_forms_are_unit_free(gpu, prec, unit):
+    """The 16-bit forms prescale the centred points by a power of two chosen
+    from the bandwidth, so data in any units (here 1e-6 and 1e5 times the
+    unit-scale case, bandwidth scaled alike) reach the same accuracy against
+    FP64: without it FP16 pieces of 1e-6-scale data fall into subnormals."""
+    rng = np.random.default_rng(17)
+    x = (rng.normal(size=(1500, 16)) * unit).astype(np.float32)
+    h = 1.1 * unit
+    pdf, w = _ops.score16(torch.as_tensor(x, device=gpu), h, variant="ordered", precision=prec)
+    pe, we = fp64.score_sums(x.astype(np.float64), h)
+    tol = 5e-2 if prec == "fp16" else 1e-3
+    np.testing.assert_allclose(pdf.cpu().numpy(), pe, rtol=tol)
+    np.testing.assert_allclose(w.cpu().numpy(), we, rtol=tol, atol=tol * np.abs(we).max())
+    q = torch.as_tensor(x[:300] + np.float32(0.3 * unit), device=gpu)
+    lp = _ops.kde16_logdensity(torch.as_tensor(x, device=gpu), q, h, precision=prec)
+    lref = fp64.log_density(x[:300].astype(np.float64) + 0.3 * unit, x.astype(np.float64), h)
+    rel = np.abs(lp.cpu().numpy() - lref) / np.abs(lref)
+    assert rel.max() < (1e-2 if prec == "fp16" else 1e-4), rel.max()
+
+
 @pytest.mark.parametrize("prec", ["bf16x3", "fp16x3", "fp16x3s", "bf16x3s", "bf16x6"])
 def test_score_rows_on_the_data_equal_ordered(gpu, prec):
     """rows = data runs the same plan and kernel as the ordered score: the
