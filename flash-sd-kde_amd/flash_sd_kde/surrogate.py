@@ -297,13 +297,14 @@ def surrogate_fit_transform(data, bandwidth: float, *, n_label: int = 262144,
                             n_holdout: int = 16384, n_probe: int = 16384, tol: float = 1e-3,
                             check_margin: float = 0.5, steps: int = 8000, batch: int = 8192,
                             lr: float = 6e-3, seed: int = 0, device="cuda",
-                            net: ScoreMLP | None = None, label_precision: str = "fp16x3"):
+                            net: ScoreMLP | None = None, label_precision: str = "fp16x3s"):
     """One-step SD-KDE debiasing with the learned shift field.
 
     The training labels come from the score kernel in `label_precision`
-    (default fp16x3: FP32-level accuracy, about 1e-6 relative, at twice the
-    FP32 form's rate; FP32 when the inputs leave FP16's range). The held-out
-    check and the fallback use the FP32 form.
+    (default fp16x3s: FP32-class log-density accuracy at three times the FP32
+    form's rate; the label rows are data points, so each row's self pair is
+    in its sums; FP32 when the inputs leave FP16's range). The held-out check
+    and the fallback use the FP32 form.
 
     Returns (debiased (n,16) tensor, info dict). info["fallback"] is True when
     the held-out log-density error exceeded check_margin * tol and the exact
