@@ -33,7 +33,7 @@ constexpr int kScoreLpQT = FSK_LP_QT;
 constexpr int kScoreLpQTx6 = FSK_LP_QT_X6;
 inline int score_lp_qt(unsigned prec) { return prec == FSK_PREC_BF16X6 ? kScoreLpQTx6 : kScoreLpQT; }
 // Symmetric score: I block = rows of one workgroup (4 waves x kSymQT tiles).
-constexpr int kSymQT = 4;
+constexpr int kSymQT = 8;
 // Target number of (query block, split) tasks per launch: a few per CU slot.
 constexpr int64_t kTargetTasks = 8192;
 // Cap on split-K partial workspace for the ordered score (bytes).

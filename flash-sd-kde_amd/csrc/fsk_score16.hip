@@ -12,8 +12,8 @@
 //
 // Ordered variant: deterministic split-K, FP32 partials per split, FP64
 // combine (plus FP64 outer sums inside each split).
-// Symmetric variant: each unordered 256x256 block pair once; the column side
-// (W_J += phi^T X_I) needs phi transposed, done through a per-wave LDS tile;
+// Symmetric variant: each unordered 512x512 block pair once; the column side
+// (W_J += phi^T X_I) needs phi transposed, done through per-wave LDS slots;
 // column sums are gathered in LDS per J block and scattered with FP64 global
 // atomics, then converted to the FP32 outputs.
 #include <type_traits>
@@ -28,11 +28,10 @@ namespace fsk {
 constexpr int kScoreFlush = 32;
 static_assert(kNormPad % kScoreRowsPerWG == 0, "row norms must cover whole row blocks");
 
+// Symmetric score: J blocks per task. The row sums of a task stay in FP32
+// over jc blocks before their FP64 atomic add.
 #ifndef FSK_SYM_JC
-#define FSK_SYM_JC 8
-#endif
-#ifndef FSK_SYM_BAND
-#define FSK_SYM_BAND 1
+#define FSK_SYM_JC 2
 #endif
 
 // Largest spread of the query terms A_i = -c|q_i|^2 (c < 0) inside one
@@ -177,174 +176,292 @@ __global__ __launch_bounds__(kWG) void score16_ordered_kernel(
   }
 }
 
-// Symmetric variant. Task = (I block, window of jc J blocks), blocks of B
-// rows, only J >= I. For J > I both sides accumulate; J == I runs the I side
-// over the whole diagonal block (every ordered pair inside it exactly once).
-// Windows are placed relative to I (chunk k covers J in [I + k jc, I + (k+1) jc)),
-// so the workgroups an XCD runs together (consecutive I, same k) scatter
-// their column sums into staggered J blocks instead of the same ones.
-template <int QT, int WAVES>
-__global__ __launch_bounds__(64 * WAVES) void score16_sym_kernel(
-    const float *__restrict__ data, int64_t ld, int64_t n, const float *__restrict__ bn,
-    float qscale, int64_t nb, int64_t jc, int64_t ntasks, int64_t per_xcd,
+// Symmetric variant: every unordered pair of blocks (I, J), J >= I, of
+// B = 4 waves x 16 QT points, once. The workgroup holds block I in registers
+// (a wave owns QT row tiles) and streams the data tiles of up to jc blocks J.
+//  * Row side (pdf_I, W_I): as the ordered kernel; phi is the A operand of
+//    W_I += phi . X_J in the layout the cross-term MFMA leaves it in.
+//  * Column side (W_J += phi^T . X_I): each phi tile goes to LDS (four
+//    ds_write_b32 per lane) and comes back transposed (one ds_read_b128);
+//    rows padded to kSymTS floats keep both accesses free of bank conflicts.
+//    The column pdf sums are FMAs in the original layout, reduced over the 16
+//    lanes of a DPP row once per data tile.
+//  * Each wave stores its column sums of a data tile in its own LDS region;
+//    every kSymFlushTiles tiles the workgroup adds the four regions and sends
+//    the totals to the FP64 accumulators with global atomics. Row sums go out
+//    once per task.
+// The diagonal block J = I runs the row side only, over every ordered pair in
+// it. Points are centred on the data mean (zero rows pad the last block). The
+// factored form of the ordered kernel applies per workgroup (same spread
+// guard): the per-row factor f_i = exp2(qa_i + K) enters the column side
+// through its B operand (f_i x_i) and the column pdf FMA, and the row sums
+// once at the end.
+// Tasks: window k of block I covers J in [I + k jc, I + (k+1) jc). Tasks are
+// numbered window-major (window k holds the nb - k jc blocks I with
+// I + k jc < nb), so the workgroups one XCD runs together have neighbouring I
+// and the same k: they share J data in L2 and scatter into staggered blocks.
+__host__ __device__ inline int64_t sym_window_offset(int64_t k, int64_t nb, int64_t jc) {
+  return k * nb - jc * (k * (k - 1) / 2);
+}
+
+template <int CTRL>
+__device__ __forceinline__ float dpp_f32(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL,
+                                                               0xF, 0xF, false));
+}
+// Sum over the 16 lanes of a DPP row (lanes with equal lane / 16); every lane
+// of the row gets the total.
+__device__ __forceinline__ float row16_sum(float v) {
+  v += dpp_f32<0xB1>(v);   // quad_perm [1,0,3,2]
+  v += dpp_f32<0x4E>(v);   // quad_perm [2,3,0,1]
+  v += dpp_f32<0x141>(v);  // row_half_mirror
+  v += dpp_f32<0x140>(v);  // row_mirror
+  return v;
+}
+
+constexpr int kSymTS = 20;        // LDS row stride (floats) of transposed tiles and column sums
+constexpr int kSymSlots = 4;      // phi tiles a wave keeps in LDS at a time
+constexpr int kSymFlushTiles = 8;  // data tiles between two flushes of the column sums
+
+template <int QT>
+__global__ __launch_bounds__(kWG) void score16_sym_kernel(
+    const float *__restrict__ xc, const float *__restrict__ bn, int64_t n, float qscale,
+    int64_t nb, int64_t jc, int64_t nwin, int64_t ntasks, int64_t per_xcd,
     double *__restrict__ pdf64, double *__restrict__ w64) {
-  constexpr int B = WAVES * 16 * QT;
-  constexpr int TS = 17;  // padded transpose row
-  __shared__ float tbuf[WAVES][16 * TS];
-  __shared__ float jw[B * kD];
-  __shared__ float jp[B];
+  constexpr int B = 4 * 16 * QT;
+  constexpr int TS = kSymTS;
+  constexpr int NS = kSymSlots;
+  constexpr int FT = kSymFlushTiles;
+  static_assert(QT % NS == 0, "row tiles are processed in groups of the slot count");
+  static_assert((B / kTile) % FT == 0, "a block holds whole flush groups");
+  __shared__ float tslot[4][NS][16 * TS];
+  __shared__ float colw[4][FT * kTile * TS];  // per wave: column sums W_J of FT tiles
+  __shared__ float colp[4][FT * kTile];       // per wave: column pdf sums of FT tiles
+  __shared__ float red[8];
 
   const int64_t task = xcd_task(blockIdx.x, per_xcd);
   if (task >= ntasks) return;
-  const int64_t chunk = task / nb;
-  const int64_t I = task - chunk * nb;
-#if FSK_SYM_BAND
-  const int64_t Jlo = I + chunk * jc;
-  int64_t Jhi = Jlo + jc;
-  if (Jhi > nb) Jhi = nb;
-#else
-  int64_t Jlo = chunk * jc;
-  int64_t Jhi = Jlo + jc;
-  if (Jhi > nb) Jhi = nb;
-  if (Jlo < I) Jlo = I;
-#endif
-  if (Jlo >= Jhi) return;
+  int64_t k_lo = 0, k_hi = nwin - 1;  // largest window k with offset(k) <= task
+  while (k_lo < k_hi) {
+    const int64_t mid = (k_lo + k_hi + 1) >> 1;
+    if (sym_window_offset(mid, nb, jc) <= task)
+      k_lo = mid;
+    else
+      k_hi = mid - 1;
+  }
+  const int64_t I = task - sym_window_offset(k_lo, nb, jc);
+  const int64_t Jlo = I + k_lo * jc;
+  const int64_t Jhi = Jlo + jc < nb ? Jlo + jc : nb;
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
   const int g = lane >> 4, c = lane & 15;
-  for (int e = tid; e < B * kD; e += 64 * WAVES) jw[e] = 0.f;
-  for (int e = tid; e < B; e += 64 * WAVES) jp[e] = 0.f;
-  __syncthreads();
 
+  // xc and bn cover nb * B rows (zero points, kPadNeg norms past n)
   const int64_t row0 = I * B + int64_t(wave) * (16 * QT);
-  f4 qv[QT], wI[QT];
-  float qav[QT], pI[QT], xIt[QT][4];
+  f4 qv[QT], wI[QT], xit[QT];
+  float qav[QT], fI[QT], pI[QT];
 #pragma unroll
   for (int R = 0; R < QT; ++R) {
-    int64_t i = row0 + 16 * R + c;
-    int64_t ic = i > n - 1 ? n - 1 : i;
-    qv[R] = ld4(data + ic * ld + 4 * g) * qscale;
-    qav[R] = bn[i];  // kPadNeg for padded rows: their phi is exactly 0
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      int64_t it = row0 + 16 * R + 4 * g + r;
-      if (it > n - 1) it = n - 1;
-      xIt[R][r] = data[it * ld + c];
-    }
+    const int64_t i = row0 + 16 * R + c;
+    qv[R] = ld4(xc + i * kD + 4 * g) * qscale;
+    qav[R] = bn[i];
     wI[R] = f4{0.f, 0.f, 0.f, 0.f};
     pI[R] = 0.f;
   }
-  float *tb = tbuf[wave];
 
-  for (int64_t J = Jlo; J < Jhi; ++J) {
-    const bool diag = (J == I);
-    for (int jt = 0; jt < B / kTile; ++jt) {
-      const int64_t j0 = J * B + jt * kTile;
-      if (j0 >= n) break;  // uniform across the workgroup
-      int64_t jr = j0 + c;
-      if (jr > n - 1) jr = n - 1;
-      const f4 xa = ld4(data + jr * ld + 4 * g);
-      const f4 bb = ld4(bn + j0 + 4 * g);
-      float xt[4];
+  // workgroup range of A_i = -qa_i over its real rows (row I*B is always real)
+  float amin = INFINITY, amax = -INFINITY;
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        int64_t jj = j0 + 4 * g + r;
-        if (jj > n - 1) jj = n - 1;
-        xt[r] = data[jj * ld + c];
-      }
-      f4 d[QT];
-#pragma unroll
-      for (int R = 0; R < QT; ++R) {
-        d[R][0] = qav[R] + bb[0];
-        d[R][1] = qav[R] + bb[1];
-        d[R][2] = qav[R] + bb[2];
-        d[R][3] = qav[R] + bb[3];
-      }
-#pragma unroll
-      for (int s = 0; s < 4; ++s) {
-#pragma unroll
-        for (int R = 0; R < QT; ++R) d[R] = mfma_f32(xa[s], qv[R][s], d[R]);
-      }
-#pragma unroll
-      for (int R = 0; R < QT; ++R) {
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          float e = fexp2_le1(d[R][r]);
-          d[R][r] = e;
-          pI[R] += e;
-        }
-      }
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-#pragma unroll
-        for (int R = 0; R < QT; ++R) wI[R] = mfma_f32(d[R][r], xt[r], wI[R]);
-      }
-      if (!diag) {
-        f4 wJ = f4{0.f, 0.f, 0.f, 0.f};
-        float pj = 0.f;
-#pragma unroll
-        for (int R = 0; R < QT; ++R) {
-          // lane holds phi[i=c][j=4g+r]; W_J needs phi[i=4g+r'][j=c]
-#pragma unroll
-          for (int r = 0; r < 4; ++r) tb[c * TS + 4 * g + r] = d[R][r];
-          __builtin_amdgcn_wave_barrier();
-          float et[4];
-#pragma unroll
-          for (int r = 0; r < 4; ++r) et[r] = tb[(4 * g + r) * TS + c];
-          __builtin_amdgcn_wave_barrier();
-          pj += (et[0] + et[1]) + (et[2] + et[3]);
-#pragma unroll
-          for (int r = 0; r < 4; ++r) wJ = mfma_f32(et[r], xIt[R][r], wJ);
-        }
-        atomicAdd(&jp[jt * kTile + c], pj);
-#pragma unroll
-        for (int r = 0; r < 4; ++r) atomicAdd(&jw[(jt * kTile + 4 * g + r) * kD + c], wJ[r]);
-      }
-    }
-    if (!diag) {
-      __syncthreads();
-      const int64_t jbase = J * B;
-      for (int e = tid; e < B * kD; e += 64 * WAVES) {
-        int64_t j = jbase + e / kD;
-        if (j < n) atomicAdd(&w64[j * kD + (e % kD)], double(jw[e]));
-        jw[e] = 0.f;
-      }
-      for (int e = tid; e < B; e += 64 * WAVES) {
-        int64_t j = jbase + e;
-        if (j < n) atomicAdd(&pdf64[j], double(jp[e]));
-        jp[e] = 0.f;
-      }
-      __syncthreads();
+  for (int R = 0; R < QT; ++R) {
+    if (row0 + 16 * R + c < n) {
+      amin = fminf(amin, -qav[R]);
+      amax = fmaxf(amax, -qav[R]);
     }
   }
 #pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) {
+    amin = fminf(amin, __shfl_xor(amin, o));
+    amax = fmaxf(amax, __shfl_xor(amax, o));
+  }
+  if (lane == 0) {
+    red[wave] = amin;
+    red[4 + wave] = amax;
+  }
+  __syncthreads();  // also orders the LDS zero fill before the first use
+  amin = fminf(fminf(red[0], red[1]), fminf(red[2], red[3]));
+  amax = fmaxf(fmaxf(red[4], red[5]), fmaxf(red[6], red[7]));
+  const bool fact = amax - amin <= kFactSpread;
+  const float kq = 0.5f * (amax + amin);
+
+  // per-row factors (padded rows: f = 0 in the factored form; their points are
+  // zero, so x f = 0 in both forms) and the column side's B operand f_i x_i
+#pragma unroll
   for (int R = 0; R < QT; ++R) {
-    float p = sum_groups(pI[R]);
-    int64_t i = row0 + 16 * R + c;
-    if (g == 0 && i < n) atomicAdd(&pdf64[i], double(p));
+    fI[R] = fact ? fexp2(qav[R] + kq) : 1.f;
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const int64_t it = row0 + 16 * R + 4 * g + s;
+      const float f = fact ? fexp2(bn[it] + kq) : 1.f;
+      xit[R][s] = xc[it * kD + c] * f;
+    }
+  }
+
+  auto run = [&](auto factored) {
+    constexpr bool F = decltype(factored)::value;
+    // cross term of row tile R into d: factored form starts from the data-side
+    // term ci = c|x_j|^2 - K, the direct form from qa_i + c|x_j|^2
+    auto cross = [&](const TileData &t, const f4 &ci, int R, f4 &d) {
+      if constexpr (F) {
+        d = mfma_f32(t.xa[0], qv[R][0], ci);
+      } else {
+        d = f4{qav[R] + t.bb[0], qav[R] + t.bb[1], qav[R] + t.bb[2], qav[R] + t.bb[3]};
+        d = mfma_f32(t.xa[0], qv[R][0], d);
+      }
+    };
+    auto kern = [&](float v) { return F ? fexp2(v) : fexp2_le1(v); };
+    for (int64_t J = Jlo; J < Jhi; ++J) {
+      const int64_t jbase = J * B;
+      if (J == I) {
+        // diagonal block: row side over every ordered pair inside it
+        for (int jt = 0; jt < B / kTile; ++jt) {
+          TileData t;
+          load_tile<false, true>(xc, kD, nb * B, bn, jbase + jt * kTile, g, c, t);
+          const f4 ci = t.bb - f4{kq, kq, kq, kq};
+          f4 d[QT];
+#pragma unroll
+          for (int R = 0; R < QT; ++R) cross(t, ci, R, d[R]);
+#pragma unroll
+          for (int s = 1; s < 4; ++s) {
+#pragma unroll
+            for (int R = 0; R < QT; ++R) d[R] = mfma_f32(t.xa[s], qv[R][s], d[R]);
+          }
+#pragma unroll
+          for (int R = 0; R < QT; ++R) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              const float e = kern(d[R][r]);
+              pI[R] += e;
+              d[R][r] = e;
+            }
+          }
+          phi_x_tile<QT>(d, t, wI);
+        }
+        continue;
+      }
+      for (int jt = 0; jt < B / kTile; ++jt) {
+        TileData t;
+        load_tile<false, true>(xc, kD, nb * B, bn, jbase + jt * kTile, g, c, t);
+        const f4 ci = t.bb - f4{kq, kq, kq, kq};
+        f4 wJ = f4{0.f, 0.f, 0.f, 0.f};
+        float pj[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int hb = 0; hb < QT / NS; ++hb) {
+          f4 d[NS];
+#pragma unroll
+          for (int q = 0; q < NS; ++q) cross(t, ci, hb * NS + q, d[q]);
+#pragma unroll
+          for (int s = 1; s < 4; ++s) {
+#pragma unroll
+            for (int q = 0; q < NS; ++q) d[q] = mfma_f32(t.xa[s], qv[hb * NS + q][s], d[q]);
+          }
+          // lane holds phi(i = c, j = 4g + r): row sums, column pdf FMAs, and
+          // the transposed copy T[j][i] in this wave's LDS slot q
+#pragma unroll
+          for (int q = 0; q < NS; ++q) {
+            const int R = hb * NS + q;
+            float *sb = tslot[wave][q];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              const float e = kern(d[q][r]);
+              pI[R] += e;
+              pj[r] = fmaf(e, fI[R], pj[r]);
+              d[q][r] = e;
+              sb[(4 * g + r) * TS + c] = e;
+            }
+          }
+          __builtin_amdgcn_wave_barrier();
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+#pragma unroll
+            for (int q = 0; q < NS; ++q) wI[hb * NS + q] = mfma_f32(d[q][r], t.xt[r], wI[hb * NS + q]);
+          }
+          // column side: A = phi(i = 4g + s, j = c), B = f_i x_i of the same rows
+#pragma unroll
+          for (int q = 0; q < NS; ++q) {
+            const f4 et = ld4(&tslot[wave][q][c * TS + 4 * g]);
+#pragma unroll
+            for (int s = 0; s < 4; ++s) wJ = mfma_f32(et[s], xit[hb * NS + q][s], wJ);
+          }
+          __builtin_amdgcn_wave_barrier();
+        }
+        // pj[r]: this lane's part of the column pdf of point 4g + r. The
+        // wave's column sums of this tile go to its own LDS region (plain
+        // stores: LDS float atomics measured 40 % slower for the whole kernel)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) pj[r] = row16_sum(pj[r]);
+        const float pv = (c & 2) ? ((c & 1) ? pj[3] : pj[2]) : ((c & 1) ? pj[1] : pj[0]);
+        const int lt = jt % FT;
+        if (c < 4) colp[wave][lt * kTile + 4 * g + c] = pv;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) colw[wave][(lt * kTile + 4 * g + r) * TS + c] = wJ[r];
+        if (lt == FT - 1) {
+          // the four waves' sums of the last FT tiles -> FP64 accumulators
+          __syncthreads();
+          const int64_t p0 = jbase + (jt - (FT - 1)) * kTile;
+          for (int e = tid; e < FT * kTile * kD; e += kWG) {
+            const int pl = e >> 4, f = e & 15;
+            const float v = (colw[0][pl * TS + f] + colw[1][pl * TS + f]) +
+                            (colw[2][pl * TS + f] + colw[3][pl * TS + f]);
+            if (p0 + pl < n) atomicAdd(&w64[(p0 + pl) * kD + f], double(v));
+          }
+          for (int e = tid; e < FT * kTile; e += kWG) {
+            const float v = (colp[0][e] + colp[1][e]) + (colp[2][e] + colp[3][e]);
+            if (p0 + e < n) atomicAdd(&pdf64[p0 + e], double(v));
+          }
+          __syncthreads();
+        }
+      }
+    }
+  };
+  if (fact)
+    run(std::true_type{});
+  else
+    run(std::false_type{});
+
+  // row sums, times the per-row factor of the factored form (FP64)
+#pragma unroll
+  for (int R = 0; R < QT; ++R) {
+    const double p = sum_groups(double(pI[R]));
+    const int64_t i = row0 + 16 * R + c;
+    if (g == 0 && i < n) atomicAdd(&pdf64[i], fact ? p * exp2(double(qav[R]) + double(kq)) : p);
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      int64_t iw = row0 + 16 * R + 4 * g + r;
-      if (iw < n) atomicAdd(&w64[iw * kD + c], double(wI[R][r]));
+      const int64_t iw = row0 + 16 * R + 4 * g + r;
+      if (iw < n) {
+        const double sc = fact ? exp2(double(bn[iw]) + double(kq)) : 1.0;
+        atomicAdd(&w64[iw * kD + c], double(wI[R][r]) * sc);
+      }
     }
   }
 }
 
+// pdf = pdf64, w = w64 + mu pdf64 (FP64) for sums over points centred on mu.
 __global__ void score_finish_kernel(const double *__restrict__ pdf64,
                                     const double *__restrict__ w64, int64_t n,
+                                    const double *__restrict__ mu_sum, double inv_n,
                                     float *__restrict__ pdf, float *__restrict__ w,
                                     int64_t ldw_row, int64_t ldw_col) {
   int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (e >= n * kD) return;
   int64_t i = e / kD, f = e % kD;
-  w[i * ldw_row + f * ldw_col] = float(w64[e]);
+  w[i * ldw_row + f * ldw_col] = float(w64[e] + mu_sum[f] * inv_n * pdf64[i]);
   if (f == 0) pdf[i] = float(pdf64[i]);
 }
 
-constexpr int kSymWaves = 4;
-constexpr int kSymB = kSymWaves * 16 * kSymQT;
+constexpr int kSymRows = 4 * 16 * kSymQT;
+static_assert(kNormPad % kSymRows == 0, "padded rows must cover whole symmetric blocks");
 constexpr int64_t kSymJC = FSK_SYM_JC;  // J blocks per task
 
 // Points centred on the data mean: x - mu (FP64 difference, FP32 result),
@@ -376,12 +493,11 @@ static size_t score_layout(void *ws, int64_t n, bool ordered, ScoreLayout *L, in
   L->qa = nullptr;
   L->pdf_part = L->w_part = nullptr;
   L->pdf64 = L->w64 = nullptr;
-  L->mu = nullptr;
-  L->xc = L->rc = nullptr;
-  if (m >= 0 || ordered) {
-    L->mu = cv.take<double>(kMeanWs);
-    L->xc = cv.take<float>(round_up(n, kTile) * kD);
-  }
+  L->rc = nullptr;
+  // data mean and centred data (the symmetric kernel reads whole blocks of
+  // kSymRows points, so its copy is padded with zero rows to kNormPad)
+  L->mu = cv.take<double>(kMeanWs);
+  L->xc = cv.take<float>(round_up(n, (m < 0 && !ordered) ? kNormPad : kTile) * kD);
   if (m >= 0) {  // separate rows
     L->qa = cv.take<float>(round_up(m, kNormPad));
     L->rc = cv.take<float>(m * kD);
@@ -492,25 +608,34 @@ extern "C" int fsk_score16_sym_atomic(const float *data, int64_t ld, int64_t n,
   hipStream_t s = as_stream(stream);
   const double coef = exp2_coef(bandwidth);
   const int64_t n_norm = round_up(n, kNormPad);
-  hipLaunchKernelGGL(prep_norms16, dim3(cdiv(n_norm, 256)), dim3(256), 0, s, data, ld, n, n_norm,
-                     coef, L.bn);
+  const double inv_n = 1.0 / double(n);
+  // centre on the data mean; rows [n, n_norm) of the copy are zero
+  launch_mean16(data, ld, n, L.mu, s);
+  hipLaunchKernelGGL(center16_kernel, dim3(cdiv(n_norm * kD, 256)), dim3(256), 0, s, data, ld, n,
+                     n_norm, L.mu, inv_n, L.xc);
+  hipLaunchKernelGGL(prep_norms16, dim3(cdiv(n_norm, 256)), dim3(256), 0, s, L.xc, int64_t(kD), n,
+                     n_norm, coef, L.bn);
   hipLaunchKernelGGL(fill_f64, dim3(cdiv(n, 256) < 4096 ? cdiv(n, 256) : 4096), dim3(256), 0, s,
                      L.pdf64, n, 0.0);
   hipLaunchKernelGGL(fill_f64, dim3(cdiv(n * kD, 256) < 4096 ? cdiv(n * kD, 256) : 4096),
                      dim3(256), 0, s, L.w64, n * kD, 0.0);
-  const int64_t nb = cdiv(n, kSymB);
-  const int64_t nchunks = cdiv(nb, kSymJC);
-  const int64_t ntasks = nb * nchunks;
+  const int64_t nb = cdiv(n, kSymRows);
+  const int64_t nwin = cdiv(nb, kSymJC);
+  const int64_t ntasks = sym_window_offset(nwin, nb, kSymJC);
   int64_t per_xcd;
   const int64_t grid = xcd_grid(ntasks, &per_xcd);
+  if (grid > int64_t(0xFFFFFFFFu)) {
+    set_error("%s: n = %lld needs %lld workgroups, above the grid limit", what, (long long)n,
+              (long long)grid);
+    return FSK_EUNSUPPORTED;
+  }
   prof_begin(FSK_OP_SCORE16_SYM, s);
-  hipLaunchKernelGGL((score16_sym_kernel<kSymQT, kSymWaves>), dim3(grid), dim3(64 * kSymWaves), 0,
-                     s, data, ld, n, L.bn, float(-2.0 * coef), nb, kSymJC, ntasks, per_xcd,
-                     L.pdf64, L.w64);
+  hipLaunchKernelGGL((score16_sym_kernel<kSymQT>), dim3(grid), dim3(kWG), 0, s, L.xc, L.bn, n,
+                     float(-2.0 * coef), nb, kSymJC, nwin, ntasks, per_xcd, L.pdf64, L.w64);
   prof_end(FSK_OP_SCORE16_SYM, s);
   if ((st = launch_status(what)) != FSK_OK) return st;
   hipLaunchKernelGGL(score_finish_kernel, dim3(cdiv(n * kD, 256)), dim3(256), 0, s, L.pdf64,
-                     L.w64, n, pdf, w, ldw_row, ldw_col);
+                     L.w64, n, L.mu, inv_n, pdf, w, ldw_row, ldw_col);
   return launch_status(what);
 }
 
