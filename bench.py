@@ -4,9 +4,9 @@
 One step = the plain-FP32 SD-KDE pass at n = 1,048,576 training points drawn
 from an 8-component isotropic Gaussian mixture (means ~ N(0, 9 I), sigma 1,
 seed 0) and m = 65,536 queries from the same mixture (seed 1):
-  1. empirical-score accumulators over all n^2 ordered pairs (deterministic
-     ordered split-K kernel by default; --variant symmetric evaluates each
-     unordered pair once and scatters with atomics),
+  1. empirical-score accumulators over all n^2 ordered pairs (symmetric
+     kernel by default: each unordered pair once, column sums scattered with
+     FP64 atomics; --variant ordered runs the deterministic split-K kernel),
   2. FP64 debias x + (h^2/2) score,
   3. KDE of the m queries against the n debiased points.
 value = (m n + n^2) / step time, inputs resident in HBM.
@@ -41,7 +41,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=1 << 20)
     ap.add_argument("--m", type=int, default=65536)
-    ap.add_argument("--variant", choices=["symmetric", "ordered"], default="ordered")
+    ap.add_argument("--variant", choices=["symmetric", "ordered"], default="symmetric")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-precision-forms", action="store_true")
@@ -120,15 +120,19 @@ def sklearn_baseline(x, q, h, budget_s):
 
 
 # HBM bytes per launch of the dominant kernel, from two rocprofv3 --pmc runs of
-# `bench.py --steps 1 --warmup 0 --no-cpu-baseline --no-batched
+# `bench.py --steps 1 --warmup 0 [--variant ordered] --no-cpu-baseline --no-batched
 # --no-precision-forms --no-surrogate` (FETCH_SIZE, then WRITE_SIZE, KiB per
 # dispatch), each scaled by what the same counter reads for the library's copy
 # microbenchmark in the same run (512 MiB read and 512 MiB written per
 # dispatch: FETCH_SIZE reads 262169 KiB -> scale 2.0, WRITE_SIZE reads
 # 524288 KiB -> scale 1.0). Keyed by (score kernel, n); other sizes report null.
+# The symmetric kernel's figure is dominated by its FP64 device-scope atomic
+# scatters, which are performed past the L2 (8 bytes each).
 PMC_TRAFFIC = {
     ("ordered", 1 << 20): {"fetch_kib": 2355570.0, "write_kib": 278528.0,
                            "fetch_scale": 524288.0 / 262169.0, "write_scale": 1.0},
+    ("symmetric", 1 << 20): {"fetch_kib": 17589769.0, "write_kib": 178257920.0,
+                             "fetch_scale": 524288.0 / 262168.0, "write_scale": 1.0},
 }
 
 
@@ -465,7 +469,8 @@ def main():
     if args.variant == "symmetric":
         units = n * (n + 1) / 2.0
         flops = units * 96.0
-        exp_valu_s = units / R_EXP + units * 5.0 / R_VALU
+        # per unique pair: 1 exp + 2 VALU ops (row-sum add, column-pdf FMA)
+        exp_valu_s = units / R_EXP + units * 2.0 / R_VALU
     else:
         units = float(n) * n
         flops = units * 64.0
@@ -539,21 +544,26 @@ def main():
     if rates.get("exp_per_s") and rates.get("fma_per_s") and score_ms > 0:
         # exp/VALU roofline with the measured issue rates (per ordered pair of
         # the ordered kernel in factored form: 1 exp + 1 VALU op, the row-sum
-        # add; symmetric: per unique pair 1 + 5)
-        v_ops = 5.0 if args.variant == "symmetric" else 1.0
+        # add; symmetric: per unique pair 1 exp + 2, the row-sum add and the
+        # column-pdf FMA)
+        v_ops = 2.0 if args.variant == "symmetric" else 1.0
         t = units / rates["exp_per_s"] + units * v_ops / rates["fma_per_s"]
         line["roofline"]["exp_valu_frac_measured_rates"] = round(t / (score_ms * 1e-3), 4)
         line["roofline"]["mfma_frac_of_measured_peak"] = round(
             flops / (score_ms * 1e-3) / rates["fp32_mfma_flops"], 4)
-        if args.variant == "ordered" and rates.get("fp32_mfma_flops"):
+        if rates.get("fp32_mfma_flops"):
             # FP32 MFMA never co-executes with VALU work on this GPU (counter
-            # SQ_VALU_MFMA_COEXEC_CYCLES = 0), so the issue bound of the
-            # ordered kernel is the SUM of its MFMA time and its VALU time:
+            # SQ_VALU_MFMA_COEXEC_CYCLES = 0), so the issue bound of a score
+            # kernel is the SUM of its MFMA time and its VALU time. Ordered:
             # per ordered pair 64 flop MFMA, 1 exp and 1 other VALU op (the
             # row-sum add; the factored form starts the accumulator from the
-            # data-side norm term, 4 subtractions per 2048 pairs).
+            # data-side norm term, 4 subtractions per 2048 pairs). Symmetric:
+            # per unique pair 96 flop, 1 exp, 2 VALU ops, plus per data tile
+            # of 2048 unique pairs the column-pdf reduction (16 DPP adds,
+            # 5 selects) and the 4 subtractions of the factored form.
+            v_pair = 2.0 + 25.0 / 2048 if args.variant == "symmetric" else 1.0 + 4.0 / 2048
             t_add = (flops / rates["fp32_mfma_flops"] + units / rates["exp_per_s"]
-                     + (1.0 + 4.0 / 2048) * units / rates["fma_per_s"])
+                     + v_pair * units / rates["fma_per_s"])
             line["roofline"]["additive_issue_bound_frac"] = round(t_add / (score_ms * 1e-3), 4)
     if not args.no_batched:
         line["batched_loo"] = batched_loo(rates, args.steps)

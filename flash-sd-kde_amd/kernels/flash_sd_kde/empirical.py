@@ -2,9 +2,10 @@
 
 emp_score_16d_flash_sd_kde returns (pdf_sum (n,), weighted_sum (n, 16)) with
 weighted_sum a transposed view of a (16, n) buffer, as the reference does
-(:180,223). The accumulators come from the ordered split-K HIP score kernel
-(FP32 MFMA cross term and phi.X, FP64 combine), which on MI355X is faster
-than the symmetric atomic kernel and deterministic; the debias step runs in
+(:180,223). The accumulators come from the symmetric HIP score kernel (each
+unordered pair once: FP32 MFMA cross term and phi.X on both sides, FP64
+atomic accumulation), the fastest FP32 score on MI355X; like the reference's
+atomic kernel it is reproducible to rounding only. The debias step runs in
 FP64 on the GPU.
 """
 
@@ -41,7 +42,7 @@ def emp_score_16d_flash_sd_kde(
     train = _ops.as_points16(data, device)
     if train.shape[0] == 0:
         raise ValueError("data must contain at least one element.")
-    pdf, weighted = _ops.score16(train, bandwidth, variant="ordered", transposed_w=True)
+    pdf, weighted = _ops.score16(train, bandwidth, variant="symmetric", transposed_w=True)
     if synchronize:
         torch.cuda.synchronize(device)
     return pdf, weighted
