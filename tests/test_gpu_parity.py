@@ -247,21 +247,23 @@ def test_score_rows_matches_fp64(gpu, m, n):
     assert np.max(np.abs(cpu(w) - we)) / scale < 3e-5
 
 
+@pytest.mark.parametrize("variant", ["ordered", "symmetric"])
 @pytest.mark.parametrize("shift", [0.0, 6.0, 25.0])
-def test_score_ordered_factored_and_direct_rows_agree_with_fp64(gpu, shift):
-    """The ordered score evaluates a workgroup's rows in factored form when
+def test_score_factored_and_direct_rows_agree_with_fp64(gpu, shift, variant):
+    """Both score kernels evaluate a workgroup's rows in factored form when
     their norm terms c|q - mu|^2 span a narrow range and directly otherwise; an
     outlier (shifted by `shift` bandwidths along every axis) forces the direct
     form in its workgroup while the others stay factored (shift = 0: all
-    factored). Every row agrees with the FP64 sums to 2e-5, except that the
-    outlier's own row carries the FP32 cancellation error of the norm
-    expansion t = c|q|^2 + c|x|^2 - 2c q.x, a few units of 2^-24 times its
-    norm terms (|c||x - mu|^2 ~ 7200 at shift 25)."""
+    factored). In the symmetric kernel the outlier's block is also the column
+    block of a factored workgroup. Every row agrees with the FP64 sums to
+    2e-5, except that the outlier's own row carries the FP32 cancellation
+    error of the norm expansion t = c|q|^2 + c|x|^2 - 2c q.x, a few units of
+    2^-24 times its norm terms (|c||x - mu|^2 ~ 7200 at shift 25)."""
     rng = np.random.default_rng(21)
     x = rng.normal(size=(2600, 16)).astype(np.float32)
     x[700] += np.float32(shift)
     h = 1.0
-    po, wo = _ops.score16(torch.as_tensor(x, device=gpu), h, variant="ordered")
+    po, wo = _ops.score16(torch.as_tensor(x, device=gpu), h, variant=variant)
     pe, we = score_exact(x, h)
     xd = x.astype(np.float64)
     cnorm = np.log2(np.e) / (2 * h * h) * ((xd - xd.mean(0)) ** 2).sum(1)
@@ -274,11 +276,33 @@ def test_score_ordered_factored_and_direct_rows_agree_with_fp64(gpu, shift):
     assert np.all(relw <= tol), (relw.max(), int(relw.argmax()))
 
 
-def test_score_permutation_equivariance(gpu):
+@pytest.mark.parametrize("variant", ["ordered", "symmetric"])
+def test_score_large_ragged_vs_fp64_gpu(gpu, variant):
+    """n = 40,001 points of the benchmark's 8-component mixture (79 blocks of
+    512 for the symmetric kernel: several task windows, a ragged last block)
+    against the FP64 GPU score (direct differences): the pdf sums to 2e-5
+    relative, the mean shift w/pdf to 2e-5 bandwidths."""
+    from flash_sd_kde.bandwidth import scott_mean_std_bandwidth
+    from flash_sd_kde.datasets import gmm_isotropic_split
+
+    x, _, _ = gmm_isotropic_split(40001, 1, k=8, mean_scale=3.0, seed=4)
+    h = scott_mean_std_bandwidth(x)
+    xd = torch.as_tensor(x, device=gpu)
+    p64, w64 = _ops.ref64_score(xd.double(), h)
+    p, w = _ops.score16(xd, h, variant=variant)
+    p64, w64, p, w = cpu(p64), cpu(w64), cpu(p).astype(np.float64), cpu(w).astype(np.float64)
+    e_pdf = float(np.max(np.abs(p - p64) / p64))
+    e_shift = float(np.max(np.abs(w / p[:, None] - w64 / p64[:, None])) / h)
+    record(f"score_large_{variant}", pdf_rel_vs_fp64=e_pdf, shift_vs_fp64_in_h=e_shift)
+    assert e_pdf <= 2e-5 and e_shift <= 2e-5, (e_pdf, e_shift)
+
+
+@pytest.mark.parametrize("variant", ["ordered", "symmetric"])
+def test_score_permutation_equivariance(gpu, variant):
     x = np.random.default_rng(9).normal(size=(1500, 16)).astype(np.float32)
     perm = np.random.default_rng(10).permutation(1500)
-    p1, w1 = _ops.score16(torch.as_tensor(x, device=gpu), 1.0, variant="ordered")
-    p2, w2 = _ops.score16(torch.as_tensor(x[perm], device=gpu), 1.0, variant="ordered")
+    p1, w1 = _ops.score16(torch.as_tensor(x, device=gpu), 1.0, variant=variant)
+    p2, w2 = _ops.score16(torch.as_tensor(x[perm], device=gpu), 1.0, variant=variant)
     np.testing.assert_allclose(cpu(p1)[perm], cpu(p2), rtol=1e-5)
     np.testing.assert_allclose(cpu(w1)[perm], cpu(w2), rtol=1e-5, atol=1e-5)
 
