@@ -72,6 +72,46 @@ __device__ __forceinline__ float fexp2_le1(float x) {
   return __builtin_amdgcn_fmed3f(__builtin_amdgcn_exp2f(x), 0.f, 1.f);
 }
 
+// Symmetric score tasks: window k of block I covers blocks J in
+// [I + k jc, I + (k+1) jc) (J >= I). Tasks are numbered window-major (window k
+// holds the nb - k jc blocks I with I + k jc < nb), so the workgroups one XCD
+// runs together have neighbouring I and the same k: they share J data in L2
+// and scatter into staggered blocks. sym_window_offset(k) = first task of
+// window k; sym_window_offset(nwin) = number of tasks.
+__host__ __device__ inline int64_t sym_window_offset(int64_t k, int64_t nb, int64_t jc) {
+  return k * nb - jc * (k * (k - 1) / 2);
+}
+// Block I and block range [Jlo, Jhi) of a task (binary search over windows).
+__device__ __forceinline__ void sym_task(int64_t task, int64_t nb, int64_t jc, int64_t nwin,
+                                         int64_t *I, int64_t *Jlo, int64_t *Jhi) {
+  int64_t k_lo = 0, k_hi = nwin - 1;  // largest window k with offset(k) <= task
+  while (k_lo < k_hi) {
+    const int64_t mid = (k_lo + k_hi + 1) >> 1;
+    if (sym_window_offset(mid, nb, jc) <= task)
+      k_lo = mid;
+    else
+      k_hi = mid - 1;
+  }
+  *I = task - sym_window_offset(k_lo, nb, jc);
+  *Jlo = *I + k_lo * jc;
+  *Jhi = *Jlo + jc < nb ? *Jlo + jc : nb;
+}
+
+template <int CTRL>
+__device__ __forceinline__ float dpp_f32(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL,
+                                                               0xF, 0xF, false));
+}
+// Sum over the 16 lanes of a DPP row (lanes with equal lane / 16); every lane
+// of the row gets the total.
+__device__ __forceinline__ float row16_sum(float v) {
+  v += dpp_f32<0xB1>(v);   // quad_perm [1,0,3,2]
+  v += dpp_f32<0x4E>(v);   // quad_perm [2,3,0,1]
+  v += dpp_f32<0x141>(v);  // row_half_mirror
+  v += dpp_f32<0x140>(v);  // row_mirror
+  return v;
+}
+
 __device__ __forceinline__ f4 ld4(const float *p) {
   return *reinterpret_cast<const f4 *>(p);
 }

@@ -34,6 +34,19 @@ constexpr int kScoreLpQTx6 = FSK_LP_QT_X6;
 inline int score_lp_qt(unsigned prec) { return prec == FSK_PREC_BF16X6 ? kScoreLpQTx6 : kScoreLpQT; }
 // Symmetric score: I block = rows of one workgroup (4 waves x kSymQT tiles).
 constexpr int kSymQT = 8;
+constexpr int kSymRows = 4 * 16 * kSymQT;  // 512 points per block
+// J blocks per task. The row sums of a task stay in FP32 over jc blocks
+// before their FP64 atomic add, so jc trades task overhead for accuracy: at
+// n = 2^20 (8-component mixture) jc = 2, 4, 8 ran the FP32 kernel in 508,
+// 486, 475 ms with largest mean-shift errors (w/pdf vs FP64) of 0.96, 1.13
+// and 1.46e-5 h; the ordered kernel has 0.92e-5 h.
+#ifndef FSK_SYM_JC
+#define FSK_SYM_JC 4
+#endif
+constexpr int64_t kSymJC = FSK_SYM_JC;
+constexpr int kSymTS = 20;         // LDS row stride (floats) of transposed tiles and column sums
+constexpr int kSymSlots = 2;       // phi tiles a wave keeps in LDS at a time
+constexpr int kSymFlushTiles = 8;  // data tiles between two flushes of the column sums
 // Target number of (query block, split) tasks per launch: a few per CU slot.
 constexpr int64_t kTargetTasks = 8192;
 // Cap on split-K partial workspace for the ordered score (bytes).

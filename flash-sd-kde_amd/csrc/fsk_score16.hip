@@ -28,15 +28,6 @@ namespace fsk {
 constexpr int kScoreFlush = 32;
 static_assert(kNormPad % kScoreRowsPerWG == 0, "row norms must cover whole row blocks");
 
-// Symmetric score: J blocks per task. The row sums of a task stay in FP32
-// over jc blocks before their FP64 atomic add, so jc trades task overhead for
-// accuracy: at n = 2^20 (8-component mixture) jc = 2, 4, 8 ran 508, 486,
-// 475 ms with largest mean-shift errors (w/pdf vs FP64) of 0.96, 1.13 and
-// 1.46e-5 h; the ordered kernel has 0.92e-5 h.
-#ifndef FSK_SYM_JC
-#define FSK_SYM_JC 4
-#endif
-
 // Largest spread of the query terms A_i = -c|q_i|^2 (c < 0) inside one
 // workgroup for which the factored form below is used.
 constexpr float kFactSpread = 160.f;
@@ -199,33 +190,8 @@ __global__ __launch_bounds__(kWG) void score16_ordered_kernel(
 // guard): the per-row factor f_i = exp2(qa_i + K) enters the column side
 // through its B operand (f_i x_i) and the column pdf FMA, and the row sums
 // once at the end.
-// Tasks: window k of block I covers J in [I + k jc, I + (k+1) jc). Tasks are
-// numbered window-major (window k holds the nb - k jc blocks I with
-// I + k jc < nb), so the workgroups one XCD runs together have neighbouring I
-// and the same k: they share J data in L2 and scatter into staggered blocks.
-__host__ __device__ inline int64_t sym_window_offset(int64_t k, int64_t nb, int64_t jc) {
-  return k * nb - jc * (k * (k - 1) / 2);
-}
-
-template <int CTRL>
-__device__ __forceinline__ float dpp_f32(float v) {
-  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL,
-                                                               0xF, 0xF, false));
-}
-// Sum over the 16 lanes of a DPP row (lanes with equal lane / 16); every lane
-// of the row gets the total.
-__device__ __forceinline__ float row16_sum(float v) {
-  v += dpp_f32<0xB1>(v);   // quad_perm [1,0,3,2]
-  v += dpp_f32<0x4E>(v);   // quad_perm [2,3,0,1]
-  v += dpp_f32<0x141>(v);  // row_half_mirror
-  v += dpp_f32<0x140>(v);  // row_mirror
-  return v;
-}
-
-constexpr int kSymTS = 20;        // LDS row stride (floats) of transposed tiles and column sums
-constexpr int kSymSlots = 2;      // phi tiles a wave keeps in LDS at a time
-constexpr int kSymFlushTiles = 8;  // data tiles between two flushes of the column sums
-
+// Tasks: window k of block I covers J in [I + k jc, I + (k+1) jc), decoded by
+// sym_task (fsk_common.h).
 template <int QT>
 __global__ __launch_bounds__(kWG) void score16_sym_kernel(
     const float *__restrict__ xc, const float *__restrict__ bn, int64_t n, float qscale,
@@ -244,17 +210,8 @@ __global__ __launch_bounds__(kWG) void score16_sym_kernel(
 
   const int64_t task = xcd_task(blockIdx.x, per_xcd);
   if (task >= ntasks) return;
-  int64_t k_lo = 0, k_hi = nwin - 1;  // largest window k with offset(k) <= task
-  while (k_lo < k_hi) {
-    const int64_t mid = (k_lo + k_hi + 1) >> 1;
-    if (sym_window_offset(mid, nb, jc) <= task)
-      k_lo = mid;
-    else
-      k_hi = mid - 1;
-  }
-  const int64_t I = task - sym_window_offset(k_lo, nb, jc);
-  const int64_t Jlo = I + k_lo * jc;
-  const int64_t Jhi = Jlo + jc < nb ? Jlo + jc : nb;
+  int64_t I, Jlo, Jhi;
+  sym_task(task, nb, jc, nwin, &I, &Jlo, &Jhi);
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -463,9 +420,7 @@ __global__ void score_finish_kernel(const double *__restrict__ pdf64,
   if (f == 0) pdf[i] = float(pdf64[i]);
 }
 
-constexpr int kSymRows = 4 * 16 * kSymQT;
 static_assert(kNormPad % kSymRows == 0, "padded rows must cover whole symmetric blocks");
-constexpr int64_t kSymJC = FSK_SYM_JC;  // J blocks per task
 
 // Points centred on the data mean: x - mu (FP64 difference, FP32 result),
 // rows [n, n_pad) zero.
