@@ -257,7 +257,7 @@ def surrogate_form(fp32_pair_rate):
     torch.cuda.synchronize()
     t_sur = time.perf_counter() - t0
     t0 = time.perf_counter()
-    pdf, w = _ops.score16(xd, h, variant="ordered")
+    pdf, w = _ops.score16(xd, h, variant="symmetric")  # the fastest exact FP32 score
     deb_e = _ops.debias(xd, pdf, w, h, 1e-12)
     torch.cuda.synchronize()
     t_exact = time.perf_counter() - t0

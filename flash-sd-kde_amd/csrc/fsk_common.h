@@ -102,14 +102,21 @@ __device__ __forceinline__ float dpp_f32(float v) {
   return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL,
                                                                0xF, 0xF, false));
 }
-// Sum over the 16 lanes of a DPP row (lanes with equal lane / 16); every lane
-// of the row gets the total.
-__device__ __forceinline__ float row16_sum(float v) {
-  v += dpp_f32<0xB1>(v);   // quad_perm [1,0,3,2]
-  v += dpp_f32<0x4E>(v);   // quad_perm [2,3,0,1]
-  v += dpp_f32<0x141>(v);  // row_half_mirror
-  v += dpp_f32<0x140>(v);  // row_mirror
-  return v;
+
+// Sums over the 16 lanes of a DPP row of four per-lane values v[0..3], folded
+// so that lane c ends with the sum of v[c & 3]: two exchange stages (lane
+// bits 0 and 1, each keeping the value the lane is to reduce and sending the
+// other to its partner: 2 selects + 1 DPP add per pair) halve the values to
+// reduce, then rotations by 4 and 8 lanes finish the sum (11 instructions
+// against 16 DPP adds and 3 selects for four separate butterfly row sums).
+__device__ __forceinline__ float row16_sum4(const float (&v)[4], int c) {
+  const bool b0 = c & 1, b1 = c & 2;
+  const float a = (b0 ? v[1] : v[0]) + dpp_f32<0xB1>(b0 ? v[0] : v[1]);  // quad_perm [1,0,3,2]
+  const float b = (b0 ? v[3] : v[2]) + dpp_f32<0xB1>(b0 ? v[2] : v[3]);
+  float e = (b1 ? b : a) + dpp_f32<0x4E>(b1 ? a : b);  // quad_perm [2,3,0,1]
+  e += dpp_f32<0x124>(e);                             // row_ror:4
+  e += dpp_f32<0x128>(e);                             // row_ror:8
+  return e;
 }
 
 __device__ __forceinline__ f4 ld4(const float *p) {

@@ -179,7 +179,7 @@ __global__ __launch_bounds__(kWG) void score16_ordered_kernel(
 //    ds_write_b32 per lane) and comes back transposed (one ds_read_b128);
 //    rows padded to kSymTS floats keep both accesses free of bank conflicts.
 //    The column pdf sums are FMAs in the original layout, reduced over the 16
-//    lanes of a DPP row once per data tile.
+//    lanes of a DPP row once per data tile (row16_sum4).
 //  * Each wave stores its column sums of a data tile in its own LDS region;
 //    every kSymFlushTiles tiles the workgroup adds the four regions and sends
 //    the totals to the FP64 accumulators with global atomics. Row sums go out
@@ -356,18 +356,19 @@ __global__ __launch_bounds__(kWG) void score16_sym_kernel(
           }
           __builtin_amdgcn_wave_barrier();
         }
-        // pj[r]: this lane's part of the column pdf of point 4g + r. The
-        // wave's column sums of this tile go to its own LDS region (plain
-        // stores: LDS float atomics measured 40 % slower for the whole kernel)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) pj[r] = row16_sum(pj[r]);
-        const float pv = (c & 2) ? ((c & 1) ? pj[3] : pj[2]) : ((c & 1) ? pj[1] : pj[0]);
+        // pj[r]: this lane's part of the column pdf of point 4g + r; lanes
+        // c < 4 get the total for point 4g + c. The wave's column sums of this
+        // tile go to its own LDS region (plain stores: LDS float atomics
+        // measured 40 % slower for the whole kernel)
+        const float pv = row16_sum4(pj, c);
         const int lt = jt % FT;
         if (c < 4) colp[wave][lt * kTile + 4 * g + c] = pv;
 #pragma unroll
         for (int r = 0; r < 4; ++r) colw[wave][(lt * kTile + 4 * g + r) * TS + c] = wJ[r];
         if (lt == FT - 1) {
           // the four waves' sums of the last FT tiles -> FP64 accumulators
+          // (two alternating region sets with one barrier per flush measured
+          // no faster)
           __syncthreads();
           const int64_t p0 = jbase + (jt - (FT - 1)) * kTile;
           for (int e = tid; e < FT * kTile * kD; e += kWG) {

@@ -347,7 +347,7 @@ def surrogate_fit_transform(data, bandwidth: float, *, n_label: int = 262144,
     info["fallback"] = info["holdout_max_rel_logp_err"] > check_margin * tol
     t0 = time.perf_counter()
     if info["fallback"]:
-        pdf, w = _ops.score16(x, h, variant="ordered")
+        pdf, w = _ops.score16(x, h, variant="symmetric")
         deb = _ops.debias(x, pdf, w, h, 1e-12)
     else:
         deb = x + 0.5 * net.predict(x)
