@@ -298,6 +298,22 @@ def test_score_large_ragged_vs_fp64_gpu(gpu, variant):
 
 
 @pytest.mark.parametrize("variant", ["ordered", "symmetric"])
+def test_score_strided_input(gpu, variant):
+    """Points read through a row stride of 20 floats give the contiguous
+    result (both kernels centre a copy of the data first)."""
+    x = np.random.default_rng(12).normal(size=(1100, 16)).astype(np.float32)
+    xt = torch.zeros((1100, 20), device=gpu)
+    xt[:, :16] = torch.as_tensor(x, device=gpu)
+    p1, w1 = _ops.score16(xt[:, :16], 0.9, variant=variant)
+    p2, w2 = _ops.score16(torch.as_tensor(x, device=gpu), 0.9, variant=variant)
+    pe, we = score_exact(x, 0.9)
+    np.testing.assert_allclose(cpu(p1), pe, rtol=2e-5)
+    assert np.max(np.abs(cpu(w1) - we)) / max(1.0, float(np.max(np.abs(we)))) < 2e-5
+    np.testing.assert_allclose(cpu(p1), cpu(p2), rtol=1e-6)
+    np.testing.assert_allclose(cpu(w1), cpu(w2), rtol=1e-6, atol=1e-6)
+
+
+@pytest.mark.parametrize("variant", ["ordered", "symmetric"])
 def test_score_permutation_equivariance(gpu, variant):
     x = np.random.default_rng(9).normal(size=(1500, 16)).astype(np.float32)
     perm = np.random.default_rng(10).permutation(1500)
