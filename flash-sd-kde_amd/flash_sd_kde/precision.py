@@ -20,7 +20,8 @@ operands and FP32 accumulation:
   "bf16"    single BF16 operands
   "fp32", "fp32_valu"  FP32 MFMA / FP32 VALU cross term (log-density only)
 with points centred on the data mean, log-densities by log-sum-exp
-(per-split running maximum, FP64 combine) and FP64 debias.
+(per-split running maximum, FP64 combine) and FP64 debias. The score and the
+log-density may use different forms (`sdkde_logdensity(score_precision=...)`).
 
 `ref64_*` are FP64 direct-difference GPU kernels used to measure the error of
 every form at sizes the CPU oracle cannot reach.
@@ -62,11 +63,26 @@ def sdkde_fit_transform(data, bandwidth: float, *, precision: str = "fp32", eps:
 
 
 def sdkde_logdensity(data, queries, bandwidth: float, *, precision: str = "fp32",
-                     device="cuda") -> torch.Tensor:
-    """SD-KDE log-density of `queries`: debias `data` (score in `precision`), then
-    the log-sum-exp KDE (cross term in `precision`)."""
+                     score_precision: str | None = None, device="cuda") -> torch.Tensor:
+    """SD-KDE log-density of `queries`: debias `data` (score in `score_precision`,
+    default `precision`), then the log-sum-exp KDE (cross term in `precision`).
+
+    The log-density error is set by the KDE's cross term far more than by the
+    score's: the score only enters through the debias shift
+    (h^2/2) score = (sum_j phi_ij x_j / sum_j phi_ij - x_i) / 2, a weighted mean
+    of neighbours in which independent rounding errors of the phi_ij average
+    out. So a single-FP16 score under an FP16-split KDE ("fp16" score,
+    "fp16x3" KDE) stays within 1e-3 relative log-density error of FP64 at
+    h in {0.5, 1, 2} x Silverman on the 8-component 16-D mixture at
+    n = 2^20, where the single-FP16 KDE does not (measured errors by bandwidth:
+    3.7e-4, 4.6e-6 and 1.7e-7 against 6.0e-3, 1.1e-3 and 1.6e-4). The score's
+    share grows as h shrinks; a single-BF16 score exceeds 1e-3 at 0.5 x h.
+    """
     device = torch.device(device)
-    deb = sdkde_fit_transform(data, bandwidth, precision=precision, device=device)
+    sp = precision if score_precision is None else score_precision
+    if sp not in SCORE_PRECISIONS:
+        raise ValueError(f"score_precision must be one of {SCORE_PRECISIONS}, got {sp!r}")
+    deb = sdkde_fit_transform(data, bandwidth, precision=sp, device=device)
     return _ops.kde16_logdensity(deb, _ops.as_points16(queries, device), bandwidth,
                                  precision=precision)
 

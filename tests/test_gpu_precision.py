@@ -133,3 +133,16 @@ def test_sdkde_logdensity_precision_forms(gpu, golden, prec, tol):
     mx, med = precision.rel_log_error(lp, ref)
     record(f"sdkde_logdensity_{prec}", max_rel=mx, median_rel=med)
     assert mx <= tol
+
+
+@pytest.mark.parametrize("score_prec,prec,tol", [("fp16", "fp16x3", 1e-3), ("fp16x3s", "fp32", 1e-5)])
+def test_sdkde_logdensity_mixed_forms(gpu, golden, score_prec, prec, tol):
+    """Score and KDE in different forms: the single-FP16 score under the
+    FP16-split KDE stays inside the 1e-3 log-density bound."""
+    c = "cfg_gmm_small"
+    x, q, h = golden(c, "x"), golden(c, "q"), golden(c, "h")
+    ref = precision.ref64_sdkde_logdensity(x, q, h, device=gpu)
+    lp = precision.sdkde_logdensity(x, q, h, precision=prec, score_precision=score_prec, device=gpu)
+    mx, med = precision.rel_log_error(lp, ref)
+    record(f"sdkde_logdensity_{score_prec}_score_{prec}", max_rel=mx, median_rel=med)
+    assert mx <= tol

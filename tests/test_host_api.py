@@ -115,3 +115,12 @@ def test_fp16_forms_refuse_out_of_range_inputs():
         y = torch.zeros((4, 16))
         y[0, 0] = 1e5
         _ops.check_fp16_range(y, y, 1.0, nat.PREC_BF16X3S)
+
+
+def test_mixed_precision_form_rejects_unknown_score_precision():
+    from flash_sd_kde import precision
+
+    x = np.zeros((4, 16), np.float32)
+    with pytest.raises(ValueError, match="score_precision"):
+        precision.sdkde_logdensity(x, x, 1.0, precision="fp16x3", score_precision="fp8",
+                                   device="cpu")
