@@ -1,0 +1,113 @@
+"""GPU parity at the full 16-D SD-KDE workload: n = 2^20 training points of the
+8-component isotropic mixture (means ~ N(0, 9 I), sigma = 1), m = 65 536
+queries, the mean-std bandwidth rule. The CPU oracle cannot reach this size,
+so the checks are properties that hold at any n plus the FP64 GPU kernels
+(direct differences, themselves checked against the CPU FP64 oracle at small n):
+
+* both score kernels against the FP64 score: pdf to 5e-5 relative, the mean
+  shift w/pdf to 3e-5 bandwidths;
+* the score's antisymmetry identity sum_i w_i = sum_i pdf_i x_i (the double
+  sum of phi_ij (x_j - x_i) vanishes because phi is symmetric), evaluated in
+  FP64 over each kernel's outputs: a dropped block, a misplaced column scatter
+  of pdf or a wrong centring breaks it;
+* 1 <= pdf_i <= n (the self term phi_ii = 1 is included);
+* the drop-in SD-KDE pipeline (emp_sd_kde_fit_transform, kde_eval) against
+  the FP64 pipeline in log space on 1024 queries;
+* the KDE split-K and atomic backends agree, and chunking the queries
+  changes split-K results only by FP32 rounding: the split count is planned
+  from (n, m) to fill the GPU, so half the queries get more, shorter data
+  splits (results are bitwise equal only between calls of the same sizes).
+"""
+
+import pytest
+import torch
+
+import flash_sd_kde
+from flash_sd_kde import _ops
+
+pytestmark = pytest.mark.gpu
+
+N, M = 1 << 20, 65536
+IEEE = "fp32_ieee"
+
+
+def record(name, **vals):
+    print(name, vals, flush=True)
+
+
+@pytest.fixture(scope="module")
+def work(gpu):
+    from flash_sd_kde.bandwidth import scott_mean_std_bandwidth
+    from flash_sd_kde.datasets import gmm_isotropic_split
+
+    x, q, _ = gmm_isotropic_split(N, M, k=8, mean_scale=3.0, seed=0)
+    h = scott_mean_std_bandwidth(x)
+    xd = torch.as_tensor(x, device=gpu)
+    qd = torch.as_tensor(q, device=gpu)
+    p64, w64 = _ops.ref64_score(xd.double(), h)
+    scores = {v: _ops.score16(xd, h, variant=v) for v in ("symmetric", "ordered")}
+    torch.cuda.synchronize()
+    return {"x": xd, "q": qd, "h": h, "p64": p64, "w64": w64, "scores": scores}
+
+
+@pytest.mark.parametrize("variant", ["symmetric", "ordered"])
+def test_fullsize_score_vs_fp64(work, variant):
+    p, w = (t.double() for t in work["scores"][variant])
+    p64, w64, h = work["p64"], work["w64"], work["h"]
+    e_pdf = float(((p - p64).abs() / p64).max())
+    e_shift = float((w / p[:, None] - w64 / p64[:, None]).abs().max()) / h
+    record(f"fullsize_score_{variant}", pdf_rel_vs_fp64=e_pdf, shift_vs_fp64_in_h=e_shift)
+    assert e_pdf <= 5e-5 and e_shift <= 3e-5, (e_pdf, e_shift)
+
+
+@pytest.mark.parametrize("variant", ["symmetric", "ordered"])
+def test_fullsize_score_antisymmetry_identity(work, variant):
+    p, w = (t.double() for t in work["scores"][variant])
+    x = work["x"].double()
+    lhs = w.sum(0)
+    rhs = (p[:, None] * x).sum(0)
+    scale = (p[:, None] * x.abs()).sum(0)
+    e = float(((lhs - rhs).abs() / scale).max())
+    # the FP64 kernel's own residual shows the size of the rounding floor
+    e64 = float(((work["w64"].sum(0) - (work["p64"][:, None] * x).sum(0)).abs() / scale).max())
+    record(f"fullsize_antisymmetry_{variant}", rel_residual=e, fp64_rel_residual=e64)
+    assert e <= 1e-6, e
+
+
+@pytest.mark.parametrize("variant", ["symmetric", "ordered"])
+def test_fullsize_pdf_bounds(work, variant):
+    p = work["scores"][variant][0]
+    assert bool(torch.isfinite(p).all())
+    assert float(p.min()) >= 1.0 - 1e-5 and float(p.max()) <= N
+
+
+def test_fullsize_sdkde_pipeline_vs_fp64(work, gpu):
+    x, q, h = work["x"], work["q"], work["h"]
+    deb = flash_sd_kde.emp_sd_kde_fit_transform(x, h, device=gpu, precision_mode=IEEE)
+    dens = flash_sd_kde.kde_eval(deb, q, h, device=gpu, precision_mode=IEEE)
+    assert dens.shape == (M,) and bool(torch.isfinite(dens).all())
+    sub = q[:1024]
+    x64, p64, w64 = x.double(), work["p64"], work["w64"]
+    deb64 = x64 + 0.5 * (w64 / (p64[:, None] + 1e-12) - x64)
+    ref = _ops.ref64_logdensity(deb64, sub.double(), h)
+    lp = torch.log(dens[:1024].double())
+    e = float(((lp - ref).abs() / ref.abs()).max())
+    shift = float((deb.double() - deb64).abs().max()) / h
+    record("fullsize_sdkde_pipeline", max_rel_logp_err=e, debias_vs_fp64_in_h=shift)
+    assert e <= 2e-6 and shift <= 3e-5, (e, shift)
+
+
+def test_fullsize_kde_backends_and_chunking(work, gpu):
+    x, q, h = work["x"], work["q"], work["h"]
+    full = flash_sd_kde.kde_eval(x, q, h, device=gpu, precision_mode=IEEE)
+    halves = [flash_sd_kde.kde_eval(x, q[s], h, device=gpu, precision_mode=IEEE)
+              for s in (slice(0, M // 2 + 37), slice(M // 2 + 37, M))]
+    e_chunk = float(((torch.cat(halves).double() - full.double()).abs() / full.double()).max())
+    assert torch.equal(full, flash_sd_kde.kde_eval(x, q, h, device=gpu, precision_mode=IEEE))
+    atomic = flash_sd_kde.kde_eval(x, q, h, device=gpu, precision_mode=IEEE, kde_backend="atomic")
+    e = float(((atomic.double() - full.double()).abs() / full.double()).max())
+    ref = _ops.ref64_logdensity(x.double(), q[:1024].double(), h)
+    e64 = float(((torch.log(full[:1024].double()) - ref).abs() / ref.abs()).max())
+    record("fullsize_kde", chunked_vs_whole_rel=e_chunk, atomic_vs_splitk_rel=e,
+           max_rel_logp_err_vs_fp64=e64)
+    assert e_chunk <= 2e-6 and e <= 1e-5 and e64 <= 2e-6, (e_chunk, e, e64)
