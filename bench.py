@@ -269,7 +269,7 @@ def surrogate_form(fp32_pair_rate):
     truth = gmm_full_cov_logpdf(q, params)
     mae = lambda a: float(np.mean(np.abs(a.cpu().numpy() - truth)))
     return {
-        "workload": "n=4194304 full-cov GMM32, 65536 queries, MLP 16-384-384-384-16 SiLU bf16",
+        "workload": "n=4194304 full-cov GMM32, 65536 queries, MLP 16-256-256-256-16 SiLU bf16",
         "bandwidth": h,
         "exact_debias_s": round(t_exact, 3),
         "surrogate_debias_s": round(t_sur, 3),

@@ -2,8 +2,8 @@
 
 The exact one-step SD-KDE shifts every point by half its mean shift,
     x_i -> x_i + (h^2 / 2) score_i = x_i + delta_i / 2,   delta_i = w_i / pdf_i - x_i,
-which costs n^2 kernel evaluations. Here a small MLP, 16 -> 384 -> 384 ->
-384 -> 16 with SiLU (BF16 weights, FP32 accumulation), learns x -> delta from
+which costs n^2 kernel evaluations. Here a small MLP, 16 -> 256 -> 256 ->
+256 -> 16 with SiLU (BF16 weights, FP32 accumulation), learns x -> delta from
 exact labels on a subset (the score of the subset rows under the KDE of ALL
 points, fsk_score16_rows) and then shifts all n points at O(n) cost.
 
@@ -32,7 +32,7 @@ import torch
 from flash_sd_kde import _native as nat
 from flash_sd_kde import _ops
 
-DIMS = (16, 384, 384, 384, 16)
+DIMS = (16, 256, 256, 256, 16)
 
 
 def _bf16_buffer(shape, device):
@@ -296,7 +296,7 @@ def exact_shift(rows: torch.Tensor, data: torch.Tensor, h: float, eps: float = 1
 def surrogate_fit_transform(data, bandwidth: float, *, n_label: int = 262144,
                             n_holdout: int = 16384, n_probe: int = 16384, tol: float = 1e-3,
                             check_margin: float = 0.5, steps: int = 8000, batch: int = 8192,
-                            lr: float = 6e-3, seed: int = 0, device="cuda",
+                            lr: float = 1e-2, seed: int = 0, device="cuda",
                             net: ScoreMLP | None = None, label_precision: str = "fp16x3s"):
     """One-step SD-KDE debiasing with the learned shift field.
 
