@@ -13,6 +13,9 @@ so the checks are properties that hold at any n plus the FP64 GPU kernels
 * 1 <= pdf_i <= n (the self term phi_ii = 1 is included);
 * the drop-in SD-KDE pipeline (emp_sd_kde_fit_transform, kde_eval) against
   the FP64 pipeline in log space on 1024 queries;
+* the fastest precision form inside the 1e-3 log-density bound and the
+  FP32-class FP16x3S form against the FP64 pipeline at 0.5, 1 and 2 times the
+  bandwidth;
 * the KDE split-K and atomic backends agree, and chunking the queries
   changes split-K results only by FP32 rounding: the split count is planned
   from (n, m) to fill the GPU, so half the queries get more, shorter data
@@ -111,3 +114,26 @@ def test_fullsize_kde_backends_and_chunking(work, gpu):
     record("fullsize_kde", chunked_vs_whole_rel=e_chunk, atomic_vs_splitk_rel=e,
            max_rel_logp_err_vs_fp64=e64)
     assert e_chunk <= 2e-6 and e <= 1e-5 and e64 <= 2e-6, (e_chunk, e, e64)
+
+
+@pytest.mark.parametrize("mult", [0.5, 1.0, 2.0])
+def test_fullsize_precision_forms_vs_fp64(work, gpu, mult):
+    """The fastest form inside the 1e-3 bound (single-FP16 score under the
+    FP16-split KDE) and the FP32-class FP16x3S form, against the FP64
+    pipeline on 1024 queries at 0.5, 1 and 2 times the bandwidth."""
+    from flash_sd_kde import precision
+
+    x, sub, h = work["x"], work["q"][:1024], work["h"] * mult
+    if mult == 1.0:
+        p64, w64 = work["p64"], work["w64"]
+    else:
+        p64, w64 = _ops.ref64_score(x.double(), h)
+    x64 = x.double()
+    ref = _ops.ref64_logdensity(x64 + 0.5 * (w64 / (p64[:, None] + 1e-12) - x64), sub.double(), h)
+    errs = {}
+    for name, sp, ep in (("fp16+fp16x3", "fp16", "fp16x3"), ("fp16x3s", "fp16x3s", "fp16x3")):
+        lp = precision.sdkde_logdensity(x, sub, h, precision=ep, score_precision=sp, device=gpu)
+        errs[name] = precision.rel_log_error(lp, ref)[0]
+    record(f"fullsize_precision_forms_{mult}h", **errs)
+    assert errs["fp16+fp16x3"] <= 1e-3, errs
+    assert errs["fp16x3s"] <= (1e-4 if mult < 1.0 else 2e-6), errs
