@@ -47,6 +47,11 @@ constexpr int64_t kSymJC = FSK_SYM_JC;
 constexpr int kSymTS = 20;         // LDS row stride (floats) of transposed tiles and column sums
 constexpr int kSymSlots = 2;       // phi tiles a wave keeps in LDS at a time
 constexpr int kSymFlushTiles = 8;  // data tiles between two flushes of the column sums
+// Tasks (workgroups) per launch of the symmetric kernel. Their number grows as
+// n^2 / 2^21 and a launch may hold at most 2^32 work-items (2^24 workgroups
+// of 256), so larger n runs as several launches over consecutive task ranges;
+// n = 2^20 (525,312 tasks) is one launch, n = 2^22 nine.
+constexpr int64_t kSymTasksPerLaunch = int64_t(1) << 20;
 // Target number of (query block, split) tasks per launch: a few per CU slot.
 constexpr int64_t kTargetTasks = 8192;
 // Cap on split-K partial workspace for the ordered score (bytes).

@@ -195,7 +195,7 @@ __global__ __launch_bounds__(kWG) void score16_ordered_kernel(
 template <int QT>
 __global__ __launch_bounds__(kWG) void score16_sym_kernel(
     const float *__restrict__ xc, const float *__restrict__ bn, int64_t n, float qscale,
-    int64_t nb, int64_t jc, int64_t nwin, int64_t ntasks, int64_t per_xcd,
+    int64_t nb, int64_t jc, int64_t nwin, int64_t task0, int64_t ntasks, int64_t per_xcd,
     double *__restrict__ pdf64, double *__restrict__ w64) {
   constexpr int B = 4 * 16 * QT;
   constexpr int TS = kSymTS;
@@ -208,7 +208,8 @@ __global__ __launch_bounds__(kWG) void score16_sym_kernel(
   __shared__ float colp[4][FT * kTile];       // per wave: column pdf sums of FT tiles
   __shared__ float red[8];
 
-  const int64_t task = xcd_task(blockIdx.x, per_xcd);
+  // this launch covers tasks [task0, task0 + gridDim.x) of [0, ntasks)
+  const int64_t task = task0 + xcd_task(blockIdx.x, per_xcd);
   if (task >= ntasks) return;
   int64_t I, Jlo, Jhi;
   sym_task(task, nb, jc, nwin, &I, &Jlo, &Jhi);
@@ -581,18 +582,18 @@ extern "C" int fsk_score16_sym_atomic(const float *data, int64_t ld, int64_t n,
   const int64_t nb = cdiv(n, kSymRows);
   const int64_t nwin = cdiv(nb, kSymJC);
   const int64_t ntasks = sym_window_offset(nwin, nb, kSymJC);
-  int64_t per_xcd;
-  const int64_t grid = xcd_grid(ntasks, &per_xcd);
-  if (grid > int64_t(0xFFFFFFFFu)) {
-    set_error("%s: n = %lld needs %lld workgroups, above the grid limit", what, (long long)n,
-              (long long)grid);
-    return FSK_EUNSUPPORTED;
-  }
   prof_begin(FSK_OP_SCORE16_SYM, s);
-  hipLaunchKernelGGL((score16_sym_kernel<kSymQT>), dim3(grid), dim3(kWG), 0, s, L.xc, L.bn, n,
-                     float(-2.0 * coef), nb, kSymJC, nwin, ntasks, per_xcd, L.pdf64, L.w64);
+  for (int64_t t0 = 0; t0 < ntasks; t0 += kSymTasksPerLaunch) {
+    const int64_t nt = ntasks - t0 < kSymTasksPerLaunch ? ntasks - t0 : kSymTasksPerLaunch;
+    int64_t per_xcd;
+    const int64_t grid = xcd_grid(nt, &per_xcd);
+    hipLaunchKernelGGL((score16_sym_kernel<kSymQT>), dim3(grid), dim3(kWG), 0, s, L.xc, L.bn, n,
+                       float(-2.0 * coef), nb, kSymJC, nwin, t0, t0 + nt, per_xcd, L.pdf64,
+                       L.w64);
+    if ((st = launch_status(what)) != FSK_OK) break;
+  }
   prof_end(FSK_OP_SCORE16_SYM, s);
-  if ((st = launch_status(what)) != FSK_OK) return st;
+  if (st != FSK_OK) return st;
   hipLaunchKernelGGL(score_finish_kernel, dim3(cdiv(n * kD, 256)), dim3(256), 0, s, L.pdf64,
                      L.w64, n, L.mu, inv_n, pdf, w, ldw_row, ldw_col);
   return launch_status(what);

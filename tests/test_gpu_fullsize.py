@@ -137,3 +137,23 @@ def test_fullsize_precision_forms_vs_fp64(work, gpu, mult):
     record(f"fullsize_precision_forms_{mult}h", **errs)
     assert errs["fp16+fp16x3"] <= 1e-3, errs
     assert errs["fp16x3s"] <= (1e-4 if mult < 1.0 else 2e-6), errs
+
+
+def test_symmetric_multi_launch_equals_ordered(gpu):
+    """n = 2^21: the symmetric kernel's 2,099,200 tasks run as three launches
+    over consecutive task ranges; every block pair must be covered exactly
+    once, so its accumulators match the ordered kernel's (each sits ~1.4e-5
+    from FP64 at n = 2^20)."""
+    from flash_sd_kde.bandwidth import scott_mean_std_bandwidth
+    from flash_sd_kde.datasets import gmm_isotropic_split
+
+    x, _, _ = gmm_isotropic_split(1 << 21, 1, k=8, mean_scale=3.0, seed=3)
+    h = scott_mean_std_bandwidth(x)
+    xd = torch.as_tensor(x, device=gpu)
+    ps, ws = (t.double() for t in _ops.score16(xd, h, variant="symmetric"))
+    po, wo = (t.double() for t in _ops.score16(xd, h, variant="ordered"))
+    e_pdf = float(((ps - po).abs() / po).max())
+    e_shift = float((ws / ps[:, None] - wo / po[:, None]).abs().max()) / h
+    record("symmetric_multi_launch_n2097152", pdf_rel_vs_ordered=e_pdf,
+           shift_vs_ordered_in_h=e_shift)
+    assert e_pdf <= 5e-5 and e_shift <= 5e-5, (e_pdf, e_shift)
