@@ -37,10 +37,16 @@ constexpr float kPadNeg = -1.0e30f;
 // bounds.
 constexpr int64_t kNormPad = 512;
 
+// Elementwise passes over (row, feature) pairs launch one thread each; a
+// launch holds at most 2^32 work-items, so the paths that run them (score,
+// debias, weighted combine) take at most this many rows.
+constexpr int64_t kMaxElemRows = int64_t(1) << 27;
+
 // ---- error plumbing (host) ----
 void set_error(const char *fmt, ...);
 int launch_status(const char *what);  // FSK_OK or FSK_ELAUNCH after a launch
 int check_points16(const char *what, const float *p, int64_t ld, int64_t rows);
+int check_elem_rows(const char *what, int64_t rows);  // FSK_EUNSUPPORTED above kMaxElemRows
 
 inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
 inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }

@@ -26,6 +26,15 @@ int launch_status(const char *what) {
   return FSK_OK;
 }
 
+int check_elem_rows(const char *what, int64_t rows) {
+  if (rows > kMaxElemRows) {
+    set_error("%s: %lld rows, above the supported %lld", what, (long long)rows,
+              (long long)kMaxElemRows);
+    return FSK_EUNSUPPORTED;
+  }
+  return FSK_OK;
+}
+
 int check_points16(const char *what, const float *p, int64_t ld, int64_t rows) {
   if (rows <= 0) {
     set_error("%s: need at least one row, got %lld", what, (long long)rows);
@@ -210,6 +219,7 @@ extern "C" int fsk_reduce_weighted_partials_f64(const float *partials, int64_t n
     set_error("fsk_reduce_weighted_partials_f64: bad arguments");
     return FSK_EINVAL;
   }
+  if (int st = check_elem_rows("fsk_reduce_weighted_partials_f64", m)) return st;
   hipLaunchKernelGGL(reduce_weighted_f64_kernel, dim3(cdiv(m * kD, 256)), dim3(256), 0,
                      as_stream(stream), partials, n_splits, m, ld_split, out, ldo_row,
                      ldo_col);
@@ -228,6 +238,7 @@ extern "C" int fsk_sdkde_debias_f64(const float *x, int64_t ldx, int64_t n, cons
     set_error("fsk_sdkde_debias_f64: bad arguments");
     return FSK_EINVAL;
   }
+  if (int st = check_elem_rows("fsk_sdkde_debias_f64", n)) return st;
   double inv_h2 = 1.0 / (bandwidth * bandwidth);
   double delta = 0.5 * bandwidth * bandwidth;
   hipLaunchKernelGGL(debias_f64_kernel, dim3(cdiv(n * kD, 256)), dim3(256), 0,

@@ -489,6 +489,7 @@ static int score_check(const char *what, const float *data, int64_t ld, int64_t 
                        unsigned flags, const float *pdf, const float *w, bool lowp_ok) {
   int st;
   if ((st = check_points16(what, data, ld, n)) != FSK_OK) return st;
+  if ((st = check_elem_rows(what, n)) != FSK_OK) return st;
   if (!(h > 0.0) || !isfinite(h)) {
     set_error("%s: bandwidth must be positive, got %g", what, h);
     return FSK_EINVAL;
@@ -607,6 +608,7 @@ extern "C" int fsk_score16_rows(const float *rows, int64_t ldr, int64_t m, const
   int st = score_check(what, data, ld, n, bandwidth, flags, pdf, w, true);
   if (st != FSK_OK) return st;
   if ((st = check_points16(what, rows, ldr, m)) != FSK_OK) return st;
+  if ((st = check_elem_rows(what, m)) != FSK_OK) return st;
   if ((flags & FSK_PREC_MASK) != FSK_PREC_F32)
     return score16_lp_run(what, rows, ldr, m, data, ld, n, bandwidth, flags & FSK_PREC_MASK, ws,
                           ws_bytes, pdf, w, ldw_row, ldw_col, stream);

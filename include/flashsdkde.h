@@ -15,6 +15,8 @@
  *   - Accumulator outputs (pdf sums, weighted sums) are fully overwritten:
  *     callers never pre-zero them (the reference zero-fills with torch first,
  *     kernels/emp_score_16d_symmetric_atomic.py:189-190).
+ *   - The score, debias and weighted-combine entry points take at most 2^27
+ *     rows (FSK_EUNSUPPORTED above); the KDE entry points have no such limit.
  *
  * The reference (Flash-SD-KDE, Python + Triton) has no C interface; each entry
  * point below replaces the Python launch wrapper cited next to it, which is
@@ -33,7 +35,7 @@ extern "C" {
 /* ---- status codes ---- */
 #define FSK_OK 0
 #define FSK_EINVAL -1       /* bad argument (shape, stride, bandwidth, pointer) */
-#define FSK_EUNSUPPORTED -2 /* flag combination not implemented */
+#define FSK_EUNSUPPORTED -2 /* flag combination or size not implemented */
 #define FSK_ELAUNCH -3      /* HIP launch / runtime error */
 #define FSK_EWORKSPACE -4   /* workspace missing or too small */
 

@@ -106,3 +106,19 @@ def test_check_maps_status_to_python_errors():
         nat.check(nat.FSK_EINVAL, "fsk_kde16_splitk")
     with pytest.raises(RuntimeError):
         nat.check(nat.FSK_EWORKSPACE, "x")
+
+
+def test_row_counts_above_the_elementwise_limit_are_rejected():
+    """Score, debias and weighted-combine paths run one thread per (row,
+    feature); above 2^27 rows they refuse before any launch."""
+    lib = nat.load_library()
+    big = (1 << 27) + 1
+    assert lib.fsk_score16_sym_atomic(_fake(), 16, big, 1.0, 0, _fake(), 1 << 20, _fake(),
+                                      _fake(), 16, 1, None) == nat.FSK_EUNSUPPORTED
+    assert "rows" in nat.last_error()
+    assert lib.fsk_score16_ordered(_fake(), 16, big, 1.0, 0, _fake(), 1 << 20, _fake(),
+                                   _fake(), 16, 1, None) == nat.FSK_EUNSUPPORTED
+    assert lib.fsk_sdkde_debias_f64(_fake(), 16, big, _fake(), _fake(), 16, 1, 1.0, 1e-12,
+                                    _fake(), 16, None, None) == nat.FSK_EUNSUPPORTED
+    assert lib.fsk_reduce_weighted_partials_f64(_fake(), 2, big, 16 * big, _fake(), 16, 1,
+                                                None) == nat.FSK_EUNSUPPORTED
