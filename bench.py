@@ -433,10 +433,11 @@ def precision_forms(xd, qd, h, fp32_ms, fp32_step, steps, warmup, n_err, rates=N
             # ordered pair from its instruction mix: 1 exp, the row-sum add, and
             # the phi operand (single forms: half a v_cvt_pk; split forms: and,
             # sub, half a v_perm, half a v_cvt_pk; fp16x3s: half a v_cvt_pk and
-            # half a copy). The MFMA work co-issues.
+            # half a copy; single FP16: the row sums as a quarter of a packed
+            # FP16 add and half a mixed FMA). The MFMA work co-issues.
             units = float(n) * n
             v_ops = {"bf16x3": 4.0, "fp16x3": 4.0, "bf16x6": 4.0, "fp16x3s": 2.0,
-                     "bf16x3s": 2.0}.get(sp, 1.5)
+                     "bf16x3s": 2.0, "fp16": 1.25}.get(sp, 1.5)
             t = units / rates["exp_per_s"] + v_ops * units / rates["fma_per_s"]
             out[name]["score_exp_valu_frac_measured_rates"] = round(t / (score_ms * 1e-3), 4)
     ok = [p for p in out if p != "reference" and out[p]["within_1e-3"]]
