@@ -307,7 +307,9 @@ __global__ __launch_bounds__(kWG) void score16_lp_kernel(
         // Single FP16 form: the row sums add the FP16 phi that phi.X
         // multiplies, two at a time: one packed FP16 add of the lane's two
         // phi pairs, then both halves into the FP32 sum (v_fma_mix_f32),
-        // 3 instructions instead of 4 FP32 adds.
+        // 3 instructions instead of 4 FP32 adds. (The one-phi forms keep FP32
+        // row sums: there the FP16 phi, subnormal below 2^-14, would raise
+        // the pdf error of an FP32-class form about six-fold.)
         constexpr bool PAIR_SUM = !SPLIT_PHI && !DUP_PHI && PX == FSK_PREC_F16;
         float e[4];
 #pragma unroll
