@@ -345,14 +345,23 @@ def one_d_sdkde(rates, steps):
 
 
 # Reduced-precision forms timed beside the FP32 step: (name, score precision,
-# log-density precision). The last one mixes forms: the log-density error is
-# set by the KDE's cross term far more than by the score's, so a single-FP16
-# score under the FP16-split KDE stays within the error bound.
-PRECISION_FORMS = (("fp16x3", "fp16x3", "fp16x3"), ("fp16x3s", "fp16x3s", "fp16x3"),
+# log-density precision). The first keeps the FP32 score and evaluates the
+# log-density cross term on the vector ALU (FP32 FMAs). The last one mixes
+# forms: the log-density error is set by the KDE's cross term far more than by
+# the score's, so a single-FP16 score under the FP16-split KDE stays within
+# the error bound.
+PRECISION_FORMS = (("fp32_valu", "fp32", "fp32_valu"),
+                   ("fp16x3", "fp16x3", "fp16x3"), ("fp16x3s", "fp16x3s", "fp16x3"),
                    ("bf16x6", "bf16x6", "bf16x6"), ("bf16x3", "bf16x3", "bf16x3"),
                    ("bf16x3s", "bf16x3s", "bf16x3"), ("fp16", "fp16", "fp16"),
                    ("bf16", "bf16", "bf16"), ("fp16+fp16x3", "fp16", "fp16x3"))
 ERROR_BANDWIDTHS = (0.5, 1.0, 2.0)  # multiples of the workload's bandwidth
+
+
+def score_variant(prec):
+    """The FP32 score runs on the symmetric kernel (as the FP32 step), the
+    16-bit forms on the ordered one."""
+    return "symmetric" if prec == "fp32" else "ordered"
 
 
 def precision_forms(xd, qd, h, fp32_ms, fp32_step, steps, warmup, n_err, rates=None):
@@ -379,7 +388,7 @@ def precision_forms(xd, qd, h, fp32_ms, fp32_step, steps, warmup, n_err, rates=N
         log_norm = -math.log(n) - 16 * math.log(hh) - 8 * math.log(2 * math.pi)
         res = {"fp32": torch.log(fp32_step(hh)[:n_err].double()) + log_norm}
         for name, sp, ep in PRECISION_FORMS:
-            pdf, w = _ops.score16(xd, hh, variant="ordered", precision=sp)
+            pdf, w = _ops.score16(xd, hh, variant=score_variant(sp), precision=sp)
             deb = _ops.debias(xd, pdf, w, hh, 1e-12)
             res[name] = _ops.kde16_logdensity(deb, sub, hh, precision=ep)
         return res
@@ -409,7 +418,7 @@ def precision_forms(xd, qd, h, fp32_ms, fp32_step, steps, warmup, n_err, rates=N
     pairs = float(m) * n + float(n) * n
     for name, sp, ep in PRECISION_FORMS:
         def step():
-            pdf, w = _ops.score16(xd, h, variant="ordered", precision=sp)
+            pdf, w = _ops.score16(xd, h, variant=score_variant(sp), precision=sp)
             deb = _ops.debias(xd, pdf, w, h, 1e-12)
             return _ops.kde16_logdensity(deb, qd, h, precision=ep)
         for _ in range(warmup):
@@ -421,14 +430,16 @@ def precision_forms(xd, qd, h, fp32_ms, fp32_step, steps, warmup, n_err, rates=N
             step()
         torch.cuda.synchronize()
         ms = (time.perf_counter() - t0) / steps * 1e3
-        score_ms = lib.fsk_profile_last_ms(_native.OP_SCORE16_ORDERED)
+        score_ms = lib.fsk_profile_last_ms(_native.OP_SCORE16_SYM if score_variant(sp) ==
+                                           "symmetric" else _native.OP_SCORE16_ORDERED)
         kde_ms = lib.fsk_profile_last_ms(_native.OP_KDE16_LOGDENSITY)
         lib.fsk_profile_enable(0)
         out[name] = {"score_precision": sp, "kde_precision": ep,
                      "ms_per_step": round(ms, 3), "gpair_evals_per_s": round(pairs / ms / 1e6, 3),
                      "speedup_vs_fp32": round(fp32_ms / ms, 3), "score_kernel_ms": round(score_ms, 3),
                      "kde_kernel_ms": round(kde_ms, 3), **err_fields(name)}
-        if rates and rates.get("exp_per_s") and rates.get("fma_per_s") and score_ms > 0:
+        if (rates and rates.get("exp_per_s") and rates.get("fma_per_s") and score_ms > 0
+                and sp != "fp32"):
             # exp/VALU issue time of the score kernel at the measured rates, per
             # ordered pair from its instruction mix: 1 exp, the row-sum add, and
             # the phi operand (single forms: half a v_cvt_pk; split forms: and,
