@@ -157,3 +157,40 @@ def test_symmetric_multi_launch_equals_ordered(gpu):
     record("symmetric_multi_launch_n2097152", pdf_rel_vs_ordered=e_pdf,
            shift_vs_ordered_in_h=e_shift)
     assert e_pdf <= 5e-5 and e_shift <= 5e-5, (e_pdf, e_shift)
+
+
+def test_one_step_sdkde_configuration_n65536(gpu):
+    """The one-step empirical SD-KDE configuration: n = 65,536 points of the
+    8-component mixture, accumulators from both the symmetric-atomic and the
+    ordered split-K kernels (through the reference's wrapper names), debias,
+    KDE of 16,384 fresh queries, against the FP64 pipeline on every query."""
+    import math
+
+    from flash_sd_kde.bandwidth import scott_mean_std_bandwidth
+    from flash_sd_kde.datasets import gmm_isotropic_split
+    from kernels.emp_score_16d_ordered_splitk import emp_score_16d_ordered_splitk
+    from kernels.emp_score_16d_symmetric_atomic import emp_score_16d_symmetric_atomic
+
+    x, q, _ = gmm_isotropic_split(65536, 16384, k=8, mean_scale=3.0, seed=0)
+    h = scott_mean_std_bandwidth(x)
+    xd, qd = torch.as_tensor(x, device=gpu), torch.as_tensor(q, device=gpu)
+    p64, w64 = _ops.ref64_score(xd.double(), h)
+    x64 = xd.double()
+    deb64 = x64 + 0.5 * (w64 / (p64[:, None] + 1e-12) - x64)
+    ref = _ops.ref64_logdensity(deb64, qd.double(), h)
+    kw = dict(device=gpu, precision_mode=IEEE, use_precomputed_norms=True, autotune=True)
+    out = {}
+    for name, fn in (("symmetric", emp_score_16d_symmetric_atomic),
+                     ("ordered", emp_score_16d_ordered_splitk)):
+        pdf, w = fn(xd, h, **kw)
+        p, ww = pdf.double(), w.double()
+        e_pdf = float(((p - p64).abs() / p64).max())
+        e_shift = float((ww / p[:, None] - w64 / p64[:, None]).abs().max()) / h
+        deb = _ops.debias(xd, pdf, w, h, 1e-12)
+        dens = flash_sd_kde.kde_eval(deb, qd, h, device=gpu, precision_mode=IEEE)
+        e_lp = float(((torch.log(dens.double()) - ref).abs() / ref.abs()).max())
+        out[name] = (e_pdf, e_shift, e_lp)
+        assert math.isfinite(e_lp)
+    record("sdkde_configuration_n65536", **{f"{k}_pdf_shift_logp": v for k, v in out.items()})
+    for e_pdf, e_shift, e_lp in out.values():
+        assert e_pdf <= 2e-5 and e_shift <= 2e-5 and e_lp <= 2e-6, out
