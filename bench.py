@@ -131,8 +131,8 @@ def sklearn_baseline(x, q, h, budget_s):
 PMC_TRAFFIC = {
     ("ordered", 1 << 20): {"fetch_kib": 2355570.0, "write_kib": 278528.0,
                            "fetch_scale": 524288.0 / 262169.0, "write_scale": 1.0},
-    ("symmetric", 1 << 20): {"fetch_kib": 17591795.0, "write_kib": 178257920.0,
-                             "fetch_scale": 524288.0 / 262168.0, "write_scale": 1.0},
+    ("symmetric", 1 << 20): {"fetch_kib": 8815652.0, "write_kib": 160432128.0,
+                             "fetch_scale": 524288.0 / 262169.0, "write_scale": 1.0},
 }
 
 

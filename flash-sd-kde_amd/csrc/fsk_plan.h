@@ -37,20 +37,23 @@ constexpr int kSymQT = 8;
 constexpr int kSymRows = 4 * 16 * kSymQT;  // 512 points per block
 // J blocks per task. The row sums of a task stay in FP32 over jc blocks
 // before their FP64 atomic add, so jc trades task overhead for accuracy: at
-// n = 2^20 (8-component mixture) jc = 2, 4, 8 ran the FP32 kernel in 508,
-// 486, 475 ms with largest mean-shift errors (w/pdf vs FP64) of 0.96, 1.13
-// and 1.46e-5 h; the ordered kernel has 0.92e-5 h.
+// n = 2^20 (8-component mixture) jc = 4, 8, 16 ran the FP32 kernel in 478,
+// 465, 459 ms with largest mean-shift errors (w/pdf vs FP64) of 1.13, 1.46
+// and 2.07e-5 h (the ordered kernel: 0.92e-5 h). Flushing the row sums every
+// 4 blocks inside an 8-block task would keep the smaller error, but the
+// flush inside the block loop took the kernel past 256 VGPRs.
 #ifndef FSK_SYM_JC
-#define FSK_SYM_JC 4
+#define FSK_SYM_JC 8
 #endif
 constexpr int64_t kSymJC = FSK_SYM_JC;
 constexpr int kSymTS = 20;         // LDS row stride (floats) of transposed tiles and column sums
 constexpr int kSymSlots = 2;       // phi tiles a wave keeps in LDS at a time
 constexpr int kSymFlushTiles = 8;  // data tiles between two flushes of the column sums
 // Tasks (workgroups) per launch of the symmetric kernel. Their number grows as
-// n^2 / 2^21 and a launch may hold at most 2^32 work-items (2^24 workgroups
-// of 256), so larger n runs as several launches over consecutive task ranges;
-// n = 2^20 (525,312 tasks) is one launch, n = 2^22 nine.
+// n^2 / 2^22 (jc = 8) and a launch may hold at most 2^32 work-items (2^24
+// workgroups of 256), so larger n runs as several launches over consecutive
+// task ranges; n = 2^20 (263,168 tasks) is one launch, n = 2^21 two, n = 2^22
+// five.
 constexpr int64_t kSymTasksPerLaunch = int64_t(1) << 20;
 // Target number of (query block, split) tasks per launch: a few per CU slot.
 constexpr int64_t kTargetTasks = 8192;

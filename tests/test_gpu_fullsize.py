@@ -140,7 +140,7 @@ def test_fullsize_precision_forms_vs_fp64(work, gpu, mult):
 
 
 def test_symmetric_multi_launch_equals_ordered(gpu):
-    """n = 2^21: the symmetric kernel's 2,099,200 tasks run as three launches
+    """n = 2^21: the symmetric kernel's 1,050,624 tasks run as two launches
     over consecutive task ranges; every block pair must be covered exactly
     once, so its accumulators match the ordered kernel's (each sits ~1.4e-5
     from FP64 at n = 2^20)."""
