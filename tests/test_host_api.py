@@ -124,3 +124,18 @@ def test_mixed_precision_form_rejects_unknown_score_precision():
     with pytest.raises(ValueError, match="score_precision"):
         precision.sdkde_logdensity(x, x, 1.0, precision="fp16x3", score_precision="fp8",
                                    device="cpu")
+
+
+def test_bench_precision_forms_name_valid_forms():
+    """Every (score, log-density) pair the benchmark times is a form the
+    precision module accepts, and the metric's bound is checked at the
+    workload's bandwidth and at half and twice it."""
+    import bench
+    from flash_sd_kde import precision
+
+    names = [f[0] for f in bench.PRECISION_FORMS]
+    assert len(names) == len(set(names))
+    for _, sp, ep in bench.PRECISION_FORMS:
+        assert sp in precision.SCORE_PRECISIONS and ep in precision.LOGDENSITY_PRECISIONS
+    assert bench.ERROR_BANDWIDTHS == (0.5, 1.0, 2.0)
+    assert bench.score_variant("fp32") == "symmetric" and bench.score_variant("fp16") == "ordered"
