@@ -162,9 +162,16 @@ size_t kde1_ws_bytes(int64_t n, int64_t m, bool partials, bool two);
 __global__ void fill_f32(float *p, int64_t n, float v);
 __global__ void fill_f64(double *p, int64_t n, double v);
 
-// Per-row scaled squared norms: out[j] = coef * |x_j|^2 (FP64 accumulation),
-// rows [n, n_pad) get kPadNeg so padded tail points contribute exactly 0.
+// Per-row scaled squared norms: out[j] = coef * |x_j|^2 (FP64 accumulation,
+// one rounding), rows [n, n_pad) get `pad` (kPadNeg for norm terms in base-2
+// exponent units) so padded tail points contribute exactly 0.
 __global__ void prep_norms16(const float *__restrict__ x, int64_t ld, int64_t n,
-                             int64_t n_pad, double coef, float *__restrict__ out);
+                             int64_t n_pad, double coef, float pad, float *__restrict__ out);
+
+// Points centred on the mean mu = mu_sum * inv_n: xc = x - mu (FP64
+// difference, FP32 result), 16 floats per row, rows [n, n_pad) zero.
+__global__ void center16_kernel(const float *__restrict__ x, int64_t ld, int64_t n, int64_t n_pad,
+                                const double *__restrict__ mu_sum, double inv_n,
+                                float *__restrict__ xc);
 
 }  // namespace fsk

@@ -95,11 +95,11 @@ __global__ void fill_f64(double *p, int64_t n, double v) {
 }
 
 __global__ void prep_norms16(const float *__restrict__ x, int64_t ld, int64_t n,
-                             int64_t n_pad, double coef, float *__restrict__ out) {
+                             int64_t n_pad, double coef, float pad, float *__restrict__ out) {
   int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (j >= n_pad) return;
   if (j >= n) {
-    out[j] = kPadNeg;
+    out[j] = pad;
     return;
   }
   const float *r = x + j * ld;
@@ -110,6 +110,16 @@ __global__ void prep_norms16(const float *__restrict__ x, int64_t ld, int64_t n,
     s += double(v[0]) * v[0] + double(v[1]) * v[1] + double(v[2]) * v[2] + double(v[3]) * v[3];
   }
   out[j] = float(coef * s);
+}
+
+__global__ void center16_kernel(const float *__restrict__ x, int64_t ld, int64_t n, int64_t n_pad,
+                                const double *__restrict__ mu_sum, double inv_n,
+                                float *__restrict__ xc) {
+  const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= n_pad * kD) return;
+  const int64_t j = e / kD;
+  const int f = int(e % kD);
+  xc[e] = j < n ? float(double(x[j * ld + f]) - mu_sum[f] * inv_n) : 0.f;
 }
 
 // out[i] = sum_s partials[s*ld + i], FP64, fixed order over s.

@@ -1,10 +1,14 @@
 // fsk_kde1.hip - 1-D Gaussian KDE sums and density-derivative sums (VALU).
 //
-// Points are pre-scaled by k = sqrt(log2(e) / (2 h^2)), so with
-// u = k q - k x the kernel value is exp(-(q-x)^2/(2h^2)) = exp2(-u^2):
-// one subtract, one multiply, one exp2 and one add per pair. A data chunk is
-// staged in LDS and read as wave-wide broadcasts; each thread carries
-// k1dQPT queries so every LDS read feeds several pairs.
+// Data points are pre-scaled by k = sqrt(log2(e) / (2 h^2)) (x' = k x, one
+// rounding); with u = fma(q, k, -x') (the query's product enters unrounded)
+// the kernel value is exp(-(q-x)^2/(2h^2)) = exp2(-u^2): one FMA, one
+// multiply, one exp2 and one add per pair. The Laplacian factor
+// (1.5 - ln2 u^2) multiplies each pair before the sum, as the reference's
+// phi * (1 + 1/2 - z^2/2) does (flash_sd_kde/reference.py:200-213), so the
+// sum carries no cancellation between two separately rounded totals. A data
+// chunk is staged in LDS and read as wave-wide broadcasts; each thread
+// carries k1dQPT queries so every LDS read feeds several pairs.
 #include "fsk_common.h"
 #include "fsk_plan.h"
 
@@ -20,14 +24,15 @@ __global__ void prep_scale1(const float *__restrict__ x, int64_t n, int64_t n_pa
   out[j] = j < n ? x[j] * k : kFar1d;
 }
 
-// MODE 0: plain sums; MODE 1: Laplacian factor (1.5 - ln2 u^2);
+// MODE 0: plain sums; MODE 1: sums of phi (1.5 - ln2 u^2);
 // MODE 2: pdf sums and derivative sums sum(-z phi) (two outputs).
 template <int MODE, bool ATOMIC>
 __global__ __launch_bounds__(256) void kde1_kernel(const float *__restrict__ xs, int64_t n_pad,
                                                    const float *__restrict__ qs, int64_t m,
                                                    int64_t split_len, int64_t qblocks,
                                                    int64_t ntasks, int64_t per_xcd,
-                                                   float zscale, float *__restrict__ out,
+                                                   float kscale, float zscale,
+                                                   float *__restrict__ out,
                                                    float *__restrict__ out2, int64_t ld_out) {
   __shared__ f4 chunk[k1dChunk / 4];
   const int64_t task = xcd_task(blockIdx.x, per_xcd);
@@ -60,11 +65,13 @@ __global__ __launch_bounds__(256) void kde1_kernel(const float *__restrict__ xs,
       for (int s = 0; s < 4; ++s) {
 #pragma unroll
         for (int k = 0; k < k1dQPT; ++k) {
-          float u = q[k] - xv[s];
-          float t = u * u;
-          float e = fexp2(-t);
-          acc[k] += e;
-          if (MODE == 1) acc2[k] = fmaf(e, t, acc2[k]);
+          const float u = fmaf(q[k], kscale, -xv[s]);
+          const float t = u * u;
+          const float e = fexp2(-t);
+          if (MODE == 1)
+            acc[k] = fmaf(e, fmaf(t, -float(kLn2), 1.5f), acc[k]);
+          else
+            acc[k] += e;
           if (MODE == 2) acc2[k] = fmaf(e, u, acc2[k]);
         }
       }
@@ -80,8 +87,7 @@ __global__ __launch_bounds__(256) void kde1_kernel(const float *__restrict__ xs,
   for (int k = 0; k < k1dQPT; ++k) {
     int64_t i = qb * k1dRowsPerWG + k * 256 + tid;
     if (i >= m) continue;
-    double v = a64[k];
-    if (MODE == 1) v = 1.5 * v - kLn2 * b64[k];
+    const double v = a64[k];
     float v2 = 0.f;
     // sum(-z phi) with z = u / sqrt(log2 e / 2): -z = -u * sqrt(2 ln 2)
     if (MODE == 2) v2 = float(-b64[k] * double(zscale));
@@ -96,7 +102,7 @@ __global__ __launch_bounds__(256) void kde1_kernel(const float *__restrict__ xs,
 }
 
 struct Kde1Layout {
-  float *xs, *qs, *part, *part2;
+  float *xs, *part, *part2;
 };
 
 static size_t kde1_layout(void *ws, int64_t n, int64_t m, bool partials, bool two,
@@ -104,7 +110,6 @@ static size_t kde1_layout(void *ws, int64_t n, int64_t m, bool partials, bool tw
   Carve cv(ws);
   SplitPlan p = kde1_plan(n, m);
   L->xs = cv.take<float>(round_up(n, k1dChunk));
-  L->qs = cv.take<float>(m);
   L->part = partials ? cv.take<float>(p.splits * m) : nullptr;
   L->part2 = (partials && two) ? cv.take<float>(p.splits * m) : nullptr;
   return cv.used;
@@ -137,8 +142,6 @@ static int kde1_run(const char *what, int mode, bool atomic, const float *data, 
   const int64_t n_pad = round_up(n, k1dChunk);
   hipLaunchKernelGGL(prep_scale1, dim3(cdiv(n_pad, 256)), dim3(256), 0, s, data, n, n_pad,
                      float(k), L.xs);
-  hipLaunchKernelGGL(prep_scale1, dim3(cdiv(m, 256)), dim3(256), 0, s, query, m, m, float(k),
-                     L.qs);
   SplitPlan p = kde1_plan(n, m);
   const int64_t ntasks = p.qblocks * p.splits;
   int64_t per_xcd;
@@ -154,8 +157,8 @@ static int kde1_run(const char *what, int mode, bool atomic, const float *data, 
   const int op = mode == 2 ? FSK_OP_KDE1_SCORE : (atomic ? FSK_OP_KDE1_ATOMIC : FSK_OP_KDE1_SPLITK);
   prof_begin(op, s);
 #define FSK_K1(MODE, AT)                                                                      \
-  hipLaunchKernelGGL((kde1_kernel<MODE, AT>), dim3(grid), dim3(256), 0, s, L.xs, n_pad, L.qs, m, \
-                     p.split_len, p.qblocks, ntasks, per_xcd, zscale, o1, o2, m)
+  hipLaunchKernelGGL((kde1_kernel<MODE, AT>), dim3(grid), dim3(256), 0, s, L.xs, n_pad, query, m, \
+                     p.split_len, p.qblocks, ntasks, per_xcd, float(k), zscale, o1, o2, m)
   if (atomic) {
     if (mode == 0) FSK_K1(0, true);
     else if (mode == 1) FSK_K1(1, true);

@@ -1,15 +1,20 @@
 // fsk_kde16.hip - 16-D Gaussian KDE sums on MFMA (split-K and atomic forms).
 //
-// Per pair (i, j) the kernel evaluates
-//     t_ij = c |q_i|^2 + c |x_j|^2 - 2 c q_i . x_j,   c = -log2(e) / (2 h^2)
+// Data and queries are first centred on the data mean (FP64 mean, one FP32
+// rounding of each difference): distances are translation invariant and the
+// norm terms, whose cancellation sets the FP32 error, shrink. Per pair (i, j)
+// of centred points the kernel evaluates
+//     D_ij = |x_j|^2 - 2 q_i . x_j                  (MFMA, natural units)
+//     t_ij = c D_ij + c |q_i|^2,   c = -log2(e) / (2 h^2)   (one FMA)
 //     phi  = exp2(min(t_ij, 0))
 // which is exp(-max(|q|^2 + |x|^2 - 2 q.x, 0) / (2 h^2)), the reference's
 // clamped norm expansion (kernels/kde_eval_16d_stream_splitk.py:71-75).
-// The two norm terms initialise the MFMA accumulator (one VALU add), the
-// cross term is four chained v_mfma_f32_16x16x4_f32 with the query side
-// pre-scaled by -2c, and min / exp2 / add finish the pair on the VALU.
-// Padded tail points carry c|x|^2 = -1e30, so they contribute exactly 0.
+// |x_j|^2 (FP64, rounded once) initialises the MFMA accumulator, the cross
+// term is four chained v_mfma_f32_16x16x4_f32 against -2 q (an exact
+// scaling), and fma / exp2 / add finish the pair on the VALU. Padded tail
+// points carry a norm term P with c P = -1e30, so they contribute exactly 0.
 #include "fsk_common.h"
+#include "fsk_lowp.h"
 #include "fsk_plan.h"
 #include "fsk_tile.h"
 
@@ -23,7 +28,7 @@ template <int QT, bool LAPL, bool ATOMIC>
 __global__ __launch_bounds__(kWG) void kde16_f32_kernel(
     const float *__restrict__ data, int64_t ld, int64_t n, const float *__restrict__ bn,
     int64_t n_pad, const float *__restrict__ query, int64_t ldq, int64_t m,
-    const float *__restrict__ qa, float qscale, int64_t split_len, int64_t qblocks,
+    const float *__restrict__ qa, float coef, int64_t split_len, int64_t qblocks,
     int64_t ntasks, int64_t per_xcd, float *__restrict__ out, int64_t ld_out) {
   const int64_t task = xcd_task(blockIdx.x, per_xcd);
   if (task >= ntasks) return;
@@ -41,7 +46,7 @@ __global__ __launch_bounds__(kWG) void kde16_f32_kernel(
   for (int R = 0; R < QT; ++R) {
     int64_t i = row0 + 16 * R + c;
     if (i > m - 1) i = m - 1;
-    qv[R] = ld4(query + i * ldq + 4 * g) * qscale;
+    qv[R] = ld4(query + i * ldq + 4 * g) * -2.f;
     qav[R] = qa[i];
     acc[R] = 0.f;
     accT[R] = 0.f;
@@ -55,8 +60,8 @@ __global__ __launch_bounds__(kWG) void kde16_f32_kernel(
   int tc = 0;
   for_each_tile<false>(data, ld, n, bn, j_begin, j_end, g, c, [&](const TileData &t) {
     f4 d[QT];
-    cross_tile<QT>(qv, qav, t, d);
-    kernel_tile<QT, LAPL>(d, acc, accT);
+    cross_tile<QT>(qv, t, d);
+    kernel_tile<QT, LAPL, true>(d, qav, coef, acc, accT);
     if (++tc == kFlushTiles) {
       tc = 0;
 #pragma unroll
@@ -86,6 +91,9 @@ __global__ __launch_bounds__(kWG) void kde16_f32_kernel(
 }
 
 struct Kde16Layout {
+  double *mu;   // data mean (column sums) workspace
+  float *xc;    // centred data, zero rows up to a multiple of the tile
+  float *qc;    // centred queries
   float *bn;
   float *qa;
   float *partials;
@@ -95,6 +103,9 @@ static size_t kde16_layout(void *ws, int64_t n, int64_t m, bool with_partials,
                            Kde16Layout *L) {
   Carve cv(ws);
   SplitPlan p = kde16_plan(n, m);
+  L->mu = cv.take<double>(kMeanWs);
+  L->xc = cv.take<float>(round_up(n, kTile) * kD);
+  L->qc = cv.take<float>(m * kD);
   L->bn = cv.take<float>(round_up(n, kNormPad));
   L->qa = cv.take<float>(m);
   L->partials = with_partials ? cv.take<float>(p.splits * m) : nullptr;
@@ -135,26 +146,35 @@ static int kde16_run(const char *what, bool atomic, const float *data, int64_t l
   hipStream_t s = as_stream(stream);
   const double coef = exp2_coef(h);
   const int64_t n_pad = round_up(n, kTile);
+  const double inv_n = 1.0 / double(n);
+  // centre data and queries on the data mean; data norms in natural units
+  // (padded rows: c P = -1e30, P kept finite), query norms times c
+  const double pad = fmin(1e30 / -coef, 1e38);
+  launch_mean16(data, ld, n, L.mu, s);
+  hipLaunchKernelGGL(center16_kernel, dim3(cdiv(n_pad * kD, 256)), dim3(256), 0, s, data, ld, n,
+                     n_pad, L.mu, inv_n, L.xc);
+  hipLaunchKernelGGL(center16_kernel, dim3(cdiv(m * kD, 256)), dim3(256), 0, s, query, ldq, m, m,
+                     L.mu, inv_n, L.qc);
   hipLaunchKernelGGL(prep_norms16, dim3(cdiv(round_up(n, kNormPad), 256)), dim3(256), 0, s,
-                     data, ld, n, round_up(n, kNormPad), coef, L.bn);
-  hipLaunchKernelGGL(prep_norms16, dim3(cdiv(m, 256)), dim3(256), 0, s, query, ldq, m, m,
-                     coef, L.qa);
+                     L.xc, int64_t(kD), n, round_up(n, kNormPad), 1.0, float(pad), L.bn);
+  hipLaunchKernelGGL(prep_norms16, dim3(cdiv(m, 256)), dim3(256), 0, s, L.qc, int64_t(kD), m, m,
+                     coef, kPadNeg, L.qa);
   SplitPlan p = kde16_plan(n, m);
   const int64_t ntasks = p.qblocks * p.splits;
   int64_t per_xcd;
   const int64_t grid = xcd_grid(ntasks, &per_xcd);
-  const float qscale = float(-2.0 * coef);
+  const float fcoef = float(coef);
   const int op = atomic ? FSK_OP_KDE16_ATOMIC : FSK_OP_KDE16_SPLITK;
   if (atomic) {
     hipLaunchKernelGGL(fill_f32, dim3(cdiv(m, 256)), dim3(256), 0, s, out, m, 0.f);
     prof_begin(op, s);
     if (lapl)
       hipLaunchKernelGGL((kde16_f32_kernel<kKdeQT, true, true>), dim3(grid), dim3(kWG), 0, s,
-                         data, ld, n, L.bn, n_pad, query, ldq, m, L.qa, qscale, p.split_len,
+                         L.xc, int64_t(kD), n, L.bn, n_pad, L.qc, int64_t(kD), m, L.qa, fcoef, p.split_len,
                          p.qblocks, ntasks, per_xcd, out, int64_t(0));
     else
       hipLaunchKernelGGL((kde16_f32_kernel<kKdeQT, false, true>), dim3(grid), dim3(kWG), 0, s,
-                         data, ld, n, L.bn, n_pad, query, ldq, m, L.qa, qscale, p.split_len,
+                         L.xc, int64_t(kD), n, L.bn, n_pad, L.qc, int64_t(kD), m, L.qa, fcoef, p.split_len,
                          p.qblocks, ntasks, per_xcd, out, int64_t(0));
     prof_end(op, s);
     return launch_status(what);
@@ -162,11 +182,11 @@ static int kde16_run(const char *what, bool atomic, const float *data, int64_t l
   prof_begin(op, s);
   if (lapl)
     hipLaunchKernelGGL((kde16_f32_kernel<kKdeQT, true, false>), dim3(grid), dim3(kWG), 0, s,
-                       data, ld, n, L.bn, n_pad, query, ldq, m, L.qa, qscale, p.split_len,
+                       L.xc, int64_t(kD), n, L.bn, n_pad, L.qc, int64_t(kD), m, L.qa, fcoef, p.split_len,
                        p.qblocks, ntasks, per_xcd, L.partials, m);
   else
     hipLaunchKernelGGL((kde16_f32_kernel<kKdeQT, false, false>), dim3(grid), dim3(kWG), 0, s,
-                       data, ld, n, L.bn, n_pad, query, ldq, m, L.qa, qscale, p.split_len,
+                       L.xc, int64_t(kD), n, L.bn, n_pad, L.qc, int64_t(kD), m, L.qa, fcoef, p.split_len,
                        p.qblocks, ntasks, per_xcd, L.partials, m);
   prof_end(op, s);
   if ((st = launch_status(what)) != FSK_OK) return st;

@@ -29,8 +29,18 @@ constexpr int kScoreFlush = 32;
 static_assert(kNormPad % kScoreRowsPerWG == 0, "row norms must cover whole row blocks");
 
 // Largest spread of the query terms A_i = -c|q_i|^2 (c < 0) inside one
-// workgroup for which the factored form below is used.
+// workgroup for which the factored form below is used, and the largest
+// exponent the form lets exp2 see.
 constexpr float kFactSpread = 160.f;
+constexpr float kFactHalf = 80.f;
+
+// Shift K of the factored form for a workgroup whose A_i lie in [amin, amax]
+// (amax - amin <= kFactSpread): the largest d of row i is A_i - K and the
+// per-row factor is exp2(K - A_i), both within +-kFactHalf. K is kept as small
+// as that allows (0 while amax <= kFactHalf): the accumulator then starts from
+// c|x_j|^2 - K with the least magnitude, and the chained FP32 roundings of the
+// cross term happen at the smallest partial sums.
+__device__ __forceinline__ float fact_shift(float amax) { return fmaxf(0.f, amax - kFactHalf); }
 
 // Rows (queries) and data may differ: rows = data gives the empirical score of
 // the data set itself; a row subset gives the score of those rows under the
@@ -40,11 +50,12 @@ constexpr float kFactSpread = 160.f;
 // 2c q_i.x_j is evaluated as exp2(d_ij) * exp2(qa_i + K), d_ij = (c|x_j|^2 - K)
 // - 2c q_i.x_j: the MFMA accumulator starts from the data-side term (shared by
 // every row tile of a data tile), so no per-pair add is needed, and the
-// per-row factor multiplies the finished FP64 sums once. K is the midpoint of
-// the workgroup's A_i range; with the self pair t_ii = 0 in every row, the
-// largest d of row i is A_i - K, within +-kFactSpread/2, so exp2 neither
-// overflows nor leaves the normal range for any pair that matters. A
-// workgroup whose rows spread wider takes the direct form.
+// per-row factor multiplies the finished FP64 sums once. K = fact_shift(max
+// A_i); with the self pair t_ii = 0 in every row, the largest d of row i is
+// A_i - K, within +-kFactHalf, so exp2 neither overflows nor leaves the
+// normal range for any pair that matters. A workgroup whose rows spread wider
+// takes the direct form (accumulator from c|x_j|^2, qa_i added after the
+// cross term).
 template <int QT, bool FACT>
 __global__ __launch_bounds__(kWG) void score16_ordered_kernel(
     const float *__restrict__ data, int64_t ld, int64_t n, const float *__restrict__ bn,
@@ -101,7 +112,7 @@ __global__ __launch_bounds__(kWG) void score16_ordered_kernel(
     amin = fminf(fminf(red[0], red[1]), fminf(red[2], red[3]));
     amax = fmaxf(fmaxf(red[4], red[5]), fmaxf(red[6], red[7]));
     fact = amax - amin <= kFactSpread;  // finite and narrow (padded-only rows give -inf)
-    kq = 0.5f * (amax + amin);
+    kq = fact_shift(amax);
   }
 
   const int64_t j_begin = split * split_len;
@@ -132,8 +143,8 @@ __global__ __launch_bounds__(kWG) void score16_ordered_kernel(
           }
         }
       } else {
-        cross_tile<QT>(qv, qav, t, d);
-        kernel_tile<QT, false>(d, pdf, unused);
+        cross_tile<QT>(qv, t, d);
+        kernel_tile<QT, false, false>(d, qav, 1.f, pdf, unused);
       }
       phi_x_tile<QT>(d, t, w);
       if (++tc == kScoreFlush) {
@@ -254,7 +265,7 @@ __global__ __launch_bounds__(kWG) void score16_sym_kernel(
   amin = fminf(fminf(red[0], red[1]), fminf(red[2], red[3]));
   amax = fmaxf(fmaxf(red[4], red[5]), fmaxf(red[6], red[7]));
   const bool fact = amax - amin <= kFactSpread;
-  const float kq = 0.5f * (amax + amin);
+  const float kq = fact_shift(amax);
 
   // per-row factors (padded rows: f = 0 in the factored form; their points are
   // zero, so x f = 0 in both forms) and the column side's B operand f_i x_i
@@ -271,17 +282,13 @@ __global__ __launch_bounds__(kWG) void score16_sym_kernel(
 
   auto run = [&](auto factored) {
     constexpr bool F = decltype(factored)::value;
-    // cross term of row tile R into d: factored form starts from the data-side
-    // term ci = c|x_j|^2 - K, the direct form from qa_i + c|x_j|^2
+    // cross term of row tile R into d: the factored form starts from the
+    // data-side term ci = c|x_j|^2 - K, the direct form from c|x_j|^2 (qa_i is
+    // added after the cross term, in kern)
     auto cross = [&](const TileData &t, const f4 &ci, int R, f4 &d) {
-      if constexpr (F) {
-        d = mfma_f32(t.xa[0], qv[R][0], ci);
-      } else {
-        d = f4{qav[R] + t.bb[0], qav[R] + t.bb[1], qav[R] + t.bb[2], qav[R] + t.bb[3]};
-        d = mfma_f32(t.xa[0], qv[R][0], d);
-      }
+      d = mfma_f32(t.xa[0], qv[R][0], F ? ci : t.bb);
     };
-    auto kern = [&](float v) { return F ? fexp2(v) : fexp2_le1(v); };
+    auto kern = [&](float v, int R) { return F ? fexp2(v) : fexp2_le1(v + qav[R]); };
     for (int64_t J = Jlo; J < Jhi; ++J) {
       const int64_t jbase = J * B;
       if (J == I) {
@@ -302,7 +309,7 @@ __global__ __launch_bounds__(kWG) void score16_sym_kernel(
           for (int R = 0; R < QT; ++R) {
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-              const float e = kern(d[R][r]);
+              const float e = kern(d[R][r], R);
               pI[R] += e;
               d[R][r] = e;
             }
@@ -335,7 +342,7 @@ __global__ __launch_bounds__(kWG) void score16_sym_kernel(
             float *sb = tslot[wave][q];
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-              const float e = kern(d[q][r]);
+              const float e = kern(d[q][r], R);
               pI[R] += e;
               pj[r] = fmaf(e, fI[R], pj[r]);
               d[q][r] = e;
@@ -423,18 +430,6 @@ __global__ void score_finish_kernel(const double *__restrict__ pdf64,
 }
 
 static_assert(kNormPad % kSymRows == 0, "padded rows must cover whole symmetric blocks");
-
-// Points centred on the data mean: x - mu (FP64 difference, FP32 result),
-// rows [n, n_pad) zero.
-__global__ void center16_kernel(const float *__restrict__ x, int64_t ld, int64_t n, int64_t n_pad,
-                                const double *__restrict__ mu_sum, double inv_n,
-                                float *__restrict__ xc) {
-  const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (e >= n_pad * kD) return;
-  const int64_t j = e / kD;
-  const int f = int(e % kD);
-  xc[e] = j < n ? float(double(x[j * ld + f]) - mu_sum[f] * inv_n) : 0.f;
-}
 
 struct ScoreLayout {
   float *bn;
@@ -537,7 +532,7 @@ extern "C" int fsk_score16_ordered(const float *data, int64_t ld, int64_t n, dou
   hipLaunchKernelGGL(center16_kernel, dim3(cdiv(n16 * kD, 256)), dim3(256), 0, s, data, ld, n,
                      n16, L.mu, inv_n, L.xc);
   hipLaunchKernelGGL(prep_norms16, dim3(cdiv(n_norm, 256)), dim3(256), 0, s, L.xc, int64_t(kD), n,
-                     n_norm, coef, L.bn);
+                     n_norm, coef, kPadNeg, L.bn);
   SplitPlan p = score16_ordered_plan(n);
   const int64_t ntasks = p.qblocks * p.splits;
   int64_t per_xcd;
@@ -575,7 +570,7 @@ extern "C" int fsk_score16_sym_atomic(const float *data, int64_t ld, int64_t n,
   hipLaunchKernelGGL(center16_kernel, dim3(cdiv(n_norm * kD, 256)), dim3(256), 0, s, data, ld, n,
                      n_norm, L.mu, inv_n, L.xc);
   hipLaunchKernelGGL(prep_norms16, dim3(cdiv(n_norm, 256)), dim3(256), 0, s, L.xc, int64_t(kD), n,
-                     n_norm, coef, L.bn);
+                     n_norm, coef, kPadNeg, L.bn);
   hipLaunchKernelGGL(fill_f64, dim3(cdiv(n, 256) < 4096 ? cdiv(n, 256) : 4096), dim3(256), 0, s,
                      L.pdf64, n, 0.0);
   hipLaunchKernelGGL(fill_f64, dim3(cdiv(n * kD, 256) < 4096 ? cdiv(n * kD, 256) : 4096),
@@ -630,9 +625,9 @@ extern "C" int fsk_score16_rows(const float *rows, int64_t ldr, int64_t m, const
   hipLaunchKernelGGL(center16_kernel, dim3(cdiv(m * kD, 256)), dim3(256), 0, s, rows, ldr, m, m,
                      L.mu, inv_n, L.rc);
   hipLaunchKernelGGL(prep_norms16, dim3(cdiv(n_norm, 256)), dim3(256), 0, s, L.xc, int64_t(kD), n,
-                     n_norm, coef, L.bn);
+                     n_norm, coef, kPadNeg, L.bn);
   hipLaunchKernelGGL(prep_norms16, dim3(cdiv(m_norm, 256)), dim3(256), 0, s, L.rc, int64_t(kD), m,
-                     m_norm, coef, L.qa);
+                     m_norm, coef, kPadNeg, L.qa);
   SplitPlan p = score16_rows_plan(n, m);
   const int64_t ntasks = p.qblocks * p.splits;
   int64_t per_xcd;

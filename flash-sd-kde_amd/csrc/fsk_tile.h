@@ -1,13 +1,23 @@
 // fsk_tile.h - the per-tile building blocks of the 16-D pair kernels.
 //
-// A wave holds QT row tiles of 16 queries (B operand, pre-scaled by -2c) and
+// A wave holds QT row tiles of 16 queries (B operand, pre-scaled) and
 // streams data in tiles of 16 points (A operand). For one data tile:
-//   cross_tile : D[R] = (c|q|^2 + c|x|^2) + 4 chained v_mfma_f32_16x16x4_f32
-//                -> lane holds t(i = lane%16, j = 4(lane/16) + r), r = 0..3
-//   kernel_tile: phi = exp2(min(t, 0)) in place, row sums into acc[R]
+//   cross_tile : D[R] = bb_j + 4 chained v_mfma_f32_16x16x4_f32
+//                -> lane holds D(i = lane%16, j = 4(lane/16) + r), r = 0..3
+//   kernel_tile: t = D * cs + qa_i (NAT) or D + qa_i, phi = exp2(min(t, 0)) in
+//                place, row sums into acc[R]
 //   phi_x_tile : W[R] += phi . X_J  (4 MFMAs; phi is already the A operand)
+// The accumulator starts from the data-side norm term only and the query-side
+// term is added once after the cross term: the chained FP32 roundings of the
+// MFMA then happen at |bb_j + partial cross term|, which stays well below the
+// full |qa_i + bb_j| that a combined initialiser would carry through all four
+// steps (on 16-D N(0, I) data this halves the FP32 error of the densities).
+// Two unit systems share the code:
+//   log2 units: bb = c|x|^2, query operand -2c q, t = D + qa       (score paths)
+//   natural   : bb = |x|^2,  query operand -2 q (exact), t = fma(D, c, qa)
+//               (the KDE: no rounded query pre-scale, one rounding for c)
 // TileSrc addresses one data tile: xa = features 4g..4g+3 of point j0 + c,
-// bb = c|x|^2 of points j0 + 4g .. +3, xt[r] = feature c of point j0 + 4g + r.
+// bb = norm terms of points j0 + 4g .. +3, xt[r] = feature c of point j0 + 4g + r.
 #pragma once
 
 #include "fsk_common.h"
@@ -20,7 +30,7 @@ struct TileData {
 };
 
 // Loads for tile j0. CLAMP = true for the one tile that crosses n (rows >= n
-// are replaced by row n-1; their norm term is already kPadNeg, so phi = 0).
+// are replaced by row n-1; their norm term is already the pad value, so phi = 0).
 template <bool CLAMP, bool WITH_XT>
 __device__ __forceinline__ void load_tile(const float *__restrict__ data, int64_t ld, int64_t n,
                                           const float *__restrict__ bn, int64_t j0, int g, int c,
@@ -40,37 +50,33 @@ __device__ __forceinline__ void load_tile(const float *__restrict__ data, int64_
 }
 
 template <int QT>
-__device__ __forceinline__ void cross_tile(const f4 (&qv)[QT], const float (&qav)[QT],
-                                           const TileData &t, f4 (&d)[QT]) {
+__device__ __forceinline__ void cross_tile(const f4 (&qv)[QT], const TileData &t, f4 (&d)[QT]) {
 #pragma unroll
-  for (int R = 0; R < QT; ++R) {
-    d[R][0] = qav[R] + t.bb[0];
-    d[R][1] = qav[R] + t.bb[1];
-    d[R][2] = qav[R] + t.bb[2];
-    d[R][3] = qav[R] + t.bb[3];
-  }
+  for (int R = 0; R < QT; ++R) d[R] = mfma_f32(t.xa[0], qv[R][0], t.bb);
 #pragma unroll
-  for (int s = 0; s < 4; ++s) {
+  for (int s = 1; s < 4; ++s) {
 #pragma unroll
     for (int R = 0; R < QT; ++R) d[R] = mfma_f32(t.xa[s], qv[R][s], d[R]);
   }
 }
 
-// phi = exp2(min(t, 0)) written back into d; acc[R] += row sums.
-// LAPL also accumulates accT[R] += phi * min(t, 0).
-template <int QT, bool LAPL>
-__device__ __forceinline__ void kernel_tile(f4 (&d)[QT], float (&acc)[QT], float (&accT)[QT]) {
+// t = D * cs + qa (NAT) or D + qa; phi = exp2(min(t, 0)) written back into d;
+// acc[R] += row sums. LAPL also accumulates accT[R] += phi * min(t, 0).
+template <int QT, bool LAPL, bool NAT>
+__device__ __forceinline__ void kernel_tile(f4 (&d)[QT], const float (&qav)[QT], float cs,
+                                            float (&acc)[QT], float (&accT)[QT]) {
 #pragma unroll
   for (int R = 0; R < QT; ++R) {
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
+      const float v = NAT ? fmaf(d[R][r], cs, qav[R]) : d[R][r] + qav[R];
       float e;
       if (LAPL) {
-        const float t = fminf(d[R][r], 0.f);
+        const float t = fminf(v, 0.f);
         e = fexp2(t);
         accT[R] = fmaf(e, t, accT[R]);
       } else {
-        e = fexp2_le1(d[R][r]);
+        e = fexp2_le1(v);
       }
       acc[R] += e;
       d[R][r] = e;

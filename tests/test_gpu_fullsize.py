@@ -4,8 +4,11 @@ queries, the mean-std bandwidth rule. The CPU oracle cannot reach this size,
 so the checks are properties that hold at any n plus the FP64 GPU kernels
 (direct differences, themselves checked against the CPU FP64 oracle at small n):
 
-* both score kernels against the FP64 score: pdf to 5e-5 relative, the mean
-  shift w/pdf to 3e-5 bandwidths;
+* both score kernels against the FP64 score: pdf to 2e-5 relative, the mean
+  shift w/pdf to 1.5e-5 bandwidths (measured 1.4e-5 / 1.25e-5; the
+  reference's FP32 operation order, emulated on the 64 most offset rows of
+  two 512-row blocks, errs by up to 3.0e-5 in pdf where this one errs by
+  5.1e-6);
 * the score's antisymmetry identity sum_i w_i = sum_i pdf_i x_i (the double
   sum of phi_ij (x_j - x_i) vanishes because phi is symmetric), evaluated in
   FP64 over each kernel's outputs: a dropped block, a misplaced column scatter
@@ -22,6 +25,9 @@ so the checks are properties that hold at any n plus the FP64 GPU kernels
   splits (results are bitwise equal only between calls of the same sizes).
 """
 
+import json
+import os
+
 import pytest
 import torch
 
@@ -36,6 +42,10 @@ IEEE = "fp32_ieee"
 
 def record(name, **vals):
     print(name, vals, flush=True)
+    path = os.environ.get("FSK_METRICS_OUT")
+    if path:
+        with open(path, "a") as f:
+            f.write(json.dumps({"test": name, **vals}) + "\n")
 
 
 @pytest.fixture(scope="module")
@@ -60,7 +70,7 @@ def test_fullsize_score_vs_fp64(work, variant):
     e_pdf = float(((p - p64).abs() / p64).max())
     e_shift = float((w / p[:, None] - w64 / p64[:, None]).abs().max()) / h
     record(f"fullsize_score_{variant}", pdf_rel_vs_fp64=e_pdf, shift_vs_fp64_in_h=e_shift)
-    assert e_pdf <= 5e-5 and e_shift <= 3e-5, (e_pdf, e_shift)
+    assert e_pdf <= 2e-5 and e_shift <= 1.5e-5, (e_pdf, e_shift)
 
 
 @pytest.mark.parametrize("variant", ["symmetric", "ordered"])
@@ -97,7 +107,7 @@ def test_fullsize_sdkde_pipeline_vs_fp64(work, gpu):
     e = float(((lp - ref).abs() / ref.abs()).max())
     shift = float((deb.double() - deb64).abs().max()) / h
     record("fullsize_sdkde_pipeline", max_rel_logp_err=e, debias_vs_fp64_in_h=shift)
-    assert e <= 2e-6 and shift <= 3e-5, (e, shift)
+    assert e <= 1e-6 and shift <= 1.5e-5, (e, shift)
 
 
 def test_fullsize_kde_backends_and_chunking(work, gpu):

@@ -1,12 +1,16 @@
 """GPU parity: the HIP kernels against the reference's golden vectors and the
 FP64 oracle, on the reference tests' seeds/shapes and the small workloads.
 
-Tolerances (FP32 forms): relative error vs the reference's FP32 outputs
-<= 2e-5, and relative error vs the FP64 oracle <= 2x the reference's own
-FP32 error vs FP64 (+1e-6). Both evaluate the clamped norm expansion
-|q|^2 + |x|^2 - 2 q.x in FP32, whose rounding at magnitude |q|^2/h^2 sets a
-noise floor of ~1e-5 relative for 16-D N(0, I) data; the test requires the
-GPU to sit on the same floor, and every case records both errors.
+Bounds (FP32 forms), with e_x = max relative error of x against FP64:
+* at least as accurate as the reference: e_gpu <= e_reference + 1e-6;
+* within 1e-5 of the reference's own outputs on every element where the
+  reference itself is within 5e-6 of FP64 (where it is further off, the two
+  FP32 results may differ by up to the sum of their errors; the cases record
+  the largest difference over all elements as well).
+Both evaluate the clamped norm expansion |q|^2 + |x|^2 - 2 q.x in FP32, whose
+cancellation sets a noise floor of ~1e-5 relative for 16-D N(0, I) data (the
+reference's own error on the 4096 x 1024 workload is 8.9e-6 / 9.9e-6 at the
+two bandwidth rules); every case records both errors.
 """
 
 import json
@@ -55,13 +59,20 @@ def cpu(t):
     return t.detach().cpu().numpy()
 
 
-def check_vs_ref(name, got, ref, exact, tol=2e-5, slack=3e-6):
-    e_ref = rel(got, ref)
+def check_vs_ref(name, got, ref, exact, tol=1e-5, slack=1e-6, room=5e-6):
+    got, ref, exact = (np.asarray(a, np.float64) for a in (got, ref, exact))
+    scale = np.maximum(np.abs(exact), 1e-300)
+    ref_err = np.abs(ref - exact) / scale
+    diff = np.abs(got - ref) / scale
     e_gpu64 = rel(got, exact)
-    e_ref64 = rel(ref, exact)
-    record(name, rel_vs_reference=e_ref, rel_vs_fp64=e_gpu64, reference_rel_vs_fp64=e_ref64)
-    assert e_ref <= tol, (e_ref, tol)
-    assert e_gpu64 <= 2.0 * e_ref64 + slack, (e_gpu64, e_ref64)
+    e_ref64 = float(ref_err.max())
+    inside = ref_err <= room
+    e_ref_room = float(diff[inside].max()) if inside.any() else 0.0
+    record(name, rel_vs_reference=float(diff.max()), rel_vs_reference_where_room=e_ref_room,
+           rel_vs_fp64=e_gpu64, reference_rel_vs_fp64=e_ref64)
+    assert e_gpu64 <= e_ref64 + slack, (e_gpu64, e_ref64)
+    assert e_ref_room <= tol, (e_ref_room, tol)
+    assert float(diff.max()) <= e_gpu64 + e_ref64 + 1e-7
 
 
 # ---------------- 16-D KDE ----------------
@@ -81,7 +92,7 @@ def test_kde16_linearized_matches_reference(gpu, golden, case):
     x, q, h = golden(case, "x"), golden(case, "q"), golden(case, "h")
     out = flash_sd_kde.kde_eval_linearized(x, q, h, device=gpu, precision_mode=IEEE)
     check_vs_ref(f"{case}_fused", cpu(out), golden(case, "density"),
-                 fp64.kde_density(q, x, h, laplacian=True), tol=5e-5, slack=5e-6)
+                 fp64.kde_density(q, x, h, laplacian=True))
     out2 = flash_sd_kde.kde_eval_linearized_nonfused(x, q, h, device=gpu, precision_mode=IEEE,
                                                      chunk_size=64)
     np.testing.assert_allclose(cpu(out2), golden(case, "density"), rtol=2e-3, atol=2e-3)
@@ -160,7 +171,7 @@ def test_kde1d_matches_reference(gpu, golden, backend):
     x, q, h = golden("kde1d_lin", "x"), golden("kde1d_lin", "q"), golden("kde1d_lin", "h")
     out = cpu(flash_sd_kde.kde_eval_linearized(x, q, h, device=gpu, kde_backend=backend))
     check_vs_ref(f"kde1d_lin_{backend}", out, golden("kde1d_lin", "density"),
-                 fp64.kde_density(q, x, h, laplacian=True), tol=5e-5, slack=5e-6)
+                 fp64.kde_density(q, x, h, laplacian=True))
     out = cpu(flash_sd_kde.kde_eval_linearized_nonfused(x, q, h, device=gpu, chunk_size=64))
     np.testing.assert_allclose(out, golden("kde1d_lin", "density"), rtol=2e-3, atol=2e-3)
 
@@ -200,12 +211,18 @@ def test_score_matches_reference(gpu, golden, case, variant):
         assert w.shape == (x.shape[0], 16) and w.stride() == (1, x.shape[0])
     pe, we = score_exact(x, h)
     check_vs_ref(f"{case}_{variant}_pdf", cpu(pdf), golden(case, "pdf"), pe)
-    wg = cpu(w)
-    err_w = float(np.max(np.abs(wg - we)) / np.max(np.abs(we)))
-    err_w_ref = float(np.max(np.abs(golden(case, "weighted") - we)) / np.max(np.abs(we)))
-    record(f"{case}_{variant}_w", err_vs_fp64=err_w, reference_err_vs_fp64=err_w_ref)
-    np.testing.assert_allclose(wg, golden(case, "weighted"), rtol=1e-4, atol=2e-5)
-    assert err_w <= 2.0 * err_w_ref + 1e-6
+    # weighted sums: errors relative to the largest |w| of the row (a row's
+    # features can sum to near zero, so elementwise relative errors would
+    # measure cancellation, not the kernel)
+    wg, wr = cpu(w).astype(np.float64), golden(case, "weighted").astype(np.float64)
+    row = np.maximum(np.abs(we).max(1, keepdims=True), 1e-300)
+    err_w = float((np.abs(wg - we) / row).max())
+    err_w_ref = float((np.abs(wr - we) / row).max())
+    diff_w = float((np.abs(wg - wr) / row).max())
+    record(f"{case}_{variant}_w", rel_vs_fp64=err_w, reference_rel_vs_fp64=err_w_ref,
+           rel_vs_reference=diff_w)
+    assert err_w <= err_w_ref + 1e-6, (err_w, err_w_ref)
+    assert diff_w <= 1e-5, diff_w
 
 
 def test_score_ordered_is_deterministic(gpu):
@@ -337,7 +354,7 @@ def test_fit_transform_matches_reference(gpu, golden, backend):
     e_ref = float(np.max(np.abs(golden("transform", "debiased") - exact)))
     record(f"transform_{backend}", abs_err_vs_fp64=e_gpu, reference_abs_err_vs_fp64=e_ref)
     np.testing.assert_allclose(deb, golden("transform", "debiased"), rtol=1e-5, atol=1e-5)
-    assert e_gpu <= 2.0 * e_ref + 1e-6
+    assert e_gpu <= e_ref + 1e-6, (e_gpu, e_ref)
 
 
 def test_sdkde_pipeline_gmm_small(gpu, golden):
@@ -352,10 +369,10 @@ def test_sdkde_pipeline_gmm_small(gpu, golden):
     e_gpu = float(np.max(np.abs(cpu(deb) - exact_deb)))
     e_ref = float(np.max(np.abs(golden(c, "debiased") - exact_deb)))
     record("gmm_small_debias", abs_err_vs_fp64=e_gpu, reference_abs_err_vs_fp64=e_ref)
-    assert e_gpu <= 2.0 * e_ref + 1e-6
+    assert e_gpu <= e_ref + 1e-6, (e_gpu, e_ref)
     dens = cpu(flash_sd_kde.kde_eval(deb, q, h, device=gpu, precision_mode=IEEE))
     check_vs_ref("gmm_small_density", dens, golden(c, "density"),
-                 fp64.kde_density(q, exact_deb.astype(np.float32), h), tol=5e-5, slack=5e-6)
+                 fp64.kde_density(q, exact_deb.astype(np.float32), h))
     deb2, h2 = empirical_sd_kde_triton_nd(x, h, device=gpu)
     np.testing.assert_allclose(deb2, golden(c, "debiased"), rtol=1e-5, atol=1e-5)
 
