@@ -13,6 +13,8 @@
 // term is four chained v_mfma_f32_16x16x4_f32 against -2 q (an exact
 // scaling), and fma / exp2 / add finish the pair on the VALU. Padded tail
 // points carry a norm term P with c P = -1e30, so they contribute exactly 0.
+#include <mutex>
+
 #include "fsk_common.h"
 #include "fsk_lowp.h"
 #include "fsk_plan.h"
@@ -193,9 +195,69 @@ static int kde16_run(const char *what, bool atomic, const float *data, int64_t l
   return fsk_reduce_partials_f64(L.partials, p.splits, m, m, out, stream);
 }
 
+// ---- allocation-free variant of the split-K KDE ----
+// Its workspace is a file-scope device array instead of a caller buffer, for
+// callers that cannot allocate (code under graph capture, serving loops).
+// Every device the library runs on holds its own copy until the process ends.
+// The host never uses the array's name as an address: the per-device address
+// comes from hipGetSymbolAddress on the first call on that device.
+constexpr size_t kNoAllocScratchBytes = size_t(FSK_NOALLOC_SCRATCH_BYTES);
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[kMaxDevices];
+
+// Device address of `scratch` on the current device (looked up once per device).
+static int scratch_for_current_device(const char *what, void **out) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+    set_error("%s: no usable current device (%s, id %d)", what, hipGetErrorString(e), dev);
+    return FSK_ELAUNCH;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      set_error("%s: hipGetSymbolAddress(scratch) failed on device %d: %s", what, dev,
+                hipGetErrorString(e));
+      return FSK_ELAUNCH;
+    }
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return FSK_OK;
+}
+
 }  // namespace fsk
 
 using namespace fsk;
+
+extern "C" int fsk_kde16_splitk_noalloc(const float *data, int64_t ld, int64_t n,
+                                        const float *query, int64_t ldq, int64_t m,
+                                        double bandwidth, unsigned flags, float *out,
+                                        void *stream) {
+  const char *what = "fsk_kde16_splitk_noalloc";
+  int st;
+  if ((st = check_points16(what, data, ld, n)) != FSK_OK) return st;
+  if ((st = check_points16(what, query, ldq, m)) != FSK_OK) return st;
+  const size_t need = kde16_ws_bytes(n, m, true);
+  if (need > kNoAllocScratchBytes) {
+    set_error("%s: n = %lld points and m = %lld queries need %zu bytes of scratch, above the "
+              "%zu-byte static buffer; use fsk_kde16_splitk with a workspace",
+              what, (long long)n, (long long)m, need, kNoAllocScratchBytes);
+    return FSK_EUNSUPPORTED;
+  }
+  void *ws = nullptr;
+  if ((st = scratch_for_current_device(what, &ws)) != FSK_OK) return st;
+  // every byte the kernels read from the scratch is written earlier in the
+  // same call (mean, centred copies, norms, all split partials): nothing
+  // needs clearing between calls
+  return kde16_run(what, false, data, ld, n, query, ldq, m, bandwidth, flags, ws,
+                   kNoAllocScratchBytes, out, stream);
+}
 
 extern "C" int fsk_kde16_splitk(const float *data, int64_t ld, int64_t n, const float *query,
                                 int64_t ldq, int64_t m, double bandwidth, unsigned flags,

@@ -62,6 +62,7 @@ SIGNATURES = {
     "fsk_microbench": [ctypes.c_int, ctypes.c_int, ctypes.c_int, _p, _p, _p],
     "fsk_kde16_splitk": [_p, _i64, _i64, _p, _i64, _i64, _d, _u, _p, _sz, _p, _p],
     "fsk_kde16_atomic": [_p, _i64, _i64, _p, _i64, _i64, _d, _u, _p, _sz, _p, _p],
+    "fsk_kde16_splitk_noalloc": [_p, _i64, _i64, _p, _i64, _i64, _d, _u, _p, _p],
     "fsk_kde16_logdensity": [_p, _i64, _i64, _p, _i64, _i64, _d, _u, _p, _sz, _p, _p],
     "fsk_kde1_splitk": [_p, _i64, _p, _i64, _d, _u, _p, _sz, _p, _p],
     "fsk_kde1_atomic": [_p, _i64, _p, _i64, _d, _u, _p, _sz, _p, _p],

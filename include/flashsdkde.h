@@ -119,6 +119,26 @@ int fsk_kde16_atomic(const float *data, int64_t ld, int64_t n,
                      double bandwidth, unsigned flags,
                      void *ws, size_t ws_bytes, float *out, void *stream);
 
+/* Size of the static device scratch behind fsk_kde16_splitk_noalloc (per GPU). */
+#define FSK_NOALLOC_SCRATCH_BYTES (32u << 20)
+
+/* fsk_kde16_splitk without a workspace argument: the same kernels and results
+ * (bitwise), with their scratch in a file-scope device array of
+ * FSK_NOALLOC_SCRATCH_BYTES that each GPU holds until the process ends. The
+ * call allocates nothing (no hipMalloc / hipFree), for callers that cannot
+ * allocate: code under HIP graph capture, tight serving loops, harnesses
+ * that forbid allocation. No counterpart in the reference.
+ *  - Sizes whose workspace (fsk_workspace_bytes(FSK_OP_KDE16_SPLITK, n, m, 0))
+ *    exceeds FSK_NOALLOC_SCRATCH_BYTES return FSK_EUNSUPPORTED (e.g. 200 000
+ *    points with 65 536 queries fit; 2^20 points do not).
+ *  - The scratch of the current device is used (the stream must belong to it);
+ *    its address is looked up on the first call on each device.
+ *  - All calls on one GPU share that array: calls that overlap in time on one
+ *    GPU (other streams, other host threads) race. Serialise them. */
+int fsk_kde16_splitk_noalloc(const float *data, int64_t ld, int64_t n,
+                             const float *query, int64_t ldq, int64_t m,
+                             double bandwidth, unsigned flags, float *out, void *stream);
+
 /* 16-D log-density by log-sum-exp (precision form):
  * out[i] = log( 1/(n (2 pi)^8 h^16) * sum_j exp(-|q_i - x_j|^2/(2h^2)) ),
  * computed as per-split (max, sum) pairs in base 2 and an FP64 combine.
