@@ -12,6 +12,11 @@
 // neighbour contributes 1, and no bandwidth of the grid can underflow.
 // One distance feeds all n_bw (<= 32) exponentials; the kernel is exp-bound,
 // so the cross term stays FP32 MFMA (it is < 10 % of the issue slots).
+// Each dataset is first centred on its own FP64 mean (distances are
+// translation invariant; the norm terms, whose cancellation sets the FP32
+// error of |x_i|^2 + |x_j|^2 - 2 x_i.x_j, shrink to the dataset's spread), and
+// the MFMA accumulator starts from |x_j|^2 alone, |x_i|^2 being added after
+// the cross term.
 // Tasks come from an atomic work queue, largest datasets first; per task the
 // row sums of log S_ib and of dmin_i are reduced in LDS and added to FP64
 // per-(dataset, bandwidth) accumulators; a finish kernel forms LL.
@@ -33,17 +38,58 @@ struct LooMeta {
   int order[kLooMaxSets];            // processing order (largest first)
 };
 
-__global__ void loo_norms_kernel(const float *__restrict__ x, int64_t ld, int64_t N,
-                                 float *__restrict__ nrm) {
-  const int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (j >= N) return;
-  double s = 0.0;
+// One workgroup per dataset: its FP64 mean (fixed summation order: per
+// thread over a strided row set, then a fixed LDS tree), then the centred
+// copy xc (16 floats per row, one rounding of the FP64 difference) and the
+// squared norms of the centred rows.
+__global__ __launch_bounds__(256) void loo_center_kernel(const float *__restrict__ x, int64_t ld,
+                                                         LooMeta meta, float *__restrict__ xc,
+                                                         float *__restrict__ nrm) {
+  __shared__ double red[256][kD + 1];
+  __shared__ double mu[kD];
+  const int d = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int64_t beg = meta.off[d], end = meta.off[d + 1];
+  double acc[kD];
 #pragma unroll
-  for (int f = 0; f < kD; ++f) {
-    const double v = x[j * ld + f];
-    s += v * v;
+  for (int f = 0; f < kD; ++f) acc[f] = 0.0;
+  for (int64_t j = beg + tid; j < end; j += 256) {
+#pragma unroll
+    for (int f = 0; f < kD; f += 4) {
+      const f4 v = ld4(x + j * ld + f);
+      acc[f] += v[0];
+      acc[f + 1] += v[1];
+      acc[f + 2] += v[2];
+      acc[f + 3] += v[3];
+    }
   }
-  nrm[j] = float(s);
+#pragma unroll
+  for (int f = 0; f < kD; ++f) red[tid][f] = acc[f];
+  __syncthreads();
+  for (int w = 128; w >= 1; w >>= 1) {
+    if (tid < w) {
+#pragma unroll
+      for (int f = 0; f < kD; ++f) red[tid][f] += red[tid + w][f];
+    }
+    __syncthreads();
+  }
+  if (tid < kD) mu[tid] = red[0][tid] / double(end - beg);
+  __syncthreads();
+  for (int64_t j = beg + tid; j < end; j += 256) {
+    double s = 0.0;
+#pragma unroll
+    for (int f = 0; f < kD; f += 4) {
+      const f4 v = ld4(x + j * ld + f);
+      f4 o;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        o[e] = float(double(v[e]) - mu[f + e]);
+        s += double(o[e]) * double(o[e]);
+      }
+      *reinterpret_cast<f4 *>(xc + j * kD + f) = o;
+    }
+    nrm[j] = float(s);
+  }
 }
 
 template <int WAVES>
@@ -102,9 +148,10 @@ __global__ __launch_bounds__(64 * WAVES) void loo_grid_kernel(
         const int64_t jb = j0 + 4 * g + r;
         bb[r] = jb < end ? nrm[jb] : kLooFar;
       }
-      f4 dd = {qa + bb[0], qa + bb[1], qa + bb[2], qa + bb[3]};
+      f4 dd = bb;
 #pragma unroll
       for (int s = 0; s < 4; ++s) dd = mfma_f32(xa[s], qv[s], dd);
+      dd += qa;
       if (j0 == rbase) {  // diagonal tile: drop j == i
 #pragma unroll
         for (int r = 0; r < 4; ++r)
@@ -137,9 +184,10 @@ __global__ __launch_bounds__(64 * WAVES) void loo_grid_kernel(
         const int64_t jb = j0 + 4 * g + r;
         bb[r] = jb < end ? nrm[jb] : kLooFar;
       }
-      f4 dd = {qad + bb[0], qad + bb[1], qad + bb[2], qad + bb[3]};
+      f4 dd = bb;
 #pragma unroll
       for (int s = 0; s < 4; ++s) dd = mfma_f32(xa[s], qv[s], dd);
+      dd += qad;
       if (j0 == rbase) {
 #pragma unroll
         for (int r = 0; r < 4; ++r)
@@ -212,6 +260,7 @@ __global__ void loo_pack_bw_kernel(const double *__restrict__ bw_in, int n_sets,
 
 struct LooLayout {
   int *queue;
+  float *xc;  // rows centred on their dataset's mean
   float *nrm;
   double *bw, *sumlog, *sumdmin;
 };
@@ -219,6 +268,7 @@ struct LooLayout {
 static size_t loo_layout(void *ws, int64_t N, int n_sets, LooLayout *L) {
   Carve cv(ws);
   L->queue = cv.take<int>(64);
+  L->xc = cv.take<float>(N * kD);
   L->nrm = cv.take<float>(N);
   L->bw = cv.take<double>(int64_t(n_sets) * kLooMaxBw);
   L->sumlog = cv.take<double>(int64_t(n_sets) * kLooMaxBw);
@@ -309,7 +359,8 @@ extern "C" int fsk_loo_bandwidth_grid(const float *data, int64_t ld, const int64
   hipLaunchKernelGGL(fill_f64, dim3(1), dim3(64), 0, s, L.sumdmin, int64_t(n_sets), 0.0);
   hipLaunchKernelGGL(loo_pack_bw_kernel, dim3(cdiv(int64_t(n_sets) * kLooMaxBw, 256)), dim3(256),
                      0, s, bandwidths, n_sets, n_bw, L.bw);
-  hipLaunchKernelGGL(loo_norms_kernel, dim3(cdiv(N, 256)), dim3(256), 0, s, data, ld, N, L.nrm);
+  hipLaunchKernelGGL(loo_center_kernel, dim3(n_sets), dim3(256), 0, s, data, ld, meta, L.xc,
+                     L.nrm);
   static int cus_cached = 0;  // compute units of the current device (queried once)
   if (cus_cached == 0) {
     int dev = 0, cus = 0;
@@ -325,10 +376,10 @@ extern "C" int fsk_loo_bandwidth_grid(const float *data, int64_t ld, const int64
   if (grid > meta.ntasks) grid = meta.ntasks;
   prof_begin(FSK_OP_LOO_GRID, s);
   if (waves_per_wg == 4)
-    hipLaunchKernelGGL(loo_grid_kernel<4>, dim3(grid), dim3(256), 0, s, data, ld, L.nrm, L.bw, meta,
+    hipLaunchKernelGGL(loo_grid_kernel<4>, dim3(grid), dim3(256), 0, s, L.xc, int64_t(kD), L.nrm, L.bw, meta,
                        L.queue, L.sumlog, L.sumdmin);
   else
-    hipLaunchKernelGGL(loo_grid_kernel<8>, dim3(grid), dim3(512), 0, s, data, ld, L.nrm, L.bw, meta,
+    hipLaunchKernelGGL(loo_grid_kernel<8>, dim3(grid), dim3(512), 0, s, L.xc, int64_t(kD), L.nrm, L.bw, meta,
                        L.queue, L.sumlog, L.sumdmin);
   prof_end(FSK_OP_LOO_GRID, s);
   if ((st = launch_status(what)) != FSK_OK) return st;

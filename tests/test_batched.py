@@ -90,3 +90,18 @@ def test_batched_agrees_with_score_kernel_identity(gpu):
         log_norm = -np.log(n - 1) - 16 * np.log(h) - 8 * np.log(2 * np.pi)
         ref = float(np.mean(np.log(s) + log_norm))
         assert abs(got - ref) <= 1e-5 * abs(ref), (f, got, ref)
+
+
+@pytest.mark.gpu
+def test_batched_offset_datasets_match_fp64(gpu):
+    """Datasets far from the origin (every coordinate shifted by +50, |x|^2 ~
+    4e4): each dataset is centred on its own mean before the FP32 cross term,
+    so the result matches FP64 as for data near 0, over the whole grid down
+    to 0.05 h where the exponent coefficient is 400 times that at h."""
+    sets = [x + np.float32(50.0) for x in _sets([3, 40, 333, 1000], seed=3)]
+    sets[1] -= np.float32(120.0)  # another dataset on the other side of the origin
+    factors = np.geomspace(0.05, 2.0, 32)
+    ll = loo_loglik_grid(sets, factors=factors, device=gpu)
+    for k, x in enumerate(sets):
+        ref = fp64.loo_loglik(x, scott_mean_std_bandwidth(x) * factors)
+        np.testing.assert_allclose(ll[k], ref, rtol=1e-5)

@@ -65,19 +65,42 @@ def test_mlp_forward_and_backward_match_oracle(gpu, M, splits):
 
 
 def test_mlp_adam_matches_oracle(gpu):
-    net = ScoreMLP(seed=4, device=gpu)
-    W0 = net.layers[0]["W"].cpu().double()
-    grads = [(torch.randn(2, L["N"], L["K"], device=gpu), torch.randn(2, L["N"], device=gpu))
-             for L in net.layers]
-    net.adam_step(grads, 1e-2)
-    g0 = grads[0][0].sum(0).cpu().double()
-    w_ref, _, _ = mlp.adam(W0, torch.zeros_like(W0), torch.zeros_like(W0), g0, 1, 1e-2)
-    assert rel(net.layers[0]["W"].cpu(), w_ref) < 1e-6
-    # BF16 copies follow the master weights (and the transposed copy its transpose)
-    Wb = net.layers[0]["Wb"].cpu().view(torch.bfloat16).double()
-    Wt = net.layers[0]["Wt"].cpu().view(torch.bfloat16).double()
-    assert torch.equal(Wb, mlp.bf16(net.layers[0]["W"].cpu().double()))
-    assert torch.equal(Wt, Wb.T)
+    """Three Adam steps of every tensor against the oracle (FP64, pinned to
+    torch.optim.Adam on the CPU), once with the step count passed from the
+    host and once with the device schedule [lr, 1 - b1^t, 1 - b2^t] the
+    captured training step reads; the learning rate changes every step.
+    Steps 2 and 3 exercise the bias corrections beyond their step-1 values."""
+    g = torch.Generator(device=gpu).manual_seed(5)
+    steps = [[(torch.randn(2, L["N"], L["K"], device=gpu, generator=g),
+               torch.randn(2, L["N"], device=gpu, generator=g))
+              for L in ScoreMLP(seed=4, device=gpu).layers] for _ in range(3)]
+    lrs = [1e-2, 7e-3, 4e-3]
+    for mode in ("host_step", "device_schedule"):
+        net = ScoreMLP(seed=4, device=gpu)
+        ref = [[L["W"].cpu().double(), L["b"].cpu().double()] for L in net.layers]
+        mom = [[torch.zeros_like(t) for t in pair] for pair in ref]
+        vel = [[torch.zeros_like(t) for t in pair] for pair in ref]
+        sched = torch.empty(3, dtype=torch.float32, device=gpu)
+        for k, (grads, lr) in enumerate(zip(steps, lrs), start=1):
+            if mode == "host_step":
+                net.adam_step(grads, lr)
+            else:
+                sched.copy_(torch.tensor([lr, 1 - 0.9 ** k, 1 - 0.999 ** k]))
+                net.adam_step(grads, 123.0, sched=sched)  # host lr ignored
+            for l, (dW, db) in enumerate(grads):
+                for j, part in enumerate((dW, db)):
+                    ref[l][j], mom[l][j], vel[l][j] = mlp.adam(
+                        ref[l][j], mom[l][j], vel[l][j], part.sum(0).cpu().double(), k, lr)
+            errs = [rel(L["W"].cpu(), r[0]) for L, r in zip(net.layers, ref)]
+            errs += [rel(L["b"].cpu(), r[1]) for L, r in zip(net.layers, ref)]
+            record(f"mlp_adam_{mode}_step{k}", max_rel=max(errs))
+            assert max(errs) < 2e-6, (mode, k, errs)
+        # BF16 copies follow the master weights (and the transposed copy its transpose)
+        for L in net.layers:
+            Wb = L["Wb"].cpu().view(torch.bfloat16).double()
+            Wt = L["Wt"].cpu().view(torch.bfloat16).double()
+            assert torch.equal(Wb, mlp.bf16(L["W"].cpu().double()))
+            assert torch.equal(Wt, Wb.T)
 
 
 def test_adam_multi_equals_per_tensor_updates(gpu):
