@@ -283,8 +283,9 @@ __global__ __launch_bounds__(256) void mlp_wgrad_kernel(const float *__restrict_
 
 __global__ void mlp_adam_kernel(float *__restrict__ w, float *__restrict__ mom,
                                 float *__restrict__ vel, const float *__restrict__ part, int S,
-                                int64_t count, int cols, float lr, float b1, float b2, float eps,
-                                float bc1, float bc2, unsigned short *__restrict__ wb,
+                                int64_t count, int cols, float lr, float b1, float omb1, float b2,
+                                float omb2, float eps, float bc1, float bc2,
+                                unsigned short *__restrict__ wb,
                                 unsigned short *__restrict__ wtb, int rows,
                                 const float *__restrict__ sched) {
   const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -297,8 +298,8 @@ __global__ void mlp_adam_kernel(float *__restrict__ w, float *__restrict__ mom,
   float gsum = 0.f;
 #pragma unroll 8
   for (int s = 0; s < S; ++s) gsum += part[int64_t(s) * count + e];
-  const float m = b1 * mom[e] + (1.f - b1) * gsum;
-  const float v = b2 * vel[e] + (1.f - b2) * gsum * gsum;
+  const float m = b1 * mom[e] + omb1 * gsum;
+  const float v = b2 * vel[e] + omb2 * gsum * gsum;
   mom[e] = m;
   vel[e] = v;
   const float upd = lr * (m / bc1) / (sqrtf(v / bc2) + eps);
@@ -319,8 +320,11 @@ struct AdamBatch {
   int n;
 };
 
-__global__ void mlp_adam_multi_kernel(AdamBatch batch, float lr, float b1, float b2, float eps,
-                                      float bc1, float bc2, const float *__restrict__ sched) {
+// omb1 / omb2 = 1 - beta1 / 1 - beta2, formed in double on the host: FP32
+// 1 - float(0.999) would be 1.3e-5 off 0.001 in relative terms.
+__global__ void mlp_adam_multi_kernel(AdamBatch batch, float lr, float b1, float omb1, float b2,
+                                      float omb2, float eps, float bc1, float bc2,
+                                      const float *__restrict__ sched) {
   int k = 0;
   while (k + 1 < batch.n && int64_t(blockIdx.x) >= batch.block_start[k + 1]) ++k;
   const fsk_adam_tensor &T = batch.t[k];
@@ -334,8 +338,8 @@ __global__ void mlp_adam_multi_kernel(AdamBatch batch, float lr, float b1, float
   float gsum = 0.f;
 #pragma unroll 8
   for (int s = 0; s < T.splits; ++s) gsum += T.grad_part[int64_t(s) * T.count + e];
-  const float m = b1 * T.m[e] + (1.f - b1) * gsum;
-  const float v = b2 * T.v[e] + (1.f - b2) * gsum * gsum;
+  const float m = b1 * T.m[e] + omb1 * gsum;
+  const float v = b2 * T.v[e] + omb2 * gsum * gsum;
   T.m[e] = m;
   T.v[e] = v;
   const float nw = T.w[e] - lr * (m / bc1) / (sqrtf(v / bc2) + eps);
@@ -481,8 +485,9 @@ extern "C" int fsk_mlp_adam(float *w, float *m, float *v, const float *grad_part
   const float bc1 = step > 0 ? float(1.0 - pow(beta1, step)) : 1.f;
   const float bc2 = step > 0 ? float(1.0 - pow(beta2, step)) : 1.f;
   hipLaunchKernelGGL(mlp_adam_kernel, dim3(cdiv(count, 256)), dim3(256), 0, as_stream(stream), w, m,
-                     v, grad_part, splits, count, cols, float(lr), float(beta1), float(beta2),
-                     float(eps), bc1, bc2, w_bf16, wt_bf16, rows, sched);
+                     v, grad_part, splits, count, cols, float(lr), float(beta1),
+                     float(1.0 - beta1), float(beta2), float(1.0 - beta2), float(eps), bc1, bc2,
+                     w_bf16, wt_bf16, rows, sched);
   return launch_status("fsk_mlp_adam");
 }
 
@@ -510,8 +515,8 @@ extern "C" int fsk_mlp_adam_multi(const fsk_adam_tensor *tensors, int n_tensors,
   const float bc1 = step > 0 ? float(1.0 - pow(beta1, step)) : 1.f;
   const float bc2 = step > 0 ? float(1.0 - pow(beta2, step)) : 1.f;
   hipLaunchKernelGGL(mlp_adam_multi_kernel, dim3(unsigned(b.block_start[n_tensors])), dim3(256), 0,
-                     as_stream(stream), b, float(lr), float(beta1), float(beta2), float(eps), bc1,
-                     bc2, sched);
+                     as_stream(stream), b, float(lr), float(beta1), float(1.0 - beta1),
+                     float(beta2), float(1.0 - beta2), float(eps), bc1, bc2, sched);
   return launch_status(what);
 }
 

@@ -1,13 +1,19 @@
-"""Validators, backend selection and heuristics (reference flash_sd_kde/config.py).
+"""Option names, their validation, and the empirical-score backend choice.
 
-`select_emp_score_backend` keeps the reference's rule (:70-75): under the
-default heuristics it always returns the "flash_sd_kde" backend.
+Interface of the reference's flash_sd_kde/config.py (:46-75): the validators
+raise ValueError naming the option, the allowed values and the value given;
+KernelHeuristics carries the same fields, and select_emp_score_backend makes
+the same choice (the flash backend whenever it is the configured default,
+else the symmetric kernel up to n_sym_max points and the ordered kernel
+above). The reference's Triton tile records and split-count helpers have no
+counterpart: the HIP library plans its own tiles and splits from (n, m)
+(flash-sd-kde_amd/csrc/fsk_plan.h), which is what keeps the split-K paths
+bitwise reproducible.
 """
 
 from __future__ import annotations
 
 from dataclasses import dataclass
-from typing import Iterable
 
 from globals import (
     DEFAULT_EMP_SCORE_BACKEND,
@@ -19,27 +25,51 @@ from globals import (
     EMP_SCORE_BACKEND_SYMMETRIC_ATOMIC,
     KDE_BACKEND_ATOMIC,
     KDE_BACKEND_SPLITK_STREAM,
+    PRECISION_FAST_TF32,
+    PRECISION_FP32_IEEE,
 )
 
+# Accepted values of each string option. Both precision modes run the FP32
+# MFMA cross term on gfx950 (it has no TF32 MFMA); the names stay so that
+# callers passing either keep working.
+CHOICES: dict[str, frozenset[str]] = {
+    "precision_mode": frozenset({PRECISION_FAST_TF32, PRECISION_FP32_IEEE}),
+    "kde_backend": frozenset({KDE_BACKEND_SPLITK_STREAM, KDE_BACKEND_ATOMIC}),
+    "emp_score_backend": frozenset({EMP_SCORE_BACKEND_ORDERED_SPLITK,
+                                    EMP_SCORE_BACKEND_SYMMETRIC_ATOMIC,
+                                    EMP_SCORE_BACKEND_FLASH_SD_KDE}),
+}
 
-@dataclass(frozen=True)
-class SplitKParams:
-    block_m: int
-    block_n_iter: int
-    iters_per_split: int
-    num_warps: int
-    num_stages: int
+
+def check_choice(option: str, value: str) -> None:
+    """Raise ValueError unless `value` is one of the accepted values of `option`."""
+    allowed = CHOICES[option]
+    if value not in allowed:
+        raise ValueError(f"{option} must be one of {sorted(allowed)}, got {value}.")
 
 
-@dataclass(frozen=True)
-class SymmetricParams:
-    block_size: int
-    num_warps: int
-    num_stages: int
+def validate_precision_mode(precision_mode: str) -> None:
+    check_choice("precision_mode", precision_mode)
+
+
+def validate_kde_backend(kde_backend: str) -> None:
+    check_choice("kde_backend", kde_backend)
+
+
+def validate_emp_score_backend(emp_score_backend: str) -> None:
+    check_choice("emp_score_backend", emp_score_backend)
 
 
 @dataclass(frozen=True)
 class KernelHeuristics:
+    """Defaults a caller may override and pass to the selection rule.
+
+    n_sym_max: largest n for which the symmetric kernel is chosen when the
+        configured default is not the flash backend.
+    max_partial_weighted_bytes: accepted for callers that set it; the
+        library caps its own split-K partial workspace (2 GiB).
+    """
+
     n_sym_max: int = 50_000
     max_partial_weighted_bytes: int = 256 * 1024 * 1024
     eps: float = DEFAULT_EPS
@@ -51,55 +81,10 @@ class KernelHeuristics:
 DEFAULT_HEURISTICS = KernelHeuristics()
 
 
-def validate_precision_mode(precision_mode: str) -> None:
-    valid = {"fast_tf32", "fp32_ieee"}
-    if precision_mode not in valid:
-        raise ValueError(f"precision_mode must be one of {sorted(valid)}, got {precision_mode}.")
-
-
-def validate_kde_backend(kde_backend: str) -> None:
-    valid = {KDE_BACKEND_SPLITK_STREAM, KDE_BACKEND_ATOMIC}
-    if kde_backend not in valid:
-        raise ValueError(f"kde_backend must be one of {sorted(valid)}, got {kde_backend}.")
-
-
-def validate_emp_score_backend(emp_score_backend: str) -> None:
-    valid = {
-        EMP_SCORE_BACKEND_ORDERED_SPLITK,
-        EMP_SCORE_BACKEND_SYMMETRIC_ATOMIC,
-        EMP_SCORE_BACKEND_FLASH_SD_KDE,
-    }
-    if emp_score_backend not in valid:
-        raise ValueError(
-            f"emp_score_backend must be one of {sorted(valid)}, got {emp_score_backend}."
-        )
-
-
 def select_emp_score_backend(n: int, *, heuristics: KernelHeuristics = DEFAULT_HEURISTICS) -> str:
-    if heuristics.default_emp_score_backend == EMP_SCORE_BACKEND_FLASH_SD_KDE:
-        return EMP_SCORE_BACKEND_FLASH_SD_KDE
-    if n <= heuristics.n_sym_max:
-        return EMP_SCORE_BACKEND_SYMMETRIC_ATOMIC
-    return EMP_SCORE_BACKEND_ORDERED_SPLITK
-
-
-def compute_splitk_splits(n_data: int, *, block_n_iter: int, iters_per_split: int) -> int:
-    block_n_total = block_n_iter * iters_per_split
-    if block_n_total <= 0:
-        raise ValueError("block_n_total must be positive.")
-    return int((n_data + block_n_total - 1) // block_n_total)
-
-
-def cap_splits_for_memory(n_query: int, n_splits: int, *, features: int,
-                          heuristics: KernelHeuristics = DEFAULT_HEURISTICS) -> int:
-    if n_splits <= 1:
-        return n_splits
-    bytes_per_split = n_query * features * 4
-    if bytes_per_split <= 0:
-        return n_splits
-    max_splits = max(1, heuristics.max_partial_weighted_bytes // bytes_per_split)
-    return int(min(n_splits, max_splits))
-
-
-def iter_autotune_candidates(candidates: Iterable[SplitKParams]) -> list[SplitKParams]:
-    return list(candidates)
+    """Backend used when the caller names none."""
+    configured = heuristics.default_emp_score_backend
+    if configured == EMP_SCORE_BACKEND_FLASH_SD_KDE:
+        return configured
+    by_size = {True: EMP_SCORE_BACKEND_SYMMETRIC_ATOMIC, False: EMP_SCORE_BACKEND_ORDERED_SPLITK}
+    return by_size[n <= heuristics.n_sym_max]

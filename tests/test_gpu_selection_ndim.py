@@ -74,3 +74,26 @@ def test_sd_kde_nd_by_padding_matches_fp64(gpu, d):
     pdf, w = fp64.score_sums(x, h)
     ref, _ = fp64.debias(x, pdf, w, h)
     assert np.max(np.abs(deb - ref)) < 2e-5 * max(1.0, float(np.abs(ref).max()))
+
+
+@pytest.mark.parametrize("mode", ["kde", "sd_kde"])
+def test_estimator_other_dimensions(gpu, mode):
+    """FlashSDKDE with 5 features (the reference accepts only 16 for SD-KDE
+    and 1 or 16 for KDE): log(density + eps) of the padded kernels against the
+    FP64 pipeline; scoring twice reuses the device copy of the evaluation set."""
+    from flash_sd_kde import FlashSDKDE
+
+    rng = np.random.default_rng(31)
+    x = rng.standard_normal((2000, 5)).astype(np.float32)
+    q = rng.standard_normal((300, 5)).astype(np.float32)
+    est = FlashSDKDE(mode=mode, bandwidth=0.6, device=gpu).fit(x)
+    ls = est.score_samples(q)
+    assert est.n_features_in_ == 5 and est.X_fit_.shape == (2000, 5)
+    data = x
+    if mode == "sd_kde":
+        pdf, w = fp64.score_sums(x, 0.6)
+        data, _ = fp64.debias(x, pdf, w, 0.6)
+        data = data.astype(np.float32)
+    ref = np.log(fp64.kde_density(q, data, 0.6) + 1e-12)
+    assert np.max(np.abs(ls - ref)) < 5e-5
+    np.testing.assert_array_equal(est.score_samples(q), ls)

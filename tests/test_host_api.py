@@ -70,8 +70,10 @@ def test_estimator_errors():
         est.fit(np.zeros((0, 16)))
     with pytest.raises(ValueError, match="mode"):
         flash_sd_kde.FlashSDKDE(mode="bogus", device="cpu").fit(np.zeros((3, 16)))
-    with pytest.raises(ValueError, match="16 features"):
-        flash_sd_kde.FlashSDKDE(mode="sd_kde").fit(np.zeros((3, 4)))
+    est = flash_sd_kde.FlashSDKDE(mode="kde", bandwidth=-1.0)
+    with pytest.raises(ValueError, match="bandwidth"):
+        est.fit(np.zeros((3, 16)))
+    assert est.bandwidth_ is None and est.X_fit_ is None  # a failed fit sets nothing
     with pytest.raises(ValueError, match="bandwidth"):
         flash_sd_kde.FlashSDKDE(bandwidth=-1.0).fit(np.zeros((3, 16)))
     with pytest.raises(ValueError, match="Unsupported bandwidth"):
