@@ -45,6 +45,9 @@ OP_KDE16_LOGDENSITY = 7
 OP_KDE1_SCORE = 8
 OP_LOO_GRID = 9
 OP_SCORE16_ROWS = 10
+OP_KDE_ND_SPLITK = 11
+OP_SCORE_ND_ORDERED = 12
+ND_MAX_DIM = 64
 
 _p = ctypes.c_void_p
 _i64 = ctypes.c_int64
@@ -91,6 +94,12 @@ SIGNATURES = {
                                ctypes.c_int, _p, _sz, _p, _p],
     "fsk_ref64_logdensity16": [_p, _i64, _p, _i64, _d, _p, _sz, _p, _p],
     "fsk_ref64_score16": [_p, _i64, _d, _p, _p, _p],
+    "fsk_nd_workspace_bytes": [ctypes.c_int, _i64, _i64, ctypes.c_int],
+    "fsk_kde_nd_splitk": [_p, _i64, _i64, _p, _i64, _i64, ctypes.c_int, _d, _u, _p, _sz, _p, _p],
+    "fsk_score_nd_ordered": [_p, _i64, _i64, ctypes.c_int, _d, _u, _p, _sz, _p, _p, _i64, _i64,
+                             _p],
+    "fsk_sdkde_debias_nd_f64": [_p, _i64, _i64, ctypes.c_int, _p, _p, _i64, _i64, _d, _d, _p,
+                                _i64, _p],
 }
 _RESTYPES = {
     "fsk_last_error": ctypes.c_char_p,
@@ -99,6 +108,7 @@ _RESTYPES = {
     "fsk_profile_last_ms": _d,
     "fsk_ref64_workspace_bytes": _sz,
     "fsk_loo_workspace_bytes": _sz,
+    "fsk_nd_workspace_bytes": _sz,
 }
 
 class AdamTensor(ctypes.Structure):

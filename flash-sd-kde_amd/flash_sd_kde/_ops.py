@@ -272,6 +272,73 @@ def debias(train: torch.Tensor, pdf: torch.Tensor, weighted: torch.Tensor, bandw
     return (out, score) if return_score else out
 
 
+# ---- any dimension d <= 64 (zero-padded 16-wide slices inside the library) ----
+
+def as_points_nd(array_like, device: torch.device) -> torch.Tensor:
+    """(rows, d) float32 tensor on `device` with unit feature stride."""
+    if isinstance(array_like, torch.Tensor):
+        t = array_like.to(device=device, dtype=torch.float32, copy=False)
+    else:
+        t = torch.as_tensor(array_like, dtype=torch.float32, device=device)
+    if t.ndim != 2 or t.shape[0] == 0 or not 1 <= t.shape[1] <= nat.ND_MAX_DIM:
+        raise ValueError(f"expected a non-empty (rows, d) array with 1 <= d <= {nat.ND_MAX_DIM}, "
+                         f"got shape {tuple(t.shape)}")
+    return t if t.stride(1) == 1 else t.contiguous()
+
+
+def kde_nd_sums(train: torch.Tensor, query: torch.Tensor, bandwidth: float, *,
+                laplacian: bool = False) -> torch.Tensor:
+    """sum_j exp(-|q_i - x_j|^2 / 2h^2) in d dimensions (unnormalised)."""
+    h = _check_h(bandwidth)
+    dev = train.device
+    nat.require_gpu(dev, "kde_nd_sums")
+    n, d = train.shape
+    m = query.shape[0]
+    if query.shape[1] != d:
+        raise ValueError(f"data has {d} features but queries have {query.shape[1]}.")
+    flags = nat.FLAG_LAPLACIAN if laplacian else 0
+    out = torch.empty(m, dtype=torch.float32, device=dev)
+    nbytes = int(nat.lib().fsk_nd_workspace_bytes(nat.OP_KDE_ND_SPLITK, n, m, d))
+    ws = torch.empty(max(nbytes, 256), dtype=torch.uint8, device=dev)
+    st = nat.lib().fsk_kde_nd_splitk(train.data_ptr(), train.stride(0), n, query.data_ptr(),
+                                     query.stride(0), m, d, h, flags, ws.data_ptr(), nbytes,
+                                     out.data_ptr(), nat.stream_handle(dev))
+    nat.check(st, "fsk_kde_nd_splitk")
+    return out
+
+
+def score_nd(train: torch.Tensor, bandwidth: float):
+    """(pdf (n,), weighted (n, d)) empirical-score accumulators in d dimensions."""
+    h = _check_h(bandwidth)
+    dev = train.device
+    nat.require_gpu(dev, "score_nd")
+    n, d = train.shape
+    pdf = torch.empty(n, dtype=torch.float32, device=dev)
+    w = torch.empty((n, d), dtype=torch.float32, device=dev)
+    nbytes = int(nat.lib().fsk_nd_workspace_bytes(nat.OP_SCORE_ND_ORDERED, n, n, d))
+    ws = torch.empty(max(nbytes, 256), dtype=torch.uint8, device=dev)
+    st = nat.lib().fsk_score_nd_ordered(train.data_ptr(), train.stride(0), n, d, h, 0,
+                                        ws.data_ptr(), nbytes, pdf.data_ptr(), w.data_ptr(), d, 1,
+                                        nat.stream_handle(dev))
+    nat.check(st, "fsk_score_nd_ordered")
+    return pdf, w
+
+
+def debias_nd(train: torch.Tensor, pdf: torch.Tensor, weighted: torch.Tensor, bandwidth: float,
+              eps: float) -> torch.Tensor:
+    """x + (h^2/2) ((w/(pdf+eps) - x)/h^2) in d dimensions, FP64 on the GPU."""
+    h = _check_h(bandwidth)
+    dev = train.device
+    n, d = train.shape
+    out = torch.empty((n, d), dtype=torch.float32, device=dev)
+    st = nat.lib().fsk_sdkde_debias_nd_f64(train.data_ptr(), train.stride(0), n, d,
+                                           pdf.contiguous().data_ptr(), weighted.data_ptr(),
+                                           weighted.stride(0), weighted.stride(1), h, float(eps),
+                                           out.data_ptr(), d, nat.stream_handle(dev))
+    nat.check(st, "fsk_sdkde_debias_nd_f64")
+    return out
+
+
 def reduce_partials(partials: torch.Tensor) -> torch.Tensor:
     if partials.ndim != 2:
         raise ValueError("partials must be 2D (n_splits, n_query).")

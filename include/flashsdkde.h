@@ -73,6 +73,8 @@ extern "C" {
 #define FSK_OP_KDE1_SCORE 8
 #define FSK_OP_LOO_GRID 9 /* profiling id only; workspace: fsk_loo_workspace_bytes */
 #define FSK_OP_SCORE16_ROWS 10
+#define FSK_OP_KDE_ND_SPLITK 11    /* workspace: fsk_nd_workspace_bytes */
+#define FSK_OP_SCORE_ND_ORDERED 12 /* workspace: fsk_nd_workspace_bytes */
 
 const char *fsk_last_error(void);
 int fsk_version(void);
@@ -118,6 +120,40 @@ int fsk_kde16_atomic(const float *data, int64_t ld, int64_t n,
                      const float *query, int64_t ldq, int64_t m,
                      double bandwidth, unsigned flags,
                      void *ws, size_t ws_bytes, float *out, void *stream);
+
+/* ---- any dimension d <= FSK_ND_MAX_DIM ----
+ * Points are (rows, d) float32 with row stride ld >= d (no alignment
+ * requirement). The library centres them on the data mean into a copy padded
+ * with zero coordinates to 16 ceil(d/16) features (distances, hence every
+ * kernel value, are unchanged) and runs the FP32 MFMA pair kernels over all
+ * 16-wide slices at once. The reference fixes d = 16 (globals.py:25,
+ * flash_sd_kde/kde.py:120); its paper describes other dimensions as padding
+ * to the kernel width (paper/rebuttal.md:253,505), which is what this is. */
+#define FSK_ND_MAX_DIM 64
+
+/* Workspace of FSK_OP_KDE_ND_SPLITK (m queries) or FSK_OP_SCORE_ND_ORDERED. */
+size_t fsk_nd_workspace_bytes(int op, int64_t n, int64_t m, int d);
+
+/* KDE sums out[i] = sum_j exp(-|q_i - x_j|^2 / (2 h^2)) in d dimensions
+ * (FSK_FLAG_LAPLACIAN: times 1 + d/2 - |q-x|^2/(2h^2)); split-K, FP64 combine.
+ * Generalises fsk_kde16_splitk (kernels/kde_eval_16d_stream_splitk.py:263-370). */
+int fsk_kde_nd_splitk(const float *data, int64_t ld, int64_t n, const float *query,
+                      int64_t ldq, int64_t m, int d, double bandwidth, unsigned flags,
+                      void *ws, size_t ws_bytes, float *out, void *stream);
+
+/* Empirical-score accumulators in d dimensions: pdf[i] = sum_j phi_ij (j = i
+ * included), w[i, f] = sum_j phi_ij x_j[f] written at w + i ldw_row + f ldw_col.
+ * Deterministic split-K; generalises fsk_score16_ordered
+ * (kernels/emp_score_16d_ordered_splitk.py:162-263). flags must be 0 (FP32). */
+int fsk_score_nd_ordered(const float *data, int64_t ld, int64_t n, int d, double bandwidth,
+                         unsigned flags, void *ws, size_t ws_bytes, float *pdf, float *w,
+                         int64_t ldw_row, int64_t ldw_col, void *stream);
+
+/* FP64 debias x + (h^2/2) (w/(pdf + eps) - x)/h^2 in d dimensions (as
+ * fsk_sdkde_debias_f64; flash_sd_kde/kde.py:319-322). */
+int fsk_sdkde_debias_nd_f64(const float *x, int64_t ldx, int64_t n, int d, const float *pdf,
+                            const float *w, int64_t ldw_row, int64_t ldw_col, double bandwidth,
+                            double eps, float *out, int64_t ldo, void *stream);
 
 /* Size of the static device scratch behind fsk_kde16_splitk_noalloc (per GPU). */
 #define FSK_NOALLOC_SCRATCH_BYTES (32u << 20)

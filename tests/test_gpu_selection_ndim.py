@@ -97,3 +97,75 @@ def test_estimator_other_dimensions(gpu, mode):
     ref = np.log(fp64.kde_density(q, data, 0.6) + 1e-12)
     assert np.max(np.abs(ls - ref)) < 5e-5
     np.testing.assert_array_equal(est.score_samples(q), ls)
+
+
+WIDE = [17, 32, 40, 64]
+
+
+def _wide_case(d, n=3001, m=700, seed=0):
+    rng = np.random.default_rng(100 + d + seed)
+    x = (rng.standard_normal((n, d)) + 2.0).astype(np.float32)  # off-centre: centring matters
+    q = (rng.standard_normal((m, d)) + 2.0).astype(np.float32)
+    return x, q, 0.35 * math.sqrt(d)
+
+
+@pytest.mark.parametrize("d", WIDE)
+def test_kde_wide_dimensions_match_fp64(gpu, d):
+    """d in 17..64: the library pads to 16-wide slices inside (one MFMA chain
+    over all slices); sums against FP64 direct differences, with a ragged n,
+    and the Laplacian form with its 1 + d/2 factor."""
+    from flash_sd_kde.ndim import kde_sums_nd
+
+    # unnormalised sums: the normalised densities (2 pi)^(-d/2) h^(-d) / n sum
+    # fall below FP32's range at d = 64 (1e-58 here), so the comparison is on
+    # the sums and on the log-densities
+    x, q, h = _wide_case(d)
+    sums, n, dd = kde_sums_nd(x, q, h, device=gpu)
+    assert (n, dd) == (x.shape[0], d)
+    ref = fp64.kde_sums(q, x, h)
+    e = float(np.max(np.abs(sums.cpu().numpy() - ref) / ref))
+    print(f"kde d={d} rel err vs FP64 {e:.2e}", flush=True)
+    assert e < 1e-5
+    lap = kde_sums_nd(x, q, h, device=gpu, laplacian=True)[0].cpu().numpy()
+    lref = fp64.kde_sums(q, x, h, laplacian=True)
+    assert np.max(np.abs(lap - lref)) <= 1e-5 * np.max(np.abs(lref))
+    lp = kde_logdensity_nd(x, q, h, device=gpu).cpu().numpy()
+    assert np.max(np.abs(lp - fp64.log_density(q, x, h))) < 2e-5
+
+
+def test_kde_wide_strided_rows(gpu):
+    """Rows read through a stride of d + 7 floats (a column slice of a wider
+    array) give the contiguous result bit for bit."""
+    x, q, h = _wide_case(40, n=999, m=130)
+    xb = torch.zeros((999, 47), device=gpu)
+    xb[:, :40] = torch.as_tensor(x, device=gpu)
+    a = kde_eval_nd(xb[:, :40], q, h, device=gpu)
+    b = kde_eval_nd(x, q, h, device=gpu)
+    assert torch.equal(a, b)
+
+
+@pytest.mark.parametrize("d", [17, 40, 64])
+def test_sd_kde_wide_dimensions_match_fp64(gpu, d):
+    x, _, h = _wide_case(d, n=2003)
+    deb = sd_kde_fit_transform_nd(x, h, device=gpu).cpu().numpy()
+    assert deb.shape == (2003, d)
+    pdf, w = fp64.score_sums(x, h)
+    ref, _ = fp64.debias(x, pdf, w, h)
+    e = float(np.max(np.abs(deb - ref))) / h
+    print(f"sd-kde d={d} debias err vs FP64 {e:.2e} h", flush=True)
+    assert e < 1e-5
+
+
+@pytest.mark.parametrize("mode", ["kde", "sd_kde"])
+def test_estimator_wide_dimension(gpu, mode):
+    from flash_sd_kde import FlashSDKDE
+
+    x, q, h = _wide_case(40, n=1500, m=200)
+    est = FlashSDKDE(mode=mode, bandwidth=h, device=gpu).fit(x)
+    ls = est.score_samples(q)
+    data = x
+    if mode == "sd_kde":
+        pdf, w = fp64.score_sums(x, h)
+        data = fp64.debias(x, pdf, w, h)[0].astype(np.float32)
+    ref = np.log(fp64.kde_density(q, data, h) + 1e-12)
+    assert np.max(np.abs(ls - ref)) < 5e-5

@@ -102,6 +102,29 @@ def test_ndim_rejects_bad_dimensions():
         kde_eval_nd(np.zeros((4, 3)), np.zeros((2, 3)), 0.0)
 
 
+def test_wide_dimensions_are_bounded():
+    from flash_sd_kde import _native as nat
+    from flash_sd_kde.ndim import kde_eval_nd
+
+    with pytest.raises(ValueError, match="dimension"):
+        kde_eval_nd(np.zeros((4, 65), np.float32), np.zeros((2, 65), np.float32), 1.0)
+    with pytest.raises(ValueError, match="features"):
+        kde_eval_nd(np.zeros((4, 20), np.float32), np.zeros((2, 21), np.float32), 1.0)
+    lib = nat.load_library()
+    assert lib.fsk_nd_workspace_bytes(nat.OP_KDE_ND_SPLITK, 1000, 10, 65) == 0
+    assert lib.fsk_nd_workspace_bytes(nat.OP_KDE_ND_SPLITK, 1000, 10, 40) > 1000 * 48 * 4
+    # the C entry rejects a stride below the dimension before any GPU work
+    st = lib.fsk_kde_nd_splitk(ctypes_ptr(), 30, 10, ctypes_ptr(), 40, 10, 40, 1.0, 0,
+                               ctypes_ptr(), 1 << 20, ctypes_ptr(), None)
+    assert st == nat.FSK_EINVAL and "stride" in nat.last_error()
+
+
+def ctypes_ptr():
+    import ctypes
+
+    return ctypes.c_void_p(0x1000)
+
+
 def test_fp16_forms_refuse_out_of_range_inputs():
     from flash_sd_kde import _native as nat
     from flash_sd_kde import _ops
