@@ -350,7 +350,7 @@ def one_d_sdkde(rates, steps):
 # forms: the log-density error is set by the KDE's cross term far more than by
 # the score's, so a single-FP16 score under the FP16-split KDE stays within
 # the error bound.
-PRECISION_FORMS = (("fp32_valu", "fp32", "fp32_valu"),
+PRECISION_FORMS = (("fp32_valu", "fp32_valu", "fp32_valu"),
                    ("fp16x3", "fp16x3", "fp16x3"), ("fp16x3s", "fp16x3s", "fp16x3"),
                    ("bf16x6", "bf16x6", "bf16x6"), ("bf16x3", "bf16x3", "bf16x3"),
                    ("bf16x3s", "bf16x3s", "bf16x3"), ("fp16", "fp16", "fp16"),
@@ -445,10 +445,12 @@ def precision_forms(xd, qd, h, fp32_ms, fp32_step, steps, warmup, n_err, rates=N
             # the phi operand (single forms: half a v_cvt_pk; split forms: and,
             # sub, half a v_perm, half a v_cvt_pk; fp16x3s: half a v_cvt_pk and
             # half a copy; single FP16: the row sums as a quarter of a packed
-            # FP16 add and half a mixed FMA). The MFMA work co-issues.
+            # FP16 add and half a mixed FMA). The MFMA work co-issues. The
+            # FP32-VALU score has no MFMA: 16 FMAs of the cross term, the FMA
+            # applying c, the row-sum add and 16 FMAs of phi.X per pair.
             units = float(n) * n
             v_ops = {"bf16x3": 4.0, "fp16x3": 4.0, "bf16x6": 4.0, "fp16x3s": 2.0,
-                     "bf16x3s": 2.0, "fp16": 1.25}.get(sp, 1.5)
+                     "bf16x3s": 2.0, "fp16": 1.25, "fp32_valu": 34.0}.get(sp, 1.5)
             t = units / rates["exp_per_s"] + v_ops * units / rates["fma_per_s"]
             out[name]["score_exp_valu_frac_measured_rates"] = round(t / (score_ms * 1e-3), 4)
     ok = [p for p in out if p != "reference" and out[p]["within_1e-3"]]

@@ -152,6 +152,11 @@ size_t kde16_lse_ws_bytes(int64_t n, int64_t m, unsigned flags);
 size_t score16_ws_bytes(int64_t n, bool ordered);
 size_t score16_lp_ws_bytes(int64_t n, int64_t m, unsigned prec);
 size_t score16_rows_ws_bytes(int64_t n, int64_t m);
+size_t score16_valu_ws_bytes(int64_t n, int64_t m, bool same_rows);
+int score16_valu_run(const char *what, const float *rows, int64_t ldr, int64_t m,
+                     const float *data, int64_t ld, int64_t n, double h, bool same_rows, void *ws,
+                     size_t ws_bytes, float *pdf, float *w, int64_t ldw_row, int64_t ldw_col,
+                     void *stream);
 int score16_lp_run(const char *what, const float *rows, int64_t ldr, int64_t m,
                    const float *data, int64_t ld, int64_t n, double h, unsigned prec, void *ws,
                    size_t ws_bytes, float *pdf, float *w, int64_t ldw_row, int64_t ldw_col,

@@ -198,10 +198,12 @@ extern "C" size_t fsk_workspace_bytes(int op, int64_t n, int64_t m, unsigned fla
     case FSK_OP_SCORE16_SYM: return score16_ws_bytes(n, false);
     case FSK_OP_SCORE16_ROWS:
       if (m <= 0) return 0;
+      if ((flags & FSK_PREC_MASK) == FSK_PREC_F32_VALU) return score16_valu_ws_bytes(n, m, false);
       return (flags & FSK_PREC_MASK) == FSK_PREC_F32
                  ? score16_rows_ws_bytes(n, m)
                  : score16_lp_ws_bytes(n, m, flags & FSK_PREC_MASK);
     case FSK_OP_SCORE16_ORDERED:
+      if ((flags & FSK_PREC_MASK) == FSK_PREC_F32_VALU) return score16_valu_ws_bytes(n, n, true);
       return (flags & FSK_PREC_MASK) == FSK_PREC_F32
                  ? score16_ws_bytes(n, true)
                  : score16_lp_ws_bytes(n, n, flags & FSK_PREC_MASK);

@@ -25,7 +25,8 @@ __global__ void prep_scale1(const float *__restrict__ x, int64_t n, int64_t n_pa
 }
 
 // MODE 0: plain sums; MODE 1: sums of phi (1.5 - ln2 u^2);
-// MODE 2: pdf sums and derivative sums sum(-z phi) (two outputs).
+// MODE 2: pdf sums and derivative sums sum(-z phi) (two outputs);
+// MODE 3: sums of phi z^2 = phi 2 ln2 u^2 (the Laplacian correction's moment).
 template <int MODE, bool ATOMIC>
 __global__ __launch_bounds__(256) void kde1_kernel(const float *__restrict__ xs, int64_t n_pad,
                                                    const float *__restrict__ qs, int64_t m,
@@ -70,6 +71,8 @@ __global__ __launch_bounds__(256) void kde1_kernel(const float *__restrict__ xs,
           const float e = fexp2(-t);
           if (MODE == 1)
             acc[k] = fmaf(e, fmaf(t, -float(kLn2), 1.5f), acc[k]);
+          else if (MODE == 3)
+            acc[k] = fmaf(e, t, acc[k]);
           else
             acc[k] += e;
           if (MODE == 2) acc2[k] = fmaf(e, u, acc2[k]);
@@ -87,7 +90,7 @@ __global__ __launch_bounds__(256) void kde1_kernel(const float *__restrict__ xs,
   for (int k = 0; k < k1dQPT; ++k) {
     int64_t i = qb * k1dRowsPerWG + k * 256 + tid;
     if (i >= m) continue;
-    const double v = a64[k];
+    const double v = MODE == 3 ? 2.0 * kLn2 * a64[k] : a64[k];
     float v2 = 0.f;
     // sum(-z phi) with z = u / sqrt(log2 e / 2): -z = -u * sqrt(2 ln 2)
     if (MODE == 2) v2 = float(-b64[k] * double(zscale));
@@ -162,10 +165,12 @@ static int kde1_run(const char *what, int mode, bool atomic, const float *data, 
   if (atomic) {
     if (mode == 0) FSK_K1(0, true);
     else if (mode == 1) FSK_K1(1, true);
+    else if (mode == 3) FSK_K1(3, true);
     else FSK_K1(2, true);
   } else {
     if (mode == 0) FSK_K1(0, false);
     else if (mode == 1) FSK_K1(1, false);
+    else if (mode == 3) FSK_K1(3, false);
     else FSK_K1(2, false);
   }
 #undef FSK_K1
@@ -184,22 +189,24 @@ using namespace fsk;
 extern "C" int fsk_kde1_splitk(const float *data, int64_t n, const float *query, int64_t m,
                                double bandwidth, unsigned flags, void *ws, size_t ws_bytes,
                                float *out, void *stream) {
-  if (flags & ~FSK_FLAG_LAPLACIAN) {
+  if ((flags & ~(FSK_FLAG_LAPLACIAN | FSK_FLAG_PHI_S)) ||
+      (flags & (FSK_FLAG_LAPLACIAN | FSK_FLAG_PHI_S)) == (FSK_FLAG_LAPLACIAN | FSK_FLAG_PHI_S)) {
     set_error("fsk_kde1_splitk: flags 0x%x not available on the 1-D path", flags);
     return FSK_EUNSUPPORTED;
   }
-  return kde1_run("fsk_kde1_splitk", (flags & FSK_FLAG_LAPLACIAN) ? 1 : 0, false, data, n,
+  return kde1_run("fsk_kde1_splitk", (flags & FSK_FLAG_LAPLACIAN) ? 1 : ((flags & FSK_FLAG_PHI_S) ? 3 : 0), false, data, n,
                   query, m, bandwidth, ws, ws_bytes, out, nullptr, stream);
 }
 
 extern "C" int fsk_kde1_atomic(const float *data, int64_t n, const float *query, int64_t m,
                                double bandwidth, unsigned flags, void *ws, size_t ws_bytes,
                                float *out, void *stream) {
-  if (flags & ~FSK_FLAG_LAPLACIAN) {
+  if ((flags & ~(FSK_FLAG_LAPLACIAN | FSK_FLAG_PHI_S)) ||
+      (flags & (FSK_FLAG_LAPLACIAN | FSK_FLAG_PHI_S)) == (FSK_FLAG_LAPLACIAN | FSK_FLAG_PHI_S)) {
     set_error("fsk_kde1_atomic: flags 0x%x not available on the 1-D path", flags);
     return FSK_EUNSUPPORTED;
   }
-  return kde1_run("fsk_kde1_atomic", (flags & FSK_FLAG_LAPLACIAN) ? 1 : 0, true, data, n, query,
+  return kde1_run("fsk_kde1_atomic", (flags & FSK_FLAG_LAPLACIAN) ? 1 : ((flags & FSK_FLAG_PHI_S) ? 3 : 0), true, data, n, query,
                   m, bandwidth, ws, ws_bytes, out, nullptr, stream);
 }
 

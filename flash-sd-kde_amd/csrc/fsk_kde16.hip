@@ -26,12 +26,17 @@ constexpr int kFlushTiles = 32;  // FP32 inner sums of 4*32 terms, then FP64
 
 // One workgroup = 4 waves; a wave owns QT row tiles of 16 queries and streams
 // the data range of its split in tiles of 16 points.
-template <int QT, bool LAPL, bool ATOMIC>
+// MODE 0: sums of phi; 1: sums of phi (1 + 8 - |q-x|^2/(2h^2)) (the fused
+// Laplacian correction); 2: sums of phi |q-x|^2/h^2 alone (the correction's
+// moment, for the two-pass form). With t = log2(phi) = -|q-x|^2 log2(e)/(2h^2),
+// |q-x|^2/(2h^2) = -ln2 t, so modes 1 and 2 need the one extra FMA phi t.
+template <int QT, int MODE, bool ATOMIC>
 __global__ __launch_bounds__(kWG) void kde16_f32_kernel(
     const float *__restrict__ data, int64_t ld, int64_t n, const float *__restrict__ bn,
     int64_t n_pad, const float *__restrict__ query, int64_t ldq, int64_t m,
     const float *__restrict__ qa, float coef, int64_t split_len, int64_t qblocks,
     int64_t ntasks, int64_t per_xcd, float *__restrict__ out, int64_t ld_out) {
+  constexpr bool LAPL = MODE != 0;
   const int64_t task = xcd_task(blockIdx.x, per_xcd);
   if (task >= ntasks) return;
   const int64_t split = task / qblocks;
@@ -80,7 +85,8 @@ __global__ __launch_bounds__(kWG) void kde16_f32_kernel(
 #pragma unroll
   for (int R = 0; R < QT; ++R) {
     double v = acc64[R] + acc[R];
-    if (LAPL) v = 9.0 * v + kLn2 * (accT64[R] + accT[R]);
+    if (MODE == 1) v = 9.0 * v + kLn2 * (accT64[R] + accT[R]);
+    if (MODE == 2) v = -2.0 * kLn2 * (accT64[R] + accT[R]);
     v = sum_groups(v);
     int64_t i = row0 + 16 * R + c;
     if (g == 0 && i < m) {
@@ -134,11 +140,16 @@ static int kde16_run(const char *what, bool atomic, const float *data, int64_t l
     return FSK_EINVAL;
   }
   const unsigned prec = flags & FSK_PREC_MASK;
-  if (prec != FSK_PREC_F32) {
-    set_error("%s: precision %u not available on this path", what, prec);
+  if (prec != FSK_PREC_F32 || (flags & ~(FSK_PREC_MASK | FSK_FLAG_LAPLACIAN | FSK_FLAG_PHI_S))) {
+    set_error("%s: flags 0x%x not available on this path", what, flags);
     return FSK_EUNSUPPORTED;
   }
-  const bool lapl = (flags & FSK_FLAG_LAPLACIAN) != 0;
+  const bool lapl = (flags & FSK_FLAG_LAPLACIAN) != 0, moment = (flags & FSK_FLAG_PHI_S) != 0;
+  if (lapl && moment) {
+    set_error("%s: FSK_FLAG_LAPLACIAN and FSK_FLAG_PHI_S exclude each other", what);
+    return FSK_EINVAL;
+  }
+  const int mode = lapl ? 1 : (moment ? 2 : 0);
   Kde16Layout L;
   size_t need = kde16_layout(ws, n, m, !atomic, &L);
   if (!ws || ws_bytes < need) {
@@ -167,29 +178,28 @@ static int kde16_run(const char *what, bool atomic, const float *data, int64_t l
   const int64_t grid = xcd_grid(ntasks, &per_xcd);
   const float fcoef = float(coef);
   const int op = atomic ? FSK_OP_KDE16_ATOMIC : FSK_OP_KDE16_SPLITK;
+  float *dst = atomic ? out : L.partials;
+  const int64_t ld_out = atomic ? 0 : m;
+  if (atomic) hipLaunchKernelGGL(fill_f32, dim3(cdiv(m, 256)), dim3(256), 0, s, out, m, 0.f);
+  prof_begin(op, s);
+#define FSK_KDE16(MODE, AT)                                                                     \
+  hipLaunchKernelGGL((kde16_f32_kernel<kKdeQT, MODE, AT>), dim3(grid), dim3(kWG), 0, s, L.xc,  \
+                     int64_t(kD), n, L.bn, n_pad, L.qc, int64_t(kD), m, L.qa, fcoef,            \
+                     p.split_len, p.qblocks, ntasks, per_xcd, dst, ld_out)
   if (atomic) {
-    hipLaunchKernelGGL(fill_f32, dim3(cdiv(m, 256)), dim3(256), 0, s, out, m, 0.f);
-    prof_begin(op, s);
-    if (lapl)
-      hipLaunchKernelGGL((kde16_f32_kernel<kKdeQT, true, true>), dim3(grid), dim3(kWG), 0, s,
-                         L.xc, int64_t(kD), n, L.bn, n_pad, L.qc, int64_t(kD), m, L.qa, fcoef, p.split_len,
-                         p.qblocks, ntasks, per_xcd, out, int64_t(0));
-    else
-      hipLaunchKernelGGL((kde16_f32_kernel<kKdeQT, false, true>), dim3(grid), dim3(kWG), 0, s,
-                         L.xc, int64_t(kD), n, L.bn, n_pad, L.qc, int64_t(kD), m, L.qa, fcoef, p.split_len,
-                         p.qblocks, ntasks, per_xcd, out, int64_t(0));
+    if (mode == 0) FSK_KDE16(0, true);
+    else if (mode == 1) FSK_KDE16(1, true);
+    else FSK_KDE16(2, true);
+  } else {
+    if (mode == 0) FSK_KDE16(0, false);
+    else if (mode == 1) FSK_KDE16(1, false);
+    else FSK_KDE16(2, false);
+  }
+#undef FSK_KDE16
+  if (atomic) {
     prof_end(op, s);
     return launch_status(what);
   }
-  prof_begin(op, s);
-  if (lapl)
-    hipLaunchKernelGGL((kde16_f32_kernel<kKdeQT, true, false>), dim3(grid), dim3(kWG), 0, s,
-                       L.xc, int64_t(kD), n, L.bn, n_pad, L.qc, int64_t(kD), m, L.qa, fcoef, p.split_len,
-                       p.qblocks, ntasks, per_xcd, L.partials, m);
-  else
-    hipLaunchKernelGGL((kde16_f32_kernel<kKdeQT, false, false>), dim3(grid), dim3(kWG), 0, s,
-                       L.xc, int64_t(kD), n, L.bn, n_pad, L.qc, int64_t(kD), m, L.qa, fcoef, p.split_len,
-                       p.qblocks, ntasks, per_xcd, L.partials, m);
   prof_end(op, s);
   if ((st = launch_status(what)) != FSK_OK) return st;
   return fsk_reduce_partials_f64(L.partials, p.splits, m, m, out, stream);

@@ -496,7 +496,7 @@ static int score_check(const char *what, const float *data, int64_t ld, int64_t 
   const unsigned prec = flags & FSK_PREC_MASK;
   const bool lowp = prec == FSK_PREC_BF16 || prec == FSK_PREC_F16 || prec == FSK_PREC_BF16X3 ||
                     prec == FSK_PREC_BF16X6 || prec == FSK_PREC_F16X3 || prec == FSK_PREC_F16X3S ||
-                    prec == FSK_PREC_BF16X3S;
+                    prec == FSK_PREC_BF16X3S || prec == FSK_PREC_F32_VALU;
   if ((prec != FSK_PREC_F32 && !(lowp && lowp_ok)) || (flags & ~FSK_PREC_MASK)) {
     set_error("%s: flags 0x%x not available on this path", what, flags);
     return FSK_EUNSUPPORTED;
@@ -514,6 +514,9 @@ extern "C" int fsk_score16_ordered(const float *data, int64_t ld, int64_t n, dou
   const char *what = "fsk_score16_ordered";
   int st = score_check(what, data, ld, n, bandwidth, flags, pdf, w, true);
   if (st != FSK_OK) return st;
+  if ((flags & FSK_PREC_MASK) == FSK_PREC_F32_VALU)
+    return score16_valu_run(what, data, ld, n, data, ld, n, bandwidth, true, ws, ws_bytes, pdf, w,
+                            ldw_row, ldw_col, stream);
   if ((flags & FSK_PREC_MASK) != FSK_PREC_F32)
     return score16_lp_run(what, data, ld, n, data, ld, n, bandwidth, flags & FSK_PREC_MASK, ws,
                           ws_bytes, pdf, w, ldw_row, ldw_col, stream);
@@ -604,6 +607,9 @@ extern "C" int fsk_score16_rows(const float *rows, int64_t ldr, int64_t m, const
   if (st != FSK_OK) return st;
   if ((st = check_points16(what, rows, ldr, m)) != FSK_OK) return st;
   if ((st = check_elem_rows(what, m)) != FSK_OK) return st;
+  if ((flags & FSK_PREC_MASK) == FSK_PREC_F32_VALU)
+    return score16_valu_run(what, rows, ldr, m, data, ld, n, bandwidth, false, ws, ws_bytes, pdf,
+                            w, ldw_row, ldw_col, stream);
   if ((flags & FSK_PREC_MASK) != FSK_PREC_F32)
     return score16_lp_run(what, rows, ldr, m, data, ld, n, bandwidth, flags & FSK_PREC_MASK, ws,
                           ws_bytes, pdf, w, ldw_row, ldw_col, stream);

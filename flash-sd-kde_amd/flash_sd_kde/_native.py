@@ -34,6 +34,7 @@ PREC_F16X3 = 6
 PREC_F16X3S = 7
 PREC_BF16X3S = 8
 FLAG_LAPLACIAN = 1 << 8
+FLAG_PHI_S = 1 << 9
 
 OP_KDE16_SPLITK = 1
 OP_KDE16_ATOMIC = 2

@@ -45,6 +45,19 @@ def precision_flag(precision_mode: str) -> int:
         ) from None
 
 
+def check_finite(t: torch.Tensor, what: str = "points") -> torch.Tensor:
+    """Refuse NaN / inf coordinates. The kernels clamp exponents (exp2 of
+    min(t, 0)) with NaN-ignoring min/max, so a NaN point would otherwise turn
+    into a finite kernel value and silently change every density (the
+    reference returns NaN there). Skipped under HIP graph capture, where the
+    check's device-to-host read cannot run; the caller vouches for the data."""
+    if t.is_cuda and torch.cuda.is_current_stream_capturing():
+        return t
+    if t.numel() and not bool(torch.isfinite(t).all()):
+        raise ValueError(f"{what} contain NaN or infinite values.")
+    return t
+
+
 def as_points16(array_like, device: torch.device) -> torch.Tensor:
     """(n, 16) float32 contiguous tensor on `device` (reference _to_matrix_tensor,
     kernels/flash_sd_kde/common.py:28-43)."""
@@ -54,7 +67,7 @@ def as_points16(array_like, device: torch.device) -> torch.Tensor:
         t = torch.as_tensor(array_like, dtype=torch.float32, device=device)
     if t.ndim != 2 or t.shape[1] != ND:
         raise ValueError(f"expected tensor with shape (n, {ND}), got {tuple(t.shape)}")
-    return t.contiguous()
+    return check_finite(t.contiguous())
 
 
 def as_points1(array_like, device: torch.device) -> torch.Tensor:
@@ -64,7 +77,7 @@ def as_points1(array_like, device: torch.device) -> torch.Tensor:
         t = torch.as_tensor(array_like, dtype=torch.float32, device=device)
     if t.ndim != 1:
         raise ValueError("expected 1D tensor for KDE eval.")
-    return t.contiguous()
+    return check_finite(t.contiguous())
 
 
 def _check_h(bandwidth: float) -> float:
@@ -76,15 +89,17 @@ def _check_h(bandwidth: float) -> float:
 
 def kde16_sums(train: torch.Tensor, query: torch.Tensor, bandwidth: float, *,
                precision_mode: str = "fp32_ieee", laplacian: bool = False,
-               atomic: bool = False) -> torch.Tensor:
-    """sum_j exp(-|q_i - x_j|^2 / 2h^2) for each query (unnormalised)."""
+               atomic: bool = False, moment: bool = False) -> torch.Tensor:
+    """sum_j exp(-|q_i - x_j|^2 / 2h^2) for each query (unnormalised); with
+    `moment`, sum_j phi |q_i - x_j|^2 / h^2 instead."""
     h = _check_h(bandwidth)
     dev = train.device
     nat.require_gpu(dev, "kde16_sums")
     n, m = train.shape[0], query.shape[0]
     if n == 0 or m == 0:
         raise ValueError("data and queries must be non-empty.")
-    flags = precision_flag(precision_mode) | (nat.FLAG_LAPLACIAN if laplacian else 0)
+    flags = precision_flag(precision_mode) | (nat.FLAG_LAPLACIAN if laplacian else 0) \
+        | (nat.FLAG_PHI_S if moment else 0)
     if (flags & 0xFF) != nat.PREC_F32:
         raise ValueError(f"precision_mode {precision_mode} is only available for log-densities")
     out = torch.empty(m, dtype=torch.float32, device=dev)
@@ -152,14 +167,15 @@ def kde16_logdensity(train: torch.Tensor, query: torch.Tensor, bandwidth: float,
 
 
 def kde1_sums(train: torch.Tensor, query: torch.Tensor, bandwidth: float, *,
-              laplacian: bool = False, atomic: bool = False) -> torch.Tensor:
+              laplacian: bool = False, atomic: bool = False,
+              moment: bool = False) -> torch.Tensor:
     h = _check_h(bandwidth)
     dev = train.device
     nat.require_gpu(dev, "kde1_sums")
     n, m = train.numel(), query.numel()
     if n == 0 or m == 0:
         raise ValueError("data and queries must be non-empty.")
-    flags = nat.FLAG_LAPLACIAN if laplacian else 0
+    flags = (nat.FLAG_LAPLACIAN if laplacian else 0) | (nat.FLAG_PHI_S if moment else 0)
     out = torch.empty(m, dtype=torch.float32, device=dev)
     op = nat.OP_KDE1_ATOMIC if atomic else nat.OP_KDE1_SPLITK
     ws, nbytes = nat.workspace(op, n, m, flags, dev)
@@ -215,8 +231,9 @@ def score16(train: torch.Tensor, bandwidth: float, *, variant: str = "symmetric"
     else:
         raise ValueError(f"unknown score variant {variant}")
     flags = precision_flag(precision)
-    if flags not in (nat.PREC_F32, nat.PREC_BF16, nat.PREC_F16, nat.PREC_BF16X3, nat.PREC_BF16X6,
-                     nat.PREC_F16X3, nat.PREC_F16X3S, nat.PREC_BF16X3S):
+    if flags not in (nat.PREC_F32, nat.PREC_F32_VALU, nat.PREC_BF16, nat.PREC_F16,
+                     nat.PREC_BF16X3, nat.PREC_BF16X6, nat.PREC_F16X3, nat.PREC_F16X3S,
+                     nat.PREC_BF16X3S):
         raise ValueError(f"precision {precision} is not available for the score pass")
     if flags != nat.PREC_F32 and variant != "ordered":
         raise ValueError("reduced-precision score accumulators use the ordered kernel")
@@ -239,8 +256,9 @@ def score16_rows(rows: torch.Tensor, train: torch.Tensor, bandwidth: float, *,
     if n == 0 or m == 0:
         raise ValueError("rows and data must be non-empty.")
     flags = precision_flag(precision)
-    if flags not in (nat.PREC_F32, nat.PREC_BF16, nat.PREC_F16, nat.PREC_BF16X3, nat.PREC_BF16X6,
-                     nat.PREC_F16X3, nat.PREC_F16X3S, nat.PREC_BF16X3S):
+    if flags not in (nat.PREC_F32, nat.PREC_F32_VALU, nat.PREC_BF16, nat.PREC_F16,
+                     nat.PREC_BF16X3, nat.PREC_BF16X6, nat.PREC_F16X3, nat.PREC_F16X3S,
+                     nat.PREC_BF16X3S):
         raise ValueError(f"precision {precision} is not available for the score pass")
     check_fp16_range(train, rows, h, flags)
     pdf = torch.empty(m, dtype=torch.float32, device=dev)
@@ -283,7 +301,7 @@ def as_points_nd(array_like, device: torch.device) -> torch.Tensor:
     if t.ndim != 2 or t.shape[0] == 0 or not 1 <= t.shape[1] <= nat.ND_MAX_DIM:
         raise ValueError(f"expected a non-empty (rows, d) array with 1 <= d <= {nat.ND_MAX_DIM}, "
                          f"got shape {tuple(t.shape)}")
-    return t if t.stride(1) == 1 else t.contiguous()
+    return check_finite(t if t.stride(1) == 1 else t.contiguous())
 
 
 def kde_nd_sums(train: torch.Tensor, query: torch.Tensor, bandwidth: float, *,

@@ -120,29 +120,37 @@ def kde_eval_linearized_nonfused(
     precision_mode: str = DEFAULT_PRECISION_MODE, kde_backend: str = DEFAULT_KDE_BACKEND,
     use_precomputed_norms: bool = True, autotune: bool = True, chunk_size: int | None = None,
 ) -> torch.Tensor:
-    """Linearized density from two separate passes (for comparing against the
-    fused kernel): base density, then the correction sum S = sum_j phi * s with
-    s = |q - x|^2 / h^2, obtained from the Laplacian-weighted pass L through
-    S = 2 ((1 + d/2) sum phi - L). Result = base (1 + d/2) - norm S / 2.
-    `chunk_size` is accepted for signature compatibility (the reference chunks
-    its torch pass over the data)."""
+    """Linearized density from two separate passes, as the reference's
+    non-fused form (flash_sd_kde/kde.py:152-255) for timing against the fused
+    kernel: the base density, then a second pair pass for the correction's
+    moment S = sum_j phi |q - x_j|^2 / h^2 (FSK_FLAG_PHI_S), combined in FP64
+    as base (1 + d/2) - norm S / 2. `chunk_size` is accepted for signature
+    compatibility (the reference chunks its torch pass over the data; the
+    moment pass here streams the data through the kernel's own splits)."""
     del chunk_size
+    validate_precision_mode(precision_mode)
+    validate_kde_backend(kde_backend)
     if bandwidth <= 0:
         raise ValueError("bandwidth must be positive.")
     device = torch.device(device)
+    _require_cuda(device, "kde_eval_linearized_nonfused")
     base = kde_eval(data, queries, bandwidth, device=device, precision_mode=precision_mode,
                     kde_backend=kde_backend, use_precomputed_norms=use_precomputed_norms,
-                    autotune=autotune, apply_laplacian_correction=False)
-    lap = kde_eval(data, queries, bandwidth, device=device, precision_mode=precision_mode,
-                   kde_backend=kde_backend, use_precomputed_norms=use_precomputed_norms,
-                   autotune=autotune, apply_laplacian_correction=True)
+                    autotune=autotune)
+    atomic = kde_backend != KDE_BACKEND_SPLITK_STREAM
     ndim = _ops.infer_ndim(data)
-    if ndim not in (1, 2):
+    if ndim == 1:
+        train, query = _ops.as_points1(data, device), _ops.as_points1(queries, device)
+        s_sum = _ops.kde1_sums(train, query, bandwidth, atomic=atomic, moment=True)
+        dim = 1.0
+    elif ndim == 2:
+        train, query = _ops.as_points16(data, device), _ops.as_points16(queries, device)
+        s_sum = _ops.kde16_sums(train, query, bandwidth, atomic=atomic, moment=True)
+        dim = float(ND_FEATURES)
+    else:
         raise ValueError("data must be 1D or 2D.")
-    dim = 1.0 if ndim == 1 else float(ND_FEATURES)
-    # base and lap already carry the normalisation `norm`
-    sum_phi_scaled_norm = 2.0 * ((1.0 + 0.5 * dim) * base - lap)
-    return base * (1.0 + 0.5 * dim) - 0.5 * sum_phi_scaled_norm
+    norm = 1.0 / ((2.0 * math.pi) ** (dim / 2.0) * bandwidth ** dim * train.shape[0])
+    return (base.double() * (1.0 + 0.5 * dim) - 0.5 * norm * s_sum.double()).float()
 
 
 def emp_sd_kde_fit_transform(

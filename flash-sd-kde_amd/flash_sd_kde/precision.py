@@ -18,7 +18,8 @@ operands and FP32 accumulation:
   "bf16x3"  BF16 hi/lo split, three products (hi.hi + lo.hi + hi.lo)
   "fp16"    single FP16 operands
   "bf16"    single BF16 operands
-  "fp32", "fp32_valu"  FP32 MFMA / FP32 VALU cross term (log-density only)
+  "fp32", "fp32_valu"  FP32 MFMA / FP32 VALU (16 FMAs per pair) cross term;
+            the VALU score also forms phi.X with FMAs (one thread per row)
 with points centred on the data mean, log-densities by log-sum-exp
 (per-split running maximum, FP64 combine) and FP64 debias. The score and the
 log-density may use different forms (`sdkde_logdensity(score_precision=...)`).
@@ -33,7 +34,8 @@ import torch
 
 from flash_sd_kde import _ops
 
-SCORE_PRECISIONS = ("fp32", "fp16x3", "fp16x3s", "bf16x6", "bf16x3", "bf16x3s", "fp16", "bf16")
+SCORE_PRECISIONS = ("fp32", "fp32_valu", "fp16x3", "fp16x3s", "bf16x6", "bf16x3", "bf16x3s",
+                    "fp16", "bf16")
 LOGDENSITY_PRECISIONS = ("fp32", "fp32_valu", "fp16x3", "fp16x3s", "bf16x6", "bf16x3", "bf16x3s",
                          "fp16", "bf16")
 

@@ -17,6 +17,11 @@
  *     kernels/emp_score_16d_symmetric_atomic.py:189-190).
  *   - The score, debias and weighted-combine entry points take at most 2^27
  *     rows (FSK_EUNSUPPORTED above); the KDE entry points have no such limit.
+ *   - Inputs must be finite. Device code clamps exponents with min/max that
+ *     ignore NaN, so a NaN or inf coordinate is not propagated: it yields
+ *     finite but meaningless sums. The entry points do not scan their inputs
+ *     (that would need a device-to-host read per call); the Python layer
+ *     refuses non-finite points with ValueError before launching.
  *
  * The reference (Flash-SD-KDE, Python + Triton) has no C interface; each entry
  * point below replaces the Python launch wrapper cited next to it, which is
@@ -61,6 +66,8 @@ extern "C" {
  * inputs above 3e4). */
 #define FSK_PREC_MASK 0xffu
 #define FSK_FLAG_LAPLACIAN (1u << 8) /* phi * (1 + d/2 - |q-x|^2/(2h^2)) */
+#define FSK_FLAG_PHI_S (1u << 9)     /* phi * |q-x|^2/h^2 alone: the Laplacian correction's
+                                      * moment, for the two-pass (non-fused) form */
 
 /* ---- operation ids for fsk_workspace_bytes ---- */
 #define FSK_OP_KDE16_SPLITK 1
@@ -104,7 +111,8 @@ size_t fsk_workspace_bytes(int op, int64_t n, int64_t m, unsigned flags);
 
 /* 16-D Gaussian KDE sums, deterministic split-K.
  * out[i] = sum_j exp(-|q_i - x_j|^2 / (2 h^2))            (no normalisation)
- * (with FSK_FLAG_LAPLACIAN each term is multiplied by 1 + 8 - |q-x|^2/(2h^2)).
+ * (with FSK_FLAG_LAPLACIAN each term is multiplied by 1 + 8 - |q-x|^2/(2h^2); with
+ * FSK_FLAG_PHI_S by |q-x|^2/h^2).
  * Per-split FP32 partials are combined in FP64.
  * Replaces kde_eval_16d_splitk, kernels/kde_eval_16d_stream_splitk.py:263-370
  * (pass A :14-81 and reduce_pdf_partials, kernels/reduce_partials.py:61-79). */

@@ -146,3 +146,25 @@ def test_sdkde_logdensity_mixed_forms(gpu, golden, score_prec, prec, tol):
     mx, med = precision.rel_log_error(lp, ref)
     record(f"sdkde_logdensity_{score_prec}_score_{prec}", max_rel=mx, median_rel=med)
     assert mx <= tol
+
+
+@pytest.mark.parametrize("n", [1, 17, 1000, 2500])
+def test_valu_score_matches_fp64(gpu, n):
+    """FP32-VALU score form (cross term and phi.X as FMAs, one thread per row):
+    the data set's own accumulators and separate rows, against FP64, to the
+    FP32 MFMA form's bounds."""
+    rng = np.random.default_rng(40 + n)
+    x = (rng.normal(size=(n, 16)) + 3.0).astype(np.float32)
+    h = 0.95
+    xd = torch.as_tensor(x, device=gpu)
+    pdf, w = _ops.score16(xd, h, variant="ordered", precision="fp32_valu")
+    pe, we = fp64.score_sums(x, h)
+    np.testing.assert_allclose(pdf.cpu().numpy(), pe, rtol=1e-5)
+    scale = np.maximum(np.abs(we).max(1, keepdims=True), 1e-30)
+    assert np.max(np.abs(w.cpu().numpy() - we) / scale) < 1e-5
+    rows = (rng.normal(size=(333, 16)) + 3.0).astype(np.float32)
+    pr, wr = _ops.score16_rows(torch.as_tensor(rows, device=gpu), xd, h, precision="fp32_valu")
+    pre, wre = fp64.score_rows(rows, x, h)
+    np.testing.assert_allclose(pr.cpu().numpy(), pre, rtol=1e-5, atol=1e-30)
+    scale = np.maximum(np.abs(wre).max(1, keepdims=True), 1e-30)
+    assert np.max(np.abs(wr.cpu().numpy() - wre) / scale) < 1e-5

@@ -164,3 +164,19 @@ def test_bench_precision_forms_name_valid_forms():
         assert sp in precision.SCORE_PRECISIONS and ep in precision.LOGDENSITY_PRECISIONS
     assert bench.ERROR_BANDWIDTHS == (0.5, 1.0, 2.0)
     assert bench.score_variant("fp32") == "symmetric" and bench.score_variant("fp16") == "ordered"
+
+
+def test_non_finite_points_are_refused():
+    """NaN / inf coordinates raise ValueError in the input conversion (the
+    kernels would otherwise clamp them into finite, meaningless sums)."""
+    from flash_sd_kde import _ops
+
+    x = np.zeros((5, 16), np.float32)
+    x[2, 7] = np.nan
+    with pytest.raises(ValueError, match="NaN or infinite"):
+        _ops.as_points16(x, torch.device("cpu"))
+    with pytest.raises(ValueError, match="NaN or infinite"):
+        _ops.as_points1(np.array([0.0, np.inf], np.float32), torch.device("cpu"))
+    with pytest.raises(ValueError, match="NaN or infinite"):
+        _ops.as_points_nd(np.full((3, 20), -np.inf, np.float32), torch.device("cpu"))
+    _ops.as_points16(np.zeros((5, 16), np.float32), torch.device("cpu"))  # finite passes
