@@ -165,6 +165,184 @@ __global__ __launch_bounds__(256) void mlp_gemm_kernel(const float *__restrict__
     }
 }
 
+// Whole-K GEMM for K <= 256 (every layer of the 16-256-256-256-16 surrogate):
+// the K loop of mlp_gemm_kernel waits on one global->LDS round trip per 32-wide
+// K step, which at K = 256 leaves each GEMM latency-bound. Here every thread
+// first issues all of its global loads at once into registers - 16 float4 of
+// the workgroup's 64 x K slice of X (coalesced along rows) and 8 u16x8 of the
+// 64 x K weight slice - then rounds X to BF16 and stores both slices in LDS
+// (67.6 KB: two workgroups per CU), and after one barrier the 4 waves (2 x 2
+// sub-tiles of 32 x 32) run their MFMAs from LDS back to back. Tasks
+// (row tile, column tile) go out in XCD-grouped order so the column tiles of a
+// row tile read its X rows from one L2.
+// ACT 0..2 as mlp_gemm_kernel; ACT 3 (output layer of a training step): with
+// d = Z - T, Y = scale d and loss += sum d^2 (FP64), Z not stored - the loss
+// and its gradient fused into the output GEMM.
+constexpr int FK_BM = 64, FK_BN = 64, FK_KMAX = 256;
+constexpr int FK_LDS = FK_KMAX + 8;  // u16 per LDS row (528 B: 4-bank stagger between rows)
+constexpr int FK_A4 = FK_BM * FK_KMAX / 4 / 256;  // float4 loads of X per thread (16)
+constexpr int FK_B8 = FK_BN * FK_KMAX / 8 / 256;  // u16x8 loads of W per thread (8)
+
+// KT: K as a compile-time constant (16 or 256, the surrogate's widths: the
+// load-index divisions become shifts), or 0 for any K <= 256 at run time.
+constexpr int FK_LDS_U16 = (FK_BM + FK_BN) * FK_LDS;  // both slices (67.6 KB)
+
+// One (row tile, column tile) task of the whole-K GEMM; lds holds FK_LDS_U16
+// u16 and sl 4 doubles (shared with the weight-gradient body in the fused
+// backward kernel). Returns at once for task >= ntasks (uniform per workgroup).
+template <int ACT, bool STORE_Z, int KT>
+__device__ __forceinline__ void fullk_body(
+    int64_t task, unsigned short *__restrict__ lds, double *__restrict__ sl,
+    const float *__restrict__ X, int64_t ldx, int64_t M, int K_rt,
+    const unsigned short *__restrict__ Wt, const float *__restrict__ bias, int N,
+    float *__restrict__ Z, float *__restrict__ Y, int64_t ldy, int ntiles_n, int64_t ntasks,
+    const float *__restrict__ T, float mse_scale, double *__restrict__ loss) {
+  unsigned short *sA = lds, *sB = lds + FK_BM * FK_LDS;
+  const int K = KT ? KT : K_rt;
+  if (task >= ntasks) return;  // uniform per workgroup
+  const int64_t bm = (task / ntiles_n) * FK_BM;
+  const int bn = int(task % ntiles_n) * FK_BN;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int g = lane >> 4, c = lane & 15;
+  const int ksteps = (K + 31) >> 5;
+  const int k4 = K >> 2, k8 = K >> 3;  // float4 / u16x8 chunks per row (K % 8 == 0)
+  // 1. all global loads of the workgroup in flight at once
+  f4 ra[FK_A4];
+  u16x8 rb[FK_B8];
+#pragma unroll
+  for (int i = 0; i < FK_A4; ++i) {
+    const int e = tid + 256 * i;
+    const int r = e / k4, q = e - r * k4;
+    const int64_t row = bm + r < M ? bm + r : M - 1;
+    if (e < FK_BM * k4) ra[i] = ld4(X + row * ldx + 4 * q);
+  }
+#pragma unroll
+  for (int i = 0; i < FK_B8; ++i) {
+    const int e = tid + 256 * i;
+    const int r = e / k8, q = e - r * k8;
+    const int n = bn + r < N ? bn + r : N - 1;
+    if (e < FK_BN * k8) rb[i] = *reinterpret_cast<const u16x8 *>(Wt + int64_t(n) * K + 8 * q);
+  }
+  // 2. BF16 X and W slices into LDS; zero K slots up to the last 32-wide step
+#pragma unroll
+  for (int i = 0; i < FK_A4; ++i) {
+    const int e = tid + 256 * i;
+    const int r = e / k4, q = e - r * k4;
+    if (e < FK_BM * k4) {
+      u16x4 w;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) w[j] = to16<FSK_PREC_BF16>(ra[i][j]);
+      *reinterpret_cast<u16x4 *>(sA + r * FK_LDS + 4 * q) = w;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < FK_B8; ++i) {
+    const int e = tid + 256 * i;
+    const int r = e / k8, q = e - r * k8;
+    if (e < FK_BN * k8) *reinterpret_cast<u16x8 *>(sB + r * FK_LDS + 8 * q) = rb[i];
+  }
+  const int kpad = ksteps * 4 - k8;  // u16x8 chunks of zeros after K in each row
+  const u16x8 zero8 = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int e = tid; e < (FK_BM + FK_BN) * kpad; e += 256) {
+    const int r = e / kpad, q = k8 + (e - r * kpad);
+    unsigned short *dst = r < FK_BM ? sA + r * FK_LDS : sB + (r - FK_BM) * FK_LDS;
+    *reinterpret_cast<u16x8 *>(dst + 8 * q) = zero8;
+  }
+  __syncthreads();
+  // 3. MFMAs from LDS
+  const int wr = (wave & 1) * 32, wc = (wave >> 1) * 32;
+  f4 acc[2][2];
+#pragma unroll
+  for (int x = 0; x < 2; ++x)
+#pragma unroll
+    for (int y = 0; y < 2; ++y) acc[x][y] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int ks = 0; ks < FK_KMAX / 32; ++ks) {
+    if (ks < ksteps) {  // uniform
+      u16x8 A[2], B[2];
+#pragma unroll
+      for (int x = 0; x < 2; ++x)
+        A[x] = *reinterpret_cast<const u16x8 *>(sA + (wr + 16 * x + c) * FK_LDS + 32 * ks + 8 * g);
+#pragma unroll
+      for (int y = 0; y < 2; ++y)
+        B[y] = *reinterpret_cast<const u16x8 *>(sB + (wc + 16 * y + c) * FK_LDS + 32 * ks + 8 * g);
+#pragma unroll
+      for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+          acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+              __builtin_bit_cast(bf16x8, A[x]), __builtin_bit_cast(bf16x8, B[y]), acc[x][y], 0,
+              0, 0);
+    }
+  }
+  const int64_t m0 = bm + wr;
+  const int n0 = bn + wc;
+  double lsum = 0.0;
+#pragma unroll
+  for (int x = 0; x < 2; ++x)
+#pragma unroll
+    for (int y = 0; y < 2; ++y) {
+      const int n = n0 + 16 * y + c;
+      const float bv = (bias && n < N) ? bias[n] : 0.f;
+#pragma unroll
+      for (int rr = 0; rr < 4; ++rr) {
+        const int64_t row = m0 + 16 * x + 4 * g + rr;
+        if (row < M && n < N) {
+          const float z = acc[x][y][rr] + bv;
+          if (ACT == 3) {
+            const float d = z - T[row * N + n];
+            Y[row * ldy + n] = mse_scale * d;
+            lsum += double(d) * d;
+          } else if (ACT == 2) {
+            Y[row * ldy + n] = z * silu_grad(Z[row * N + n]);
+          } else {
+            if (STORE_Z) Z[row * N + n] = z;
+            Y[row * ldy + n] = ACT ? silu(z) : z;
+          }
+        }
+      }
+    }
+  if (ACT == 3) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) lsum += __shfl_xor(lsum, o);
+    if (lane == 0) sl[wave] = lsum;
+    __syncthreads();
+    if (tid == 0 && loss) atomicAdd(loss, (sl[0] + sl[1]) + (sl[2] + sl[3]));
+  }
+}
+
+template <int ACT, bool STORE_Z, int KT>
+__global__ __launch_bounds__(256) void mlp_gemm_fullk_kernel(
+    const float *__restrict__ X, int64_t ldx, int64_t M, int K_rt,
+    const unsigned short *__restrict__ Wt, const float *__restrict__ bias, int N,
+    float *__restrict__ Z, float *__restrict__ Y, int64_t ldy, int ntiles_n, int64_t ntasks,
+    int64_t per_xcd, const float *__restrict__ T, float mse_scale, double *__restrict__ loss) {
+  __shared__ __align__(16) unsigned short lds[FK_LDS_U16];
+  __shared__ double sl[4];
+  fullk_body<ACT, STORE_Z, KT>(xcd_task(blockIdx.x, per_xcd), lds, sl, X, ldx, M, K_rt, Wt, bias,
+                               N, Z, Y, ldy, ntiles_n, ntasks, T, mse_scale, loss);
+}
+
+template <int ACT, bool STORE_Z>
+static void launch_fullk(const float *x, int64_t ldx, int64_t M, int K, const unsigned short *wt,
+                         const float *bias, int N, float *z, float *y, int64_t ldy,
+                         const float *t, float scale, double *loss, hipStream_t s) {
+  const int ntn = int(cdiv(N, FK_BN));
+  const int64_t ntasks = cdiv(M, FK_BM) * ntn;
+  int64_t per_xcd;
+  const int64_t grid = xcd_grid(ntasks, &per_xcd);
+#define FSK_FULLK(KT)                                                                          \
+  hipLaunchKernelGGL((mlp_gemm_fullk_kernel<ACT, STORE_Z, KT>), dim3(grid), dim3(256), 0, s, x, \
+                     ldx, M, K, wt, bias, N, z, y, ldy, ntn, ntasks, per_xcd, t, scale, loss)
+  if (K == 256)
+    FSK_FULLK(256);
+  else if (K == 16)
+    FSK_FULLK(16);
+  else
+    FSK_FULLK(0);
+#undef FSK_FULLK
+}
+
 // dW_part[s] (N x K) = sum_{m in split s} bf16(dZ[m, n]) bf16(X[m, k]) and,
 // from the workgroups of the first K block, db_part[s][n] = sum dZ[m, n] in
 // FP32. Workgroup tile 64 (n) x 64 (k), wave tile 32 x 32, batch chunk 64.
@@ -200,20 +378,25 @@ __device__ __forceinline__ void wgrad_stage(unsigned short *sT, int b, int q, co
   }
 }
 
-__global__ __launch_bounds__(256) void mlp_wgrad_kernel(const float *__restrict__ dZ, int64_t M,
-                                                        int N, const float *__restrict__ X,
-                                                        int64_t ldx, int K, int64_t split_len,
-                                                        float *__restrict__ dW_part,
-                                                        float *__restrict__ db_part) {
-  __shared__ __align__(16) unsigned short sA[64 * WG_LDS_STRIDE];  // dZ^T chunk [n][m]
-  __shared__ __align__(16) unsigned short sB[64 * WG_LDS_STRIDE];  // X^T chunk [k][m]
-  __shared__ float sbias[16][64];
+// LDS of one weight-gradient task: dZ^T and X^T chunks and the bias partials.
+constexpr int WG_LDS_U16 = 2 * 64 * WG_LDS_STRIDE + 2 * 16 * 64;
+static_assert(WG_LDS_U16 <= FK_LDS_U16, "the fused backward kernel shares one LDS buffer");
+
+// Weight gradient of the (n tile nb, k tile kb, batch split s) task.
+__device__ __forceinline__ void wgrad_body(int nb, int kb, int64_t s,
+                                           unsigned short *__restrict__ lds,
+                                           const float *__restrict__ dZ, int64_t M, int N,
+                                           const float *__restrict__ X, int64_t ldx, int K,
+                                           int64_t split_len, float *__restrict__ dW_part,
+                                           float *__restrict__ db_part) {
+  unsigned short *sA = lds;                           // dZ^T chunk [n][m]
+  unsigned short *sB = lds + 64 * WG_LDS_STRIDE;      // X^T chunk [k][m]
+  float(*sbias)[64] = reinterpret_cast<float(*)[64]>(lds + 2 * 64 * WG_LDS_STRIDE);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int g = lane >> 4, c = lane & 15;
   const int q = tid >> 4, b = tid & 15;
-  const int n0 = blockIdx.x * 64, k0 = blockIdx.y * 64;
+  const int n0 = nb * 64, k0 = kb * 64;
   const int wn = (wave & 1) * 32, wk = (wave >> 1) * 32;
-  const int64_t s = blockIdx.z;
   const int64_t mb = s * split_len;
   const int64_t me = (mb + split_len < M) ? mb + split_len : M;
   const bool a_ok = n0 + 4 * b < N, b_ok = k0 + 4 * b < K;  // N, K multiples of 8
@@ -268,7 +451,7 @@ __global__ __launch_bounds__(256) void mlp_wgrad_kernel(const float *__restrict_
         if (n < N && k < K) dW_part[(s * N + n) * K + k] = acc[x][y][r];
       }
     }
-  if (blockIdx.y == 0) {  // bias gradient: fixed-order sum over the 16 row groups
+  if (kb == 0) {  // bias gradient: fixed-order sum over the 16 row groups
 #pragma unroll
     for (int i = 0; i < 4; ++i) sbias[q][4 * b + i] = bsum[i];
     __syncthreads();
@@ -279,6 +462,66 @@ __global__ __launch_bounds__(256) void mlp_wgrad_kernel(const float *__restrict_
       db_part[s * N + n0 + tid] = t;
     }
   }
+}
+
+__global__ __launch_bounds__(256) void mlp_wgrad_kernel(const float *__restrict__ dZ, int64_t M,
+                                                        int N, const float *__restrict__ X,
+                                                        int64_t ldx, int K, int64_t split_len,
+                                                        float *__restrict__ dW_part,
+                                                        float *__restrict__ db_part) {
+  __shared__ __align__(16) unsigned short lds[WG_LDS_U16];
+  wgrad_body(blockIdx.x, blockIdx.y, blockIdx.z, lds, dZ, M, N, X, ldx, K, split_len, dW_part,
+             db_part);
+}
+
+// Backward of one hidden layer in one launch: workgroups [0, grid_d) run the
+// data gradient dX = (dZ W) * silu'(Z_prev) (whole-K tasks, XCD-grouped),
+// the rest the weight-gradient tasks (n tile, k tile, split) - the two are
+// independent given dZ, so they need not be separate launches.
+template <int KT>
+__global__ __launch_bounds__(256) void mlp_bwd_kernel(
+    const float *__restrict__ dZ, int64_t M, int N, const float *__restrict__ X, int64_t ldx,
+    int K, int64_t split_len, float *__restrict__ dW_part, float *__restrict__ db_part,
+    const unsigned short *__restrict__ Wt, const float *__restrict__ Zp, float *__restrict__ dX,
+    int64_t grid_d, int ntn_d, int64_t ntasks_d, int64_t per_xcd_d, int ntn_w, int ntk_w) {
+  __shared__ __align__(16) unsigned short lds[FK_LDS_U16];
+  __shared__ double sl[4];
+  const int64_t b = blockIdx.x;
+  if (b < grid_d) {
+    // dX (M x K) = dZ (M x N) . W (N x K): the whole-K GEMM over N with Wt = W^T (K x N)
+    fullk_body<2, false, KT>(xcd_task(b, per_xcd_d), lds, sl, dZ, N, M, N, Wt, nullptr, K,
+                             const_cast<float *>(Zp), dX, K, ntn_d, ntasks_d, nullptr, 0.f,
+                             nullptr);
+  } else {
+    const int64_t t = b - grid_d;
+    const int nb = int(t % ntn_w);
+    const int kb = int((t / ntn_w) % ntk_w);
+    wgrad_body(nb, kb, t / (int64_t(ntn_w) * ntk_w), lds, dZ, M, N, X, ldx, K, split_len, dW_part,
+               db_part);
+  }
+}
+
+// Weight gradients of up to 4 layers in one launch: block b belongs to layer
+// k when start[k] <= b < start[k + 1]; within it (n tile, k tile, split).
+struct WgradBatch {
+  fsk_mlp_wgrad_layer l[4];
+  int64_t split_len[4];
+  int ntn[4], ntk[4];
+  int64_t start[5];
+  int n;
+  int64_t M;
+};
+
+__global__ __launch_bounds__(256) void mlp_wgrad_multi_kernel(WgradBatch wb) {
+  __shared__ __align__(16) unsigned short lds[WG_LDS_U16];
+  int k = 0;
+  while (k + 1 < wb.n && int64_t(blockIdx.x) >= wb.start[k + 1]) ++k;
+  const fsk_mlp_wgrad_layer &L = wb.l[k];
+  const int64_t t = int64_t(blockIdx.x) - wb.start[k];
+  const int nb = int(t % wb.ntn[k]);
+  const int kb = int((t / wb.ntn[k]) % wb.ntk[k]);
+  wgrad_body(nb, kb, t / (int64_t(wb.ntn[k]) * wb.ntk[k]), lds, L.dz, wb.M, L.N, L.x, L.ldx, L.K,
+             wb.split_len[k], L.dw_part, L.db_part);
 }
 
 __global__ void mlp_adam_kernel(float *__restrict__ w, float *__restrict__ mom,
@@ -324,7 +567,10 @@ struct AdamBatch {
 // 1 - float(0.999) would be 1.3e-5 off 0.001 in relative terms.
 __global__ void mlp_adam_multi_kernel(AdamBatch batch, float lr, float b1, float omb1, float b2,
                                       float omb2, float eps, float bc1, float bc2,
-                                      const float *__restrict__ sched) {
+                                      const float *__restrict__ sched, int64_t *counter) {
+  // the training step's device counter (read by the next step's gather; this
+  // step's schedule entry was copied to sched before the step began)
+  if (counter && blockIdx.x == 0 && threadIdx.x == 0) *counter += 1;
   int k = 0;
   while (k + 1 < batch.n && int64_t(blockIdx.x) >= batch.block_start[k + 1]) ++k;
   const fsk_adam_tensor &T = batch.t[k];
@@ -433,6 +679,19 @@ extern "C" int fsk_mlp_linear_fwd(const float *x, int64_t ldx, int64_t M, int K,
     return FSK_EINVAL;
   }
   hipStream_t s = as_stream(stream);
+  if (K <= FK_KMAX) {  // every load issued up front (the surrogate's layers)
+    if (act == 2)
+      launch_fullk<2, false>(x, ldx, M, K, wt_bf16, bias, N, z, y, ldy, nullptr, 0.f, nullptr, s);
+    else if (act == 1 && z)
+      launch_fullk<1, true>(x, ldx, M, K, wt_bf16, bias, N, z, y, ldy, nullptr, 0.f, nullptr, s);
+    else if (act == 1)
+      launch_fullk<1, false>(x, ldx, M, K, wt_bf16, bias, N, z, y, ldy, nullptr, 0.f, nullptr, s);
+    else if (z)
+      launch_fullk<0, true>(x, ldx, M, K, wt_bf16, bias, N, z, y, ldy, nullptr, 0.f, nullptr, s);
+    else
+      launch_fullk<0, false>(x, ldx, M, K, wt_bf16, bias, N, z, y, ldy, nullptr, 0.f, nullptr, s);
+    return launch_status(what);
+  }
   dim3 grid(unsigned(cdiv(M, GM_BM)), unsigned(cdiv(N, GM_BN)));
   if (act == 2) {
     hipLaunchKernelGGL((mlp_gemm_kernel<2, false>), grid, dim3(256), 0, s, x, ldx, M, K, wt_bf16,
@@ -452,6 +711,92 @@ extern "C" int fsk_mlp_linear_fwd(const float *x, int64_t ldx, int64_t M, int K,
       hipLaunchKernelGGL((mlp_gemm_kernel<0, false>), grid, dim3(256), 0, s, x, ldx, M, K, wt_bf16,
                          bias, N, z, y, ldy);
   }
+  return launch_status(what);
+}
+
+extern "C" int fsk_mlp_linear_fwd_mse(const float *x, int64_t ldx, int64_t M, int K,
+                                      const unsigned short *wt_bf16, const float *bias, int N,
+                                      const float *t, double scale, float *dy, double *loss,
+                                      void *stream) {
+  const char *what = "fsk_mlp_linear_fwd_mse";
+  int st = mlp_dims_ok(what, M, K, N);
+  if (st != FSK_OK) return st;
+  if (!x || !wt_bf16 || !t || !dy || ldx < K || (ldx & 3) || K > FK_KMAX ||
+      (reinterpret_cast<uintptr_t>(x) & 15) || (reinterpret_cast<uintptr_t>(wt_bf16) & 15)) {
+    set_error("%s: bad pointers or strides, or K = %d above %d", what, K, FK_KMAX);
+    return FSK_EINVAL;
+  }
+  launch_fullk<3, false>(x, ldx, M, K, wt_bf16, bias, N, nullptr, dy, N, t, float(scale), loss,
+                         as_stream(stream));
+  return launch_status(what);
+}
+
+extern "C" int fsk_mlp_linear_bwd(const float *dz, int64_t M, int N, const float *x, int64_t ldx,
+                                  int K, int splits, float *dw_part, float *db_part,
+                                  const unsigned short *wt_bf16, const float *z_prev, float *dx,
+                                  void *stream) {
+  const char *what = "fsk_mlp_linear_bwd";
+  int st = mlp_dims_ok(what, M, K, N);
+  if (st != FSK_OK) return st;
+  if (!dz || !x || !dw_part || !db_part || !wt_bf16 || !z_prev || !dx || splits <= 0 ||
+      splits > 1024 || ldx < K || (ldx & 3) || N > FK_KMAX ||
+      (reinterpret_cast<uintptr_t>(x) & 15) || (reinterpret_cast<uintptr_t>(dz) & 15) ||
+      (reinterpret_cast<uintptr_t>(wt_bf16) & 15)) {
+    set_error("%s: bad arguments (16-byte aligned pointers, ldx a multiple of 4, N <= %d)", what,
+              FK_KMAX);
+    return FSK_EINVAL;
+  }
+  const int ntn_d = int(cdiv(K, FK_BN));
+  const int64_t ntasks_d = cdiv(M, FK_BM) * ntn_d;
+  int64_t per_xcd_d;
+  const int64_t grid_d = xcd_grid(ntasks_d, &per_xcd_d);
+  const int ntn_w = int(cdiv(N, 64)), ntk_w = int(cdiv(K, 64));
+  const int64_t grid = grid_d + int64_t(ntn_w) * ntk_w * splits;
+  const int64_t split_len = round_up(cdiv(M, splits), 64);
+  hipStream_t s = as_stream(stream);
+#define FSK_BWD(KT)                                                                             \
+  hipLaunchKernelGGL((mlp_bwd_kernel<KT>), dim3(unsigned(grid)), dim3(256), 0, s, dz, M, N, x,  \
+                     ldx, K, split_len, dw_part, db_part, wt_bf16, z_prev, dx, grid_d, ntn_d,   \
+                     ntasks_d, per_xcd_d, ntn_w, ntk_w)
+  if (N == 256)
+    FSK_BWD(256);
+  else if (N == 16)
+    FSK_BWD(16);
+  else
+    FSK_BWD(0);
+#undef FSK_BWD
+  return launch_status(what);
+}
+
+extern "C" int fsk_mlp_wgrad_multi(const fsk_mlp_wgrad_layer *layers, int n_layers, int64_t M,
+                                   void *stream) {
+  const char *what = "fsk_mlp_wgrad_multi";
+  if (!layers || n_layers <= 0 || n_layers > 4 || M <= 0) {
+    set_error("%s: need 1..4 layers and M > 0", what);
+    return FSK_EINVAL;
+  }
+  WgradBatch wb{};
+  wb.n = n_layers;
+  wb.M = M;
+  wb.start[0] = 0;
+  for (int k = 0; k < n_layers; ++k) {
+    const fsk_mlp_wgrad_layer &L = layers[k];
+    int st = mlp_dims_ok(what, M, L.K, L.N);
+    if (st != FSK_OK) return st;
+    if (!L.dz || !L.x || !L.dw_part || !L.db_part || L.splits <= 0 || L.splits > 1024 ||
+        L.ldx < L.K || (L.ldx & 3) || (reinterpret_cast<uintptr_t>(L.x) & 15) ||
+        (reinterpret_cast<uintptr_t>(L.dz) & 15)) {
+      set_error("%s: layer %d: bad arguments", what, k);
+      return FSK_EINVAL;
+    }
+    wb.l[k] = L;
+    wb.split_len[k] = round_up(cdiv(M, L.splits), 64);
+    wb.ntn[k] = int(cdiv(L.N, 64));
+    wb.ntk[k] = int(cdiv(L.K, 64));
+    wb.start[k + 1] = wb.start[k] + int64_t(wb.ntn[k]) * wb.ntk[k] * L.splits;
+  }
+  hipLaunchKernelGGL(mlp_wgrad_multi_kernel, dim3(unsigned(wb.start[n_layers])), dim3(256), 0,
+                     as_stream(stream), wb);
   return launch_status(what);
 }
 
@@ -491,9 +836,9 @@ extern "C" int fsk_mlp_adam(float *w, float *m, float *v, const float *grad_part
   return launch_status("fsk_mlp_adam");
 }
 
-extern "C" int fsk_mlp_adam_multi(const fsk_adam_tensor *tensors, int n_tensors, double lr,
-                                  double beta1, double beta2, double eps, int step,
-                                  const float *sched, void *stream) {
+extern "C" int fsk_mlp_adam_multi_step(const fsk_adam_tensor *tensors, int n_tensors, double lr,
+                                       double beta1, double beta2, double eps, int step,
+                                       const float *sched, int64_t *counter, void *stream) {
   const char *what = "fsk_mlp_adam_multi";
   if (!tensors || n_tensors <= 0 || n_tensors > FSK_ADAM_MAX_TENSORS || (step <= 0 && !sched)) {
     set_error("%s: need 1..%d tensors and step > 0 or a schedule", what, FSK_ADAM_MAX_TENSORS);
@@ -516,8 +861,15 @@ extern "C" int fsk_mlp_adam_multi(const fsk_adam_tensor *tensors, int n_tensors,
   const float bc2 = step > 0 ? float(1.0 - pow(beta2, step)) : 1.f;
   hipLaunchKernelGGL(mlp_adam_multi_kernel, dim3(unsigned(b.block_start[n_tensors])), dim3(256), 0,
                      as_stream(stream), b, float(lr), float(beta1), float(1.0 - beta1),
-                     float(beta2), float(1.0 - beta2), float(eps), bc1, bc2, sched);
+                     float(beta2), float(1.0 - beta2), float(eps), bc1, bc2, sched, counter);
   return launch_status(what);
+}
+
+extern "C" int fsk_mlp_adam_multi(const fsk_adam_tensor *tensors, int n_tensors, double lr,
+                                  double beta1, double beta2, double eps, int step,
+                                  const float *sched, void *stream) {
+  return fsk_mlp_adam_multi_step(tensors, n_tensors, lr, beta1, beta2, eps, step, sched, nullptr,
+                                 stream);
 }
 
 extern "C" int fsk_mlp_gather_batch(const float *x, const float *t, int64_t n, int dx, int dt,

@@ -80,11 +80,18 @@ SIGNATURES = {
     "fsk_ref64_workspace_bytes": [_i64, _i64],
     "fsk_mlp_linear_fwd": [_p, _i64, _i64, ctypes.c_int, _p, _p, ctypes.c_int, ctypes.c_int, _p,
                            _p, _i64, _p],
+    "fsk_mlp_linear_fwd_mse": [_p, _i64, _i64, ctypes.c_int, _p, _p, ctypes.c_int, _p, _d, _p,
+                               _p, _p],
+    "fsk_mlp_linear_bwd": [_p, _i64, ctypes.c_int, _p, _i64, ctypes.c_int, ctypes.c_int, _p, _p,
+                           _p, _p, _p, _p],
+    "fsk_mlp_rows_step": [_p, _p],
+    "fsk_mlp_wgrad_multi": [_p, ctypes.c_int, _i64, _p],
     "fsk_mlp_linear_bwd_weight": [_p, _i64, ctypes.c_int, _p, _i64, ctypes.c_int, ctypes.c_int,
                                   _p, _p, _p],
     "fsk_mlp_adam": [_p, _p, _p, _p, ctypes.c_int, _i64, ctypes.c_int, ctypes.c_int, _d, _d, _d,
                      _d, ctypes.c_int, _p, _p, _p, _p],
     "fsk_mlp_adam_multi": [_p, ctypes.c_int, _d, _d, _d, _d, ctypes.c_int, _p, _p],
+    "fsk_mlp_adam_multi_step": [_p, ctypes.c_int, _d, _d, _d, _d, ctypes.c_int, _p, _p, _p],
     "fsk_mlp_gather_batch": [_p, _p, _i64, ctypes.c_int, ctypes.c_int, _p, _i64, _p, _i64, _p,
                              _p, _p, _p, _p],
     "fsk_mlp_step_advance": [_p, _p],
@@ -121,6 +128,27 @@ class AdamTensor(ctypes.Structure):
 
 
 ADAM_MAX_TENSORS = 16
+
+
+class WgradLayer(ctypes.Structure):
+    """fsk_mlp_wgrad_layer of include/flashsdkde.h."""
+    _fields_ = [("dz", ctypes.c_void_p), ("x", ctypes.c_void_p), ("ldx", ctypes.c_int64),
+                ("N", ctypes.c_int), ("K", ctypes.c_int), ("splits", ctypes.c_int),
+                ("reserved", ctypes.c_int), ("dw_part", ctypes.c_void_p),
+                ("db_part", ctypes.c_void_p)]
+
+
+class RowsStepArgs(ctypes.Structure):
+    """fsk_mlp_rows_step_args of include/flashsdkde.h."""
+    _fields_ = [("x", ctypes.c_void_p), ("t", ctypes.c_void_p), ("n", ctypes.c_int64),
+                ("idx", ctypes.c_void_p), ("batch", ctypes.c_int64), ("steps", ctypes.c_int64),
+                ("table", ctypes.c_void_p), ("counter", ctypes.c_void_p),
+                ("sched", ctypes.c_void_p), ("in_dim", ctypes.c_int), ("hidden", ctypes.c_int),
+                ("out_dim", ctypes.c_int), ("reserved", ctypes.c_int),
+                ("w_bf16", ctypes.c_void_p * 4), ("wt_bf16", ctypes.c_void_p * 4),
+                ("bias", ctypes.c_void_p * 4), ("xb", ctypes.c_void_p),
+                ("h", ctypes.c_void_p * 3), ("dz", ctypes.c_void_p * 4),
+                ("mse_scale", ctypes.c_double), ("loss", ctypes.c_void_p)]
 
 _lib: Optional[ctypes.CDLL] = None
 _lock = threading.Lock()

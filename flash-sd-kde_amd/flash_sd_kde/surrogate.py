@@ -94,13 +94,42 @@ class ScoreMLP:
             h = y
         return h
 
+    def forward_loss(self, x: torch.Tensor, t: torch.Tensor, loss_acc: torch.Tensor | None,
+                     cache: list) -> torch.Tensor:
+        """Training forward pass: the hidden layers as forward(), the output
+        layer with the MSE gradient fused into its GEMM; returns dL/dy for
+        L = mean (y - t)^2 and adds sum (y - t)^2 to loss_acc."""
+        h = x.contiguous()
+        M = h.shape[0]
+        for L in self.layers[:-1]:
+            y = torch.empty((M, L["N"]), dtype=torch.float32, device=self.device)
+            z = torch.empty_like(y)
+            st = nat.lib().fsk_mlp_linear_fwd(h.data_ptr(), h.stride(0), M, L["K"],
+                                              L["Wb"].data_ptr(), L["b"].data_ptr(), L["N"], 1,
+                                              z.data_ptr(), y.data_ptr(), L["N"], self._stream())
+            nat.check(st, "fsk_mlp_linear_fwd")
+            cache.append((h, z))
+            h = y
+        L = self.layers[-1]
+        dy = torch.empty((M, L["N"]), dtype=torch.float32, device=self.device)
+        st = nat.lib().fsk_mlp_linear_fwd_mse(h.data_ptr(), h.stride(0), M, L["K"],
+                                              L["Wb"].data_ptr(), L["b"].data_ptr(), L["N"],
+                                              t.data_ptr(), 2.0 / (M * L["N"]), dy.data_ptr(),
+                                              nat.ptr(loss_acc), self._stream())
+        nat.check(st, "fsk_mlp_linear_fwd_mse")
+        cache.append((h, None))
+        return dy
+
     @staticmethod
-    def auto_splits(M: int, N: int, K: int, target_groups: int = 512) -> int:
+    def auto_splits(M: int, N: int, K: int, target_groups: int = 512,
+                    max_splits: int = 32) -> int:
         """Batch splits of a weight-gradient launch: enough (64 x 64 tile,
         split) workgroups to fill the GPU (~512), at most one split per 64
-        batch rows."""
+        batch rows and at most `max_splits` (the optimizer sums the split
+        partials of each weight one after another: the 16-wide layers, with
+        4 tiles, would otherwise take 128)."""
         tiles = -(-N // 64) * -(-K // 64)
-        return max(1, min(-(-target_groups // tiles), max(1, M // 64)))
+        return max(1, min(-(-target_groups // tiles), max(1, M // 64), max_splits))
 
     def backward(self, dy: torch.Tensor, cache: list, splits: int | None = None):
         """Gradient partials per layer: list of (dW_part, db_part). The data
@@ -116,24 +145,27 @@ class ScoreMLP:
             S = splits or self.auto_splits(M, L["N"], L["K"])
             dWp = torch.empty((S, L["N"], L["K"]), dtype=torch.float32, device=self.device)
             dbp = torch.empty((S, L["N"]), dtype=torch.float32, device=self.device)
-            st = nat.lib().fsk_mlp_linear_bwd_weight(dz.data_ptr(), M, L["N"], h.data_ptr(),
-                                                     h.stride(0), L["K"], S, dWp.data_ptr(),
-                                                     dbp.data_ptr(), self._stream())
-            nat.check(st, "fsk_mlp_linear_bwd_weight")
             grads[l] = (dWp, dbp)
-            if l > 0:
-                z_prev = cache[l - 1][1]
-                dz_prev = torch.empty((M, L["K"]), dtype=torch.float32, device=self.device)
-                st = nat.lib().fsk_mlp_linear_fwd(dz.data_ptr(), L["N"], M, L["N"],
-                                                  L["Wt"].data_ptr(), None, L["K"], 2,
-                                                  z_prev.data_ptr(), dz_prev.data_ptr(), L["K"],
-                                                  self._stream())
-                nat.check(st, "fsk_mlp_linear_fwd (data gradient x silu')")
-                dz = dz_prev
+            if l == 0:
+                st = nat.lib().fsk_mlp_linear_bwd_weight(dz.data_ptr(), M, L["N"], h.data_ptr(),
+                                                         h.stride(0), L["K"], S, dWp.data_ptr(),
+                                                         dbp.data_ptr(), self._stream())
+                nat.check(st, "fsk_mlp_linear_bwd_weight")
+                break
+            # weight gradient and the previous layer's dZ (data gradient times
+            # silu'(z_prev)) in one launch
+            z_prev = cache[l - 1][1]
+            dz_prev = torch.empty((M, L["K"]), dtype=torch.float32, device=self.device)
+            st = nat.lib().fsk_mlp_linear_bwd(dz.data_ptr(), M, L["N"], h.data_ptr(), h.stride(0),
+                                              L["K"], S, dWp.data_ptr(), dbp.data_ptr(),
+                                              L["Wt"].data_ptr(), z_prev.data_ptr(),
+                                              dz_prev.data_ptr(), self._stream())
+            nat.check(st, "fsk_mlp_linear_bwd")
+            dz = dz_prev
         return grads
 
     def adam_step(self, grads, lr: float, beta1=0.9, beta2=0.999, eps=1e-8,
-                  sched: torch.Tensor | None = None):
+                  sched: torch.Tensor | None = None, counter: torch.Tensor | None = None):
         """One Adam update of every weight and bias in a single launch. With
         `sched` (device [lr, 1-b1^t, 1-b2^t]) the learning rate and bias
         corrections are read on the device, so the call can live inside a
@@ -152,9 +184,12 @@ class ScoreMLP:
             k += 2
         for a in range(0, k, nat.ADAM_MAX_TENSORS):
             cnt = min(nat.ADAM_MAX_TENSORS, k - a)
-            st = nat.lib().fsk_mlp_adam_multi(ctypes.addressof(arr[a]), cnt, lr, beta1, beta2,
-                                              eps, step, nat.ptr(sched), self._stream())
-            nat.check(st, "fsk_mlp_adam_multi")
+            last = a + cnt >= k  # the step counter advances once, with the last group
+            st = nat.lib().fsk_mlp_adam_multi_step(ctypes.addressof(arr[a]), cnt, lr, beta1,
+                                                   beta2, eps, step, nat.ptr(sched),
+                                                   nat.ptr(counter) if last else None,
+                                                   self._stream())
+            nat.check(st, "fsk_mlp_adam_multi_step")
 
     def mse(self, y: torch.Tensor, t: torch.Tensor, loss_acc: torch.Tensor | None):
         dy = torch.empty_like(y)
@@ -167,7 +202,7 @@ class ScoreMLP:
     def fit(self, x: torch.Tensor, target: torch.Tensor, *, steps: int = 3000, batch: int = 8192,
             lr: float = 2e-3, seed: int = 0, log_every: int = 250, graph: bool = True,
             beta1: float = 0.9, beta2: float = 0.999, splits: int | None = None,
-            warmup: int = 0):
+            warmup: int = 0, fused: bool | None = None):
         """Fit x (n,16) -> target (n,16); inputs standardised per feature and
         targets by their overall scale; cosine learning-rate decay; minibatches
         drawn uniformly with replacement; `warmup` steps of linear learning-rate
@@ -175,7 +210,13 @@ class ScoreMLP:
         (gather, forward, loss, backward, Adam, step counter) is captured once
         as a HIP graph and replayed: the step's batch indices and schedule
         entry are selected on the device by a step counter, so nothing is
-        launched from the host between replays."""
+        launched from the host between replays.
+
+        fused (default: whenever the network is 16-256-256-256-16) runs each
+        step as three launches - the row-local step (gather, forward, loss,
+        data gradients; fsk_mlp_rows_step), all weight gradients
+        (fsk_mlp_wgrad_multi) and Adam with the step counter - instead of the
+        layer-by-layer launches; the values are the same bit for bit."""
         x = x.float().contiguous()
         target = target.float().contiguous()
         self.x_mean = x.mean(0)
@@ -205,44 +246,112 @@ class ScoreMLP:
         loss_acc = torch.zeros(1, dtype=torch.float64, device=dev)
         lib = nat.lib()
 
-        def train_step():
+        if fused is None:
+            fused = self.dims == (16, 256, 256, 256, 16)
+        if fused:
+            train_step = self._fused_step(xn, tn, idx, batch, steps, table, counter, sched,
+                                          loss_acc, splits, lr, beta1, beta2)
+        else:
+            train_step = None
+
+        def layered_step():
             st = lib.fsk_mlp_gather_batch(xn.data_ptr(), tn.data_ptr(), n, xn.shape[1],
                                           tn.shape[1], idx.data_ptr(), batch, table.data_ptr(),
                                           steps, counter.data_ptr(), xb.data_ptr(), tb.data_ptr(),
                                           sched.data_ptr(), self._stream())
             nat.check(st, "fsk_mlp_gather_batch")
             cache = []
-            y = self.forward(xb, cache)
-            dy = self.mse(y, tb, loss_acc)
+            dy = self.forward_loss(xb, tb, loss_acc, cache)
             grads = self.backward(dy, cache, splits)
-            self.adam_step(grads, lr, beta1, beta2, sched=sched)
-            nat.check(lib.fsk_mlp_step_advance(counter.data_ptr(), self._stream()),
-                      "fsk_mlp_step_advance")
+            self.adam_step(grads, lr, beta1, beta2, sched=sched, counter=counter)
 
-        g = None
+        train_step = train_step or layered_step
+        # One graph holds `chunk` consecutive steps (the largest divisor of
+        # log_every up to 16), so the host issues one replay per chunk: a
+        # step takes less GPU time than a replay call from Python.
+        chunk = max(d for d in range(1, 17) if log_every % d == 0)
+        graphs = {}
+
+        def replay(k):
+            if k not in graphs:
+                torch.cuda.synchronize(dev)
+                g = torch.cuda.CUDAGraph()
+                side = torch.cuda.Stream(device=dev)
+                side.wait_stream(torch.cuda.current_stream(dev))
+                with torch.cuda.stream(side):
+                    with torch.cuda.graph(g, stream=side):
+                        for _ in range(k):
+                            train_step()
+                torch.cuda.current_stream(dev).wait_stream(side)
+                self.step_count -= k  # capture records, it does not run
+                graphs[k] = g
+            graphs[k].replay()
+            self.step_count += k
+
         history = []
-        for s in range(steps):
+        s, since_log = 0, 0
+        while s < steps:
             if not graph or s < 2:
                 train_step()  # eager (also the warm-up before capture)
+                k = 1
             else:
-                if g is None:
-                    torch.cuda.synchronize(dev)
-                    g = torch.cuda.CUDAGraph()
-                    side = torch.cuda.Stream(device=dev)
-                    side.wait_stream(torch.cuda.current_stream(dev))
-                    with torch.cuda.stream(side):
-                        with torch.cuda.graph(g, stream=side):
-                            train_step()
-                    torch.cuda.current_stream(dev).wait_stream(side)
-                    self.step_count -= 1  # capture records, it does not run
-                g.replay()
-                self.step_count += 1
-            if (s + 1) % log_every == 0 or s == steps - 1:
-                done = (s % log_every) + 1
-                history.append(float(loss_acc.item()) / (done * batch * tn.shape[1]))
+                # whole chunks from a multiple of `chunk`, single steps elsewhere
+                k = chunk if (s % chunk == 0 and steps - s >= chunk) else 1
+                replay(k)
+            s += k
+            since_log += k
+            if s % log_every == 0 or s == steps:
+                history.append(float(loss_acc.item()) / (since_log * batch * tn.shape[1]))
                 loss_acc.zero_()
+                since_log = 0
         torch.cuda.synchronize(dev)
         return history
+
+    def _fused_step(self, xn, tn, idx, batch, steps, table, counter, sched, loss_acc, splits,
+                    lr, beta1, beta2):
+        """Buffers and argument records of the three-launch training step."""
+        dev = self.device
+        H = self.dims[1]
+        f32 = torch.float32
+        bufs = {"xb": torch.empty((batch, 16), dtype=f32, device=dev),
+                "h": [torch.empty((batch, H), dtype=f32, device=dev) for _ in range(3)],
+                "dz": [torch.empty((batch, H), dtype=f32, device=dev) for _ in range(3)]
+                + [torch.empty((batch, 16), dtype=f32, device=dev)]}
+        grads = []
+        wl = (nat.WgradLayer * len(self.layers))()
+        inputs = [bufs["xb"]] + bufs["h"]
+        for l, L in enumerate(self.layers):
+            S = splits or self.auto_splits(batch, L["N"], L["K"])
+            dWp = torch.empty((S, L["N"], L["K"]), dtype=f32, device=dev)
+            dbp = torch.empty((S, L["N"]), dtype=f32, device=dev)
+            grads.append((dWp, dbp))
+            wl[l] = nat.WgradLayer(bufs["dz"][l].data_ptr(), inputs[l].data_ptr(),
+                                   inputs[l].stride(0), L["N"], L["K"], S, 0, dWp.data_ptr(),
+                                   dbp.data_ptr())
+        a = nat.RowsStepArgs()
+        a.x, a.t, a.n = xn.data_ptr(), tn.data_ptr(), xn.shape[0]
+        a.idx, a.batch, a.steps = idx.data_ptr(), batch, steps
+        a.table, a.counter, a.sched = table.data_ptr(), counter.data_ptr(), sched.data_ptr()
+        a.in_dim, a.hidden, a.out_dim = self.dims[0], H, self.dims[-1]
+        for l, L in enumerate(self.layers):
+            a.w_bf16[l], a.wt_bf16[l], a.bias[l] = L["Wb"].data_ptr(), L["Wt"].data_ptr(), \
+                L["b"].data_ptr()
+            a.dz[l] = bufs["dz"][l].data_ptr()
+        for l in range(3):
+            a.h[l] = bufs["h"][l].data_ptr()
+        a.xb = bufs["xb"].data_ptr()
+        a.mse_scale = 2.0 / (batch * self.dims[-1])
+        a.loss = loss_acc.data_ptr()
+        lib = nat.lib()
+
+        def step():
+            nat.check(lib.fsk_mlp_rows_step(ctypes.byref(a), self._stream()), "fsk_mlp_rows_step")
+            nat.check(lib.fsk_mlp_wgrad_multi(ctypes.byref(wl), len(self.layers), batch,
+                                              self._stream()), "fsk_mlp_wgrad_multi")
+            self.adam_step(grads, lr, beta1, beta2, sched=sched, counter=counter)
+
+        step.keep = (a, wl, bufs, grads)  # records and buffers live as long as the step
+        return step
 
     def predict(self, x: torch.Tensor, chunk: int = 1 << 20) -> torch.Tensor:
         out = torch.empty((x.shape[0], self.dims[-1]), dtype=torch.float32, device=self.device)

@@ -312,12 +312,63 @@ int fsk_loo_bandwidth_grid(const float *data, int64_t ld, const int64_t *offsets
 int fsk_mlp_linear_fwd(const float *x, int64_t ldx, int64_t M, int K, const uint16_t *wt_bf16,
                        const float *bias, int N, int act, float *z, float *y, int64_t ldy,
                        void *stream);
+/* Output layer of a training step with the loss fused in: z = x wt^T + bias,
+ * d = z - t (t row stride N), dy = scale d (row stride N) and *loss (nullable,
+ * FP64) += sum d^2; z itself is not stored. K <= 256. Same values as
+ * fsk_mlp_linear_fwd (act 0) followed by fsk_mlp_mse. */
+int fsk_mlp_linear_fwd_mse(const float *x, int64_t ldx, int64_t M, int K, const uint16_t *wt_bf16,
+                           const float *bias, int N, const float *t, double scale, float *dy,
+                           double *loss, void *stream);
 /* dw_part[s] (N x K) = sum_{m in split s} bf16(dz[m, :])^T bf16(x[m, :]),
  * db_part[s] (N) = sum_{m in split s} dz[m, :] (FP32), splits >= 1, split s
  * covering rows [s L, (s+1) L) with L = ceil(M / splits) rounded up to 64
  * (deterministic). dz and x 16-byte aligned, ldx a multiple of 4. */
 int fsk_mlp_linear_bwd_weight(const float *dz, int64_t M, int N, const float *x, int64_t ldx,
                               int K, int splits, float *dw_part, float *db_part, void *stream);
+/* Backward of a hidden layer in one launch: the weight-gradient partials of
+ * fsk_mlp_linear_bwd_weight and the data gradient of fsk_mlp_linear_fwd act 2,
+ * dx[m, k] = (sum_n dz[m, n] W[n, k]) * silu'(z_prev[m, k]) with wt_bf16 the
+ * (K x N) BF16 copy of W^T and z_prev, dx of row stride K. N <= 256. */
+int fsk_mlp_linear_bwd(const float *dz, int64_t M, int N, const float *x, int64_t ldx, int K,
+                       int splits, float *dw_part, float *db_part, const uint16_t *wt_bf16,
+                       const float *z_prev, float *dx, void *stream);
+/* Weight gradients of several layers in one launch: fsk_mlp_linear_bwd_weight
+ * for each entry (M batch rows shared by all). At most 4 layers. */
+typedef struct {
+  const float *dz, *x; /* M x N output gradients, M x K layer inputs (row stride ldx) */
+  int64_t ldx;
+  int N, K, splits, reserved;
+  float *dw_part, *db_part;
+} fsk_mlp_wgrad_layer;
+int fsk_mlp_wgrad_multi(const fsk_mlp_wgrad_layer *layers, int n_layers, int64_t M,
+                        void *stream);
+/* The row-local part of one training step of the 16-256-256-256-16 network in
+ * one launch: with c = min(*counter, steps - 1), gather batch rows
+ * x[idx[c*batch + r]] and t[...] (xb keeps the inputs), copy table[3c..3c+2]
+ * to sched, run the four forward layers (BF16 operands, FP32 accumulate,
+ * SiLU), the MSE gradient dz4 = mse_scale (y - t) (loss += sum (y - t)^2) and
+ * the data gradients dz_{l-1} = (dz_l W_l) silu'(z_{l-1}), writing the
+ * activations h1..h3 and gradients dz1..dz4 (FP32, row-major) that the
+ * weight gradients need. Values equal those of the layer-by-layer entry
+ * points. Finish the step with fsk_mlp_wgrad_multi and fsk_mlp_adam_multi_step. */
+typedef struct {
+  const float *x, *t; /* normalised training inputs / targets, n x 16 */
+  int64_t n;
+  const int32_t *idx; /* steps x batch row indices in [0, n) */
+  int64_t batch, steps;
+  const float *table; /* steps x 3 schedule (lr, 1 - beta1^t, 1 - beta2^t) */
+  const int64_t *counter;
+  float *sched;
+  int in_dim, hidden, out_dim, reserved; /* must be 16, 256, 16 */
+  const uint16_t *w_bf16[4], *wt_bf16[4];
+  const float *bias[4];
+  float *xb;    /* batch x 16 */
+  float *h[3];  /* batch x 256 */
+  float *dz[4]; /* batch x 256 (layers 1..3), batch x 16 (layer 4) */
+  double mse_scale;
+  double *loss; /* nullable */
+} fsk_mlp_rows_step_args;
+int fsk_mlp_rows_step(const fsk_mlp_rows_step_args *args, void *stream);
 /* Adam on FP32 master weights w (rows x cols) with moments m, v; the gradient
  * is the sum of `splits` partials; also writes w_bf16 (rows x cols) and
  * wt_bf16 (cols x rows) when non-null. step counts from 1. If `sched` (DEVICE,
@@ -339,6 +390,11 @@ typedef struct {
  * one launch (the host array is read at the call; capturable). */
 int fsk_mlp_adam_multi(const fsk_adam_tensor *tensors, int n_tensors, double lr, double beta1,
                        double beta2, double eps, int step, const float *sched, void *stream);
+/* fsk_mlp_adam_multi that also adds 1 to *counter (the training step counter
+ * of fsk_mlp_gather_batch), so a captured step needs no fsk_mlp_step_advance. */
+int fsk_mlp_adam_multi_step(const fsk_adam_tensor *tensors, int n_tensors, double lr,
+                            double beta1, double beta2, double eps, int step, const float *sched,
+                            int64_t *counter, void *stream);
 /* Training-step inputs read on the device, so a captured step replays
  * unchanged: with c = min(*counter, steps - 1), xb[r] = x[idx[c*batch + r]],
  * tb[r] = t[idx[c*batch + r]] (rows of dx and dt floats, multiples of 4,

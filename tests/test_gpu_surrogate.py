@@ -186,3 +186,91 @@ def test_surrogate_sdkde_gain_over_plain_kde(gpu):
     # built from it is closer to the true density than plain KDE
     assert err_s < 0.25 * err_k
     assert mae(lp_s) < mae(lp_k)
+
+
+@pytest.mark.parametrize("M", [1, 300, 8192])
+def test_fused_output_loss_equals_separate_launches(gpu, M):
+    """fsk_mlp_linear_fwd_mse (output GEMM with the MSE gradient in its
+    epilogue) against the output GEMM followed by fsk_mlp_mse: the same dy bit
+    for bit and the same loss up to FP64 summation order."""
+    from flash_sd_kde import _native as nat
+
+    net = ScoreMLP(seed=8, device=gpu)
+    L = net.layers[-1]
+    g = torch.Generator(device=gpu).manual_seed(M)
+    h = torch.randn(M, L["K"], device=gpu, generator=g)
+    t = torch.randn(M, L["N"], device=gpu, generator=g)
+    y = torch.empty(M, L["N"], device=gpu)
+    st = nat.lib().fsk_mlp_linear_fwd(h.data_ptr(), L["K"], M, L["K"], L["Wb"].data_ptr(),
+                                      L["b"].data_ptr(), L["N"], 0, None, y.data_ptr(), L["N"],
+                                      None)
+    nat.check(st, "fwd")
+    loss_a = torch.zeros(1, dtype=torch.float64, device=gpu)
+    dy_a = net.mse(y, t, loss_a)
+    loss_b = torch.zeros(1, dtype=torch.float64, device=gpu)
+    dy_b = torch.empty_like(dy_a)
+    st = nat.lib().fsk_mlp_linear_fwd_mse(h.data_ptr(), L["K"], M, L["K"], L["Wb"].data_ptr(),
+                                          L["b"].data_ptr(), L["N"], t.data_ptr(),
+                                          2.0 / y.numel(), dy_b.data_ptr(), loss_b.data_ptr(), None)
+    nat.check(st, "fwd_mse")
+    assert torch.equal(dy_a, dy_b)
+    assert abs(float(loss_a) - float(loss_b)) <= 1e-12 * float(loss_a)
+
+
+@pytest.mark.parametrize("layer,M,splits", [(1, 8192, 8), (3, 300, 3), (2, 1, 1)])
+def test_fused_backward_equals_separate_launches(gpu, layer, M, splits):
+    """fsk_mlp_linear_bwd (weight gradient and data gradient x silu' in one
+    launch) gives the partials and dZ of the two separate entry points bit for
+    bit."""
+    from flash_sd_kde import _native as nat
+
+    net = ScoreMLP(seed=9, device=gpu)
+    L = net.layers[layer]
+    g = torch.Generator(device=gpu).manual_seed(layer * 100 + M)
+    dz = torch.randn(M, L["N"], device=gpu, generator=g)
+    h = torch.randn(M, L["K"], device=gpu, generator=g)
+    zp = torch.randn(M, L["K"], device=gpu, generator=g)
+    outs = []
+    for fused in (True, False):
+        dW = torch.full((splits, L["N"], L["K"]), float("nan"), device=gpu)
+        db = torch.full((splits, L["N"]), float("nan"), device=gpu)
+        dx = torch.full((M, L["K"]), float("nan"), device=gpu)
+        if fused:
+            st = nat.lib().fsk_mlp_linear_bwd(dz.data_ptr(), M, L["N"], h.data_ptr(), L["K"],
+                                              L["K"], splits, dW.data_ptr(), db.data_ptr(),
+                                              L["Wt"].data_ptr(), zp.data_ptr(), dx.data_ptr(),
+                                              None)
+            nat.check(st, "bwd")
+        else:
+            nat.check(nat.lib().fsk_mlp_linear_bwd_weight(dz.data_ptr(), M, L["N"], h.data_ptr(),
+                                                          L["K"], L["K"], splits, dW.data_ptr(),
+                                                          db.data_ptr(), None), "wgrad")
+            nat.check(nat.lib().fsk_mlp_linear_fwd(dz.data_ptr(), L["N"], M, L["N"],
+                                                   L["Wt"].data_ptr(), None, L["K"], 2,
+                                                   zp.data_ptr(), dx.data_ptr(), L["K"], None),
+                      "dgrad")
+        outs.append((dW, db, dx))
+    for a, b in zip(*outs):
+        assert torch.equal(a, b)
+
+
+@pytest.mark.parametrize("batch,graph", [(300, False), (8192, True)])
+def test_fused_training_step_equals_layered(gpu, batch, graph):
+    """Training with the three-launch step (row-local kernel, multi-layer
+    weight gradients, Adam with the counter) gives the weights of the
+    layer-by-layer launches bit for bit, eager and graph-replayed; a ragged
+    batch (300 rows: a partial last row block) included."""
+    g = torch.Generator(device=gpu).manual_seed(batch)
+    x = torch.randn(5000, 16, device=gpu, generator=g)
+    t = torch.randn(5000, 16, device=gpu, generator=g)
+    nets = []
+    for fused in (True, False):
+        net = ScoreMLP(seed=11, device=gpu)
+        hist = net.fit(x, t, steps=6, batch=batch, lr=1e-2, graph=graph, fused=fused,
+                       log_every=3)
+        nets.append((net, hist))
+    (a, ha), (b, hb) = nets
+    for La, Lb in zip(a.layers, b.layers):
+        for k in ("W", "b", "mW", "vW", "Wb", "Wt"):
+            assert torch.equal(La[k], Lb[k]), k
+    np.testing.assert_allclose(ha, hb, rtol=1e-12)  # loss sums differ in FP64 order only
