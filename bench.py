@@ -155,8 +155,8 @@ def measure_rates():
     out = torch.empty(2048 * 256, device="cuda")
     r = ctypes.c_double()
     rates = {}
-    for which, name, iters in ((0, "fp32_mfma_flops", 20000), (2, "exp_per_s", 20000),
-                               (3, "fma_per_s", 20000)):
+    for which, name, iters in ((0, "fp32_mfma_flops", 20000), (1, "bf16_mfma_flops", 20000),
+                               (2, "exp_per_s", 20000), (3, "fma_per_s", 20000)):
         lib.fsk_microbench(which, 2048, 50, out.data_ptr(), ctypes.addressof(r), None)
         st = lib.fsk_microbench(which, 2048, iters, out.data_ptr(), ctypes.addressof(r), None)
         rates[name] = r.value if st == 0 else None
@@ -164,19 +164,26 @@ def measure_rates():
     # iteration against the two parts timed alone (2048 workgroups, 20000
     # iterations). A ratio near 1 means the two issue times add up; near
     # max/sum they overlap.
+    # The same with 4 BF16 16x16x32 MFMAs (modes 7, 8), and 8 exps with 8 FMAs
+    # (mode 9: whether transcendental and plain VALU issue overlap).
     t = {}
-    for which in (0, 2, 3, 4, 6):
+    for which in (0, 1, 2, 3, 4, 6, 7, 8, 9):
         lib.fsk_microbench(which, 2048, 50, out.data_ptr(), ctypes.addressof(r), None)
         if lib.fsk_microbench(which, 2048, 20000, out.data_ptr(), ctypes.addressof(r), None) == 0:
             t[which] = r.value
-    if len(t) == 5:
+    if len(t) == 9:
         waves = 2048 * 4
         t_mfma = waves * 20000 * 4 * 2048 / t[0]
-        t_exp16 = 2 * waves * 64 * 20000 * 8 / t[2]
-        t_fma16 = 2 * waves * 64 * 20000 * 8 / t[3]
-        rates["mfma_exp_mix_time_over_sum"] = round(t[4] / (t_mfma + t_exp16), 4)
-        rates["mfma_exp_mix_time_over_max"] = round(t[4] / max(t_mfma, t_exp16), 4)
-        rates["mfma_fma_mix_time_over_sum"] = round(t[6] / (t_mfma + t_fma16), 4)
+        t_mfma16 = waves * 20000 * 4 * 16384 / t[1]
+        t_exp8 = waves * 64 * 20000 * 8 / t[2]
+        t_fma8 = waves * 64 * 20000 * 8 / t[3]
+        rates["mfma_exp_mix_time_over_sum"] = round(t[4] / (t_mfma + 2 * t_exp8), 4)
+        rates["mfma_exp_mix_time_over_max"] = round(t[4] / max(t_mfma, 2 * t_exp8), 4)
+        rates["mfma_fma_mix_time_over_sum"] = round(t[6] / (t_mfma + 2 * t_fma8), 4)
+        rates["mfma16_exp_mix_time_over_sum"] = round(t[7] / (t_mfma16 + 2 * t_exp8), 4)
+        rates["mfma16_exp_mix_time_over_max"] = round(t[7] / max(t_mfma16, 2 * t_exp8), 4)
+        rates["mfma16_fma_mix_time_over_sum"] = round(t[8] / (t_mfma16 + 2 * t_fma8), 4)
+        rates["exp_fma_mix_time_over_sum"] = round(t[9] / (t_exp8 + t_fma8), 4)
     # HBM copy: 512 MiB read + 512 MiB written per launch (2^20 blocks x 1 KiB)
     buf = torch.empty(1 << 28, device="cuda")
     lib.fsk_microbench(5, 1 << 20, 1, buf.data_ptr(), ctypes.addressof(r), None)
@@ -356,6 +363,18 @@ PRECISION_FORMS = (("fp32_valu", "fp32_valu", "fp32_valu"),
                    ("bf16x3s", "bf16x3s", "bf16x3"), ("fp16", "fp16", "fp16"),
                    ("bf16", "bf16", "bf16"), ("fp16+fp16x3", "fp16", "fp16x3"))
 ERROR_BANDWIDTHS = (0.5, 1.0, 2.0)  # multiples of the workload's bandwidth
+# Instruction mix of each score form per ordered pair: (16-bit MFMA flop, VALU
+# ops besides the one exp). A 16x16x32 MFMA covers 256 pairs (64 flop each);
+# data tiles go in pairs, so the single forms need 1 cross-term MFMA and half
+# a phi.X MFMA per tile, the one-phi forms 2 + 1, the split x3 forms 2 + 1.5,
+# BF16X6 4 + 1.5. VALU per pair: single forms half a v_cvt_pk and half a
+# v_dot2 (row sums); one-phi forms the row-sum add and half a v_cvt_pk; split
+# forms the row-sum add, and + sub (low part), half a v_perm and half a
+# v_cvt_pk; the FP32-VALU form 16 FMAs of the cross term, the FMA applying c,
+# the row-sum add and 16 FMAs of phi.X (no MFMA).
+SCORE_ISSUE_MIX = {"fp16": (96.0, 1.0), "bf16": (96.0, 1.0), "fp16x3s": (192.0, 1.5),
+                   "bf16x3s": (192.0, 1.5), "fp16x3": (224.0, 4.0), "bf16x3": (224.0, 4.0),
+                   "bf16x6": (352.0, 4.0), "fp32_valu": (0.0, 34.0)}
 
 
 def score_variant(prec):
@@ -438,21 +457,19 @@ def precision_forms(xd, qd, h, fp32_ms, fp32_step, steps, warmup, n_err, rates=N
                      "ms_per_step": round(ms, 3), "gpair_evals_per_s": round(pairs / ms / 1e6, 3),
                      "speedup_vs_fp32": round(fp32_ms / ms, 3), "score_kernel_ms": round(score_ms, 3),
                      "kde_kernel_ms": round(kde_ms, 3), **err_fields(name)}
-        if (rates and rates.get("exp_per_s") and rates.get("fma_per_s") and score_ms > 0
-                and sp != "fp32"):
-            # exp/VALU issue time of the score kernel at the measured rates, per
-            # ordered pair from its instruction mix: 1 exp, the row-sum add, and
-            # the phi operand (single forms: half a v_cvt_pk; split forms: and,
-            # sub, half a v_perm, half a v_cvt_pk; fp16x3s: half a v_cvt_pk and
-            # half a copy; single FP16: the row sums as a quarter of a packed
-            # FP16 add and half a mixed FMA). The MFMA work co-issues. The
-            # FP32-VALU score has no MFMA: 16 FMAs of the cross term, the FMA
-            # applying c, the row-sum add and 16 FMAs of phi.X per pair.
+        if (rates and rates.get("exp_per_s") and rates.get("fma_per_s")
+                and rates.get("bf16_mfma_flops") and score_ms > 0 and sp in SCORE_ISSUE_MIX):
+            # issue time of the score kernel at the rates measured in this run,
+            # from its instruction mix per ordered pair (SCORE_ISSUE_MIX): the
+            # MFMA pipe's time and the VALU's (exps + other VALU ops). The
+            # bound lets the two overlap fully: max(MFMA, exp + VALU).
             units = float(n) * n
-            v_ops = {"bf16x3": 4.0, "fp16x3": 4.0, "bf16x6": 4.0, "fp16x3s": 2.0,
-                     "bf16x3s": 2.0, "fp16": 1.25, "fp32_valu": 34.0}.get(sp, 1.5)
-            t = units / rates["exp_per_s"] + v_ops * units / rates["fma_per_s"]
-            out[name]["score_exp_valu_frac_measured_rates"] = round(t / (score_ms * 1e-3), 4)
+            flop, v_ops = SCORE_ISSUE_MIX[sp]
+            t_mfma = flop * units / rates["bf16_mfma_flops"]
+            t_valu = units / rates["exp_per_s"] + v_ops * units / rates["fma_per_s"]
+            out[name]["score_issue_bound_ms"] = {"mfma": round(t_mfma * 1e3, 2),
+                                                 "exp_valu": round(t_valu * 1e3, 2)}
+            out[name]["score_issue_frac"] = round(max(t_mfma, t_valu) / (score_ms * 1e-3), 4)
     ok = [p for p in out if p != "reference" and out[p]["within_1e-3"]]
     best = max(ok, key=lambda p: out[p]["speedup_vs_fp32"])
     out["best_within_1e-3"] = {"form": best, "speedup_vs_fp32": out[best]["speedup_vs_fp32"],

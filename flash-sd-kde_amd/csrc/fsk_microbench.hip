@@ -56,14 +56,20 @@ __global__ __launch_bounds__(256) void mb_exp(float *out, int iters) {
   out[blockIdx.x * blockDim.x + threadIdx.x] = s;
 }
 
+// FMA operands in registers, accumulating in place: the 4-byte
+// v_fmac_f32_e32 that the pair kernels issue (a literal operand would make
+// every instruction 8 bytes long and measured ~15 % slower).
+__device__ __forceinline__ float mb_fa() { return 0.999f + threadIdx.x * 1e-9f; }
+__device__ __forceinline__ float mb_fb() { return 1e-3f + threadIdx.x * 1e-12f; }
+
 __global__ __launch_bounds__(256) void mb_fma(float *out, int iters) {
   float v[8];
-  const float a = 0.999f, b = 1e-3f;
+  const float a = mb_fa(), b = mb_fb();
 #pragma unroll
   for (int k = 0; k < 8; ++k) v[k] = (threadIdx.x + k) * 1e-3f;
   for (int i = 0; i < iters; ++i) {
 #pragma unroll
-    for (int k = 0; k < 8; ++k) v[k] = fmaf(v[k], a, b);
+    for (int k = 0; k < 8; ++k) v[k] = fmaf(a, b, v[k]);
   }
   float s = 0.f;
 #pragma unroll
@@ -100,6 +106,7 @@ __global__ __launch_bounds__(256) void mb_mix_fma(float *out, int iters) {
   const float a = 1.0f + threadIdx.x * 1e-7f, b = 0.999f;
   f4 c0 = {0, 0, 0, 0}, c1 = c0, c2 = c0, c3 = c0;
   float v[16];
+  const float fa = mb_fa(), fb = mb_fb();
 #pragma unroll
   for (int k = 0; k < 16; ++k) v[k] = (threadIdx.x + k) * 1e-3f;
   for (int i = 0; i < iters; ++i) {
@@ -108,11 +115,64 @@ __global__ __launch_bounds__(256) void mb_mix_fma(float *out, int iters) {
     c2 = mfma_f32(a, b, c2);
     c3 = mfma_f32(a, b, c3);
 #pragma unroll
-    for (int k = 0; k < 16; ++k) v[k] = fmaf(v[k], 0.999f, 1e-3f);
+    for (int k = 0; k < 16; ++k) v[k] = fmaf(fa, fb, v[k]);
   }
   float s = c0[0] + c1[1] + c2[2] + c3[3];
 #pragma unroll
   for (int k = 0; k < 16; ++k) s += v[k];
+  out[blockIdx.x * blockDim.x + threadIdx.x] = s;
+}
+
+// 4 BF16 MFMAs 16x16x32 (4 x 16384 flop) with 16 exps (EXP) or 16 FMAs per
+// lane per iteration: whether the 16-bit MFMA issue overlaps the VALU, as the
+// 16-bit score kernels would need for their MFMA time to hide.
+template <bool EXP>
+__global__ __launch_bounds__(256) void mb_mix_bf16(float *out, int iters) {
+  bf16x8_mb a, b;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    a[k] = (__bf16)(1.0f + k * 0.01f + threadIdx.x * 1e-6f);
+    b[k] = (__bf16)(0.5f);
+  }
+  f4 c0 = {0, 0, 0, 0}, c1 = c0, c2 = c0, c3 = c0;
+  float v[16];
+  const float fa = mb_fa(), fb = mb_fb();
+#pragma unroll
+  for (int k = 0; k < 16; ++k) v[k] = (threadIdx.x + k) * 1e-3f;
+  for (int i = 0; i < iters; ++i) {
+    c0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c0, 0, 0, 0);
+    c1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c1, 0, 0, 0);
+    c2 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c2, 0, 0, 0);
+    c3 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c3, 0, 0, 0);
+#pragma unroll
+    for (int k = 0; k < 16; ++k) v[k] = EXP ? fexp2(-v[k]) : fmaf(fa, fb, v[k]);
+  }
+  float s = c0[0] + c1[1] + c2[2] + c3[3];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) s += v[k];
+  out[blockIdx.x * blockDim.x + threadIdx.x] = s;
+}
+
+// 8 exps and 8 FMAs per lane per iteration on independent chains: whether the
+// transcendental issue overlaps plain VALU work.
+__global__ __launch_bounds__(256) void mb_mix_exp_fma(float *out, int iters) {
+  float v[8], f[8];
+  const float fa = mb_fa(), fb = mb_fb();
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    v[k] = (threadIdx.x + k) * 1e-3f;
+    f[k] = (threadIdx.x + k) * 2e-3f;
+  }
+  for (int i = 0; i < iters; ++i) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      v[k] = fexp2(-v[k]);
+      f[k] = fmaf(fa, fb, f[k]);
+    }
+  }
+  float s = 0.f;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) s += v[k] + f[k];
   out[blockIdx.x * blockDim.x + threadIdx.x] = s;
 }
 
@@ -131,10 +191,12 @@ using namespace fsk;
 // which: 0 = FP32 MFMA (flop/s), 1 = BF16 MFMA (flop/s), 2 = v_exp_f32 (exp/s),
 // 3 = v_fma_f32 (fma/s), 4 = mix (returns seconds per iteration-block, see header),
 // 5 = copy of the first half of `out` to its second half, `iters` launches
-// (bytes read + written per second), 6 = mix with FMAs (seconds, as 4).
+// (bytes read + written per second), 6 = mix with FMAs (seconds, as 4),
+// 7 / 8 = 4 BF16 MFMAs with 16 exps / 16 FMAs per iteration (seconds),
+// 9 = 8 exps with 8 FMAs per iteration (seconds).
 extern "C" int fsk_microbench(int which, int blocks, int iters, float *out, double *result,
                               void *stream) {
-  if (which < 0 || which > 6 || blocks <= 0 || iters <= 0 || !out || !result ||
+  if (which < 0 || which > 9 || blocks <= 0 || iters <= 0 || !out || !result ||
       (which == 5 && (reinterpret_cast<uintptr_t>(out) & 15))) {
     set_error("fsk_microbench: bad arguments");
     return FSK_EINVAL;
@@ -153,6 +215,9 @@ extern "C" int fsk_microbench(int which, int blocks, int iters, float *out, doub
     case 3: hipLaunchKernelGGL(mb_fma, dim3(blocks), dim3(256), 0, s, out, iters); break;
     case 4: hipLaunchKernelGGL(mb_mix, dim3(blocks), dim3(256), 0, s, out, iters); break;
     case 6: hipLaunchKernelGGL(mb_mix_fma, dim3(blocks), dim3(256), 0, s, out, iters); break;
+    case 7: hipLaunchKernelGGL(mb_mix_bf16<true>, dim3(blocks), dim3(256), 0, s, out, iters); break;
+    case 8: hipLaunchKernelGGL(mb_mix_bf16<false>, dim3(blocks), dim3(256), 0, s, out, iters); break;
+    case 9: hipLaunchKernelGGL(mb_mix_exp_fma, dim3(blocks), dim3(256), 0, s, out, iters); break;
     default: {
       // blocks * 256 floats = blocks * 64 float4; each half is blocks * 32 float4
       const int64_t n4 = int64_t(blocks) * 32;

@@ -243,8 +243,9 @@ __device__ __forceinline__ void fullk_body(
   }
   const int kpad = ksteps * 4 - k8;  // u16x8 chunks of zeros after K in each row
   const u16x8 zero8 = {0, 0, 0, 0, 0, 0, 0, 0};
+  const int kdiv = kpad > 0 ? kpad : 1;  // the loop is empty when kpad = 0 (K = 256)
   for (int e = tid; e < (FK_BM + FK_BN) * kpad; e += 256) {
-    const int r = e / kpad, q = k8 + (e - r * kpad);
+    const int r = e / kdiv, q = k8 + (e - r * kdiv);
     unsigned short *dst = r < FK_BM ? sA + r * FK_LDS : sB + (r - FK_BM) * FK_LDS;
     *reinterpret_cast<u16x8 *>(dst + 8 * q) = zero8;
   }

@@ -14,23 +14,32 @@
 //                (FP32-class cross term; phi.X as in BF16X3)
 // phi = exp2(min(t, 0)) comes out with lane l holding phi(i = l%16, j = 4(l/16)+u),
 // u = 0..3, which is exactly four consecutive K slots of the 16x16x32 A operand,
-// so phi.X needs no data movement:
-//   A = [phi_h(u=0..3) | phi_l(u=0..3)] per lane (phi = phi_h + phi_l,
-//   phi_h = phi truncated to BF16, phi_l = rn_bf16(phi - phi_h)),
-//   W += A . [Xh | Xh]  +  A . [Xl | 0]      (BF16x3, BF16x6, FP16x3: drops phi_l.Xl)
-//   W += [phi | 0] . [Xh | Xl]               (single forms; the zero half drops Xl)
-//   W += [phi | phi] . [Xh | Xl]             (F16X3S, BF16X3S: phi rounded to
-//                                             FP16 once, X in FP16 pieces to
-//                                             ~22 bits, one MFMA)
-// The row sums pdf_i = sum_j phi_ij are FP32 VALU adds of the unrounded phi
-// (an MFMA against a ones operand measured slower here).
+// so phi.X needs no data movement. The data tiles go in pairs (a, b), and the
+// K slots of one phi.X MFMA take the products of both tiles wherever a form
+// would otherwise leave half of them zero:
+//   single forms : [phi_a | phi_b] . [Xh_a | Xh_b]                       1 MFMA / pair
+//   split forms  : [phi_h,a | phi_l,a] . [Xh_a | Xh_a]
+//                + [phi_h,b | phi_l,b] . [Xh_b | Xh_b]
+//                + [phi_h,a | phi_h,b] . [Xl_a | Xl_b]                   3 MFMAs / pair
+//                  (phi = phi_h + phi_l, phi_h = phi truncated to BF16,
+//                   phi_l = rn_bf16(phi - phi_h); drops phi_l . Xl)
+//   one-phi forms: [phi | phi] . [Xh | Xl] per tile (F16X3S, BF16X3S: phi
+//                  rounded to FP16 once, X in FP16 pieces to ~22 bits)  2 MFMAs / pair
+// 16x16x32 MFMAs cost as much issue time as several VALU instructions here
+// (the MFMA, exp and VALU issue times add up in these kernels), so the pairing
+// takes a quarter of the single forms' MFMA work away.
+// Row sums pdf_i = sum_j phi_ij: FP32 VALU adds of the unrounded phi, except
+// in the single forms, which add the 16-bit phi that phi.X multiplies, two per
+// v_dot2_f32_f16 / v_dot2_f32_bf16 against (1, 1) (FP32 accumulation, products
+// exact): half the VALU instructions, and an error far below the 16-bit cross
+// term's.
 //
-// Data path. These kernels need several 16-byte operands per lane per 16-point
-// data tile (the FP32 kernel needs one), so fetching them per wave through the
-// vector-memory path is what limits them. Each data tile is therefore stored
-// once, by a prep pass, as ready-made MFMA operand slots (per slot 64 lanes x
-// 16 bytes, in lane order), and each workgroup copies chunks of tiles into a
-// double-buffered LDS area with direct global-to-LDS loads
+// Data path. These kernels need several 16-byte operands per lane per data
+// tile (the FP32 kernel needs one), so fetching them per wave through the
+// vector-memory path is what limits them. Each pair of data tiles is therefore
+// stored once, by a prep pass, as ready-made MFMA operand slots (per slot 64
+// lanes x 16 bytes, in lane order), and each workgroup copies chunks of pairs
+// into a double-buffered LDS area with direct global-to-LDS loads
 // (global_load_lds_dwordx4, one 1 KiB slot per wave instruction); the four
 // waves then read their operands with ds_read_b128. Global and L2 traffic per
 // workgroup drop four-fold (one copy for four waves) and every access is a
@@ -44,43 +53,44 @@
 
 namespace fsk {
 
-constexpr int kLpFlush = 32;
 static_assert(kNormPad % (64 * kScoreLpQT) == 0 && kNormPad % (64 * kScoreLpQTx6) == 0,
               "records must cover whole row blocks");
+static_assert(kNormPad % kLpPairPts == 0, "records must cover whole tile pairs");
 
 typedef __attribute__((address_space(3))) void lds_void;
 
-// Operand slots of one 16-point data tile.
+// Operand slots of one pair of 16-point data tiles (a = 2P, b = 2P + 1):
+// the cross-term A operands of tile a, then of tile b, then the phi.X B
+// operands of the pair.
 template <int PREC>
-struct LpBlock {
+struct LpPair {
   static constexpr bool kX6 = PREC == FSK_PREC_BF16X6;
   static constexpr bool kOnePhi = PREC == FSK_PREC_F16X3S || PREC == FSK_PREC_BF16X3S;
   static constexpr bool kX3 = PREC == FSK_PREC_BF16X3 || PREC == FSK_PREC_F16X3 || kOnePhi;
   static constexpr bool kSplitPhi = kX6 || (kX3 && !kOnePhi);
-  static constexpr int kCross = kX6 ? 4 : kX3 ? 2 : 1;  // cross-term A operands
-  static constexpr int kX = kSplitPhi ? 2 : 1;          // phi.X B operands
-  static constexpr int kSlots = kCross + kX;
-  static constexpr int kSlotU16 = 64 * 8;               // 16 bytes per lane
-  static constexpr int kTileU16 = kSlots * kSlotU16;
-  // tiles per LDS chunk: up to 16 KiB per buffer, two buffers, a whole number
-  // of slots per wave
-  static constexpr int kChunk0 = 16384 / (kTileU16 * 2);
-  static constexpr int kChunk =
-      kSlots % 4 == 0 ? kChunk0 : kSlots % 2 == 0 ? (kChunk0 & ~1) : (kChunk0 & ~3);
-  static constexpr int kChunkU16 = kChunk * kTileU16;
-  static_assert((kChunk * kSlots) % 4 == 0, "four waves copy whole slots");
+  static constexpr int kCross = kX6 ? 4 : kX3 ? 2 : 1;  // cross-term A operands per tile
+  // split: [Xh_a | Xh_a], [Xh_b | Xh_b], [Xl_a | Xl_b]; one-phi: [Xh_a | Xl_a],
+  // [Xh_b | Xl_b]; single: [Xh_a | Xh_b]
+  static constexpr int kX = kSplitPhi ? 3 : kOnePhi ? 2 : 1;
+  static constexpr int kSlots = 2 * kCross + kX;
+  static constexpr int kSlotU16 = 64 * 8;  // 16 bytes per lane
+  static constexpr int kPairU16 = kSlots * kSlotU16;
+  // pairs per LDS chunk: up to 16 KiB per buffer, two buffers
+  static constexpr int kChunk = 8192 / kPairU16 > 0 ? 8192 / kPairU16 : 1;
+  static constexpr int kChunkU16 = kChunk * kPairU16;
+  static constexpr int kChunkSlots = kChunk * kSlots;
 };
 
 // Record halves (lo, hi) of cross-term A operand `s` of a form.
 template <int PREC>
 __device__ __forceinline__ int cross_slot_off(int s, int g) {
   using RL = RecLayout<PREC>;
-  if (LpBlock<PREC>::kX6) {
+  if (LpPair<PREC>::kX6) {
     const int lo = s == 0 ? RL::kP1 : RL::kP2;
     const int hi = s == 0 ? RL::kP1 : s == 1 ? RL::kP1 : s == 2 ? RL::kP3 : RL::kFold;
     return rec_off(g, lo, hi);
   }
-  if (LpBlock<PREC>::kX3) return rec_off(g, RL::kP1, s == 0 ? RL::kP2 : RL::kFold);
+  if (LpPair<PREC>::kX3) return rec_off(g, RL::kP1, s == 0 ? RL::kP2 : RL::kFold);
   return rec_off(g, RL::kP1, RL::kFold);
 }
 
@@ -104,34 +114,40 @@ __global__ void prep_xt16_kernel(const float *__restrict__ x, int64_t ld, int64_
   xt[base + 4 + u] = lo;
 }
 
-// Operand slots of every data tile from the records and the transposed X;
-// tiles [ntiles_data, ntiles_alloc) are the zero slack that whole-chunk copies
-// may read past the last tile.
+// Operand slots of every tile pair from the records and the transposed X;
+// pairs [npairs_data, npairs_alloc) are the zero slack that whole-chunk copies
+// may read past the last pair. The records and xt cover 32 npairs_data points
+// (rows past n are padding: zero X, a norm term that drives phi to 0).
 template <int PREC>
-__global__ void prep_lp_blocks_kernel(const unsigned short *__restrict__ xrec,
-                                      const unsigned short *__restrict__ xt,
-                                      int64_t ntiles_data, int64_t ntiles_alloc,
-                                      unsigned short *__restrict__ blk) {
-  using B = LpBlock<PREC>;
+__global__ void prep_lp_pairs_kernel(const unsigned short *__restrict__ xrec,
+                                     const unsigned short *__restrict__ xt, int64_t npairs_data,
+                                     int64_t npairs_alloc, unsigned short *__restrict__ blk) {
+  using B = LpPair<PREC>;
   using RL = RecLayout<PREC>;
-  const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;  // (tile, slot, lane)
-  if (e >= ntiles_alloc * B::kSlots * 64) return;
+  const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;  // (pair, slot, lane)
+  if (e >= npairs_alloc * B::kSlots * 64) return;
   const int lane = int(e % 64);
   const int slot = int((e / 64) % B::kSlots);
-  const int64_t T = e / (64 * B::kSlots);
+  const int64_t P = e / (64 * B::kSlots);
   const int g = lane >> 4, c = lane & 15;
   u16x8 v = {0, 0, 0, 0, 0, 0, 0, 0};
-  if (T < ntiles_data) {
-    if (slot < B::kCross) {
-      v = ld_u16x8(xrec + (T * kTile + c) * RL::kSize + cross_slot_off<PREC>(slot, g));
+  if (P < npairs_data) {
+    if (slot < 2 * B::kCross) {
+      const int64_t T = 2 * P + slot / B::kCross;
+      v = ld_u16x8(xrec + (T * kTile + c) * RL::kSize + cross_slot_off<PREC>(slot % B::kCross, g));
     } else {
-      const u16x8 t = ld_u16x8(xt + ((T * kD + c) * 4 + g) * 8);  // [Xh | Xl]
-      if (B::kX == 1) {
-        v = t;
-      } else if (slot == B::kCross) {
-        v = u16x8{t[0], t[1], t[2], t[3], t[0], t[1], t[2], t[3]};  // [Xh | Xh]
+      const int xs = slot - 2 * B::kCross;
+      // [Xh | Xl] of feature c, lane group g, for tiles a and b
+      const u16x8 ta = ld_u16x8(xt + (((2 * P) * kD + c) * 4 + g) * 8);
+      const u16x8 tb = ld_u16x8(xt + (((2 * P + 1) * kD + c) * 4 + g) * 8);
+      if (B::kSplitPhi) {
+        if (xs == 0) v = u16x8{ta[0], ta[1], ta[2], ta[3], ta[0], ta[1], ta[2], ta[3]};
+        else if (xs == 1) v = u16x8{tb[0], tb[1], tb[2], tb[3], tb[0], tb[1], tb[2], tb[3]};
+        else v = u16x8{ta[4], ta[5], ta[6], ta[7], tb[4], tb[5], tb[6], tb[7]};
+      } else if (B::kOnePhi) {
+        v = xs == 0 ? ta : tb;
       } else {
-        v = u16x8{t[4], t[5], t[6], t[7], 0, 0, 0, 0};  // [Xl | 0]
+        v = u16x8{ta[0], ta[1], ta[2], ta[3], tb[0], tb[1], tb[2], tb[3]};
       }
     }
   }
@@ -144,7 +160,7 @@ typedef _Float16 h16x2v __attribute__((ext_vector_type(2)));
 typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 
 // Two floats rounded (nearest even) to 16 bits and packed [a | b << 16]: one
-// v_cvt_pk_bf16_f32 for BF16.
+// v_cvt_pk_bf16_f32 / v_cvt_pk_f16_f32.
 template <int P16>
 __device__ __forceinline__ unsigned pack2(float a, float b) {
   const f2v v = {a, b};
@@ -152,54 +168,87 @@ __device__ __forceinline__ unsigned pack2(float a, float b) {
   return __builtin_bit_cast(unsigned, __builtin_convertvector(v, bf16x2v));
 }
 
-// The 16-bit A operand of phi.X from a lane's four phi values (K slots
-// 0..3) and, for the split forms, their low parts (K slots 4..7). Split: the
-// high part is phi truncated to its upper 16 bits (a BF16 value, exact as
-// FP32), the low part phi - high (exact in FP32) rounded to BF16; a byte
-// permute packs two high parts, one conversion packs two low parts.
-template <int P16, bool SPLIT, bool DUP = false>
-__device__ __forceinline__ u16x8 phi_operand(const float (&e)[4]) {
-  u32x4 a;
-  if (SPLIT) {
-    unsigned b[4];
-    float lo[4];
+// BF16 split of a lane's four phi values: hi = the two packed words of phi
+// truncated to its upper 16 bits (a BF16 value, exact as FP32: one byte
+// permute per two values), lo = the two packed words of rn_bf16(phi - hi)
+// (the difference is exact in FP32).
+__device__ __forceinline__ void split_phi(const float (&e)[4], unsigned (&hi)[2],
+                                          unsigned (&lo)[2]) {
+  unsigned b[4];
+  float l[4];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      b[u] = __builtin_bit_cast(unsigned, e[u]);
-      lo[u] = e[u] - __builtin_bit_cast(float, b[u] & 0xffff0000u);
-    }
-    a[0] = __builtin_amdgcn_perm(b[1], b[0], 0x07060302u);
-    a[1] = __builtin_amdgcn_perm(b[3], b[2], 0x07060302u);
-    a[2] = pack2<FSK_PREC_BF16>(lo[0], lo[1]);
-    a[3] = pack2<FSK_PREC_BF16>(lo[2], lo[3]);
-  } else {
-    a[0] = pack2<P16>(e[0], e[1]);
-    a[1] = pack2<P16>(e[2], e[3]);
-    a[2] = DUP ? a[0] : 0u;  // DUP: [phi | phi] against [Xh | Xl]
-    a[3] = DUP ? a[1] : 0u;
+  for (int u = 0; u < 4; ++u) {
+    b[u] = __builtin_bit_cast(unsigned, e[u]);
+    l[u] = e[u] - __builtin_bit_cast(float, b[u] & 0xffff0000u);
   }
-  return __builtin_bit_cast(u16x8, a);
+  hi[0] = __builtin_amdgcn_perm(b[1], b[0], 0x07060302u);
+  hi[1] = __builtin_amdgcn_perm(b[3], b[2], 0x07060302u);
+  lo[0] = pack2<FSK_PREC_BF16>(l[0], l[1]);
+  lo[1] = pack2<FSK_PREC_BF16>(l[2], l[3]);
+}
+
+// acc + (the two 16-bit values packed in p): one v_dot2_f32_f16 / _bf16
+// against (1, 1); the products are exact, the sum FP32.
+template <int P16>
+__device__ __forceinline__ float dot2_ones(unsigned p, float acc) {
+  if (P16 == FSK_PREC_F16) {
+    const h16x2v one = {1.0, 1.0};
+    return __builtin_amdgcn_fdot2(__builtin_bit_cast(h16x2v, p), one, acc, false);
+  }
+  const bf16x2v one = {1.0, 1.0};
+  return __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(bf16x2v, p), one, acc, false);
+}
+
+__device__ __forceinline__ u16x8 words(unsigned a, unsigned b, unsigned c, unsigned d) {
+  return __builtin_bit_cast(u16x8, u32x4{a, b, c, d});
 }
 
 __device__ __forceinline__ u16x8 lds_u16x8(const unsigned short *p) {
   return *reinterpret_cast<const u16x8 *>(p);
 }
 
+// Cross term of one data tile (A operands from LDS) against one row tile.
+template <int PREC>
+__device__ __forceinline__ f4 lp_cross(const u16x8 (&A)[LpPair<PREC>::kCross], u16x8 b1,
+                                       u16x8 b2, u16x8 b3, u16x8 b4) {
+  constexpr int P16 = CrossType<PREC>::kP16;
+  f4 d = mfma16(A[0], b1, f4{0.f, 0.f, 0.f, 0.f}, P16);
+  if (LpPair<PREC>::kCross >= 2) d = mfma16(A[1 % LpPair<PREC>::kCross], b2, d, P16);
+  if (LpPair<PREC>::kCross == 4) {
+    d = mfma16(A[2 % LpPair<PREC>::kCross], b3, d, P16);
+    d = mfma16(A[3 % LpPair<PREC>::kCross], b4, d, P16);
+  }
+  return d;
+}
+
+// Waves per SIMD the compiler must leave room for: 4 (at most 128 VGPRs) for
+// every form but BF16X6, whose four query operands per row tile need more and
+// which runs best at 3 (at most 168 VGPRs, a few spilled outside the loop:
+// n = 2^20 at 2 / 3 / 4 waves took 327 / 302 / 401 ms).
+template <int PREC>
+constexpr int lp_min_waves() { return PREC == FSK_PREC_BF16X6 ? 3 : 4; }
+
 template <int PREC, int QT>
-__global__ __launch_bounds__(kWG) void score16_lp_kernel(
+__global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(lp_min_waves<PREC>())))
+void score16_lp_kernel(
     const unsigned short *__restrict__ blk, const unsigned short *__restrict__ qrec, int64_t m,
-    int64_t ntiles_data, int64_t split_tiles, int64_t qblocks, int64_t ntasks, int64_t per_xcd,
+    int64_t npairs_data, int64_t split_pairs, int64_t qblocks, int64_t ntasks, int64_t per_xcd,
     float *__restrict__ pdf_part, float *__restrict__ w_part) {
-  using BL = LpBlock<PREC>;
-  constexpr int P16 = CrossType<PREC>::kP16;  // cross-term operands
+  using BL = LpPair<PREC>;
   constexpr bool X6 = BL::kX6;
   constexpr bool X3 = BL::kX3;
   constexpr bool SPLIT_PHI = BL::kSplitPhi;  // phi.X with phi = phi_h + phi_l
-  constexpr bool DUP_PHI = BL::kOnePhi;
+  constexpr bool ONE_PHI = BL::kOnePhi;
   // phi.X operands: BF16 for the split forms (phi in [0, 1] keeps full
   // relative precision down to tiny values), FP16 for the one-phi forms (phi
   // to 2^-12, X in FP16 pieces), else the form's own type
-  constexpr int PX = SPLIT_PHI ? FSK_PREC_BF16 : DUP_PHI ? FSK_PREC_F16 : P16;
+  constexpr int PX = SPLIT_PHI ? FSK_PREC_BF16 : ONE_PHI ? FSK_PREC_F16 : CrossType<PREC>::kP16;
+  constexpr bool SINGLE = !SPLIT_PHI && !ONE_PHI;
+  // tile pairs between FP64 flushes of the FP32 sums: the single forms' own
+  // error (16-bit cross term and phi, ~1e-2 relative) dwarfs FP32 rounding
+  // over 2048 terms
+  constexpr int FLUSH = SINGLE ? 64 : 16;
+  constexpr int KC = BL::kCross;
   constexpr int CH = BL::kChunk;
   constexpr int S1 = BL::kSlotU16;
   using RL = RecLayout<PREC>;
@@ -231,39 +280,38 @@ __global__ __launch_bounds__(kWG) void score16_lp_kernel(
     } else if (X3) {
       B1[R] = ld_u16x8(rq + rec_off(g, RL::kP1, RL::kP1));
       B2[R] = ld_u16x8(rq + rec_off(g, RL::kP2, RL::kFold));
+      B3[R] = B4[R] = B2[R];
     } else {
       B1[R] = ld_u16x8(rq + rec_off(g, RL::kP1, RL::kFold));
+      B2[R] = B3[R] = B4[R] = B1[R];
     }
     w[R] = f4{0.f, 0.f, 0.f, 0.f};
     pdf[R] = 0.f;
     w64[R] = d4{0.0, 0.0, 0.0, 0.0};
     pdf64[R] = 0.0;
   }
-  // this task's data tiles [t_begin, t_end), copied to LDS CH tiles at a time
-  const int64_t t_begin = split * split_tiles;
-  int64_t t_end = t_begin + split_tiles;
-  if (t_end > ntiles_data) t_end = ntiles_data;
-  const int ntile = int(t_end - t_begin);
-  const int nchunk = (ntile + CH - 1) / CH;
-  const unsigned short *src0 = blk + t_begin * BL::kTileU16;
+  // this task's tile pairs [p_begin, p_end), copied to LDS CH pairs at a time
+  const int64_t p_begin = split * split_pairs;
+  int64_t p_end = p_begin + split_pairs;
+  if (p_end > npairs_data) p_end = npairs_data;
+  const int npair = int(p_end - p_begin);
+  const int nchunk = (npair + CH - 1) / CH;
+  const unsigned short *src0 = blk + p_begin * BL::kPairU16;
   // Wave `wave` copies slots wave, wave + 4, ... of a chunk: one 1 KiB slot per
   // instruction, lane l's 16 bytes landing at LDS slot base + 16 l.
   auto stage = [&](int ch, int buf) {
     const unsigned short *src = src0 + int64_t(ch) * BL::kChunkU16 + lane * 8;
 #pragma unroll
-    for (int q = 0; q < CH * BL::kSlots / 4; ++q) {
+    for (int q = 0; q < (BL::kChunkSlots + 3) / 4; ++q) {
       const int k = 4 * q + wave;
-      __builtin_amdgcn_global_load_lds(src + k * S1, (lds_void *)(lds + buf * BL::kChunkU16 + k * S1),
-                                       16, 0, 0);
+      if (k < BL::kChunkSlots)
+        __builtin_amdgcn_global_load_lds(src + k * S1,
+                                         (lds_void *)(lds + buf * BL::kChunkU16 + k * S1), 16, 0, 0);
     }
   };
 
   stage(0, 0);
   int tc = 0;
-  // (1, 1) in FP16, kept opaque so that the FP16 halves enter the FP32 row
-  // sums through one mixed-precision FMA each instead of a convert and an add
-  h16x2v one_h = {1.0, 1.0};
-  asm volatile("" : "+v"(one_h));
   for (int ch = 0; ch < nchunk; ++ch) {
     // this wave's copies of chunk ch have landed; the barrier makes every
     // wave's copies visible and retires all reads of the buffer staged next
@@ -271,63 +319,87 @@ __global__ __launch_bounds__(kWG) void score16_lp_kernel(
     __syncthreads();
     if (ch + 1 < nchunk) stage(ch + 1, (ch + 1) & 1);
     const unsigned short *cb = lds + (ch & 1) * BL::kChunkU16 + lane * 8;
-    const int nt = ntile - ch * CH < CH ? ntile - ch * CH : CH;
-    for (int t = 0; t < nt; ++t) {
-      const unsigned short *tb = cb + t * BL::kTileU16;
-      u16x8 A1, A2, A3, A4;
-      if (X6) {
-        A1 = lds_u16x8(tb);
-        A2 = lds_u16x8(tb + S1);
-        A3 = lds_u16x8(tb + 2 * S1);
-        A4 = lds_u16x8(tb + 3 * S1);
-      } else if (X3) {
-        A1 = lds_u16x8(tb);
-        A2 = lds_u16x8(tb + S1);
+    const int np = npair - ch * CH < CH ? npair - ch * CH : CH;
+    for (int t = 0; t < np; ++t) {
+      const unsigned short *pb = cb + t * BL::kPairU16;
+      if constexpr (SINGLE) {
+        // single forms: both tiles at once, one phi.X MFMA [phi_a | phi_b]
+        u16x8 Aa[KC], Ab[KC];
+        Aa[0] = lds_u16x8(pb);
+        Ab[0] = lds_u16x8(pb + S1);
+        const u16x8 BX = lds_u16x8(pb + 2 * S1);
+#pragma unroll
+        for (int R = 0; R < QT; ++R) {
+          const f4 da = lp_cross<PREC>(Aa, B1[R], B2[R], B3[R], B4[R]);
+          const f4 db = lp_cross<PREC>(Ab, B1[R], B2[R], B3[R], B4[R]);
+          float ea[4], eb[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            ea[u] = fexp2_le1(da[u]);
+            eb[u] = fexp2_le1(db[u]);
+          }
+          const unsigned a0 = pack2<PX>(ea[0], ea[1]), a1 = pack2<PX>(ea[2], ea[3]);
+          const unsigned b0 = pack2<PX>(eb[0], eb[1]), b1 = pack2<PX>(eb[2], eb[3]);
+          pdf[R] = dot2_ones<PX>(a0, pdf[R]);
+          pdf[R] = dot2_ones<PX>(a1, pdf[R]);
+          pdf[R] = dot2_ones<PX>(b0, pdf[R]);
+          pdf[R] = dot2_ones<PX>(b1, pdf[R]);
+          w[R] = mfma16(words(a0, a1, b0, b1), BX, w[R], PX);
+        }
+        // Issue order of one pair (3 LDS reads, 12 MFMAs, 32 exps, 32 other
+        // VALU ops per wave): the compiler's own order clusters the MFMAs and
+        // the exps; interleaving 3 exps and 2 VALU ops per MFMA keeps the
+        // matrix core busy under the transcendental work (FP16 score at
+        // n = 2^20: 125 -> 119 ms; other mixes measured 119-123 ms).
+        __builtin_amdgcn_sched_group_barrier(0x100, 3, 0);  // LDS reads
+        __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);  // MFMA
+#pragma unroll
+        for (int gi = 0; gi < 10; ++gi) {
+          __builtin_amdgcn_sched_group_barrier(0x400, 3, 0);  // transcendental
+          __builtin_amdgcn_sched_group_barrier(0x002, 2, 0);  // VALU
+          __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
+        }
+        __builtin_amdgcn_sched_group_barrier(0x002, 64, 0);
       } else {
-        A1 = lds_u16x8(tb);
-      }
-      const u16x8 BX1 = lds_u16x8(tb + BL::kCross * S1);
-      const u16x8 BX2 = SPLIT_PHI ? lds_u16x8(tb + (BL::kCross + 1) * S1) : BX1;
-
-      // Row tile by row tile: cross term, phi, its row sums, phi.X.
+        // split and one-phi forms: tile a, then tile b (one tile's operands
+        // live at a time); the split forms keep phi_h of tile a per row tile
+        // for the shared [phi_h,a | phi_h,b] . [Xl_a | Xl_b] product
+        unsigned ha[QT][2];
 #pragma unroll
-      for (int R = 0; R < QT; ++R) {
-        f4 d;
-        if (X6) {
-          d = mfma16(A1, B1[R], f4{0.f, 0.f, 0.f, 0.f}, P16);
-          d = mfma16(A2, B2[R], d, P16);
-          d = mfma16(A3, B3[R], d, P16);
-          d = mfma16(A4, B4[R], d, P16);
-        } else if (X3) {
-          d = mfma16(A1, B1[R], f4{0.f, 0.f, 0.f, 0.f}, P16);
-          d = mfma16(A2, B2[R], d, P16);
-        } else {
-          d = mfma16(A1, B1[R], f4{0.f, 0.f, 0.f, 0.f}, P16);
-        }
-        // Single FP16 form: the row sums add the FP16 phi that phi.X
-        // multiplies, two at a time: one packed FP16 add of the lane's two
-        // phi pairs, then both halves into the FP32 sum (v_fma_mix_f32),
-        // 3 instructions instead of 4 FP32 adds. (The one-phi forms keep FP32
-        // row sums: there the FP16 phi, subnormal below 2^-14, would raise
-        // the pdf error of an FP32-class form about six-fold.)
-        constexpr bool PAIR_SUM = !SPLIT_PHI && !DUP_PHI && PX == FSK_PREC_F16;
-        float e[4];
+        for (int tt = 0; tt < 2; ++tt) {
+          const unsigned short *tb = pb + tt * KC * S1;
+          u16x8 A[KC];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          e[u] = fexp2_le1(d[u]);
-          if (!PAIR_SUM) pdf[R] += e[u];
+          for (int s = 0; s < KC; ++s) A[s] = lds_u16x8(tb + s * S1);
+          const u16x8 BX = lds_u16x8(pb + (2 * KC + tt) * S1);
+          const u16x8 BXL = SPLIT_PHI && tt == 1 ? lds_u16x8(pb + (2 * KC + 2) * S1) : BX;
+#pragma unroll
+          for (int R = 0; R < QT; ++R) {
+            const f4 d = lp_cross<PREC>(A, B1[R], B2[R], B3[R], B4[R]);
+            float e[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+              e[u] = fexp2_le1(d[u]);
+              pdf[R] += e[u];
+            }
+            if constexpr (SPLIT_PHI) {
+              unsigned hi[2], lo[2];
+              split_phi(e, hi, lo);
+              w[R] = mfma16(words(hi[0], hi[1], lo[0], lo[1]), BX, w[R], PX);
+              if (tt == 0) {
+                ha[R][0] = hi[0];
+                ha[R][1] = hi[1];
+              } else {
+                w[R] = mfma16(words(ha[R][0], ha[R][1], hi[0], hi[1]), BXL, w[R], PX);
+              }
+            } else {
+              const unsigned p0 = pack2<PX>(e[0], e[1]), p1 = pack2<PX>(e[2], e[3]);
+              w[R] = mfma16(words(p0, p1, p0, p1), BX, w[R], PX);
+            }
+          }
         }
-        if constexpr (PAIR_SUM) {
-          const h16x2v sp = __builtin_bit_cast(h16x2v, pack2<PX>(e[0], e[1])) +
-                            __builtin_bit_cast(h16x2v, pack2<PX>(e[2], e[3]));
-          pdf[R] = __builtin_fmaf(float(sp.x), float(one_h.x), pdf[R]);
-          pdf[R] = __builtin_fmaf(float(sp.y), float(one_h.y), pdf[R]);
-        }
-        const u16x8 A = phi_operand<PX, SPLIT_PHI, DUP_PHI>(e);
-        w[R] = mfma16(A, BX1, w[R], PX);
-        if (SPLIT_PHI) w[R] = mfma16(A, BX2, w[R], PX);
       }
-      if (++tc == kLpFlush) {
+      if (++tc == FLUSH) {
         tc = 0;
 #pragma unroll
         for (int R = 0; R < QT; ++R) {
@@ -373,24 +445,24 @@ __global__ void score_lp_finish_kernel(const float *__restrict__ pdf_part,
   if (f == 0) pdf[i] = float(p);
 }
 
-static int lp_tile_u16(unsigned prec) {
+static int lp_pair_u16(unsigned prec) {
   switch (prec) {
-    case FSK_PREC_BF16X6: return LpBlock<FSK_PREC_BF16X6>::kTileU16;
-    case FSK_PREC_BF16X3: return LpBlock<FSK_PREC_BF16X3>::kTileU16;
-    case FSK_PREC_F16X3: return LpBlock<FSK_PREC_F16X3>::kTileU16;
-    case FSK_PREC_F16X3S: return LpBlock<FSK_PREC_F16X3S>::kTileU16;
-    case FSK_PREC_BF16X3S: return LpBlock<FSK_PREC_BF16X3S>::kTileU16;
-    default: return LpBlock<FSK_PREC_BF16>::kTileU16;
+    case FSK_PREC_BF16X6: return LpPair<FSK_PREC_BF16X6>::kPairU16;
+    case FSK_PREC_BF16X3: return LpPair<FSK_PREC_BF16X3>::kPairU16;
+    case FSK_PREC_F16X3: return LpPair<FSK_PREC_F16X3>::kPairU16;
+    case FSK_PREC_F16X3S: return LpPair<FSK_PREC_F16X3S>::kPairU16;
+    case FSK_PREC_BF16X3S: return LpPair<FSK_PREC_BF16X3S>::kPairU16;
+    default: return LpPair<FSK_PREC_BF16>::kPairU16;
   }
 }
 static int lp_chunk(unsigned prec) {
   switch (prec) {
-    case FSK_PREC_BF16X6: return LpBlock<FSK_PREC_BF16X6>::kChunk;
-    case FSK_PREC_BF16X3: return LpBlock<FSK_PREC_BF16X3>::kChunk;
-    case FSK_PREC_F16X3: return LpBlock<FSK_PREC_F16X3>::kChunk;
-    case FSK_PREC_F16X3S: return LpBlock<FSK_PREC_F16X3S>::kChunk;
-    case FSK_PREC_BF16X3S: return LpBlock<FSK_PREC_BF16X3S>::kChunk;
-    default: return LpBlock<FSK_PREC_BF16>::kChunk;
+    case FSK_PREC_BF16X6: return LpPair<FSK_PREC_BF16X6>::kChunk;
+    case FSK_PREC_BF16X3: return LpPair<FSK_PREC_BF16X3>::kChunk;
+    case FSK_PREC_F16X3: return LpPair<FSK_PREC_F16X3>::kChunk;
+    case FSK_PREC_F16X3S: return LpPair<FSK_PREC_F16X3S>::kChunk;
+    case FSK_PREC_BF16X3S: return LpPair<FSK_PREC_BF16X3S>::kChunk;
+    default: return LpPair<FSK_PREC_BF16>::kChunk;
   }
 }
 
@@ -398,19 +470,19 @@ struct ScoreLpLayout {
   double *mu;
   unsigned short *xrec, *qrec, *xt, *blk;
   float *pdf_part, *w_part;
-  int64_t ntiles_data, ntiles_alloc;
+  int64_t npairs_data, npairs_alloc;
 };
 
 static size_t score_lp_layout(void *ws, int64_t n, int64_t m, unsigned prec, ScoreLpLayout *L) {
   Carve cv(ws);
   SplitPlan p = score16_lp_plan(n, m, prec);
-  L->ntiles_data = cdiv(n, kTile);
-  L->ntiles_alloc = L->ntiles_data + lp_chunk(prec);  // whole-chunk copies may read past the end
+  L->npairs_data = cdiv(n, kLpPairPts);
+  L->npairs_alloc = L->npairs_data + lp_chunk(prec);  // whole-chunk copies may read past the end
   L->mu = cv.take<double>(kMeanWs);
   L->xrec = cv.take<unsigned short>(round_up(n, kNormPad) * rec_size(prec));
   L->qrec = cv.take<unsigned short>(round_up(m, kNormPad) * rec_size(prec));
-  L->xt = cv.take<unsigned short>(L->ntiles_data * kTile * kD * 2);
-  L->blk = cv.take<unsigned short>(L->ntiles_alloc * lp_tile_u16(prec));
+  L->xt = cv.take<unsigned short>(L->npairs_data * kLpPairPts * kD * 2);
+  L->blk = cv.take<unsigned short>(L->npairs_alloc * lp_pair_u16(prec));
   L->pdf_part = cv.take<float>(p.splits * m);
   L->w_part = cv.take<float>(p.splits * m * kD);
   return cv.used;
@@ -437,12 +509,12 @@ int score16_lp_run(const char *what, const float *rows_in, int64_t ldr, int64_t 
   const double scl = lowp_prescale(h);
   const double coef = exp2_coef(h) / (scl * scl);
   const int64_t xrows = round_up(n, kNormPad), qrows = round_up(m, kNormPad);
-  const int64_t n16 = L.ntiles_data * kTile;
+  const int64_t n32 = L.npairs_data * kLpPairPts;
   const double inv_n = 1.0 / double(n);
   launch_mean16(data, ld, n, L.mu, s);
   SplitPlan p = score16_lp_plan(n, m, prec);
   const int64_t ntasks = p.qblocks * p.splits;
-  const int64_t split_tiles = p.split_len / kTile;
+  const int64_t split_pairs = p.split_len / kLpPairPts;
   int64_t per_xcd;
   const int64_t grid = xcd_grid(ntasks, &per_xcd);
 #define FSK_LP(P, PXT, QT)                                                                       \
@@ -454,14 +526,14 @@ int score16_lp_run(const char *what, const float *rows_in, int64_t ldr, int64_t 
                        ldr, m, qrows, L.mu, inv_n, scl, coef, 1, (float *)nullptr,                \
                        (float *)nullptr,                                                         \
                        L.qrec);                                                                  \
-    hipLaunchKernelGGL(prep_xt16_kernel<PXT>, dim3(cdiv(n16 * kD, 256)), dim3(256), 0, s, data,  \
-                       ld, n, n16, L.mu, inv_n, scl, L.xt);                                      \
-    hipLaunchKernelGGL(prep_lp_blocks_kernel<P>,                                                 \
-                       dim3(cdiv(L.ntiles_alloc * LpBlock<P>::kSlots * 64, 256)), dim3(256), 0,  \
-                       s, L.xrec, L.xt, L.ntiles_data, L.ntiles_alloc, L.blk);                    \
+    hipLaunchKernelGGL(prep_xt16_kernel<PXT>, dim3(cdiv(n32 * kD, 256)), dim3(256), 0, s, data,  \
+                       ld, n, n32, L.mu, inv_n, scl, L.xt);                                      \
+    hipLaunchKernelGGL(prep_lp_pairs_kernel<P>,                                                  \
+                       dim3(cdiv(L.npairs_alloc * LpPair<P>::kSlots * 64, 256)), dim3(256), 0,   \
+                       s, L.xrec, L.xt, L.npairs_data, L.npairs_alloc, L.blk);                    \
     prof_begin(FSK_OP_SCORE16_ORDERED, s);                                                       \
     hipLaunchKernelGGL((score16_lp_kernel<P, QT>), dim3(grid), dim3(kWG), 0, s, L.blk, L.qrec,   \
-                       m, L.ntiles_data, split_tiles, p.qblocks, ntasks, per_xcd, L.pdf_part,     \
+                       m, L.npairs_data, split_pairs, p.qblocks, ntasks, per_xcd, L.pdf_part,     \
                        L.w_part);                                                                \
     prof_end(FSK_OP_SCORE16_ORDERED, s);                                                         \
   } while (0)

@@ -99,7 +99,9 @@ double fsk_profile_last_ms(int op);
  * roofline the pair kernels are judged against). which: 0 FP32 MFMA flop/s,
  * 1 BF16 MFMA flop/s, 2 v_exp_f32 per s, 3 v_fma_f32 per s, 4 seconds of the
  * MFMA+exp mix kernel, 6 seconds of the same mix with FMAs in place of the
- * exps, 5 HBM copy bytes/s (read + write): the first half of
+ * exps, 7 / 8 seconds of 4 BF16 MFMAs with 16 exps / 16 FMAs per iteration,
+ * 9 seconds of 8 exps with 8 FMAs per iteration,
+ * 5 HBM copy bytes/s (read + write): the first half of
  * `out` copied to its second half by `iters` launches of a 4096-workgroup
  * copy kernel (blocks*512 bytes read and written per launch). `blocks`
  * workgroups of 256 threads, `iters` loop trips; `out` is a 16-byte aligned
