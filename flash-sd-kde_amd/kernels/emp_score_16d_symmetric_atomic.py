@@ -1,9 +1,9 @@
 """Empirical-score accumulators, symmetric tile pairs with atomic scatter.
 
 Drop-in for the reference module kernels/emp_score_16d_symmetric_atomic.py
-(wrapper :163-234). Each unordered 256x256 block pair is evaluated once on the
-GPU; both sides are scattered with FP64 atomics (order not fixed) and rounded
-to FP32 at the end. Outputs are produced by the call (no caller zeroing).
+(wrapper :163-234). Each unordered 512x512 block pair is evaluated once on the
+GPU (a workgroup of four waves holds 512 points of one block); both sides are
+scattered with FP64 atomics (order not fixed) and rounded to FP32 at the end. Outputs are produced by the call (no caller zeroing).
 """
 
 from __future__ import annotations

@@ -16,7 +16,7 @@ def test_microbench_rates_positive(gpu):
     lib = _native.lib()
     out = torch.empty(64 * 256, device=gpu)
     r = ctypes.c_double()
-    for which in (0, 1, 2, 3, 4, 6):
+    for which in (0, 1, 2, 3, 4, 6, 7, 8, 9):
         assert lib.fsk_microbench(which, 64, 10, out.data_ptr(), ctypes.addressof(r), None) == 0
         assert r.value > 0
 
@@ -31,4 +31,5 @@ def test_microbench_copy_moves_first_half(gpu):
     assert lib.fsk_microbench(5, blocks, 2, buf.data_ptr(), ctypes.addressof(r), None) == 0
     assert r.value > 0
     assert torch.equal(buf[half:], buf[:half])
-    assert lib.fsk_microbench(7, blocks, 2, buf.data_ptr(), ctypes.addressof(r), None) != 0
+    # modes past the last one are refused before anything is launched
+    assert lib.fsk_microbench(10, blocks, 2, buf.data_ptr(), ctypes.addressof(r), None) != 0
