@@ -125,19 +125,19 @@ def sklearn_baseline(x, q, h, budget_s):
 
 
 # HBM bytes per launch of the dominant kernel, from two rocprofv3 --pmc runs of
-# `bench.py --steps 1 --warmup 0 [--variant ordered] --no-cpu-baseline --no-batched
-# --no-precision-forms --no-surrogate` (FETCH_SIZE, then WRITE_SIZE, KiB per
-# dispatch), each scaled by what the same counter reads for the library's copy
-# microbenchmark in the same run (512 MiB read and 512 MiB written per
-# dispatch: FETCH_SIZE reads 262169 KiB -> scale 2.0, WRITE_SIZE reads
+# `bench.py --steps 1 --warmup 0 --no-cpu-baseline [--variant ordered] --no-batched
+# --no-precision-forms --no-surrogate --no-one-d` (FETCH_SIZE, then WRITE_SIZE, KiB
+# per dispatch), each scaled by what the same counter reads for the library's
+# copy microbenchmark in the same run (512 MiB read and 512 MiB written per
+# dispatch: FETCH_SIZE reads 262168 KiB -> scale 2.0, WRITE_SIZE reads
 # 524288 KiB -> scale 1.0). Keyed by (score kernel, n); other sizes report null.
 # The symmetric kernel's figure is dominated by its FP64 device-scope atomic
 # scatters, which are performed past the L2 (8 bytes each).
 PMC_TRAFFIC = {
-    ("ordered", 1 << 20): {"fetch_kib": 2355570.0, "write_kib": 278528.0,
-                           "fetch_scale": 524288.0 / 262169.0, "write_scale": 1.0},
-    ("symmetric", 1 << 20): {"fetch_kib": 8815652.0, "write_kib": 160432128.0,
-                             "fetch_scale": 524288.0 / 262169.0, "write_scale": 1.0},
+    ("ordered", 1 << 20): {"fetch_kib": 2354213.0, "write_kib": 278528.0,
+                           "fetch_scale": 524288.0 / 262168.0, "write_scale": 1.0},
+    ("symmetric", 1 << 20): {"fetch_kib": 8820297.0, "write_kib": 160432128.0,
+                             "fetch_scale": 524288.0 / 262168.0, "write_scale": 1.0},
 }
 
 
