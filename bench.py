@@ -263,7 +263,14 @@ def surrogate_form(fp32_pair_rate):
     xd = torch.as_tensor(x, device="cuda")
     qd = torch.as_tensor(q, device="cuda")
     n = xd.shape[0]
+    # one untimed run first, as for every other section: the first call in a
+    # process also loads the MLP kernels and instantiates the training graphs
+    # (the exact score kernel it is compared with is already warm)
     torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    surrogate_fit_transform(xd, h, tol=1e-3)
+    torch.cuda.synchronize()
+    t_first = time.perf_counter() - t0
     t0 = time.perf_counter()
     deb_s, info = surrogate_fit_transform(xd, h, tol=1e-3)
     torch.cuda.synchronize()
@@ -285,6 +292,7 @@ def surrogate_form(fp32_pair_rate):
         "bandwidth": h,
         "exact_debias_s": round(t_exact, 3),
         "surrogate_debias_s": round(t_sur, 3),
+        "surrogate_first_call_s": round(t_first, 3),
         "speedup_vs_exact": round(t_exact / t_sur, 3),
         "label_s": round(info["label_s"], 3), "train_s": round(info["train_s"], 3),
         "check_s": round(info["check_s"], 3), "transform_s": round(info["transform_s"], 3),

@@ -379,15 +379,38 @@ __device__ __forceinline__ void wgrad_stage(unsigned short *sT, int b, int q, co
   }
 }
 
+// BF16 layer inputs: 4 rows x 4 values (8 bytes each) of a batch chunk.
+__device__ __forceinline__ void wgrad_load_u16(const unsigned short *__restrict__ p, int64_t ld,
+                                               int64_t m, int64_t me, bool col_ok,
+                                               u16x4 (&v)[4]) {
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const bool ok = col_ok && m + j < me;
+    const u16x4 x = *reinterpret_cast<const u16x4 *>(p + (m + j < me ? m + j : me - 1) * ld);
+    v[j] = ok ? x : u16x4{0, 0, 0, 0};
+  }
+}
+
+__device__ __forceinline__ void wgrad_stage_u16(unsigned short *sT, int b, int q,
+                                                const u16x4 (&v)[4]) {
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const u16x4 w = {v[0][i], v[1][i], v[2][i], v[3][i]};
+    *reinterpret_cast<u16x4 *>(sT + (4 * b + i) * WG_LDS_STRIDE + 4 * q) = w;
+  }
+}
+
 // LDS of one weight-gradient task: dZ^T and X^T chunks and the bias partials.
 constexpr int WG_LDS_U16 = 2 * 64 * WG_LDS_STRIDE + 2 * 16 * 64;
 static_assert(WG_LDS_U16 <= FK_LDS_U16, "the fused backward kernel shares one LDS buffer");
 
-// Weight gradient of the (n tile nb, k tile kb, batch split s) task.
+// Weight gradient of the (n tile nb, k tile kb, batch split s) task; X is
+// FP32 (rounded to BF16 as it is staged) or, with XB, already BF16.
+template <bool XB = false>
 __device__ __forceinline__ void wgrad_body(int nb, int kb, int64_t s,
                                            unsigned short *__restrict__ lds,
                                            const float *__restrict__ dZ, int64_t M, int N,
-                                           const float *__restrict__ X, int64_t ldx, int K,
+                                           const void *__restrict__ Xv, int64_t ldx, int K,
                                            int64_t split_len, float *__restrict__ dW_part,
                                            float *__restrict__ db_part) {
   unsigned short *sA = lds;                           // dZ^T chunk [n][m]
@@ -402,7 +425,8 @@ __device__ __forceinline__ void wgrad_body(int nb, int kb, int64_t s,
   const int64_t me = (mb + split_len < M) ? mb + split_len : M;
   const bool a_ok = n0 + 4 * b < N, b_ok = k0 + 4 * b < K;  // N, K multiples of 8
   const float *pa = dZ + (a_ok ? n0 + 4 * b : 0);
-  const float *pb = X + (b_ok ? k0 + 4 * b : 0);
+  const float *pb = static_cast<const float *>(Xv) + (b_ok ? k0 + 4 * b : 0);
+  const unsigned short *pbh = static_cast<const unsigned short *>(Xv) + (b_ok ? k0 + 4 * b : 0);
   f4 acc[2][2];
 #pragma unroll
   for (int x = 0; x < 2; ++x)
@@ -410,18 +434,28 @@ __device__ __forceinline__ void wgrad_body(int nb, int kb, int64_t s,
     for (int y = 0; y < 2; ++y) acc[x][y] = f4{0.f, 0.f, 0.f, 0.f};
   f4 bsum = {0.f, 0.f, 0.f, 0.f};
   f4 va[4], vb[4];
+  u16x4 vh[4];
   wgrad_load(pa, N, mb + 4 * q, me, a_ok, va);
-  wgrad_load(pb, ldx, mb + 4 * q, me, b_ok, vb);
+  if (XB)
+    wgrad_load_u16(pbh, ldx, mb + 4 * q, me, b_ok, vh);
+  else
+    wgrad_load(pb, ldx, mb + 4 * q, me, b_ok, vb);
   for (int64_t m0 = mb; m0 < me; m0 += 64) {
 #pragma unroll
     for (int j = 0; j < 4; ++j) bsum += va[j];
     __syncthreads();  // previous chunk's LDS reads are done
     wgrad_stage(sA, b, q, va);
-    wgrad_stage(sB, b, q, vb);
+    if (XB)
+      wgrad_stage_u16(sB, b, q, vh);
+    else
+      wgrad_stage(sB, b, q, vb);
     __syncthreads();
     if (m0 + 64 < me) {
       wgrad_load(pa, N, m0 + 64 + 4 * q, me, a_ok, va);
-      wgrad_load(pb, ldx, m0 + 64 + 4 * q, me, b_ok, vb);
+      if (XB)
+        wgrad_load_u16(pbh, ldx, m0 + 64 + 4 * q, me, b_ok, vh);
+      else
+        wgrad_load(pb, ldx, m0 + 64 + 4 * q, me, b_ok, vb);
     }
 #pragma unroll
     for (int kk = 0; kk < 64; kk += 32) {
@@ -521,8 +555,13 @@ __global__ __launch_bounds__(256) void mlp_wgrad_multi_kernel(WgradBatch wb) {
   const int64_t t = int64_t(blockIdx.x) - wb.start[k];
   const int nb = int(t % wb.ntn[k]);
   const int kb = int((t / wb.ntn[k]) % wb.ntk[k]);
-  wgrad_body(nb, kb, t / (int64_t(wb.ntn[k]) * wb.ntk[k]), lds, L.dz, wb.M, L.N, L.x, L.ldx, L.K,
-             wb.split_len[k], L.dw_part, L.db_part);
+  const int64_t s = t / (int64_t(wb.ntn[k]) * wb.ntk[k]);
+  if (L.x_bf16)  // uniform over the workgroup
+    wgrad_body<true>(nb, kb, s, lds, L.dz, wb.M, L.N, L.x, L.ldx, L.K, wb.split_len[k], L.dw_part,
+                     L.db_part);
+  else
+    wgrad_body<false>(nb, kb, s, lds, L.dz, wb.M, L.N, L.x, L.ldx, L.K, wb.split_len[k],
+                      L.dw_part, L.db_part);
 }
 
 __global__ void mlp_adam_kernel(float *__restrict__ w, float *__restrict__ mom,
@@ -785,7 +824,7 @@ extern "C" int fsk_mlp_wgrad_multi(const fsk_mlp_wgrad_layer *layers, int n_laye
     int st = mlp_dims_ok(what, M, L.K, L.N);
     if (st != FSK_OK) return st;
     if (!L.dz || !L.x || !L.dw_part || !L.db_part || L.splits <= 0 || L.splits > 1024 ||
-        L.ldx < L.K || (L.ldx & 3) || (reinterpret_cast<uintptr_t>(L.x) & 15) ||
+        L.ldx < L.K || (L.ldx & 3) || (reinterpret_cast<uintptr_t>(L.x) & (L.x_bf16 ? 7 : 15)) ||
         (reinterpret_cast<uintptr_t>(L.dz) & 15)) {
       set_error("%s: layer %d: bad arguments", what, k);
       return FSK_EINVAL;

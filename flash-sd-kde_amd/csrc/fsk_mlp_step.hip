@@ -8,8 +8,10 @@
 // through all seven GEMMs (4 forward, 3 data-gradient) with the activations
 // and pre-activations held in LDS (about 131 KB), reading only the BF16
 // weights (from L2) between stages. It writes what the weight gradients need
-// - the gathered inputs, the activations h1..h3 and the output gradients
-// dz1..dz4, FP32 - and the step then finishes with one multi-layer
+// - the gathered inputs (FP32), the activations h1..h3 as the BF16 values
+// the next layer multiplies (half the bytes of FP32; the weight-gradient GEMM
+// rounds its inputs to BF16 anyway) and the output gradients dz1..dz4 (FP32:
+// the bias gradients sum them) - and the step then finishes with one multi-layer
 // weight-gradient launch and the Adam launch: three launches per step where
 // the layer-by-layer form needs ten, each of which costs several
 // microseconds of launch and drain on its own.
@@ -25,7 +27,10 @@
 
 namespace fsk {
 
-constexpr int RS_ROWS = 32;              // batch rows per workgroup
+// batch rows per workgroup: 32 (one workgroup per CU, LDS-bound) trained the
+// learned-score configuration in 0.576 s, 16 (two per CU) in 0.62 s
+constexpr int RS_ROWS = 32;
+constexpr int RS_RX = RS_ROWS / 16;      // 16-row MFMA tiles per workgroup
 constexpr int RS_H = 256;                // hidden width
 constexpr int RS_D = 16;                 // input / output width
 constexpr int RS_ACT = RS_H + 8;         // u16 per LDS row of a BF16 activation
@@ -49,12 +54,12 @@ __device__ __forceinline__ float rs_silu_grad(float z) {
 template <int K, int N>
 __device__ __forceinline__ bool rs_gemm(const unsigned short *A, int lda,
                                         const unsigned short *__restrict__ W, int wave, int g,
-                                        int c, f4 (&acc)[2][N >= 64 ? N / 64 : 1], int &col0) {
+                                        int c, f4 (&acc)[RS_RX][N >= 64 ? N / 64 : 1], int &col0) {
   constexpr int NB = N >= 64 ? N / 64 : 1;
   constexpr int KS = (K + 31) / 32;
   col0 = wave * NB * 16;
 #pragma unroll
-  for (int x = 0; x < 2; ++x)
+  for (int x = 0; x < RS_RX; ++x)
 #pragma unroll
     for (int y = 0; y < NB; ++y) acc[x][y] = f4{0.f, 0.f, 0.f, 0.f};
   if (col0 >= N) return false;  // wave-uniform
@@ -62,15 +67,15 @@ __device__ __forceinline__ bool rs_gemm(const unsigned short *A, int lda,
 #pragma unroll
   for (int ks = 0; ks < KS; ++ks) {
     const int k = 32 * ks + 8 * g;
-    u16x8 a[2], b[NB];
+    u16x8 a[RS_RX], b[NB];
 #pragma unroll
-    for (int x = 0; x < 2; ++x)
+    for (int x = 0; x < RS_RX; ++x)
       a[x] = k < K ? *reinterpret_cast<const u16x8 *>(A + (16 * x + c) * lda + k) : zero8;
 #pragma unroll
     for (int y = 0; y < NB; ++y)
       b[y] = k < K ? *reinterpret_cast<const u16x8 *>(W + (col0 + 16 * y + c) * K + k) : zero8;
 #pragma unroll
-    for (int x = 0; x < 2; ++x)
+    for (int x = 0; x < RS_RX; ++x)
 #pragma unroll
       for (int y = 0; y < NB; ++y)
         acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
@@ -92,7 +97,7 @@ struct RowsStepArgs {
   const unsigned short *wt[4];  // data-gradient operands, K x N
   const float *bias[4];
   float *xb;      // batch x 16
-  float *h[3];    // batch x H, layers 1..3
+  unsigned short *h[3];  // batch x H, layers 1..3, BF16
   float *dz[4];   // batch x H (layers 1..3), batch x 16 (layer 4)
   float mse_scale;
   double *loss;
@@ -111,9 +116,11 @@ __global__ __launch_bounds__(256) void mlp_rows_step_kernel(RowsStepArgs a) {
   step = step < a.steps ? step : a.steps - 1;
   if (blockIdx.x == 0 && tid < 3) a.sched[tid] = a.table[step * 3 + tid];
 
-  // ---- gather: threads 0..127 the inputs, 128..255 the targets (one float4 each)
+  // ---- gather: the first 4 RS_ROWS threads the inputs, the next 4 RS_ROWS the
+  // targets (one float4 each)
   {
-    const int half = tid >> 7, e = tid & 127, row = e >> 2, q = e & 3;
+    constexpr int PER = RS_ROWS * 4;  // one float4 per thread and half
+    const int half = tid / PER, e = tid % PER, row = e >> 2, q = e & 3;
     const int64_t br = r0 + row < a.batch ? r0 + row : a.batch - 1;
     const int64_t j = a.idx[step * a.batch + br];
     if (half == 0) {
@@ -123,7 +130,7 @@ __global__ __launch_bounds__(256) void mlp_rows_step_kernel(RowsStepArgs a) {
 #pragma unroll
       for (int i = 0; i < 4; ++i) w[i] = to16<FSK_PREC_BF16>(v[i]);
       *reinterpret_cast<u16x4 *>(small + row * RS_SMALL + 4 * q) = w;
-    } else {
+    } else if (half == 1) {
       *reinterpret_cast<f4 *>(ts + row * RS_D + 4 * q) = ld4(a.t + j * RS_D + 4 * q);
     }
   }
@@ -133,21 +140,21 @@ __global__ __launch_bounds__(256) void mlp_rows_step_kernel(RowsStepArgs a) {
   // ---- forward, hidden layers: z_l = A W_l^T + b_l, h_l = silu(z_l)
   auto hidden = [&](const unsigned short *A, int lda, int l, unsigned short *out, auto kdim) {
     constexpr int K = decltype(kdim)::value;
-    f4 acc[2][4];
+    f4 acc[RS_RX][4];
     if (rs_gemm<K, RS_H>(A, lda, a.w[l], wave, g, c, acc, col0)) {
 #pragma unroll
       for (int y = 0; y < 4; ++y) {
         const int nn = col0 + 16 * y + c;
         const float bv = a.bias[l][nn];
 #pragma unroll
-        for (int x = 0; x < 2; ++x)
+        for (int x = 0; x < RS_RX; ++x)
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
             const int row = 16 * x + 4 * g + r;
             const float z = acc[x][y][r] + bv;
-            const float hv = rs_silu(z);
+            const unsigned short hv = to16<FSK_PREC_BF16>(rs_silu(z));
             zs[l][row * RS_Z + nn] = z;
-            out[row * RS_ACT + nn] = to16<FSK_PREC_BF16>(hv);
+            out[row * RS_ACT + nn] = hv;
             if (r0 + row < a.batch) a.h[l][(r0 + row) * RS_H + nn] = hv;
           }
       }
@@ -160,13 +167,13 @@ __global__ __launch_bounds__(256) void mlp_rows_step_kernel(RowsStepArgs a) {
 
   // ---- output layer with the MSE gradient: d = z - t, dz4 = scale d
   {
-    f4 acc[2][1];
+    f4 acc[RS_RX][1];
     double lsum = 0.0;
     if (rs_gemm<RS_H, RS_D>(act[0], RS_ACT, a.w[3], wave, g, c, acc, col0)) {
       const int nn = c;
       const float bv = a.bias[3][nn];
 #pragma unroll
-      for (int x = 0; x < 2; ++x)
+      for (int x = 0; x < RS_RX; ++x)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const int row = 16 * x + 4 * g + r;
@@ -189,13 +196,13 @@ __global__ __launch_bounds__(256) void mlp_rows_step_kernel(RowsStepArgs a) {
   // ---- data gradients: dz_{l-1} = (dz_l W_l) * silu'(z_{l-1}), l = 4, 3, 2
   auto backward = [&](const unsigned short *A, int lda, int l, unsigned short *out, auto kdim) {
     constexpr int K = decltype(kdim)::value;  // width of dz_l (the reduction)
-    f4 acc[2][4];
+    f4 acc[RS_RX][4];
     if (rs_gemm<K, RS_H>(A, lda, a.wt[l], wave, g, c, acc, col0)) {
 #pragma unroll
       for (int y = 0; y < 4; ++y) {
         const int nn = col0 + 16 * y + c;
 #pragma unroll
-        for (int x = 0; x < 2; ++x)
+        for (int x = 0; x < RS_RX; ++x)
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
             const int row = 16 * x + 4 * g + r;

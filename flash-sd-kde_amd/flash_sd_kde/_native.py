@@ -134,7 +134,7 @@ class WgradLayer(ctypes.Structure):
     """fsk_mlp_wgrad_layer of include/flashsdkde.h."""
     _fields_ = [("dz", ctypes.c_void_p), ("x", ctypes.c_void_p), ("ldx", ctypes.c_int64),
                 ("N", ctypes.c_int), ("K", ctypes.c_int), ("splits", ctypes.c_int),
-                ("reserved", ctypes.c_int), ("dw_part", ctypes.c_void_p),
+                ("x_bf16", ctypes.c_int), ("dw_part", ctypes.c_void_p),
                 ("db_part", ctypes.c_void_p)]
 
 

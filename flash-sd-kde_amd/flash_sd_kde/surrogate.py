@@ -314,7 +314,9 @@ class ScoreMLP:
         H = self.dims[1]
         f32 = torch.float32
         bufs = {"xb": torch.empty((batch, 16), dtype=f32, device=dev),
-                "h": [torch.empty((batch, H), dtype=f32, device=dev) for _ in range(3)],
+                # the activations as BF16: what the next layer and the weight
+                # gradients multiply (the step kernel stores them rounded)
+                "h": [torch.empty((batch, H), dtype=torch.bfloat16, device=dev) for _ in range(3)],
                 "dz": [torch.empty((batch, H), dtype=f32, device=dev) for _ in range(3)]
                 + [torch.empty((batch, 16), dtype=f32, device=dev)]}
         grads = []
@@ -326,7 +328,8 @@ class ScoreMLP:
             dbp = torch.empty((S, L["N"]), dtype=f32, device=dev)
             grads.append((dWp, dbp))
             wl[l] = nat.WgradLayer(bufs["dz"][l].data_ptr(), inputs[l].data_ptr(),
-                                   inputs[l].stride(0), L["N"], L["K"], S, 0, dWp.data_ptr(),
+                                   inputs[l].stride(0), L["N"], L["K"], S,
+                                   int(inputs[l].dtype == torch.bfloat16), dWp.data_ptr(),
                                    dbp.data_ptr())
         a = nat.RowsStepArgs()
         a.x, a.t, a.n = xn.data_ptr(), tn.data_ptr(), xn.shape[0]

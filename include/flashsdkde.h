@@ -335,11 +335,14 @@ int fsk_mlp_linear_bwd(const float *dz, int64_t M, int N, const float *x, int64_
                        int splits, float *dw_part, float *db_part, const uint16_t *wt_bf16,
                        const float *z_prev, float *dx, void *stream);
 /* Weight gradients of several layers in one launch: fsk_mlp_linear_bwd_weight
- * for each entry (M batch rows shared by all). At most 4 layers. */
+ * for each entry (M batch rows shared by all). At most 4 layers. The layer
+ * inputs are FP32 (rounded to BF16 as they are staged), or already BF16
+ * (uint16) when x_bf16 is nonzero; the products are the same either way. */
 typedef struct {
-  const float *dz, *x; /* M x N output gradients, M x K layer inputs (row stride ldx) */
+  const float *dz; /* M x N output gradients */
+  const void *x;   /* M x K layer inputs, row stride ldx (elements) */
   int64_t ldx;
-  int N, K, splits, reserved;
+  int N, K, splits, x_bf16;
   float *dw_part, *db_part;
 } fsk_mlp_wgrad_layer;
 int fsk_mlp_wgrad_multi(const fsk_mlp_wgrad_layer *layers, int n_layers, int64_t M,
@@ -349,10 +352,12 @@ int fsk_mlp_wgrad_multi(const fsk_mlp_wgrad_layer *layers, int n_layers, int64_t
  * x[idx[c*batch + r]] and t[...] (xb keeps the inputs), copy table[3c..3c+2]
  * to sched, run the four forward layers (BF16 operands, FP32 accumulate,
  * SiLU), the MSE gradient dz4 = mse_scale (y - t) (loss += sum (y - t)^2) and
- * the data gradients dz_{l-1} = (dz_l W_l) silu'(z_{l-1}), writing the
- * activations h1..h3 and gradients dz1..dz4 (FP32, row-major) that the
- * weight gradients need. Values equal those of the layer-by-layer entry
- * points. Finish the step with fsk_mlp_wgrad_multi and fsk_mlp_adam_multi_step. */
+ * the data gradients dz_{l-1} = (dz_l W_l) silu'(z_{l-1}), writing what the
+ * weight gradients need: the activations h1..h3 as BF16 (the values the next
+ * layer multiplies) and the gradients dz1..dz4 in FP32 (the bias gradients
+ * sum them), row-major. Values equal those of the layer-by-layer entry points
+ * (h rounded to BF16). Finish the step with fsk_mlp_wgrad_multi (x_bf16 = 1
+ * for the h inputs) and fsk_mlp_adam_multi_step. */
 typedef struct {
   const float *x, *t; /* normalised training inputs / targets, n x 16 */
   int64_t n;
@@ -365,7 +370,7 @@ typedef struct {
   const uint16_t *w_bf16[4], *wt_bf16[4];
   const float *bias[4];
   float *xb;    /* batch x 16 */
-  float *h[3];  /* batch x 256 */
+  uint16_t *h[3]; /* batch x 256, BF16 */
   float *dz[4]; /* batch x 256 (layers 1..3), batch x 16 (layer 4) */
   double mse_scale;
   double *loss; /* nullable */

@@ -254,6 +254,44 @@ def test_fused_backward_equals_separate_launches(gpu, layer, M, splits):
         assert torch.equal(a, b)
 
 
+@pytest.mark.parametrize("M", [8192, 300, 1])
+def test_wgrad_multi_bf16_inputs_equal_fp32_inputs(gpu, M):
+    """fsk_mlp_wgrad_multi with BF16 layer inputs (x_bf16 = 1, the step
+    kernel's activations) gives the partials of the same inputs passed as FP32
+    (rounded to BF16 inside) bit for bit, for every layer shape at once,
+    including ragged batches; a misaligned BF16 row stride is refused."""
+    import ctypes
+
+    from flash_sd_kde import _native as nat
+
+    net = ScoreMLP(seed=5, device=gpu)
+    g = torch.Generator(device=gpu).manual_seed(M)
+    dzs = [torch.randn(M, L["N"], device=gpu, generator=g) for L in net.layers]
+    xs = [torch.randn(M, L["K"], device=gpu, generator=g) for L in net.layers]
+    outs = []
+    for bf in (0, 1):
+        wl = (nat.WgradLayer * 4)()
+        keep = []
+        for l, L in enumerate(net.layers):
+            S = 3 if M > 1 else 1
+            dW = torch.full((S, L["N"], L["K"]), float("nan"), device=gpu)
+            db = torch.full((S, L["N"]), float("nan"), device=gpu)
+            xin = xs[l].to(torch.bfloat16) if bf else xs[l]
+            keep.append((dW, db, xin))
+            wl[l] = nat.WgradLayer(dzs[l].data_ptr(), xin.data_ptr(), L["K"], L["N"], L["K"], S, bf,
+                                   dW.data_ptr(), db.data_ptr())
+        nat.check(nat.lib().fsk_mlp_wgrad_multi(ctypes.byref(wl), 4, M, None), "wgrad_multi")
+        torch.cuda.synchronize(gpu)
+        outs.append([(dW, db) for dW, db, _ in keep])
+    for (a, b), (c, d) in zip(*outs):
+        assert torch.equal(a, c) and torch.equal(b, d)
+    bad = (nat.WgradLayer * 1)()
+    xin = xs[1].to(torch.bfloat16)
+    bad[0] = nat.WgradLayer(dzs[1].data_ptr(), xin.data_ptr() + 2, 256, 256, 256, 1, 1,
+                            keep[1][0].data_ptr(), keep[1][1].data_ptr())
+    assert nat.lib().fsk_mlp_wgrad_multi(ctypes.byref(bad), 1, M, None) == nat.FSK_EINVAL
+
+
 @pytest.mark.parametrize("batch,graph", [(300, False), (8192, True)])
 def test_fused_training_step_equals_layered(gpu, batch, graph):
     """Training with the three-launch step (row-local kernel, multi-layer
