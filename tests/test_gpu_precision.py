@@ -59,9 +59,13 @@ def test_score_precision_forms(gpu, golden, prec, tol_pdf, tol_w):
 
 
 @pytest.mark.parametrize("prec,tol", [("bf16x3", 1e-3), ("bf16x6", 5e-5), ("fp16x3", 5e-5),
-                                      ("fp16x3s", 1e-3), ("bf16x3s", 1e-3)])
+                                      ("fp16x3s", 1e-3), ("bf16x3s", 1e-3), ("fp16", 5e-2),
+                                      ("bf16", 0.5)])
 def test_score_lp_ragged_and_transposed_layout(gpu, prec, tol):
-    for n in (1, 17, 300, 1025):
+    """The 16-bit kernels take data tiles in pairs: n = 1 and 17 end inside
+    the first pair, 300 and 1025 inside a later one, and 5000 runs three data
+    splits whose last one ends inside a pair."""
+    for n in (1, 17, 300, 1025, 5000):
         x = np.random.default_rng(n).normal(size=(n, 16)).astype(np.float32) + 3.0
         pdf, w = _ops.score16(torch.as_tensor(x, device=gpu), 1.1, variant="ordered",
                               precision=prec, transposed_w=True)
