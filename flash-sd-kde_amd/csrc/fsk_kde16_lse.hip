@@ -5,11 +5,14 @@
 // translation invariant; centring shrinks |x|^2 and with it the cancellation
 // in |q|^2 + |x|^2 - 2 q.x). In base 2, with c = -log2(e)/(2h^2),
 //     t_ij = c|q_i|^2 + c|x_j|^2 - 2c q_i.x_j  (<= 0 up to rounding)
-// and each (query, split) keeps a running maximum m_i (shared by the four
-// lanes that hold query i) and s_i = sum_j exp2(min(t_ij, 0) - m_i). The
-// maximum is found exactly on the first tile of the split and only ever
-// raised afterwards (rare, wave-uniform branch). Partials (m, s) are combined
-// in FP64: log p_i = ln2 (M + log2 sum_s s_s 2^(m_s - M)) + log_const.
+// and each (query, split) keeps a shift m_i (shared by the four lanes that
+// hold query i) and s_i = sum_j exp2(min(t_ij, 0) - m_i). The shift is the
+// exact maximum of the split's first tile, raised afterwards whenever a term
+// exceeds it (wave-uniform branch); a query whose first-tile maximum is above
+// -100 keeps m_i = 0 (no term can overflow, none that matters underflows),
+// and a wave whose queries all do runs a loop with no maximum tracking at
+// all: one clamped exp and one add per pair. Partials (m, s) are combined in
+// FP64: log p_i = ln2 (M + log2 sum_s s_s 2^(m_s - M)) + log_const.
 //
 // 16-bit forms use v_mfma_f32_16x16x32_{bf16,f16} on 64-value records
 // (fsk_lowp.h): x = x1 + x2 + x3, q' = -2c q = q1 + q2 + q3 in 16-bit pieces,
@@ -37,6 +40,10 @@
 namespace fsk {
 
 constexpr float kLseSlack = 8.0f;  // raise m only when a term exceeds 2^8 * 2^m
+// A split whose first tile already holds a term above 2^-kLseDirect sums
+// exp2(t) unshifted (m = 0): no term can overflow (t <= 0) and the sum keeps
+// full relative precision down to its largest term.
+constexpr float kLseDirect = 100.0f;
 constexpr int kLseFlush = 32;
 
 // Column sums of the data in FP64, deterministic: block b writes its 16 sums
@@ -90,8 +97,14 @@ __device__ __forceinline__ float gmax(float v) {
   return v;
 }
 
+// Waves per SIMD the compiler must leave room for: 4 (at most 128 VGPRs), 3
+// for BF16X6 (its four query operands per row tile).
+template <int PREC>
+constexpr int lse_min_waves() { return PREC == FSK_PREC_BF16X6 ? 3 : 4; }
+
 template <int PREC, int QT>
-__global__ __launch_bounds__(kWG) void kde16_lse_mfma_kernel(
+__global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(lse_min_waves<PREC>())))
+void kde16_lse_mfma_kernel(
     const float *__restrict__ xf, const float *__restrict__ bn,
     const unsigned short *__restrict__ xrec, int64_t n_pad, const float *__restrict__ qf,
     const float *__restrict__ qa, const unsigned short *__restrict__ qrec, int64_t m,
@@ -220,52 +233,76 @@ __global__ __launch_bounds__(kWG) void kde16_lse_mfma_kernel(
   if (j_end > n_pad) j_end = n_pad;
 
   f4 d[QT];
-  // First tile: exact per-query maximum of this split's first 16 points.
+  // First tile: exact per-query maximum of this split's first 16 points; a
+  // query whose maximum is above -kLseDirect keeps m = 0.
   tile(j_begin, d);
+  bool direct = true;
 #pragma unroll
   for (int R = 0; R < QT; ++R) {
     float mx = fminf(gmax(max4(d[R])), 0.f);
+    if (mx > -kLseDirect) mx = 0.f;
+    direct = direct && mx == 0.f;
     set_m(R, mx);
     const float dm = float(mrec[R]);
 #pragma unroll
     for (int r = 0; r < 4; ++r) s[R] += fexp2(fminf(d[R][r] - dm, negm[R]));
   }
   int tc = 1;
-  for (int64_t j0 = j_begin + kTile; j0 < j_end; j0 += kTile) {
-    tile(j0, d);
-    bool hi = false;
-#pragma unroll
-    for (int R = 0; R < QT; ++R) hi |= max4(d[R]) > kLseSlack;
-    if (__any(hi)) {
+  if (__all(direct)) {
+    // every query of the wave at m = 0: per pair one clamped exp (exp2 of
+    // min(t, 0) as v_exp ... clamp) and one add; no maximum to track
+    for (int64_t j0 = j_begin + kTile; j0 < j_end; j0 += kTile) {
+      tile(j0, d);
 #pragma unroll
       for (int R = 0; R < QT; ++R) {
-        float up = gmax(max4(d[R]));
-        if (up > 0.f) {
-          // raise m by `up` (bounded by the clamp t <= 0)
-          float mnew = fminf(float(mrec[R]) + up, 0.f);
-          double delta = double(mnew) - mrec[R];
-          float old = float(mrec[R]);
-          set_m(R, mnew);
-          const float sc = fexp2(float(-(mrec[R] - double(old))));
-          s[R] *= sc;
-          s64[R] *= double(sc);
-          const float dd = float(mrec[R] - double(old));
-          d[R] -= dd;
-          (void)delta;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) s[R] += fexp2_le1(d[R][r]);
+      }
+      if (++tc == kLseFlush) {
+        tc = 0;
+#pragma unroll
+        for (int R = 0; R < QT; ++R) {
+          s64[R] += s[R];
+          s[R] = 0.f;
         }
       }
     }
+  } else {
+    // some query of the wave keeps a shift: the running maximum as above
+    for (int64_t j0 = j_begin + kTile; j0 < j_end; j0 += kTile) {
+      tile(j0, d);
+      bool hi = false;
 #pragma unroll
-    for (int R = 0; R < QT; ++R) {
+      for (int R = 0; R < QT; ++R) hi |= max4(d[R]) > kLseSlack;
+      if (__any(hi)) {
 #pragma unroll
-      for (int r = 0; r < 4; ++r) s[R] += fexp2(fminf(d[R][r], negm[R]));
-    }
-    if (++tc == kLseFlush) {
-      tc = 0;
+        for (int R = 0; R < QT; ++R) {
+          float up = gmax(max4(d[R]));
+          if (up > 0.f) {
+            // raise m by `up` (bounded by the clamp t <= 0)
+            float mnew = fminf(float(mrec[R]) + up, 0.f);
+            float old = float(mrec[R]);
+            set_m(R, mnew);
+            const float sc = fexp2(float(-(mrec[R] - double(old))));
+            s[R] *= sc;
+            s64[R] *= double(sc);
+            const float dd = float(mrec[R] - double(old));
+            d[R] -= dd;
+          }
+        }
+      }
 #pragma unroll
       for (int R = 0; R < QT; ++R) {
-        s64[R] += s[R];
-        s[R] = 0.f;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) s[R] += fexp2(fminf(d[R][r], negm[R]));
+      }
+      if (++tc == kLseFlush) {
+        tc = 0;
+#pragma unroll
+        for (int R = 0; R < QT; ++R) {
+          s64[R] += s[R];
+          s[R] = 0.f;
+        }
       }
     }
   }
