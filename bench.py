@@ -223,19 +223,23 @@ def batched_loo(rates, steps):
     ms = (time.perf_counter() - t0) / steps * 1e3
     pairs = grid.pair_count()
     exps = pairs * 32
+    done = grid.exps_evaluated()
     exp_rate = rates.get("exp_per_s") or 17e12
     fma_rate = rates.get("fma_per_s") or 54e12
-    # per pair and bandwidth: one exp plus one multiply and one add on the VALU
-    t_roof = exps / exp_rate + 2 * exps / fma_rate
+    # per evaluated exp: one multiply and one add on the VALU beside it
+    t_roof = done / exp_rate + 2 * done / fma_rate
     ll = grid.run().cpu().numpy()
     return {
         "workload": f"16 datasets, sum n = {sum(sizes)}, 32 bandwidths, LOO",
         "ms_per_launch": round(ms, 3),
         "gpair_evals_per_s": round(pairs / ms / 1e6, 3),
         "gexp_per_s": round(exps / ms / 1e6, 3),
+        "exps_evaluated_frac": round(done / exps, 4),
+        "gexp_evaluated_per_s": round(done / ms / 1e6, 3),
         "exp_valu_frac": round(t_roof / (ms * 1e-3), 4),
         "tune": list(grid.tune),
-        "tune_ms": {f"{w}x{g}": round(t, 3) for (w, g), t in getattr(grid, "tune_times", {}).items()},
+        "tune_ms": {f"{w}w{rt}r{g}g": round(t, 3)
+                    for (w, rt, g), t in getattr(grid, "tune_times", {}).items()},
         "setup_s": round(setup_s, 2),
         "best_factor_median": float(np.median(np.argmax(ll, axis=1))),
     }

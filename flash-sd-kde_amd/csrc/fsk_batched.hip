@@ -20,6 +20,8 @@
 // Tasks come from an atomic work queue, largest datasets first; per task the
 // row sums of log S_ib and of dmin_i are reduced in LDS and added to FP64
 // per-(dataset, bandwidth) accumulators; a finish kernel forms LL.
+#include <type_traits>
+
 #include "fsk_common.h"
 #include "fsk_plan.h"
 
@@ -34,6 +36,7 @@ struct LooMeta {
   int n_sets, n_bw;
   int64_t ntasks;
   int64_t off[kLooMaxSets + 1];      // dataset row offsets (original order)
+  int64_t poff[kLooMaxSets + 1];     // the same in the padded centred copy (multiples of 16)
   int64_t task_end[kLooMaxSets];     // cumulative task count in processing order
   int order[kLooMaxSets];            // processing order (largest first)
 };
@@ -41,7 +44,9 @@ struct LooMeta {
 // One workgroup per dataset: its FP64 mean (fixed summation order: per
 // thread over a strided row set, then a fixed LDS tree), then the centred
 // copy xc (16 floats per row, one rounding of the FP64 difference) and the
-// squared norms of the centred rows.
+// squared norms of the centred rows. Each dataset's rows start at poff[d] and
+// are padded to a multiple of 16 with zero rows whose norm is kLooFar, so
+// every 16-row data tile is read whole and padded points add exp2(-huge) = 0.
 __global__ __launch_bounds__(256) void loo_center_kernel(const float *__restrict__ x, int64_t ld,
                                                          LooMeta meta, float *__restrict__ xc,
                                                          float *__restrict__ nrm) {
@@ -75,7 +80,9 @@ __global__ __launch_bounds__(256) void loo_center_kernel(const float *__restrict
   }
   if (tid < kD) mu[tid] = red[0][tid] / double(end - beg);
   __syncthreads();
+  const int64_t pbeg = meta.poff[d], pend = meta.poff[d + 1];
   for (int64_t j = beg + tid; j < end; j += 256) {
+    const int64_t jp = pbeg + (j - beg);
     double s = 0.0;
 #pragma unroll
     for (int f = 0; f < kD; f += 4) {
@@ -86,18 +93,78 @@ __global__ __launch_bounds__(256) void loo_center_kernel(const float *__restrict
         o[e] = float(double(v[e]) - mu[f + e]);
         s += double(o[e]) * double(o[e]);
       }
-      *reinterpret_cast<f4 *>(xc + j * kD + f) = o;
+      *reinterpret_cast<f4 *>(xc + jp * kD + f) = o;
     }
-    nrm[j] = float(s);
+    nrm[jp] = float(s);
+  }
+  for (int64_t jp = pbeg + (end - beg) + tid; jp < pend; jp += 256) {
+#pragma unroll
+    for (int f = 0; f < kD; f += 4) *reinterpret_cast<f4 *>(xc + jp * kD + f) = f4{0.f, 0.f, 0.f, 0.f};
+    nrm[jp] = kLooFar;
   }
 }
 
-template <int WAVES>
+// Bandwidth skip groups. A term exp2(c_b delta) with c_b delta <= -64 is
+// below 2^-64, and S_ib >= 1 (the nearest neighbour's term is exp2(0)), so
+// all such terms of a row together change S_ib by less than n 2^-64 relative
+// (2^-46 at n = 2^18): far below one FP32 rounding. Per data tile the wave
+// finds the smallest delta over its pairs, dlo, and skips every group of
+// kLooGroup bandwidths whose widest member has c_b dlo <= -64. At the grid's
+// narrow end (0.05 h, c_b 400 times that at h) most tiles skip most groups.
+constexpr int kLooGroup = 4;
+constexpr int kLooGroups = kLooMaxBw / kLooGroup;
+constexpr float kLooSkipExp = 64.f;
+
+template <int CTRL>
+__device__ __forceinline__ int dpp_i32(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, false);
+}
+
+// Minimum over the wave of a non-negative float, returned as its bit pattern
+// (wave-uniform). For non-negative floats the bit patterns order like the
+// values; a -0.0 (bit pattern INT_MIN as a signed int) is taken as +0.
+__device__ __forceinline__ int wave_min_nonneg_bits(float v) {
+  int b = __builtin_bit_cast(int, v);
+  b = b < 0 ? 0 : b;
+  b = min(b, dpp_i32<0xB1>(b));   // quad_perm [1,0,3,2]
+  b = min(b, dpp_i32<0x4E>(b));   // quad_perm [2,3,0,1]
+  b = min(b, dpp_i32<0x124>(b));  // row_ror:4
+  b = min(b, dpp_i32<0x128>(b));  // row_ror:8: every lane holds its 16-lane row's minimum
+  const int r0 = __builtin_amdgcn_readlane(b, 0), r1 = __builtin_amdgcn_readlane(b, 16);
+  const int r2 = __builtin_amdgcn_readlane(b, 32), r3 = __builtin_amdgcn_readlane(b, 48);
+  return min(min(r0, r1), min(r2, r3));
+}
+
+// Sum over the four lane groups of a wave for four values at once: lane
+// (g, c) returns v[g] summed over the lanes {c, c+16, c+32, c+48}. Each of the
+// two exchange stages keeps the half of the values the lane is to reduce and
+// sends the other half to its partner (3 shuffles against 8 for four separate
+// sums).
+__device__ __forceinline__ float group_sum_exchange4(float v0, float v1, float v2, float v3,
+                                                     int g) {
+  const bool h2 = g & 2;
+  const float a0 = (h2 ? v2 : v0) + __shfl_xor(h2 ? v0 : v2, 32);
+  const float a1 = (h2 ? v3 : v1) + __shfl_xor(h2 ? v1 : v3, 32);
+  const bool h1 = g & 1;
+  return (h1 ? a1 : a0) + __shfl_xor(h1 ? a0 : a1, 16);
+}
+
+// WAVES waves per workgroup, each owning RT row tiles (16 RT rows): a task is
+// 16 RT WAVES rows of one dataset against all of its points. RT trades
+// registers for data loads and per-tile work shared by more rows; the
+// autotuner picks (WAVES, RT, workgroups per CU).
+// Per lane (g, c) and row tile: FP32 sums S[b] of its own four pairs
+// (row c, data points 4g + r) for all bandwidths, flushed every kLooFlush
+// tiles into FP64 sums that are spread over the lane groups: lane (g, c) keeps
+// row c's FP64 sums of the bandwidths b = 4q + g.
+template <int WAVES, int RT>
 __global__ __launch_bounds__(64 * WAVES) void loo_grid_kernel(
     const float *__restrict__ x, int64_t ld, const float *__restrict__ nrm,
     const double *__restrict__ bw, LooMeta meta, int *__restrict__ queue,
-    double *__restrict__ sumlog, double *__restrict__ sumdmin) {
-  constexpr int ROWS = 16 * WAVES;
+    double *__restrict__ sumlog, double *__restrict__ sumdmin,
+    unsigned long long *__restrict__ exps_done) {
+  constexpr int ROWS = 16 * RT * WAVES;
+  constexpr int kQ = kLooMaxBw / 4;
   __shared__ int task_sh;
   __shared__ float coef_sh[kLooMaxBw];
   __shared__ double red_sh[WAVES][kLooMaxBw + 1];
@@ -117,110 +184,174 @@ __global__ __launch_bounds__(64 * WAVES) void loo_grid_kernel(
     while (t >= meta.task_end[k]) ++k;
     const int d = meta.order[k];
     const int64_t blk = t - (k ? meta.task_end[k - 1] : 0);
-    const int64_t beg = meta.off[d], end = meta.off[d + 1];
-    if (tid < nbw) {
-      const double h = bw[d * kLooMaxBw + tid];
-      coef_sh[tid] = float(-kLog2e / (2.0 * h * h));
+    const int64_t n = meta.off[d + 1] - meta.off[d];
+    const int64_t pbeg = meta.poff[d], pend = meta.poff[d + 1];
+    if (tid < kLooMaxBw) {
+      const double h = tid < nbw ? bw[d * kLooMaxBw + tid] : 0.0;
+      coef_sh[tid] = tid < nbw ? float(-kLog2e / (2.0 * h * h)) : 0.f;
     }
     __syncthreads();
-    float cf[kLooMaxBw];  // wave-uniform: kept in scalar registers
+    // The coefficients stay in vector registers (the same value in every
+    // lane): as scalars they overflow the SGPR file next to the loop state.
+    float cf[kLooMaxBw];
 #pragma unroll
-    for (int b = 0; b < kLooMaxBw; ++b)
-      cf[b] = __builtin_bit_cast(
-          float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, b < nbw ? coef_sh[b] : 0.f)));
+    for (int b = 0; b < kLooMaxBw; ++b) cf[b] = coef_sh[b];
+    // skip threshold of each group: delta < 64 / min|c_b| over its bandwidths
+    // (0, never active, for a group past n_bw), as bit patterns
+    int thr[kLooGroups];
+#pragma unroll
+    for (int q = 0; q < kLooGroups; ++q) {
+      float mn = __builtin_inff();
+#pragma unroll
+      for (int e = 0; e < kLooGroup; ++e)
+        if (q * kLooGroup + e < nbw) mn = fminf(mn, -cf[q * kLooGroup + e]);
+      thr[q] = __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, kLooSkipExp / mn));
+    }
 
-    const int64_t rbase = beg + blk * ROWS + wave * 16;  // this wave's 16 rows
-    const int64_t i = rbase + c;
-    const bool row_ok = i < end;
-    const int64_t ic = row_ok ? i : end - 1;
-    const f4 qv = ld4(x + ic * ld + 4 * g) * -2.0f;
-    const float qa = nrm[ic];
+    // this wave's 16 RT rows (padded coordinates; the rows past n are idle)
+    const int64_t rbase = pbeg + blk * ROWS + wave * 16 * RT;
+    bool row_ok[RT];
+    f4 qv[RT];
+    float qa[RT];
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt) {
+      const int64_t i = rbase + 16 * rt + c;
+      row_ok[rt] = i - pbeg < n;
+      const int64_t ic = row_ok[rt] ? i : pbeg;
+      qv[rt] = ld4(x + ic * ld + 4 * g) * -2.0f;
+      qa[rt] = nrm[ic];
+    }
 
     // ---- phase 1: nearest-neighbour distance (j != i) ----
-    float dmin = kLooFar;
-    for (int64_t j0 = beg; j0 < end; j0 += kTile) {
-      int64_t ja = j0 + c;
-      if (ja > end - 1) ja = end - 1;
-      const f4 xa = ld4(x + ja * ld + 4 * g);
-      f4 bb;
+    float dmin[RT];
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int64_t jb = j0 + 4 * g + r;
-        bb[r] = jb < end ? nrm[jb] : kLooFar;
+    for (int rt = 0; rt < RT; ++rt) dmin[rt] = kLooFar;
+    for (int64_t j0 = pbeg; j0 < pend; j0 += kTile) {
+      const f4 xa = ld4(x + (j0 + c) * ld + 4 * g);
+      const f4 bb = ld4(nrm + j0 + 4 * g);
+#pragma unroll
+      for (int rt = 0; rt < RT; ++rt) {
+        f4 dd = bb;
+#pragma unroll
+        for (int s = 0; s < 4; ++s) dd = mfma_f32(xa[s], qv[rt][s], dd);
+        dd += qa[rt];
+        if (j0 == rbase + 16 * rt) {  // diagonal tile: drop j == i
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+            if (4 * g + r == c) dd[r] = kLooFar;
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) dmin[rt] = fminf(dmin[rt], dd[r]);
       }
-      f4 dd = bb;
-#pragma unroll
-      for (int s = 0; s < 4; ++s) dd = mfma_f32(xa[s], qv[s], dd);
-      dd += qa;
-      if (j0 == rbase) {  // diagonal tile: drop j == i
-#pragma unroll
-        for (int r = 0; r < 4; ++r)
-          if (4 * g + r == c) dd[r] = kLooFar;
-      }
-#pragma unroll
-      for (int r = 0; r < 4; ++r) dmin = fminf(dmin, dd[r]);
     }
-    dmin = fminf(dmin, __shfl_xor(dmin, 16));
-    dmin = fminf(dmin, __shfl_xor(dmin, 32));
-    dmin = fmaxf(dmin, 0.f);
+    float qad[RT];
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt) {
+      dmin[rt] = fminf(dmin[rt], __shfl_xor(dmin[rt], 16));
+      dmin[rt] = fminf(dmin[rt], __shfl_xor(dmin[rt], 32));
+      dmin[rt] = fmaxf(dmin[rt], 0.f);
+      // idle rows get deltas far above every skip threshold (and exps of 0)
+      qad[rt] = row_ok[rt] ? qa[rt] - dmin[rt] : kLooFar;
+    }
 
     // ---- phase 2: S_ib over the bandwidth grid ----
-    float S[kLooMaxBw];
-    double S64[kLooMaxBw];
+    float S[RT][kLooMaxBw];
+    double S64[RT][kQ];
 #pragma unroll
-    for (int b = 0; b < kLooMaxBw; ++b) {
-      S[b] = 0.f;
-      S64[b] = 0.0;
+    for (int rt = 0; rt < RT; ++rt) {
+#pragma unroll
+      for (int b = 0; b < kLooMaxBw; ++b) S[rt][b] = 0.f;
+#pragma unroll
+      for (int q = 0; q < kQ; ++q) S64[rt][q] = 0.0;
     }
-    const float qad = qa - dmin;
-    int tc = 0;
-    for (int64_t j0 = beg; j0 < end; j0 += kTile) {
-      int64_t ja = j0 + c;
-      if (ja > end - 1) ja = end - 1;
-      const f4 xa = ld4(x + ja * ld + 4 * g);
-      f4 bb;
+    auto flush = [&]() {
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int64_t jb = j0 + 4 * g + r;
-        bb[r] = jb < end ? nrm[jb] : kLooFar;
-      }
-      f4 dd = bb;
+      for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
-      for (int s = 0; s < 4; ++s) dd = mfma_f32(xa[s], qv[s], dd);
-      dd += qad;
-      if (j0 == rbase) {
+        for (int q = 0; q < kQ; ++q) {
+          const float v = group_sum_exchange4(S[rt][4 * q], S[rt][4 * q + 1], S[rt][4 * q + 2],
+                                              S[rt][4 * q + 3], g);
+          S64[rt][q] += double(v);
 #pragma unroll
-        for (int r = 0; r < 4; ++r)
-          if (4 * g + r == c) dd[r] = kLooFar;
-      }
+          for (int e = 0; e < 4; ++e) S[rt][4 * q + e] = 0.f;
+        }
+    };
+    // One data tile: cross terms, deltas, the wave's smallest delta, and the
+    // exps of the bandwidth groups it leaves active. DIAG: the tile holds the
+    // rows of one of the wave's row tiles, whose self pairs are dropped.
+    unsigned n_active = 0;  // active (tile, group) pairs of this wave and task
+    auto tile = [&](int64_t j0, auto diag) {
+      const f4 xa = ld4(x + (j0 + c) * ld + 4 * g);
+      const f4 bb = ld4(nrm + j0 + 4 * g);
+      float delta[RT][4];
+      float lo = kLooFar;
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const float delta = fmaxf(dd[r], 0.f);
+      for (int rt = 0; rt < RT; ++rt) {
+        f4 dd = bb;
 #pragma unroll
-        for (int b = 0; b < kLooMaxBw; ++b) S[b] += fexp2(delta * cf[b]);  // cf = 0 past n_bw
-      }
-      if (++tc == kLooFlush) {
-        tc = 0;
+        for (int s = 0; s < 4; ++s) dd = mfma_f32(xa[s], qv[rt][s], dd);
+        dd += qad[rt];
+        if constexpr (decltype(diag)::value) {
+          if (j0 == rbase + 16 * rt) {
 #pragma unroll
-        for (int b = 0; b < kLooMaxBw; ++b) {
-          S64[b] += S[b];
-          S[b] = 0.f;
+            for (int r = 0; r < 4; ++r)
+              if (4 * g + r == c) dd[r] = kLooFar;
+          }
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          delta[rt][r] = fmaxf(dd[r], 0.f);
+          lo = fminf(lo, delta[rt][r]);
         }
       }
-    }
-    // ---- per-task reduction: sum over rows of log S_ib and of dmin_i ----
+      const int dlo = wave_min_nonneg_bits(lo);
 #pragma unroll
-    for (int b = 0; b < kLooMaxBw; ++b) {
-      double v = sum_groups(S64[b] + double(S[b]));
-      v = (row_ok && b < nbw) ? log(v) : 0.0;
-      // sum over the 16 rows of the wave (lanes c, group 0 holds row c)
+      for (int q = 0; q < kLooGroups; ++q) {
+        if (dlo < thr[q]) {
+          ++n_active;
+#pragma unroll
+          for (int e = 0; e < kLooGroup; ++e) {
+            const int b = q * kLooGroup + e;
+#pragma unroll
+            for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+              for (int r = 0; r < 4; ++r) S[rt][b] += fexp2(delta[rt][r] * cf[b]);
+          }
+        }
+      }
+    };
+    int tc = 0;
+    for (int64_t j0 = pbeg; j0 < pend; j0 += kTile) {
+      if (j0 >= rbase && j0 < rbase + 16 * RT)
+        tile(j0, std::true_type{});
+      else
+        tile(j0, std::false_type{});
+      if (++tc == kLooFlush) {
+        tc = 0;
+        flush();
+      }
+    }
+    flush();
+    if (exps_done && lane == 0)
+      atomicAdd(exps_done, (unsigned long long)n_active * (kLooGroup * 64 * 4 * RT));
+    // ---- per-task reduction: sum over rows of log S_ib and of dmin_i ----
+    // lane (g, c) holds row c's sums of the bandwidths 4q + g
+#pragma unroll
+    for (int q = 0; q < kQ; ++q) {
+      const int b = 4 * q + g;
+      double v = 0.0;
+#pragma unroll
+      for (int rt = 0; rt < RT; ++rt) v += (row_ok[rt] && b < nbw) ? log(S64[rt][q]) : 0.0;
+      // sum over the 16 rows (lanes c of the group)
       v += __shfl_xor(v, 1);
       v += __shfl_xor(v, 2);
       v += __shfl_xor(v, 4);
       v += __shfl_xor(v, 8);
-      if (lane == 0) red_sh[wave][b] = v;
+      if (c == 0) red_sh[wave][b] = v;
     }
-    double dm = row_ok ? double(dmin) : 0.0;
+    double dm = 0.0;
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt) dm += row_ok[rt] ? double(dmin[rt]) : 0.0;
     if (g != 0) dm = 0.0;
     dm += __shfl_xor(dm, 1);
     dm += __shfl_xor(dm, 2);
@@ -265,6 +396,13 @@ struct LooLayout {
   double *bw, *sumlog, *sumdmin;
 };
 
+// Rows of the padded centred copy: every dataset rounded up to whole tiles.
+static int64_t loo_padded_rows(const int64_t *offsets, int n_sets) {
+  int64_t np = 0;
+  for (int d = 0; d < n_sets; ++d) np += round_up(offsets[d + 1] - offsets[d], kTile);
+  return np;
+}
+
 static size_t loo_layout(void *ws, int64_t N, int n_sets, LooLayout *L) {
   Carve cv(ws);
   L->queue = cv.take<int>(64);
@@ -282,14 +420,17 @@ using namespace fsk;
 
 extern "C" size_t fsk_loo_workspace_bytes(const int64_t *offsets, int n_sets, int n_bw) {
   if (!offsets || n_sets <= 0 || n_sets > kLooMaxSets || n_bw <= 0 || n_bw > kLooMaxBw) return 0;
+  for (int d = 0; d < n_sets; ++d)
+    if (offsets[d + 1] < offsets[d]) return 0;
   LooLayout L;
-  return loo_layout(nullptr, offsets[n_sets], n_sets, &L);
+  return loo_layout(nullptr, loo_padded_rows(offsets, n_sets), n_sets, &L);
 }
 
 extern "C" int fsk_loo_bandwidth_grid(const float *data, int64_t ld, const int64_t *offsets,
                                       int n_sets, const double *bandwidths, int n_bw,
-                                      int waves_per_wg, int wg_per_cu, void *ws,
-                                      size_t ws_bytes, double *out, void *stream) {
+                                      int waves_per_wg, int row_tiles, int wg_per_cu, void *ws,
+                                      size_t ws_bytes, double *out,
+                                      unsigned long long *exps_done, void *stream) {
   const char *what = "fsk_loo_bandwidth_grid";
   if (!offsets || n_sets <= 0 || n_sets > kLooMaxSets || n_bw <= 0 || n_bw > kLooMaxBw ||
       !bandwidths || !out) {
@@ -314,13 +455,16 @@ extern "C" int fsk_loo_bandwidth_grid(const float *data, int64_t ld, const int64
     set_error("%s: waves_per_wg must be 4 or 8", what);
     return FSK_EINVAL;
   }
+  if (row_tiles != 1 && row_tiles != 2) {
+    set_error("%s: row_tiles must be 1 or 2", what);
+    return FSK_EINVAL;
+  }
   if (wg_per_cu < 1 || wg_per_cu > 8) {
     set_error("%s: wg_per_cu must be in 1..8", what);
     return FSK_EINVAL;
   }
-  const int64_t N = offsets[n_sets];
   LooLayout L;
-  const size_t need = loo_layout(ws, N, n_sets, &L);
+  const size_t need = loo_layout(ws, loo_padded_rows(offsets, n_sets), n_sets, &L);
   if (!ws || ws_bytes < need) {
     set_error("%s: workspace of %zu bytes needed, got %zu", what, need, ws_bytes);
     return FSK_EWORKSPACE;
@@ -330,6 +474,9 @@ extern "C" int fsk_loo_bandwidth_grid(const float *data, int64_t ld, const int64
   meta.n_sets = n_sets;
   meta.n_bw = n_bw;
   for (int d = 0; d <= n_sets; ++d) meta.off[d] = offsets[d];
+  meta.poff[0] = 0;
+  for (int d = 0; d < n_sets; ++d)
+    meta.poff[d + 1] = meta.poff[d] + round_up(offsets[d + 1] - offsets[d], kTile);
   for (int d = 0; d < n_sets; ++d) meta.order[d] = d;
   for (int a = 0; a < n_sets; ++a)
     for (int b = a + 1; b < n_sets; ++b) {
@@ -341,7 +488,7 @@ extern "C" int fsk_loo_bandwidth_grid(const float *data, int64_t ld, const int64
         meta.order[b] = tmp;
       }
     }
-  const int rows = 16 * waves_per_wg;
+  const int rows = 16 * row_tiles * waves_per_wg;
   int64_t acc = 0;
   for (int k = 0; k < n_sets; ++k) {
     const int d = meta.order[k];
@@ -375,12 +522,19 @@ extern "C" int fsk_loo_bandwidth_grid(const float *data, int64_t ld, const int64
   int64_t grid = int64_t(cus) * wg_per_cu;
   if (grid > meta.ntasks) grid = meta.ntasks;
   prof_begin(FSK_OP_LOO_GRID, s);
-  if (waves_per_wg == 4)
-    hipLaunchKernelGGL(loo_grid_kernel<4>, dim3(grid), dim3(256), 0, s, L.xc, int64_t(kD), L.nrm, L.bw, meta,
-                       L.queue, L.sumlog, L.sumdmin);
+  const dim3 blk(64 * waves_per_wg);
+  if (waves_per_wg == 4 && row_tiles == 1)
+    hipLaunchKernelGGL((loo_grid_kernel<4, 1>), dim3(grid), blk, 0, s, L.xc, int64_t(kD), L.nrm,
+                       L.bw, meta, L.queue, L.sumlog, L.sumdmin, exps_done);
+  else if (waves_per_wg == 4)
+    hipLaunchKernelGGL((loo_grid_kernel<4, 2>), dim3(grid), blk, 0, s, L.xc, int64_t(kD), L.nrm,
+                       L.bw, meta, L.queue, L.sumlog, L.sumdmin, exps_done);
+  else if (row_tiles == 1)
+    hipLaunchKernelGGL((loo_grid_kernel<8, 1>), dim3(grid), blk, 0, s, L.xc, int64_t(kD), L.nrm,
+                       L.bw, meta, L.queue, L.sumlog, L.sumdmin, exps_done);
   else
-    hipLaunchKernelGGL(loo_grid_kernel<8>, dim3(grid), dim3(512), 0, s, L.xc, int64_t(kD), L.nrm, L.bw, meta,
-                       L.queue, L.sumlog, L.sumdmin);
+    hipLaunchKernelGGL((loo_grid_kernel<8, 2>), dim3(grid), blk, 0, s, L.xc, int64_t(kD), L.nrm,
+                       L.bw, meta, L.queue, L.sumlog, L.sumdmin, exps_done);
   prof_end(FSK_OP_LOO_GRID, s);
   if ((st = launch_status(what)) != FSK_OK) return st;
   hipLaunchKernelGGL(loo_finish_kernel, dim3(cdiv(int64_t(n_sets) * n_bw, 256)), dim3(256), 0, s,

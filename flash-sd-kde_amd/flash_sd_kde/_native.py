@@ -99,7 +99,7 @@ SIGNATURES = {
     "fsk_mlp_mse": [_p, _p, _i64, _d, _p, _p, _p],
     "fsk_loo_workspace_bytes": [_p, ctypes.c_int, ctypes.c_int],
     "fsk_loo_bandwidth_grid": [_p, _i64, _p, ctypes.c_int, _p, ctypes.c_int, ctypes.c_int,
-                               ctypes.c_int, _p, _sz, _p, _p],
+                               ctypes.c_int, ctypes.c_int, _p, _sz, _p, _p, _p],
     "fsk_ref64_logdensity16": [_p, _i64, _p, _i64, _d, _p, _sz, _p, _p],
     "fsk_ref64_score16": [_p, _i64, _d, _p, _p, _p],
     "fsk_nd_workspace_bytes": [ctypes.c_int, _i64, _i64, ctypes.c_int],

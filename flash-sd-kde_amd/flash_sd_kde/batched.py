@@ -8,12 +8,13 @@ exact LOO log-likelihood over a bandwidth grid for many datasets at once).
     best = grid.best_bandwidths()
 
 All datasets and bandwidths are evaluated by ONE persistent kernel launch
-(fsk_loo_bandwidth_grid) fed from a device work queue. The launch shape
-(waves per workgroup, workgroups per CU) is chosen by `autotune`, which times
-every candidate on the actual batch, checks that all candidates agree, and
-caches the winner per batch signature. The whole sequence (queue reset,
-norms, persistent kernel, finish) is captured once in a HIP graph and
-replayed by run().
+(fsk_loo_bandwidth_grid) fed from a device work queue. Its tile shape and
+launch shape -- waves per workgroup, row tiles of 16 rows per wave (the task
+tile is 16 x row_tiles x waves rows against all of a dataset's points) and
+workgroups per CU -- are chosen by `autotune`, which times every candidate on
+the actual batch, checks that all candidates agree, and caches the winner per
+batch signature. The whole sequence (queue reset, centring, persistent
+kernel, finish) is captured once in a HIP graph and replayed by run().
 """
 
 from __future__ import annotations
@@ -30,16 +31,20 @@ from flash_sd_kde.bandwidth import scott_mean_std_bandwidth
 
 MAX_SETS = 64
 MAX_BW = 32
-CANDIDATES = [(w, g) for w in (4, 8) for g in (1, 2, 3, 4)]
+# (waves per workgroup, row tiles per wave, workgroups per CU). Resident waves
+# per SIMD are bounded by registers: 3 for one row tile (137 VGPRs), 2 for two
+# (~204 VGPRs), so larger workgroup counts would only queue behind these.
+CANDIDATES = [(4, 1, 2), (4, 1, 3), (8, 1, 1), (8, 1, 2), (4, 2, 1), (4, 2, 2), (8, 2, 1)]
+DEFAULT_TUNE = (4, 1, 3)
 
-_tune_cache: dict[tuple, tuple[int, int]] = {}
+_tune_cache: dict[tuple, tuple[tuple[int, int, int], dict]] = {}
 _tune_lock = threading.Lock()
 
 
 class LooBandwidthGrid:
     def __init__(self, datasets: Sequence, *, factors: Sequence[float] | None = None,
                  bandwidths: np.ndarray | None = None, device="cuda", autotune: bool = True,
-                 graph: bool = True, tune: tuple[int, int] | None = None):
+                 graph: bool = True, tune: tuple[int, int, int] | None = None):
         self.device = torch.device(device)
         arrays = [np.asarray(d, dtype=np.float32) for d in datasets]
         if not 1 <= len(arrays) <= MAX_SETS:
@@ -72,31 +77,37 @@ class LooBandwidthGrid:
         nbytes = int(nat.lib().fsk_loo_workspace_bytes(self._off_c, self.n_sets, self.n_bw))
         self.ws = torch.empty(max(nbytes, 256), dtype=torch.uint8, device=self.device)
         self.ws_bytes = nbytes
-        self.tune = tune or (self.autotune() if autotune else (8, 2))
+        if tune is not None and tuple(tune) not in CANDIDATES:
+            raise ValueError(f"tune must be one of {CANDIDATES}, got {tune}")
+        self.tune = tuple(tune) if tune is not None else (
+            self.autotune() if autotune else DEFAULT_TUNE)
         self.graph = None
         if graph:
             self._capture()
 
     # ---- launching ----
-    def _launch(self, tune: tuple[int, int] | None = None) -> None:
-        w, g = tune or self.tune
+    def _launch(self, tune: tuple[int, int, int] | None = None,
+                exps_done: torch.Tensor | None = None) -> None:
+        w, rt, g = tune or self.tune
         st = nat.lib().fsk_loo_bandwidth_grid(
             self.x.data_ptr(), self.x.stride(0), self._off_c, self.n_sets, self.bw.data_ptr(),
-            self.n_bw, w, g, self.ws.data_ptr(), self.ws_bytes, self.out.data_ptr(),
-            nat.stream_handle(self.device))
+            self.n_bw, w, rt, g, self.ws.data_ptr(), self.ws_bytes, self.out.data_ptr(),
+            None if exps_done is None else exps_done.data_ptr(), nat.stream_handle(self.device))
         nat.check(st, "fsk_loo_bandwidth_grid")
 
     def signature(self) -> tuple:
         return (tuple(self.sizes), self.n_bw)
 
-    def autotune(self, repeats: int = 2) -> tuple[int, int]:
-        """Time every launch-shape candidate on this batch; keep the fastest.
-        All candidates must produce the same log-likelihoods (they differ only
-        in the order of FP64 atomic additions)."""
+    def autotune(self, repeats: int = 2) -> tuple[int, int, int]:
+        """Time every tile/launch-shape candidate on this batch; keep the
+        fastest. All candidates must produce the same log-likelihoods: they
+        differ in the order of FP64 atomic additions, and in which terms below
+        2^-64 of a row's nearest-neighbour term their tiles skip."""
         key = self.signature()
         with _tune_lock:
             if key in _tune_cache:
-                return _tune_cache[key]
+                best, self.tune_times = _tune_cache[key]
+                return best
             times, results = {}, {}
             for cand in CANDIDATES:
                 self._launch(cand)  # warm-up
@@ -115,8 +126,16 @@ class LooBandwidthGrid:
                     raise RuntimeError(f"autotune candidate {cand} disagrees with {CANDIDATES[0]}")
             best = min(times, key=times.get)
             self.tune_times = times
-            _tune_cache[key] = best
+            _tune_cache[key] = (best, times)
             return best
+
+    def exps_evaluated(self) -> int:
+        """Exponentials one launch evaluates with the chosen shape (the
+        logical count is pair_count() * n_bw; the kernel skips tile groups
+        whose terms are all below 2^-64 of the nearest-neighbour term)."""
+        cnt = torch.zeros(1, dtype=torch.int64, device=self.device)
+        self._launch(exps_done=cnt)
+        return int(cnt.item())
 
     def _capture(self) -> None:
         self._launch()  # eager run before capture (library state, allocator)

@@ -293,14 +293,20 @@ int fsk_sdkde_debias_f64(const float *x, int64_t ldx, int64_t n,
  * = 0) delimit them; `bandwidths` (DEVICE, float64, n_sets x n_bw, row-major)
  * holds each dataset's grid. out (DEVICE, float64, n_sets x n_bw):
  *   out[d, b] = 1/n_d sum_i log( 1/(n_d - 1) sum_{j != i} N(x_i; x_j, h_b^2 I) ).
- * One persistent launch over all datasets (work queue, largest first);
- * waves_per_wg in {4, 8} and wg_per_cu in 1..8 are the tunables (chosen by the
- * Python autotuner). Limits: n_sets <= 64, n_bw <= 32, every n_d >= 2.
+ * One persistent launch over all datasets (work queue, largest first). The
+ * tunables, chosen by the Python autotuner: waves_per_wg in {4, 8}, row_tiles
+ * in {1, 2} (16 or 32 rows per wave: the task tile is 16 row_tiles
+ * waves_per_wg rows) and wg_per_cu in 1..8. Terms below 2^-64 of a row's
+ * nearest-neighbour term are skipped per data tile and group of 4 bandwidths
+ * (their sum is below n 2^-64 relative). exps_done (DEVICE, nullable): the
+ * count of exponentials evaluated is ADDED to it (the caller zeroes it).
+ * Limits: n_sets <= 64, n_bw <= 32, every n_d >= 2.
  * Capturable: no allocation, no host synchronization, fills by kernel. */
 size_t fsk_loo_workspace_bytes(const int64_t *offsets, int n_sets, int n_bw);
 int fsk_loo_bandwidth_grid(const float *data, int64_t ld, const int64_t *offsets, int n_sets,
-                           const double *bandwidths, int n_bw, int waves_per_wg, int wg_per_cu,
-                           void *ws, size_t ws_bytes, double *out, void *stream);
+                           const double *bandwidths, int n_bw, int waves_per_wg, int row_tiles,
+                           int wg_per_cu, void *ws, size_t ws_bytes, double *out,
+                           unsigned long long *exps_done, void *stream);
 
 /* ---- Learned score surrogate: MLP building blocks (no reference counterpart;
  * the learned pattern). BF16 weights (uint16 bit patterns), FP32 activations

@@ -59,7 +59,14 @@ def test_batched_graph_replays_and_autotune(gpu):
         out = grid.run().clone()
         torch.cuda.synchronize()
         np.testing.assert_allclose(out.cpu().numpy(), eager.cpu().numpy(), rtol=1e-12)
-    assert grid.tune in [(w, g) for w in (4, 8) for g in (1, 2, 3, 4)]
+    from flash_sd_kde.batched import CANDIDATES
+
+    assert grid.tune in CANDIDATES
+    assert set(grid.tune_times) == set(CANDIDATES)
+    # every tile shape gives the same result (eager, no graph)
+    for cand in CANDIDATES:
+        grid._launch(cand)
+        np.testing.assert_allclose(grid.out.cpu().numpy(), eager.cpu().numpy(), rtol=1e-12)
     # the chosen bandwidth is the grid maximum of the FP64 oracle's curve
     for k, x in enumerate(sets):
         ref = fp64.loo_loglik(x, grid.bandwidths[k])
