@@ -33,6 +33,39 @@ def _sets(sizes, seed):
     return out
 
 
+def test_spatial_order_is_a_permutation_that_keeps_clusters_together():
+    from flash_sd_kde.batched import spatial_order
+
+    rng = np.random.default_rng(4)
+    a = rng.normal(size=(300, 16)) + 40.0
+    b = rng.normal(size=(200, 16)) - 40.0
+    x = np.concatenate([a, b])[rng.permutation(500)]
+    perm = spatial_order(torch.as_tensor(x, dtype=torch.float32)).numpy()
+    assert np.array_equal(np.sort(perm), np.arange(500))
+    side = x[perm, 0] > 0  # cluster of each point in the new order
+    assert np.count_nonzero(side[1:] != side[:-1]) == 1
+    for n in (1, 2):
+        assert spatial_order(torch.zeros(n, 16)).tolist() == list(range(n))
+
+
+@pytest.mark.gpu
+def test_batched_spatial_order_changes_only_rounding(gpu):
+    """The spatial order of the points only changes the summation order and
+    which negligible (< 2^-64 relative) terms are skipped: the log-likelihoods
+    agree with the given order, and fewer exponentials are evaluated."""
+    sets = _sets([3000, 700, 65], seed=6)
+    factors = np.geomspace(0.05, 2.0, 32)
+    grids = [LooBandwidthGrid(sets, factors=factors, device=gpu, autotune=False, graph=False,
+                              spatial=sp) for sp in (False, True)]
+    ll = [g.run().cpu().numpy() for g in grids]
+    np.testing.assert_allclose(ll[1], ll[0], rtol=1e-7)
+    done = [g.exps_evaluated() for g in grids]
+    logical = grids[0].pair_count() * 32
+    print("evaluated exps: given order %.3f, spatial order %.3f" % (done[0] / logical,
+                                                                    done[1] / logical))
+    assert done[1] < done[0] <= logical * 1.1
+
+
 @pytest.mark.gpu
 def test_batched_matches_fp64_bruteforce(gpu):
     sets = _sets([2, 3, 17, 100, 333, 64], seed=1)
