@@ -3,16 +3,21 @@
 //
 // For dataset d (n points) and bandwidth h_b:
 //   LL_b = 1/n sum_i log( 1/(n-1) sum_{j != i} (2 pi)^-8 h_b^-16 exp(-|x_i - x_j|^2 / 2h_b^2) )
-// Each task = (dataset, block of 16*WAVES rows) and runs two phases over all
-// points of its dataset:
-//   1. dmin_i = min_{j != i} |x_i - x_j|^2            (one v_min per pair)
-//   2. S_ib   = sum_{j != i} exp2(c_b (|x_i - x_j|^2 - dmin_i)),  c_b = -log2(e)/(2 h_b^2)
-// so log sum_j exp(-d2/2h^2) = c_b dmin_i ln 2 + log S_ib exactly (log-sum-exp
-// with the maximum known in advance): every term is <= 1, the nearest
-// neighbour contributes 1, and no bandwidth of the grid can underflow.
-// One distance feeds all n_bw (<= 32) exponentials; the kernel is exp-bound,
-// so the cross term stays FP32 MFMA (it is < 10 % of the issue slots).
-// Each dataset is first centred on its own FP64 mean (distances are
+// Each task = (dataset, block of rows) makes one pass over all points of its
+// dataset and accumulates, per row i and bandwidth b,
+//   S_ib = sum_{j != i} exp2(c_b (|x_i - x_j|^2 - m_i)),  c_b = -log2(e)/(2 h_b^2),
+// with the shift m_i the running minimum of |x_i - x_j|^2 over the points
+// seen so far: when a tile brings a smaller distance, the row's sums are
+// rescaled by exp2(c_b (m_old - m_new)) <= 1 and m_i drops. At the end m_i is
+// the nearest-neighbour distance dmin_i, so
+//   log sum_j exp(-d2/2h^2) = c_b dmin_i ln 2 + log S_ib
+// exactly (log-sum-exp): every term is <= 1, the nearest neighbour
+// contributes 1, and no bandwidth of the grid can underflow. Tiles are visited
+// from the row block's own tile onwards (wrapping around), and the points are
+// in spatial order, so the minimum is nearly found on the first tiles and
+// later rescales are rare (about 1 % of the tiles of a wave).
+// One distance feeds all n_bw (<= 32) exponentials; the cross term stays FP32
+// MFMA. Each dataset is first centred on its own FP64 mean (distances are
 // translation invariant; the norm terms, whose cancellation sets the FP32
 // error of |x_i|^2 + |x_j|^2 - 2 x_i.x_j, shrink to the dataset's spread), and
 // the MFMA accumulator starts from |x_j|^2 alone, |x_i|^2 being added after
@@ -30,7 +35,12 @@ namespace fsk {
 constexpr int kLooMaxSets = 64;
 constexpr int kLooMaxBw = 32;
 constexpr int kLooFlush = 16;
+// kLooFar: the shift m_i of a row before its first real distance. kLooPad:
+// the squared distance of padded points, of a row's own point and of idle
+// rows, four times farther, so its delta against any shift stays >= 3e30 and
+// its terms are exp2(-huge) = 0 (and it never lowers a shift).
 constexpr float kLooFar = 1.0e30f;
+constexpr float kLooPad = 4.0e30f;
 
 struct LooMeta {
   int n_sets, n_bw;
@@ -45,7 +55,7 @@ struct LooMeta {
 // thread over a strided row set, then a fixed LDS tree), then the centred
 // copy xc (16 floats per row, one rounding of the FP64 difference) and the
 // squared norms of the centred rows. Each dataset's rows start at poff[d] and
-// are padded to a multiple of 16 with zero rows whose norm is kLooFar, so
+// are padded to a multiple of 16 with zero rows whose norm is kLooPad, so
 // every 16-row data tile is read whole and padded points add exp2(-huge) = 0.
 __global__ __launch_bounds__(256) void loo_center_kernel(const float *__restrict__ x, int64_t ld,
                                                          LooMeta meta, float *__restrict__ xc,
@@ -100,7 +110,7 @@ __global__ __launch_bounds__(256) void loo_center_kernel(const float *__restrict
   for (int64_t jp = pbeg + (end - beg) + tid; jp < pend; jp += 256) {
 #pragma unroll
     for (int f = 0; f < kD; f += 4) *reinterpret_cast<f4 *>(xc + jp * kD + f) = f4{0.f, 0.f, 0.f, 0.f};
-    nrm[jp] = kLooFar;
+    nrm[jp] = kLooPad;
   }
 }
 
@@ -147,6 +157,26 @@ __device__ __forceinline__ float group_sum_exchange4(float v0, float v1, float v
   const float a1 = (h2 ? v3 : v1) + __shfl_xor(h2 ? v1 : v3, 32);
   const bool h1 = g & 1;
   return (h1 ? a1 : a0) + __shfl_xor(h1 ? a0 : a1, 16);
+}
+
+// Minimum over the four lane groups: lane (g, c) returns the minimum of v over
+// the lanes {c, c+16, c+32, c+48}. v_permlane32_swap a, b swaps the upper 32
+// lanes of a with the lower 32 lanes of b, leaving [a_lo | b_lo] in a and
+// [a_hi | b_hi] in b; v_permlane16_swap does the same with the odd 16-lane
+// rows of a and the even rows of b. Started from a = b = v, the two results
+// hold v and its partner 32 (then 16) lanes away in every lane.
+// The swaps are written as register-only inline asm: hipcc 7.2 folds a
+// combination of the two results of __builtin_amdgcn_permlane*_swap into the
+// first result alone (min(a[0], a[1]) becomes a[0]). The s_nop 1 gives the
+// two wait states a swap needs after a VALU write of its operands.
+__device__ __forceinline__ float group_min4(float v) {
+  float a = v, b = v;
+  asm volatile("s_nop 1\n\tv_permlane32_swap_b32 %0, %1" : "+v"(a), "+v"(b));
+  v = fminf(a, b);
+  a = v;
+  b = v;
+  asm volatile("s_nop 1\n\tv_permlane16_swap_b32 %0, %1" : "+v"(a), "+v"(b));
+  return fminf(a, b);
 }
 
 // WAVES waves per workgroup, each owning RT row tiles (16 RT rows): a task is
@@ -212,49 +242,26 @@ __global__ __launch_bounds__(64 * WAVES) void loo_grid_kernel(
     const int64_t rbase = pbeg + blk * ROWS + wave * 16 * RT;
     bool row_ok[RT];
     f4 qv[RT];
-    float qa[RT];
+    float qa[RT], m[RT];
 #pragma unroll
     for (int rt = 0; rt < RT; ++rt) {
       const int64_t i = rbase + 16 * rt + c;
       row_ok[rt] = i - pbeg < n;
       const int64_t ic = row_ok[rt] ? i : pbeg;
       qv[rt] = ld4(x + ic * ld + 4 * g) * -2.0f;
-      qa[rt] = nrm[ic];
+      // idle rows: distances ~kLooPad against a shift of 0, far above every
+      // skip threshold (exps of 0), and never a rescale
+      qa[rt] = row_ok[rt] ? nrm[ic] : kLooPad;
+      m[rt] = row_ok[rt] ? kLooFar : 0.f;
     }
+    // coefficients of the FP64 sums this lane keeps (bandwidths 4q + g)
+    float cfl[kQ];
+#pragma unroll
+    for (int q = 0; q < kQ; ++q) cfl[q] = coef_sh[4 * q + g];
 
-    // ---- phase 1: nearest-neighbour distance (j != i) ----
-    float dmin[RT];
-#pragma unroll
-    for (int rt = 0; rt < RT; ++rt) dmin[rt] = kLooFar;
-    for (int64_t j0 = pbeg; j0 < pend; j0 += kTile) {
-      const f4 xa = ld4(x + (j0 + c) * ld + 4 * g);
-      const f4 bb = ld4(nrm + j0 + 4 * g);
-#pragma unroll
-      for (int rt = 0; rt < RT; ++rt) {
-        f4 dd = bb;
-#pragma unroll
-        for (int s = 0; s < 4; ++s) dd = mfma_f32(xa[s], qv[rt][s], dd);
-        dd += qa[rt];
-        if (j0 == rbase + 16 * rt) {  // diagonal tile: drop j == i
-#pragma unroll
-          for (int r = 0; r < 4; ++r)
-            if (4 * g + r == c) dd[r] = kLooFar;
-        }
-#pragma unroll
-        for (int r = 0; r < 4; ++r) dmin[rt] = fminf(dmin[rt], dd[r]);
-      }
-    }
-    float qad[RT];
-#pragma unroll
-    for (int rt = 0; rt < RT; ++rt) {
-      dmin[rt] = fminf(dmin[rt], __shfl_xor(dmin[rt], 16));
-      dmin[rt] = fminf(dmin[rt], __shfl_xor(dmin[rt], 32));
-      dmin[rt] = fmaxf(dmin[rt], 0.f);
-      // idle rows get deltas far above every skip threshold (and exps of 0)
-      qad[rt] = row_ok[rt] ? qa[rt] - dmin[rt] : kLooFar;
-    }
-
-    // ---- phase 2: S_ib over the bandwidth grid ----
+    // S: per lane and row tile, FP32 sums of the lane's own four pairs (row c,
+    // data points 4g + r) for all bandwidths, flushed every kLooFlush tiles
+    // into the FP64 sums S64 spread over the lane groups.
     float S[RT][kLooMaxBw];
     double S64[RT][kQ];
 #pragma unroll
@@ -276,10 +283,12 @@ __global__ __launch_bounds__(64 * WAVES) void loo_grid_kernel(
           for (int e = 0; e < 4; ++e) S[rt][4 * q + e] = 0.f;
         }
     };
-    // One data tile: cross terms, deltas, the wave's smallest delta, and the
-    // exps of the bandwidth groups it leaves active. DIAG: the tile holds the
-    // rows of one of the wave's row tiles, whose self pairs are dropped.
     unsigned n_active = 0;  // active (tile, group) pairs of this wave and task
+    // One data tile: squared distances, the running row minima (and the
+    // rescale of a row's sums when its minimum drops), deltas, the wave's
+    // smallest delta, and the exps of the bandwidth groups it leaves active.
+    // DIAG: the tile holds the rows of one of the wave's row tiles, whose self
+    // pairs are dropped.
     auto tile = [&](int64_t j0, auto diag) {
       const f4 xa = ld4(x + (j0 + c) * ld + 4 * g);
       const f4 bb = ld4(nrm + j0 + 4 * g);
@@ -290,17 +299,27 @@ __global__ __launch_bounds__(64 * WAVES) void loo_grid_kernel(
         f4 dd = bb;
 #pragma unroll
         for (int s = 0; s < 4; ++s) dd = mfma_f32(xa[s], qv[rt][s], dd);
-        dd += qad[rt];
+        dd += qa[rt];
         if constexpr (decltype(diag)::value) {
           if (j0 == rbase + 16 * rt) {
 #pragma unroll
             for (int r = 0; r < 4; ++r)
-              if (4 * g + r == c) dd[r] = kLooFar;
+              if (4 * g + r == c) dd[r] = kLooPad;
           }
+        }
+        const float tm = group_min4(fminf(fminf(dd[0], dd[1]), fminf(dd[2], dd[3])));
+        if (tm < m[rt]) {
+          // the row's minimum drops: its sums move to the new shift
+          const float dm = m[rt] - tm;
+#pragma unroll
+          for (int b = 0; b < kLooMaxBw; ++b) S[rt][b] *= fexp2(cf[b] * dm);
+#pragma unroll
+          for (int q = 0; q < kQ; ++q) S64[rt][q] *= double(fexp2(cfl[q] * dm));
+          m[rt] = tm;
         }
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          delta[rt][r] = fmaxf(dd[r], 0.f);
+          delta[rt][r] = dd[r] - m[rt];  // >= 0: m is at most every distance seen
           lo = fminf(lo, delta[rt][r]);
         }
       }
@@ -320,12 +339,16 @@ __global__ __launch_bounds__(64 * WAVES) void loo_grid_kernel(
         }
       }
     };
+    // tiles from the wave's own rows onwards, wrapping around
+    int64_t j0 = rbase < pend ? rbase : pbeg;
     int tc = 0;
-    for (int64_t j0 = pbeg; j0 < pend; j0 += kTile) {
+    for (int64_t k = pbeg; k < pend; k += kTile) {
       if (j0 >= rbase && j0 < rbase + 16 * RT)
         tile(j0, std::true_type{});
       else
         tile(j0, std::false_type{});
+      j0 += kTile;
+      if (j0 == pend) j0 = pbeg;
       if (++tc == kLooFlush) {
         tc = 0;
         flush();
@@ -351,7 +374,7 @@ __global__ __launch_bounds__(64 * WAVES) void loo_grid_kernel(
     }
     double dm = 0.0;
 #pragma unroll
-    for (int rt = 0; rt < RT; ++rt) dm += row_ok[rt] ? double(dmin[rt]) : 0.0;
+    for (int rt = 0; rt < RT; ++rt) dm += row_ok[rt] ? double(m[rt]) : 0.0;
     if (g != 0) dm = 0.0;
     dm += __shfl_xor(dm, 1);
     dm += __shfl_xor(dm, 2);

@@ -96,10 +96,11 @@ def test_batched_graph_replays_and_autotune(gpu):
 
     assert grid.tune in CANDIDATES
     assert set(grid.tune_times) == set(CANDIDATES)
-    # every tile shape gives the same result (eager, no graph)
+    # every tile shape gives the same result up to the order of its FP32 and
+    # FP64 additions (eager, no graph)
     for cand in CANDIDATES:
         grid._launch(cand)
-        np.testing.assert_allclose(grid.out.cpu().numpy(), eager.cpu().numpy(), rtol=1e-12)
+        np.testing.assert_allclose(grid.out.cpu().numpy(), eager.cpu().numpy(), rtol=1e-9)
     # the chosen bandwidth is the grid maximum of the FP64 oracle's curve
     for k, x in enumerate(sets):
         ref = fp64.loo_loglik(x, grid.bandwidths[k])
