@@ -85,6 +85,8 @@ class LooBandwidthGrid:
                 raise ValueError(f"dataset {k}: expected shape (n, 16), got {a.shape}")
             if a.shape[0] < 2:
                 raise ValueError(f"dataset {k}: leave-one-out needs at least 2 points")
+            if not np.all(np.isfinite(a)):
+                raise ValueError(f"dataset {k}: points contain NaN or infinite values.")
         self.sizes = [a.shape[0] for a in arrays]
         self.offsets = np.zeros(len(arrays) + 1, dtype=np.int64)
         self.offsets[1:] = np.cumsum(self.sizes)
@@ -98,6 +100,8 @@ class LooBandwidthGrid:
             raise ValueError(f"bandwidths must be (n_sets, 1..{MAX_BW}), got {bandwidths.shape}")
         if not np.all(bandwidths > 0):
             raise ValueError("bandwidths must be positive")
+        if tune is not None and tuple(tune) not in CANDIDATES:
+            raise ValueError(f"tune must be one of {CANDIDATES}, got {tune}")
         self.bandwidths = bandwidths
         self.n_sets, self.n_bw = bandwidths.shape
         nat.require_gpu(self.device, "LooBandwidthGrid")
@@ -111,8 +115,6 @@ class LooBandwidthGrid:
         nbytes = int(nat.lib().fsk_loo_workspace_bytes(self._off_c, self.n_sets, self.n_bw))
         self.ws = torch.empty(max(nbytes, 256), dtype=torch.uint8, device=self.device)
         self.ws_bytes = nbytes
-        if tune is not None and tuple(tune) not in CANDIDATES:
-            raise ValueError(f"tune must be one of {CANDIDATES}, got {tune}")
         self.tune = tuple(tune) if tune is not None else (
             self.autotune() if autotune else DEFAULT_TUNE)
         self.graph = None

@@ -21,6 +21,12 @@ def test_batched_rejects_bad_input():
         LooBandwidthGrid([np.zeros((5, 16))], bandwidths=np.array([[0.0]]))
     with pytest.raises(ValueError, match="bandwidths"):
         LooBandwidthGrid([np.zeros((5, 16))], bandwidths=np.ones((1, 33)))
+    bad = np.zeros((5, 16))
+    bad[3, 7] = np.nan
+    with pytest.raises(ValueError, match="NaN"):
+        LooBandwidthGrid([np.zeros((4, 16)), bad])
+    with pytest.raises(ValueError, match="tune"):
+        LooBandwidthGrid([np.zeros((5, 16))], tune=(4, 3, 1))
 
 
 def _sets(sizes, seed):
