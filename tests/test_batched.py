@@ -26,7 +26,7 @@ def test_batched_rejects_bad_input():
     with pytest.raises(ValueError, match="NaN"):
         LooBandwidthGrid([np.zeros((4, 16)), bad])
     with pytest.raises(ValueError, match="tune"):
-        LooBandwidthGrid([np.zeros((5, 16))], tune=(4, 3, 1))
+        LooBandwidthGrid([np.random.default_rng(0).normal(size=(5, 16))], tune=(4, 3, 1))
 
 
 def _sets(sizes, seed):
