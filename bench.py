@@ -311,6 +311,28 @@ def surrogate_form(fp32_pair_rate):
     }
 
 
+def kde1_visited_pairs(data, queries, h):
+    """Pairs the 1-D kernels evaluate: per block of 2048 sorted queries, the
+    1024-point chunks of the sorted data within 12.8 / k of the block's first
+    and last query, k = sqrt(log2(e) / (2 h^2)) (beyond that every term is
+    exactly 0 in FP32); the same rule as the library's range kernel, in FP64."""
+    import numpy as np
+
+    xs = np.sort(np.asarray(data, dtype=np.float32)).astype(np.float64)
+    qs = np.sort(np.asarray(queries, dtype=np.float32)).astype(np.float64)
+    reach = 12.8 / math.sqrt(math.log2(math.e) / (2.0 * h * h))
+    n, rows, chunk = xs.size, 2048, 1024
+    n_pad = -(-n // chunk) * chunk
+    first, last = qs[::rows], qs[np.minimum(np.arange(0, qs.size, rows) + rows, qs.size) - 1]
+    jlo = np.searchsorted(xs, first - reach, side="left")
+    jhi = np.searchsorted(xs, last + reach, side="right")
+    c0 = (jlo // chunk) * chunk
+    c1 = np.minimum(-(-jhi // chunk) * chunk, n_pad)
+    width = np.where(jhi > jlo, c1 - c0, 0)
+    nrows = np.minimum(np.arange(0, qs.size, rows) + rows, qs.size) - np.arange(0, qs.size, rows)
+    return float(np.sum(width * nrows))
+
+
 def one_d_sdkde(rates, steps):
     """1-D SD-KDE as in the reference's Triton scaling sweep
     (experiments/configs/runtime/run_triton_scaling.sh, n = 2^21, m = n/8):
@@ -334,7 +356,7 @@ def one_d_sdkde(rates, steps):
     def step():
         pdf, der = _ops.kde1_score_sums(xd, xd, h)
         deb = xd + (0.5 * h) * (der / pdf)  # (h^2/2) * score, score = der / (h pdf)
-        return _ops.kde1_sums(deb, qd, h, atomic=True)
+        return _ops.kde1_sums(deb, qd, h, atomic=True), deb
 
     step()
     torch.cuda.synchronize()
@@ -342,23 +364,27 @@ def one_d_sdkde(rates, steps):
     lib.fsk_profile_enable(1)
     t0 = time.perf_counter()
     for _ in range(steps):
-        out = step()
+        out, deb = step()
     torch.cuda.synchronize()
     ms = (time.perf_counter() - t0) / steps * 1e3
     score_ms = lib.fsk_profile_last_ms(_native.OP_KDE1_SCORE)
     kde_ms = lib.fsk_profile_last_ms(_native.OP_KDE1_ATOMIC)
     lib.fsk_profile_enable(0)
     pairs_s, pairs_k = float(n) * n, float(n) * m
+    vis_s = kde1_visited_pairs(x, x, h)
+    vis_k = kde1_visited_pairs(deb.cpu().numpy(), q, h)
     res = {"workload": f"1-D SD-KDE, n = {n}, m = {m}, N(0,1)", "bandwidth": h,
            "ms_per_step": round(ms, 3), "gpair_evals_per_s": round((pairs_s + pairs_k) / ms / 1e6, 3),
            "score_kernel_ms": round(score_ms, 3), "kde_kernel_ms": round(kde_ms, 3),
+           "visited_pair_frac": {"score": round(vis_s / pairs_s, 4), "kde": round(vis_k / pairs_k, 4)},
            "outputs_finite": bool(torch.isfinite(out).all().item())}
     if rates.get("exp_per_s") and rates.get("fma_per_s"):
-        # per pair: score 1 exp + 4 VALU (sub, mul, add, fma); KDE 1 exp + 3 VALU.
-        # The transcendental issue overlaps the other VALU work in these kernels
-        # (the additive exp + VALU time exceeds the measured time), so each
-        # resource is reported as its own fraction; the larger one binds.
-        for name, pairs, v_ops, kms in (("score", pairs_s, 4, score_ms), ("kde", pairs_k, 3, kde_ms)):
+        # per visited pair: score 1 exp + 5 VALU (sub, mul by k, square, add,
+        # fma); KDE 1 exp + 4 VALU. The transcendental issue overlaps the other
+        # VALU work in these kernels (the additive exp + VALU time exceeds the
+        # measured time), so each resource is reported as its own fraction; the
+        # larger one binds.
+        for name, pairs, v_ops, kms in (("score", vis_s, 5, score_ms), ("kde", vis_k, 4, kde_ms)):
             t_exp = pairs / rates["exp_per_s"]
             t_valu = v_ops * pairs / rates["fma_per_s"]
             res[f"{name}_roofline"] = {

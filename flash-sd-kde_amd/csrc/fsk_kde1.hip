@@ -1,36 +1,110 @@
 // fsk_kde1.hip - 1-D Gaussian KDE sums and density-derivative sums (VALU).
 //
-// Data points are pre-scaled by k = sqrt(log2(e) / (2 h^2)) (x' = k x, one
-// rounding); with u = fma(q, k, -x') (the query's product enters unrounded)
-// the kernel value is exp(-(q-x)^2/(2h^2)) = exp2(-u^2): one FMA, one
-// multiply, one exp2 and one add per pair. The Laplacian factor
-// (1.5 - ln2 u^2) multiplies each pair before the sum, as the reference's
-// phi * (1 + 1/2 - z^2/2) does (flash_sd_kde/reference.py:200-213), so the
-// sum carries no cancellation between two separately rounded totals. A data
-// chunk is staged in LDS and read as wave-wide broadcasts; each thread
-// carries k1dQPT queries so every LDS read feeds several pairs.
+// Per pair, as the reference's 1-D kernels do (kernels/kde_eval_1d_splitk.py:
+// 43-46): the difference d = q - x first (exact for nearby points), then
+// u = d k with k = sqrt(log2(e) / (2 h^2)), and exp(-(q-x)^2/(2h^2)) =
+// exp2(-u^2): a subtraction, two multiplies, one exp2 and one add. Taking the
+// difference first keeps the relative error of u at a few ulp whatever |x|/h
+// is (a pre-scaled x' = k x would carry an absolute error of |k x| ulp into
+// u). The Laplacian factor (1.5 - ln2 u^2) multiplies each pair before the
+// sum, as the reference's phi * (1 + 1/2 - z^2/2) does
+// (flash_sd_kde/reference.py:200-213), so the sum carries no cancellation
+// between two separately rounded totals. A data chunk is staged in LDS and
+// read as wave-wide broadcasts; each thread carries k1dQPT queries so every
+// LDS read feeds several pairs.
+//
+// Sorted order. The data and the queries are first sorted (hipCUB radix
+// sorts, fsk_sort.hip), so a block of consecutive sorted queries spans a
+// narrow range [q_lo, q_hi]. A pair with |u| > 12.7 has u^2 > 161 and
+// exp2(-u^2) < 2^-161, which is exactly 0 in FP32 (the smallest denormal is
+// 2^-149); so each query block only visits the data chunks that meet
+// [q_lo - 12.8/k, q_hi + 12.8/k] (binary search in FP64), and every pair it
+// skips would have added exactly +0. The sums are those of all pairs, in
+// another order. Results are written at the sorted positions and scattered
+// back to the callers' order.
 #include "fsk_common.h"
 #include "fsk_plan.h"
 
 namespace fsk {
 
-// Padded data points sit this far away: u^2 ~ 1e36 stays finite and exp2(-u^2) = 0.
-constexpr float kFar1d = 1.0e18f;
+// Reach of a query in units of 1/k: beyond it every kernel term is exactly 0.
+constexpr double kReach1d = 12.8;
 
-__global__ void prep_scale1(const float *__restrict__ x, int64_t n, int64_t n_pad, float k,
-                            float *__restrict__ out) {
-  int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+size_t sort_f32_temp_bytes(int64_t n, bool pairs);
+int sort_keys_f32(const char *what, const float *in, float *out, int64_t n, void *tmp,
+                  size_t tmp_bytes, hipStream_t s);
+int sort_pairs_f32_i32(const char *what, const float *kin, float *kout, const int *vin,
+                       int *vout, int64_t n, void *tmp, size_t tmp_bytes, hipStream_t s);
+
+// Padded data points (sorted order: past the real ones) sit twice the reach
+// beyond the largest data point and query, so their terms are exactly 0 and
+// u^2 stays finite (no 0 * inf in the Laplacian and moment sums).
+__global__ void pad1_kernel(float *__restrict__ xs, int64_t n, int64_t n_pad,
+                            const float *__restrict__ qs, int64_t m, double reach) {
+  const int64_t j = n + int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (j >= n_pad) return;
-  out[j] = j < n ? x[j] * k : kFar1d;
+  const double top = fmax(double(xs[n - 1]), double(qs[m - 1]));
+  xs[j] = float(top + 2.0 * reach);
 }
+
+__global__ void iota_i32(int *p, int64_t n) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = int(i);
+}
+
+// Data-chunk range [ranges[2b], ranges[2b+1]) of query block b: the chunks
+// that hold the sorted data points within kReach1d of the block's sorted
+// queries (empty when there are none).
+__global__ void kde1_range_kernel(const float *__restrict__ xs, int64_t n, int64_t n_pad,
+                                  const float *__restrict__ qs, int64_t m, int64_t qblocks,
+                                  double reach, int64_t *__restrict__ ranges) {
+  const int64_t b = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (b >= qblocks) return;
+  const int64_t i0 = b * k1dRowsPerWG;
+  const int64_t i1 = (i0 + k1dRowsPerWG < m ? i0 + k1dRowsPerWG : m) - 1;
+  const double lo = double(qs[i0]) - reach, hi = double(qs[i1]) + reach;
+  int64_t a = 0, z = n;  // first j with xs[j] >= lo
+  while (a < z) {
+    const int64_t mid = (a + z) >> 1;
+    if (double(xs[mid]) < lo) a = mid + 1; else z = mid;
+  }
+  const int64_t jlo = a;
+  z = n;  // first j with xs[j] > hi
+  while (a < z) {
+    const int64_t mid = (a + z) >> 1;
+    if (double(xs[mid]) <= hi) a = mid + 1; else z = mid;
+  }
+  const int64_t jhi = a;
+  int64_t c0 = (jlo / k1dChunk) * k1dChunk, c1 = ((jhi + k1dChunk - 1) / k1dChunk) * k1dChunk;
+  if (c1 > n_pad) c1 = n_pad;
+  if (jhi <= jlo) c0 = c1 = 0;
+  ranges[2 * b] = c0;
+  ranges[2 * b + 1] = c1;
+}
+
+// out[perm[i]] = res[i] (and out2 from res2): sorted positions back to the
+// callers' order.
+__global__ void kde1_scatter_kernel(const float *__restrict__ res, const float *__restrict__ res2,
+                                    const int *__restrict__ perm, int64_t m,
+                                    float *__restrict__ out, float *__restrict__ out2) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const int64_t o = perm[i];
+  out[o] = res[i];
+  if (res2) out2[o] = res2[i];
+}
+
 
 // MODE 0: plain sums; MODE 1: sums of phi (1.5 - ln2 u^2);
 // MODE 2: pdf sums and derivative sums sum(-z phi) (two outputs);
 // MODE 3: sums of phi z^2 = phi 2 ln2 u^2 (the Laplacian correction's moment).
+// xs: sorted data (n_pad, padded by pad1_kernel); qs: sorted queries;
+// ranges: each query block's data-chunk range (kde1_range_kernel).
 template <int MODE, bool ATOMIC>
 __global__ __launch_bounds__(256) void kde1_kernel(const float *__restrict__ xs, int64_t n_pad,
                                                    const float *__restrict__ qs, int64_t m,
-                                                   int64_t split_len, int64_t qblocks,
+                                                   const int64_t *__restrict__ ranges,
+                                                   int64_t qblocks,
                                                    int64_t ntasks, int64_t per_xcd,
                                                    float kscale, float zscale,
                                                    float *__restrict__ out,
@@ -50,9 +124,15 @@ __global__ __launch_bounds__(256) void kde1_kernel(const float *__restrict__ xs,
     acc[k] = acc2[k] = 0.f;
     a64[k] = b64[k] = 0.0;
   }
-  const int64_t j_begin = split * split_len;
-  int64_t j_end = j_begin + split_len;
-  if (j_end > n_pad) j_end = n_pad;
+  // this split's share of the block's data range: the range's chunks in
+  // `splits` consecutive runs of equal length (the last may be shorter or
+  // empty). The XCD-aware task order gives each XCD one split of every query
+  // block, so the XCDs get equal work wherever the data is dense.
+  const int64_t splits = ntasks / qblocks;
+  const int64_t c0 = ranges[2 * qb], c1 = ranges[2 * qb + 1];
+  const int64_t per = (c1 - c0 + splits * k1dChunk - 1) / (splits * k1dChunk) * k1dChunk;
+  const int64_t j_begin = c0 + split * per;
+  const int64_t j_end = j_begin + per < c1 ? j_begin + per : c1;
   for (int64_t j0 = j_begin; j0 < j_end; j0 += k1dChunk) {
     __syncthreads();
     reinterpret_cast<float *>(chunk)[tid] = xs[j0 + tid];
@@ -66,7 +146,7 @@ __global__ __launch_bounds__(256) void kde1_kernel(const float *__restrict__ xs,
       for (int s = 0; s < 4; ++s) {
 #pragma unroll
         for (int k = 0; k < k1dQPT; ++k) {
-          const float u = fmaf(q[k], kscale, -xv[s]);
+          const float u = (q[k] - xv[s]) * kscale;
           const float t = u * u;
           const float e = fexp2(-t);
           if (MODE == 1)
@@ -105,20 +185,37 @@ __global__ __launch_bounds__(256) void kde1_kernel(const float *__restrict__ xs,
 }
 
 struct Kde1Layout {
-  float *xs, *part, *part2;
+  float *xs, *qs, *res, *res2, *part, *part2;
+  int *qiota, *qperm;
+  int64_t *ranges;
+  void *tmp;
+  size_t tmp_bytes;
 };
 
+// Returns 0 when the sort scratch size cannot be determined (no device).
 static size_t kde1_layout(void *ws, int64_t n, int64_t m, bool partials, bool two,
                           Kde1Layout *L) {
   Carve cv(ws);
   SplitPlan p = kde1_plan(n, m);
-  L->xs = cv.take<float>(round_up(n, k1dChunk));
+  const int64_t n_pad = round_up(n, k1dChunk);
+  const size_t tk = sort_f32_temp_bytes(n, false), tp = sort_f32_temp_bytes(m, true);
+  if (tk == 0 || tp == 0) return 0;
+  L->tmp_bytes = tk > tp ? tk : tp;
+  L->xs = cv.take<float>(n_pad);
+  L->qs = cv.take<float>(m);
+  L->qiota = cv.take<int>(m);
+  L->qperm = cv.take<int>(m);
+  L->ranges = cv.take<int64_t>(2 * p.qblocks);
+  L->res = cv.take<float>(m);
+  L->res2 = two ? cv.take<float>(m) : nullptr;
   L->part = partials ? cv.take<float>(p.splits * m) : nullptr;
   L->part2 = (partials && two) ? cv.take<float>(p.splits * m) : nullptr;
+  L->tmp = cv.take<char>(int64_t(L->tmp_bytes));
   return cv.used;
 }
 
 size_t kde1_ws_bytes(int64_t n, int64_t m, bool partials, bool two) {
+  if (round_up(n, k1dChunk) > INT32_MAX || m > INT32_MAX) return 0;
   Kde1Layout L;
   return kde1_layout(nullptr, n, m, partials, two, &L);
 }
@@ -134,8 +231,17 @@ static int kde1_run(const char *what, int mode, bool atomic, const float *data, 
     set_error("%s: bandwidth must be positive, got %g", what, h);
     return FSK_EINVAL;
   }
+  if (round_up(n, k1dChunk) > INT32_MAX || m > INT32_MAX) {
+    set_error("%s: the 1-D path sorts at most 2^31 - 1 points and queries (n=%lld m=%lld)",
+              what, (long long)n, (long long)m);
+    return FSK_EUNSUPPORTED;
+  }
   Kde1Layout L;
   size_t need = kde1_layout(ws, n, m, !atomic, mode == 2, &L);
+  if (need == 0) {
+    set_error("%s: sort scratch size unavailable (no HIP device)", what);
+    return FSK_ELAUNCH;
+  }
   if (!ws || ws_bytes < need) {
     set_error("%s: workspace of %zu bytes needed, got %zu", what, need, ws_bytes);
     return FSK_EWORKSPACE;
@@ -143,25 +249,36 @@ static int kde1_run(const char *what, int mode, bool atomic, const float *data, 
   hipStream_t s = as_stream(stream);
   const double k = sqrt(kLog2e / (2.0 * h * h));
   const int64_t n_pad = round_up(n, k1dChunk);
-  hipLaunchKernelGGL(prep_scale1, dim3(cdiv(n_pad, 256)), dim3(256), 0, s, data, n, n_pad,
-                     float(k), L.xs);
+  int st;
+  if ((st = sort_keys_f32(what, data, L.xs, n, L.tmp, L.tmp_bytes, s)) != FSK_OK) return st;
+  hipLaunchKernelGGL(iota_i32, dim3(cdiv(m, 256)), dim3(256), 0, s, L.qiota, m);
+  if ((st = sort_pairs_f32_i32(what, query, L.qs, L.qiota, L.qperm, m, L.tmp, L.tmp_bytes, s)) !=
+      FSK_OK)
+    return st;
+  if (n_pad > n)
+    hipLaunchKernelGGL(pad1_kernel, dim3(cdiv(n_pad - n, 256)), dim3(256), 0, s, L.xs, n, n_pad,
+                       L.qs, m, kReach1d / k);
   SplitPlan p = kde1_plan(n, m);
+  hipLaunchKernelGGL(kde1_range_kernel, dim3(cdiv(p.qblocks, 256)), dim3(256), 0, s, L.xs, n,
+                     n_pad, L.qs, m, p.qblocks, kReach1d / k, L.ranges);
   const int64_t ntasks = p.qblocks * p.splits;
   int64_t per_xcd;
   const int64_t grid = xcd_grid(ntasks, &per_xcd);
   // z = (q - x)/h = u / sqrt(log2(e)/2)  ->  -z = -u * sqrt(2 ln 2)
   const float zscale = float(sqrt(2.0 * kLn2));
-  float *o1 = atomic ? out : L.part;
-  float *o2 = atomic ? out2 : L.part2;
+  float *o1 = atomic ? L.res : L.part;
+  float *o2 = atomic ? L.res2 : L.part2;
   if (atomic) {
-    hipLaunchKernelGGL(fill_f32, dim3(cdiv(m, 256)), dim3(256), 0, s, out, m, 0.f);
-    if (mode == 2) hipLaunchKernelGGL(fill_f32, dim3(cdiv(m, 256)), dim3(256), 0, s, out2, m, 0.f);
+    hipLaunchKernelGGL(fill_f32, dim3(cdiv(m, 256)), dim3(256), 0, s, L.res, m, 0.f);
+    if (mode == 2)
+      hipLaunchKernelGGL(fill_f32, dim3(cdiv(m, 256)), dim3(256), 0, s, L.res2, m, 0.f);
   }
   const int op = mode == 2 ? FSK_OP_KDE1_SCORE : (atomic ? FSK_OP_KDE1_ATOMIC : FSK_OP_KDE1_SPLITK);
   prof_begin(op, s);
 #define FSK_K1(MODE, AT)                                                                      \
-  hipLaunchKernelGGL((kde1_kernel<MODE, AT>), dim3(grid), dim3(256), 0, s, L.xs, n_pad, query, m, \
-                     p.split_len, p.qblocks, ntasks, per_xcd, float(k), zscale, o1, o2, m)
+  hipLaunchKernelGGL((kde1_kernel<MODE, AT>), dim3(grid), dim3(256), 0, s, L.xs, n_pad, L.qs, m, \
+                     L.ranges, p.qblocks, ntasks, per_xcd, float(k), zscale, o1, o2, \
+                     m)
   if (atomic) {
     if (mode == 0) FSK_K1(0, true);
     else if (mode == 1) FSK_K1(1, true);
@@ -175,11 +292,16 @@ static int kde1_run(const char *what, int mode, bool atomic, const float *data, 
   }
 #undef FSK_K1
   prof_end(op, s);
-  int st = launch_status(what);
-  if (st != FSK_OK || atomic) return st;
-  if ((st = fsk_reduce_partials_f64(L.part, p.splits, m, m, out, stream)) != FSK_OK) return st;
-  if (mode == 2) st = fsk_reduce_partials_f64(L.part2, p.splits, m, m, out2, stream);
-  return st;
+  if ((st = launch_status(what)) != FSK_OK) return st;
+  if (!atomic) {
+    if ((st = fsk_reduce_partials_f64(L.part, p.splits, m, m, L.res, stream)) != FSK_OK) return st;
+    if (mode == 2 && (st = fsk_reduce_partials_f64(L.part2, p.splits, m, m, L.res2, stream)) !=
+                         FSK_OK)
+      return st;
+  }
+  hipLaunchKernelGGL(kde1_scatter_kernel, dim3(cdiv(m, 256)), dim3(256), 0, s, L.res,
+                     mode == 2 ? L.res2 : nullptr, L.qperm, m, out, out2);
+  return launch_status(what);
 }
 
 }  // namespace fsk

@@ -199,6 +199,11 @@ int fsk_kde16_logdensity(const float *data, int64_t ld, int64_t n,
 
 /* 1-D Gaussian KDE sums, split-K with FP64 combine.
  * out[i] = sum_j exp(-(q_i - x_j)^2 / (2 h^2))  (FSK_FLAG_LAPLACIAN: times 1.5 - (q-x)^2/(2h^2)).
+ * All 1-D entry points sort the scaled data and the queries on the device
+ * (radix sort in the workspace) and let each block of sorted queries visit
+ * only the data within the distance beyond which every term is exactly 0 in
+ * FP32; n and m are at most 2^31 - 1, and their workspace size is queried
+ * from the current device (fsk_workspace_bytes returns 0 without one).
  * Replaces kde_eval_1d_splitk, kernels/kde_eval_1d_splitk.py:141-219. */
 int fsk_kde1_splitk(const float *data, int64_t n, const float *query, int64_t m,
                     double bandwidth, unsigned flags,

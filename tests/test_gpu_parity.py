@@ -197,6 +197,37 @@ def test_kde1d_flash_and_score(gpu, golden):
     np.testing.assert_allclose(deb, golden("one_d_sd", "debiased"), rtol=1e-5, atol=1e-5)
 
 
+@pytest.mark.parametrize("backend", ["splitk_stream", "atomic"])
+def test_kde1d_sorted_ranges_match_fp64(gpu, backend):
+    """The 1-D kernels sort data and queries and visit, per block of sorted
+    queries, only the data chunks within the distance beyond which every term
+    is exactly 0 in FP32. With a bandwidth far below the data's spread most
+    chunks are skipped: the sums must still be those of all pairs (FP64
+    brute force), including queries past the data's ends, isolated outliers,
+    duplicate values and a query order that is not sorted."""
+    rng = np.random.default_rng(11)
+    x = np.concatenate([rng.normal(size=30000), [-40.0, 35.0, 35.0, 80.0],
+                        np.full(50, 0.25)]).astype(np.float32)
+    q = np.concatenate([rng.normal(size=4000) * 1.5, [-40.01, 35.0, 100.0, -100.0]])
+    q = q[rng.permutation(q.size)].astype(np.float32)
+    h = 0.01
+    out = cpu(flash_sd_kde.kde_eval(x, q, h, device=gpu, kde_backend=backend))
+    z = (q.astype(np.float64)[:, None] - x.astype(np.float64)[None, :]) / h
+    phi = np.exp(-0.5 * z * z)
+    ref = phi.sum(1) / (math.sqrt(2 * math.pi) * h * x.size)
+    assert np.all(np.abs(out - ref) <= 1e-5 * ref + 1e-30), np.max(np.abs(out - ref) / (ref + 1e-300))
+    assert out[q == np.float32(100.0)] == 0.0 and out[q == np.float32(-100.0)] == 0.0
+    # score sums with rows = data (self pairs included)
+    xs = x[rng.permutation(x.size)[:12000]]
+    score, pdf, _ = gaussian_kde_score_triton(xs, xs, h, device=gpu)
+    zz = (xs.astype(np.float64)[:, None] - xs.astype(np.float64)[None, :]) / h
+    pp = np.exp(-0.5 * zz * zz)
+    pdf_ref = pp.sum(1) / (math.sqrt(2 * math.pi) * h * xs.size)
+    np.testing.assert_allclose(cpu(pdf), pdf_ref, rtol=1e-5)
+    score_ref = (-zz * pp).sum(1) / pp.sum(1) / h
+    np.testing.assert_allclose(cpu(score), score_ref, rtol=1e-4, atol=1e-3 / h)
+
+
 # ---------------- empirical score ----------------
 
 def score_exact(x, h):
