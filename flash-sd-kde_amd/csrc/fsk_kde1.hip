@@ -1,12 +1,13 @@
 // fsk_kde1.hip - 1-D Gaussian KDE sums and density-derivative sums (VALU).
 //
-// Per pair, as the reference's 1-D kernels do (kernels/kde_eval_1d_splitk.py:
-// 43-46): the difference d = q - x first (exact for nearby points), then
-// u = d k with k = sqrt(log2(e) / (2 h^2)), and exp(-(q-x)^2/(2h^2)) =
-// exp2(-u^2): a subtraction, two multiplies, one exp2 and one add. Taking the
-// difference first keeps the relative error of u at a few ulp whatever |x|/h
-// is (a pre-scaled x' = k x would carry an absolute error of |k x| ulp into
-// u). The Laplacian factor (1.5 - ln2 u^2) multiplies each pair before the
+// Per pair, difference first as the reference's 1-D kernels do
+// (kernels/kde_eval_1d_splitk.py:43-46): d = q - x (exact for nearby
+// points), t = (d d) K with K = k^2 = log2(e) / (2 h^2) rounded once, and
+// exp(-(q-x)^2/(2h^2)) = exp2(-t): a subtraction, two multiplies, one exp2
+// and one add. t carries three roundings whatever |x|/h is (a pre-scaled
+// x' = k x would carry an absolute error of |k x| ulp into k (q - x), and
+// squaring a rounded k doubles its rounding). The Laplacian factor
+// (1.5 - ln2 t) multiplies each pair before the
 // sum, as the reference's phi * (1 + 1/2 - z^2/2) does
 // (flash_sd_kde/reference.py:200-213), so the sum carries no cancellation
 // between two separately rounded totals. A data chunk is staged in LDS and
@@ -15,8 +16,8 @@
 //
 // Sorted order. The data and the queries are first sorted (hipCUB radix
 // sorts, fsk_sort.hip), so a block of consecutive sorted queries spans a
-// narrow range [q_lo, q_hi]. A pair with |u| > 12.7 has u^2 > 161 and
-// exp2(-u^2) < 2^-161, which is exactly 0 in FP32 (the smallest denormal is
+// narrow range [q_lo, q_hi]. A pair with k |q - x| > 12.7 has t > 161 and
+// exp2(-t) < 2^-161, which is exactly 0 in FP32 (the smallest denormal is
 // 2^-149); so each query block only visits the data chunks that meet
 // [q_lo - 12.8/k, q_hi + 12.8/k] (binary search in FP64), and every pair it
 // skips would have added exactly +0. The sums are those of all pairs, in
@@ -29,6 +30,9 @@ namespace fsk {
 
 // Reach of a query in units of 1/k: beyond it every kernel term is exactly 0.
 constexpr double kReach1d = 12.8;
+// Points per FP32 run of the sums before they are added in FP64.
+constexpr int kFlush1d = 256;
+static_assert(k1dChunk % kFlush1d == 0, "a chunk holds whole runs");
 
 size_t sort_f32_temp_bytes(int64_t n, bool pairs);
 int sort_keys_f32(const char *what, const float *in, float *out, int64_t n, void *tmp,
@@ -106,7 +110,7 @@ __global__ __launch_bounds__(256) void kde1_kernel(const float *__restrict__ xs,
                                                    const int64_t *__restrict__ ranges,
                                                    int64_t qblocks,
                                                    int64_t ntasks, int64_t per_xcd,
-                                                   float kscale, float zscale,
+                                                   float ksq, float zscale,
                                                    float *__restrict__ out,
                                                    float *__restrict__ out2, int64_t ld_out) {
   __shared__ f4 chunk[k1dChunk / 4];
@@ -115,13 +119,13 @@ __global__ __launch_bounds__(256) void kde1_kernel(const float *__restrict__ xs,
   const int64_t split = task / qblocks;
   const int64_t qb = task - split * qblocks;
   const int tid = threadIdx.x;
-  float q[k1dQPT], acc[k1dQPT], acc2[k1dQPT];
+  float q[k1dQPT], acc[k1dQPT], accb[k1dQPT], acc2[k1dQPT];
   double a64[k1dQPT], b64[k1dQPT];
 #pragma unroll
   for (int k = 0; k < k1dQPT; ++k) {
     int64_t i = qb * k1dRowsPerWG + k * 256 + tid;
     q[k] = qs[i < m ? i : m - 1];
-    acc[k] = acc2[k] = 0.f;
+    acc[k] = accb[k] = acc2[k] = 0.f;
     a64[k] = b64[k] = 0.0;
   }
   // this split's share of the block's data range: the range's chunks in
@@ -140,30 +144,39 @@ __global__ __launch_bounds__(256) void kde1_kernel(const float *__restrict__ xs,
     reinterpret_cast<float *>(chunk)[tid + 512] = xs[j0 + tid + 512];
     reinterpret_cast<float *>(chunk)[tid + 768] = xs[j0 + tid + 768];
     __syncthreads();
-    for (int jj = 0; jj < k1dChunk / 4; ++jj) {
-      const f4 xv = chunk[jj];
+    // FP32 sums over runs of kFlush1d points, two per query (even and odd
+    // points: half the chain length at no extra instruction), then FP64.
+    // (Adding each four terms pairwise before the running sum, query by
+    // query, measured 23 % slower for the score: the eight queries' chains
+    // interleave less.)
+    for (int jb = 0; jb < k1dChunk / 4; jb += kFlush1d / 4) {
+      for (int jj = jb; jj < jb + kFlush1d / 4; ++jj) {
+        const f4 xv = chunk[jj];
 #pragma unroll
-      for (int s = 0; s < 4; ++s) {
+        for (int s = 0; s < 4; ++s) {
 #pragma unroll
-        for (int k = 0; k < k1dQPT; ++k) {
-          const float u = (q[k] - xv[s]) * kscale;
-          const float t = u * u;
-          const float e = fexp2(-t);
-          if (MODE == 1)
-            acc[k] = fmaf(e, fmaf(t, -float(kLn2), 1.5f), acc[k]);
-          else if (MODE == 3)
-            acc[k] = fmaf(e, t, acc[k]);
-          else
-            acc[k] += e;
-          if (MODE == 2) acc2[k] = fmaf(e, u, acc2[k]);
+          for (int k = 0; k < k1dQPT; ++k) {
+            const float d = q[k] - xv[s];
+            const float t = (d * d) * ksq;
+            const float e = fexp2(-t);
+            // the even and the odd points of a run go to separate sums
+            float &a = (s & 1) ? accb[k] : acc[k];
+            if (MODE == 1)
+              a = fmaf(e, fmaf(t, -float(kLn2), 1.5f), a);
+            else if (MODE == 3)
+              a = fmaf(e, t, a);
+            else
+              a += e;
+            if (MODE == 2) acc2[k] = fmaf(e, d, acc2[k]);
+          }
         }
       }
-    }
 #pragma unroll
-    for (int k = 0; k < k1dQPT; ++k) {
-      a64[k] += acc[k];
-      b64[k] += acc2[k];
-      acc[k] = acc2[k] = 0.f;
+      for (int k = 0; k < k1dQPT; ++k) {
+        a64[k] += double(acc[k]) + double(accb[k]);
+        b64[k] += acc2[k];
+        acc[k] = accb[k] = acc2[k] = 0.f;
+      }
     }
   }
 #pragma unroll
@@ -172,7 +185,7 @@ __global__ __launch_bounds__(256) void kde1_kernel(const float *__restrict__ xs,
     if (i >= m) continue;
     const double v = MODE == 3 ? 2.0 * kLn2 * a64[k] : a64[k];
     float v2 = 0.f;
-    // sum(-z phi) with z = u / sqrt(log2 e / 2): -z = -u * sqrt(2 ln 2)
+    // sum(-z phi) with z = (q - x) / h: acc2 holds sum(phi (q - x)), zscale = 1/h
     if (MODE == 2) v2 = float(-b64[k] * double(zscale));
     if (ATOMIC) {
       atomicAdd(out + i, float(v));
@@ -264,8 +277,8 @@ static int kde1_run(const char *what, int mode, bool atomic, const float *data, 
   const int64_t ntasks = p.qblocks * p.splits;
   int64_t per_xcd;
   const int64_t grid = xcd_grid(ntasks, &per_xcd);
-  // z = (q - x)/h = u / sqrt(log2(e)/2)  ->  -z = -u * sqrt(2 ln 2)
-  const float zscale = float(sqrt(2.0 * kLn2));
+  // the score's derivative sums are sum(phi (q - x)) times -1/h
+  const float zscale = float(1.0 / h);
   float *o1 = atomic ? L.res : L.part;
   float *o2 = atomic ? L.res2 : L.part2;
   if (atomic) {
@@ -277,7 +290,7 @@ static int kde1_run(const char *what, int mode, bool atomic, const float *data, 
   prof_begin(op, s);
 #define FSK_K1(MODE, AT)                                                                      \
   hipLaunchKernelGGL((kde1_kernel<MODE, AT>), dim3(grid), dim3(256), 0, s, L.xs, n_pad, L.qs, m, \
-                     L.ranges, p.qblocks, ntasks, per_xcd, float(k), zscale, o1, o2, \
+                     L.ranges, p.qblocks, ntasks, per_xcd, float(k * k), zscale, o1, o2, \
                      m)
   if (atomic) {
     if (mode == 0) FSK_K1(0, true);
