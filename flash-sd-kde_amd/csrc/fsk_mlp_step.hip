@@ -5,9 +5,12 @@
 // batch rows one at a time: gathering a row, the four forward layers, the
 // MSE gradient and the data-gradient chain back to layer 1 need only that
 // row and the weights. So a workgroup takes 32 batch rows and runs them
-// through all seven GEMMs (4 forward, 3 data-gradient) with the activations
-// and pre-activations held in LDS (about 131 KB), reading only the BF16
-// weights (from L2) between stages. It writes what the weight gradients need
+// through all seven GEMMs (4 forward, 3 data-gradient), reading only the BF16
+// weights (from L2) between stages. Eight waves split each layer's 256
+// columns (32 each, two per SIMD to hide the weight loads' latency); the BF16
+// activations pass between layers through LDS (34 KB) and each wave keeps
+// the pre-activations of its own columns in registers, which is where the
+// data-gradient GEMM that reads them puts the same elements. It writes what the weight gradients need
 // - the gathered inputs (FP32), the activations h1..h3 as the BF16 values
 // the next layer multiplies (half the bytes of FP32; the weight-gradient GEMM
 // rounds its inputs to BF16 anyway) and the output gradients dz1..dz4 (FP32:
@@ -27,14 +30,15 @@
 
 namespace fsk {
 
-// batch rows per workgroup: 32 (one workgroup per CU, LDS-bound) trained the
-// learned-score configuration in 0.576 s, 16 (two per CU) in 0.62 s
+// batch rows per workgroup (32: every CU gets one workgroup of a batch of
+// 8192 rows, and each reads the weights once per stage)
 constexpr int RS_ROWS = 32;
+constexpr int RS_WAVES = 8;              // waves per workgroup
+constexpr int RS_NB = 256 / (16 * RS_WAVES);  // 16-column tiles per wave in a hidden layer
 constexpr int RS_RX = RS_ROWS / 16;      // 16-row MFMA tiles per workgroup
 constexpr int RS_H = 256;                // hidden width
 constexpr int RS_D = 16;                 // input / output width
 constexpr int RS_ACT = RS_H + 8;         // u16 per LDS row of a BF16 activation
-constexpr int RS_Z = RS_H + 4;           // floats per LDS row of a pre-activation
 constexpr int RS_SMALL = RS_D + 8;       // u16 per LDS row of a 16-wide BF16 block
 
 __device__ __forceinline__ float rs_sigmoid(float z) {
@@ -48,14 +52,14 @@ __device__ __forceinline__ float rs_silu_grad(float z) {
 
 // acc (this wave's part of the 32 x N block) = A (32 x K, BF16, LDS, row
 // stride lda) . W^T with W (N x K) BF16 row-major in global memory. For
-// N >= 64 wave w owns columns [16 NB w, 16 NB (w+1)), NB = N / 64; for N = 16
+// N = 256 wave w owns columns [16 NB w, 16 NB (w+1)), NB = RS_NB; for N = 16
 // wave 0 computes the block alone. Lane (g, c) of tile (x, y) holds
 // rows 16x + 4g + r, column col0 + 16y + c.
 template <int K, int N>
 __device__ __forceinline__ bool rs_gemm(const unsigned short *A, int lda,
                                         const unsigned short *__restrict__ W, int wave, int g,
-                                        int c, f4 (&acc)[RS_RX][N >= 64 ? N / 64 : 1], int &col0) {
-  constexpr int NB = N >= 64 ? N / 64 : 1;
+                                        int c, f4 (&acc)[RS_RX][N == RS_H ? RS_NB : 1], int &col0) {
+  constexpr int NB = N == RS_H ? RS_NB : 1;
   constexpr int KS = (K + 31) / 32;
   col0 = wave * NB * 16;
 #pragma unroll
@@ -103,12 +107,13 @@ struct RowsStepArgs {
   double *loss;
 };
 
-__global__ __launch_bounds__(256) void mlp_rows_step_kernel(RowsStepArgs a) {
+__global__ __launch_bounds__(64 * RS_WAVES) void mlp_rows_step_kernel(RowsStepArgs a) {
   __shared__ __align__(16) unsigned short act[2][RS_ROWS * RS_ACT];  // BF16 ping-pong
-  __shared__ __align__(16) float zs[3][RS_ROWS * RS_Z];              // z1..z3
   __shared__ __align__(16) unsigned short small[RS_ROWS * RS_SMALL];  // x0, then dz4 (BF16)
   __shared__ __align__(16) float ts[RS_ROWS * RS_D];                  // targets
-  __shared__ double sl[4];
+  __shared__ double sl[RS_WAVES];
+  // z1..z3 of this wave's columns, in the accumulator layout
+  f4 zr[3][RS_RX][RS_NB];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int g = lane >> 4, c = lane & 15;
   const int64_t r0 = int64_t(blockIdx.x) * RS_ROWS;
@@ -140,10 +145,10 @@ __global__ __launch_bounds__(256) void mlp_rows_step_kernel(RowsStepArgs a) {
   // ---- forward, hidden layers: z_l = A W_l^T + b_l, h_l = silu(z_l)
   auto hidden = [&](const unsigned short *A, int lda, int l, unsigned short *out, auto kdim) {
     constexpr int K = decltype(kdim)::value;
-    f4 acc[RS_RX][4];
+    f4 acc[RS_RX][RS_NB];
     if (rs_gemm<K, RS_H>(A, lda, a.w[l], wave, g, c, acc, col0)) {
 #pragma unroll
-      for (int y = 0; y < 4; ++y) {
+      for (int y = 0; y < RS_NB; ++y) {
         const int nn = col0 + 16 * y + c;
         const float bv = a.bias[l][nn];
 #pragma unroll
@@ -153,7 +158,7 @@ __global__ __launch_bounds__(256) void mlp_rows_step_kernel(RowsStepArgs a) {
             const int row = 16 * x + 4 * g + r;
             const float z = acc[x][y][r] + bv;
             const unsigned short hv = to16<FSK_PREC_BF16>(rs_silu(z));
-            zs[l][row * RS_Z + nn] = z;
+            zr[l][x][y][r] = z;
             out[row * RS_ACT + nn] = hv;
             if (r0 + row < a.batch) a.h[l][(r0 + row) * RS_H + nn] = hv;
           }
@@ -190,23 +195,24 @@ __global__ __launch_bounds__(256) void mlp_rows_step_kernel(RowsStepArgs a) {
     for (int o = 32; o >= 1; o >>= 1) lsum += __shfl_xor(lsum, o);
     if (lane == 0) sl[wave] = lsum;
     __syncthreads();
-    if (tid == 0 && a.loss) atomicAdd(a.loss, (sl[0] + sl[1]) + (sl[2] + sl[3]));
+    // only wave 0 computes the 16-wide output layer; the others add 0
+    if (tid == 0 && a.loss) atomicAdd(a.loss, sl[0]);
   }
 
   // ---- data gradients: dz_{l-1} = (dz_l W_l) * silu'(z_{l-1}), l = 4, 3, 2
   auto backward = [&](const unsigned short *A, int lda, int l, unsigned short *out, auto kdim) {
     constexpr int K = decltype(kdim)::value;  // width of dz_l (the reduction)
-    f4 acc[RS_RX][4];
+    f4 acc[RS_RX][RS_NB];
     if (rs_gemm<K, RS_H>(A, lda, a.wt[l], wave, g, c, acc, col0)) {
 #pragma unroll
-      for (int y = 0; y < 4; ++y) {
+      for (int y = 0; y < RS_NB; ++y) {
         const int nn = col0 + 16 * y + c;
 #pragma unroll
         for (int x = 0; x < RS_RX; ++x)
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
             const int row = 16 * x + 4 * g + r;
-            const float v = acc[x][y][r] * rs_silu_grad(zs[l - 1][row * RS_Z + nn]);
+            const float v = acc[x][y][r] * rs_silu_grad(zr[l - 1][x][y][r]);
             if (out) out[row * RS_ACT + nn] = to16<FSK_PREC_BF16>(v);
             if (r0 + row < a.batch) a.dz[l - 1][(r0 + row) * RS_H + nn] = v;
           }
@@ -258,7 +264,7 @@ extern "C" int fsk_mlp_rows_step(const fsk_mlp_rows_step_args *p, void *stream) 
   a.xb = p->xb;
   a.mse_scale = float(p->mse_scale);
   a.loss = p->loss;
-  hipLaunchKernelGGL(mlp_rows_step_kernel, dim3(unsigned(cdiv(p->batch, RS_ROWS))), dim3(256), 0,
-                     as_stream(stream), a);
+  hipLaunchKernelGGL(mlp_rows_step_kernel, dim3(unsigned(cdiv(p->batch, RS_ROWS))),
+                     dim3(64 * RS_WAVES), 0, as_stream(stream), a);
   return launch_status(what);
 }
