@@ -249,6 +249,17 @@ static int kde1_run(const char *what, int mode, bool atomic, const float *data, 
               what, (long long)n, (long long)m);
     return FSK_EUNSUPPORTED;
   }
+  {
+    // the radix sorts record memset nodes, which do not replay correctly from
+    // a HIP graph: refuse capture rather than give wrong results later
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(as_stream(stream), &cs) == hipSuccess &&
+        cs != hipStreamCaptureStatusNone) {
+      set_error("%s: the 1-D path sorts with library kernels that cannot be captured in a "
+                "HIP graph; run it outside capture", what);
+      return FSK_EUNSUPPORTED;
+    }
+  }
   Kde1Layout L;
   size_t need = kde1_layout(ws, n, m, !atomic, mode == 2, &L);
   if (need == 0) {

@@ -203,7 +203,9 @@ int fsk_kde16_logdensity(const float *data, int64_t ld, int64_t n,
  * (radix sort in the workspace) and let each block of sorted queries visit
  * only the data within the distance beyond which every term is exactly 0 in
  * FP32; n and m are at most 2^31 - 1, and their workspace size is queried
- * from the current device (fsk_workspace_bytes returns 0 without one).
+ * from the current device (fsk_workspace_bytes returns 0 without one). The
+ * sorts record memset nodes, so the 1-D entry points refuse HIP graph
+ * capture (FSK_EUNSUPPORTED); every other entry point is capturable.
  * Replaces kde_eval_1d_splitk, kernels/kde_eval_1d_splitk.py:141-219. */
 int fsk_kde1_splitk(const float *data, int64_t n, const float *query, int64_t m,
                     double bandwidth, unsigned flags,

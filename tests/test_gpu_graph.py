@@ -74,3 +74,25 @@ def test_symmetric_sdkde_step_replays(gpu):
         torch.cuda.synchronize(gpu)
         torch.testing.assert_close(out, eager, rtol=1e-6, atol=0.0,
                                    msg=lambda m: f"replay {k}: {m}")
+
+
+def test_one_d_path_refuses_graph_capture(gpu):
+    """The 1-D kernels sort with library kernels that record memset nodes,
+    which do not replay reliably from a HIP graph: under capture the entry
+    point refuses with a clear error and launches nothing; outside capture
+    the same call works."""
+    x = torch.linspace(-3.0, 3.0, 5000, device=gpu)
+    q = torch.linspace(-2.0, 2.0, 300, device=gpu)
+    eager = _ops.kde1_sums(x, q, 0.3)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    side = torch.cuda.Stream(device=gpu)
+    side.wait_stream(torch.cuda.current_stream(gpu))
+    with pytest.raises(RuntimeError, match="capture"):
+        with torch.cuda.stream(side):
+            with torch.cuda.graph(g, stream=side):
+                _ops.kde1_sums(x, q, 0.3)
+    torch.cuda.current_stream(gpu).wait_stream(side)
+    torch.cuda.synchronize()
+    again = _ops.kde1_sums(x, q, 0.3)
+    assert torch.equal(again, eager)
