@@ -117,65 +117,55 @@ __global__ __launch_bounds__(256) void loo_center_kernel(const float *__restrict
 // Bandwidth skip groups. A term exp2(c_b delta) with c_b delta <= -64 is
 // below 2^-64, and S_ib >= 1 (the nearest neighbour's term is exp2(0)), so
 // all such terms of a row together change S_ib by less than n 2^-64 relative
-// (2^-46 at n = 2^18): far below one FP32 rounding. Per data tile the wave
-// finds the smallest delta over its pairs, dlo, and skips every group of
-// kLooGroup bandwidths whose widest member has c_b dlo <= -64. At the grid's
-// narrow end (0.05 h, c_b 400 times that at h) most tiles skip most groups.
+// (2^-46 at n = 2^18): far below one FP32 rounding. Per data tile a group of
+// kLooGroup bandwidths is evaluated only if some lane of the wave has a delta
+// below 64 / |c_b| for the group's widest member (one compare and ballot per
+// group); every other group is skipped. At the grid's narrow end (0.05 h, c_b
+// 400 times that at h) most tiles skip most groups.
 constexpr int kLooGroup = 4;
 constexpr int kLooGroups = kLooMaxBw / kLooGroup;
 constexpr float kLooSkipExp = 64.f;
 
-template <int CTRL>
-__device__ __forceinline__ int dpp_i32(int v) {
-  return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, false);
-}
-
-// Minimum over the wave of a non-negative float, returned as its bit pattern
-// (wave-uniform). For non-negative floats the bit patterns order like the
-// values; a -0.0 (bit pattern INT_MIN as a signed int) is taken as +0.
-__device__ __forceinline__ int wave_min_nonneg_bits(float v) {
-  int b = __builtin_bit_cast(int, v);
-  b = b < 0 ? 0 : b;
-  b = min(b, dpp_i32<0xB1>(b));   // quad_perm [1,0,3,2]
-  b = min(b, dpp_i32<0x4E>(b));   // quad_perm [2,3,0,1]
-  b = min(b, dpp_i32<0x124>(b));  // row_ror:4
-  b = min(b, dpp_i32<0x128>(b));  // row_ror:8: every lane holds its 16-lane row's minimum
-  const int r0 = __builtin_amdgcn_readlane(b, 0), r1 = __builtin_amdgcn_readlane(b, 16);
-  const int r2 = __builtin_amdgcn_readlane(b, 32), r3 = __builtin_amdgcn_readlane(b, 48);
-  return min(min(r0, r1), min(r2, r3));
-}
-
-// Sum over the four lane groups of a wave for four values at once: lane
-// (g, c) returns v[g] summed over the lanes {c, c+16, c+32, c+48}. Each of the
-// two exchange stages keeps the half of the values the lane is to reduce and
-// sends the other half to its partner (3 shuffles against 8 for four separate
-// sums).
-__device__ __forceinline__ float group_sum_exchange4(float v0, float v1, float v2, float v3,
-                                                     int g) {
-  const bool h2 = g & 2;
-  const float a0 = (h2 ? v2 : v0) + __shfl_xor(h2 ? v0 : v2, 32);
-  const float a1 = (h2 ? v3 : v1) + __shfl_xor(h2 ? v1 : v3, 32);
-  const bool h1 = g & 1;
-  return (h1 ? a1 : a0) + __shfl_xor(h1 ? a0 : a1, 16);
-}
-
-// Minimum over the four lane groups: lane (g, c) returns the minimum of v over
-// the lanes {c, c+16, c+32, c+48}. v_permlane32_swap a, b swaps the upper 32
-// lanes of a with the lower 32 lanes of b, leaving [a_lo | b_lo] in a and
-// [a_hi | b_hi] in b; v_permlane16_swap does the same with the odd 16-lane
-// rows of a and the even rows of b. Started from a = b = v, the two results
-// hold v and its partner 32 (then 16) lanes away in every lane.
-// The swaps are written as register-only inline asm: hipcc 7.2 folds a
+// Register swaps between lane halves (32 lanes) and between neighbouring
+// 16-lane rows. v_permlane32_swap a, b swaps the upper 32 lanes of a with the
+// lower 32 lanes of b, leaving [a_lo | b_lo] in a and [a_hi | b_hi] in b;
+// v_permlane16_swap does the same with the odd 16-lane rows of a and the even
+// rows of b. They are written as register-only inline asm: hipcc 7.2 folds a
 // combination of the two results of __builtin_amdgcn_permlane*_swap into the
 // first result alone (min(a[0], a[1]) becomes a[0]). The s_nop 1 gives the
 // two wait states a swap needs after a VALU write of its operands.
+__device__ __forceinline__ void swap32(float &a, float &b) {
+  asm volatile("s_nop 1\n\tv_permlane32_swap_b32 %0, %1" : "+v"(a), "+v"(b));
+}
+__device__ __forceinline__ void swap16(float &a, float &b) {
+  asm volatile("s_nop 1\n\tv_permlane16_swap_b32 %0, %1" : "+v"(a), "+v"(b));
+}
+
+// Sum over the four lane groups of a wave for four values at once: lane
+// (g, c) returns v[g] summed over the lanes {c, c+16, c+32, c+48}. After
+// swap32(v0, v2) the sum v0 + v2 holds, in the lanes of groups 0 and 1, v0
+// plus v0 of the lane 32 above, and in groups 2 and 3, v2 plus v2 of the lane
+// 32 below; likewise v1 + v3 after swap32(v1, v3); one swap16 of the two
+// partial sums then adds the neighbouring row's, leaving in group g the total
+// of v[g]. Three swaps and three adds for four sums.
+__device__ __forceinline__ float group_sum_exchange4(float v0, float v1, float v2, float v3) {
+  swap32(v0, v2);
+  swap32(v1, v3);
+  float a0 = v0 + v2, a1 = v1 + v3;
+  swap16(a0, a1);
+  return a0 + a1;
+}
+
+// Minimum over the four lane groups: lane (g, c) returns the minimum of v over
+// the lanes {c, c+16, c+32, c+48}: started from a = b = v, the two results
+// of each swap hold v and its partner 32 (then 16) lanes away.
 __device__ __forceinline__ float group_min4(float v) {
   float a = v, b = v;
-  asm volatile("s_nop 1\n\tv_permlane32_swap_b32 %0, %1" : "+v"(a), "+v"(b));
+  swap32(a, b);
   v = fminf(a, b);
   a = v;
   b = v;
-  asm volatile("s_nop 1\n\tv_permlane16_swap_b32 %0, %1" : "+v"(a), "+v"(b));
+  swap16(a, b);
   return fminf(a, b);
 }
 
@@ -227,15 +217,15 @@ __global__ __launch_bounds__(64 * WAVES) void loo_grid_kernel(
 #pragma unroll
     for (int b = 0; b < kLooMaxBw; ++b) cf[b] = coef_sh[b];
     // skip threshold of each group: delta < 64 / min|c_b| over its bandwidths
-    // (0, never active, for a group past n_bw), as bit patterns
-    int thr[kLooGroups];
+    // (0, never active, for a group past n_bw)
+    float thr[kLooGroups];
 #pragma unroll
     for (int q = 0; q < kLooGroups; ++q) {
       float mn = __builtin_inff();
 #pragma unroll
       for (int e = 0; e < kLooGroup; ++e)
         if (q * kLooGroup + e < nbw) mn = fminf(mn, -cf[q * kLooGroup + e]);
-      thr[q] = __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, kLooSkipExp / mn));
+      thr[q] = kLooSkipExp / mn;
     }
 
     // this wave's 16 RT rows (padded coordinates; the rows past n are idle)
@@ -277,7 +267,7 @@ __global__ __launch_bounds__(64 * WAVES) void loo_grid_kernel(
 #pragma unroll
         for (int q = 0; q < kQ; ++q) {
           const float v = group_sum_exchange4(S[rt][4 * q], S[rt][4 * q + 1], S[rt][4 * q + 2],
-                                              S[rt][4 * q + 3], g);
+                                              S[rt][4 * q + 3]);
           S64[rt][q] += double(v);
 #pragma unroll
           for (int e = 0; e < 4; ++e) S[rt][4 * q + e] = 0.f;
@@ -307,15 +297,19 @@ __global__ __launch_bounds__(64 * WAVES) void loo_grid_kernel(
               if (4 * g + r == c) dd[r] = kLooPad;
           }
         }
-        const float tm = group_min4(fminf(fminf(dd[0], dd[1]), fminf(dd[2], dd[3])));
-        if (tm < m[rt]) {
-          // the row's minimum drops: its sums move to the new shift
-          const float dm = m[rt] - tm;
+        const float lmin = fminf(fminf(dd[0], dd[1]), fminf(dd[2], dd[3]));
+        // a new minimum in some row of the tile (about 1 % of the tiles):
+        // the row's four lanes agree on it, and its sums move to the new shift
+        if (__builtin_amdgcn_ballot_w64(lmin < m[rt])) {
+          const float tm = group_min4(lmin);
+          if (tm < m[rt]) {
+            const float dm = m[rt] - tm;
 #pragma unroll
-          for (int b = 0; b < kLooMaxBw; ++b) S[rt][b] *= fexp2(cf[b] * dm);
+            for (int b = 0; b < kLooMaxBw; ++b) S[rt][b] *= fexp2(cf[b] * dm);
 #pragma unroll
-          for (int q = 0; q < kQ; ++q) S64[rt][q] *= double(fexp2(cfl[q] * dm));
-          m[rt] = tm;
+            for (int q = 0; q < kQ; ++q) S64[rt][q] *= double(fexp2(cfl[q] * dm));
+            m[rt] = tm;
+          }
         }
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
@@ -323,10 +317,9 @@ __global__ __launch_bounds__(64 * WAVES) void loo_grid_kernel(
           lo = fminf(lo, delta[rt][r]);
         }
       }
-      const int dlo = wave_min_nonneg_bits(lo);
 #pragma unroll
       for (int q = 0; q < kLooGroups; ++q) {
-        if (dlo < thr[q]) {
+        if (__builtin_amdgcn_ballot_w64(lo < thr[q])) {
           ++n_active;
 #pragma unroll
           for (int e = 0; e < kLooGroup; ++e) {
