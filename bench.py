@@ -426,6 +426,40 @@ def score_variant(prec):
     return "symmetric" if prec == "fp32" else "ordered"
 
 
+def form_score(xd, h, prec, dense=False):
+    """Score accumulators of a form: the FP32 symmetric kernel, else
+    precision.emp_score (the forms with FP16 phi run on the data in spatial
+    order and leave out negligible tile pairs unless dense)."""
+    from flash_sd_kde import _ops, precision
+
+    if score_variant(prec) == "symmetric":
+        return _ops.score16(xd, h, variant="symmetric", precision=prec)
+    return precision.emp_score(xd, h, precision=prec, dense=dense, device=xd.device)
+
+
+def negligible_tile_frac(xd, h, t_max, samples=96):
+    """Sampled share of the (16-row tile, 32-point tile pair) blocks of the
+    data in spatial order whose every exponent -|x_i - x_j|^2 log2(e)/(2h^2)
+    is at or below t_max (FP64 distances; a sample of row tiles against all
+    tile pairs)."""
+    import torch
+
+    from flash_sd_kde.spatial import spatial_order
+
+    xs = xd[spatial_order(xd)].double()
+    n = xs.shape[0]
+    c = math.log2(math.e) / (2 * h * h)
+    npair = n // 32
+    far = total = 0
+    g = torch.Generator().manual_seed(0)
+    for r in torch.randint(0, n // 16, (samples,), generator=g).tolist():
+        d2 = torch.cdist(xs[16 * r:16 * r + 16], xs[:npair * 32]) ** 2
+        blk = (-c * d2).reshape(16, npair, 32).amax(dim=(0, 2))
+        far += int((blk <= t_max).sum())
+        total += npair
+    return far / total
+
+
 def precision_forms(xd, qd, h, fp32_ms, fp32_step, steps, warmup, n_err, rates=None):
     """Reduced-precision SD-KDE forms on the same workload: score with 16-bit
     MFMA operands (FP32 accumulate), FP64 debias, log-sum-exp KDE. Errors: max /
@@ -450,7 +484,7 @@ def precision_forms(xd, qd, h, fp32_ms, fp32_step, steps, warmup, n_err, rates=N
         log_norm = -math.log(n) - 16 * math.log(hh) - 8 * math.log(2 * math.pi)
         res = {"fp32": torch.log(fp32_step(hh)[:n_err].double()) + log_norm}
         for name, sp, ep in PRECISION_FORMS:
-            pdf, w = _ops.score16(xd, hh, variant=score_variant(sp), precision=sp)
+            pdf, w = form_score(xd, hh, sp)
             deb = _ops.debias(xd, pdf, w, hh, 1e-12)
             res[name] = _ops.kde16_logdensity(deb, sub, hh, precision=ep)
         return res
@@ -478,9 +512,10 @@ def precision_forms(xd, qd, h, fp32_ms, fp32_step, steps, warmup, n_err, rates=N
            "fp32": {"ms_per_step": round(fp32_ms, 3), "speedup_vs_fp32": 1.0, **err_fields("fp32")}}
     lib = _native.lib()
     pairs = float(m) * n + float(n) * n
+    dense_ms, tile_frac = {}, {}
     for name, sp, ep in PRECISION_FORMS:
         def step():
-            pdf, w = _ops.score16(xd, h, variant=score_variant(sp), precision=sp)
+            pdf, w = form_score(xd, h, sp)
             deb = _ops.debias(xd, pdf, w, h, 1e-12)
             return _ops.kde16_logdensity(deb, qd, h, precision=ep)
         for _ in range(warmup):
@@ -500,6 +535,21 @@ def precision_forms(xd, qd, h, fp32_ms, fp32_step, steps, warmup, n_err, rates=N
                      "ms_per_step": round(ms, 3), "gpair_evals_per_s": round(pairs / ms / 1e6, 3),
                      "speedup_vs_fp32": round(fp32_ms / ms, 3), "score_kernel_ms": round(score_ms, 3),
                      "kde_kernel_ms": round(kde_ms, 3), **err_fields(name)}
+        if sp in precision.SKIP_PRECISIONS:
+            # the same kernel evaluating every tile pair (FSK_FLAG_DENSE), and
+            # the sampled share of tile pairs below the kernel's threshold
+            if sp not in dense_ms:
+                lib.fsk_profile_enable(1)
+                form_score(xd, h, sp, dense=True)
+                torch.cuda.synchronize()
+                dense_ms[sp] = lib.fsk_profile_last_ms(_native.OP_SCORE16_ORDERED)
+                lib.fsk_profile_enable(0)
+            t_skip = -27.0 if sp == "fp16" else -(math.ceil(math.log2(n)) + 29.0)
+            out[name]["score_kernel_dense_ms"] = round(dense_ms[sp], 3)
+            if t_skip not in tile_frac:
+                tile_frac[t_skip] = negligible_tile_frac(xd, h, t_skip)
+            out[name]["negligible_tile_frac_sampled"] = round(tile_frac[t_skip], 4)
+            score_ms = dense_ms[sp]  # the issue fractions below are the dense kernel's
         if (rates and rates.get("exp_per_s") and rates.get("fma_per_s")
                 and rates.get("bf16_mfma_flops") and score_ms > 0 and sp in SCORE_ISSUE_MIX):
             # issue time of the score kernel at the rates measured in this run,

@@ -158,9 +158,9 @@ int score16_valu_run(const char *what, const float *rows, int64_t ldr, int64_t m
                      size_t ws_bytes, float *pdf, float *w, int64_t ldw_row, int64_t ldw_col,
                      void *stream);
 int score16_lp_run(const char *what, const float *rows, int64_t ldr, int64_t m,
-                   const float *data, int64_t ld, int64_t n, double h, unsigned prec, void *ws,
-                   size_t ws_bytes, float *pdf, float *w, int64_t ldw_row, int64_t ldw_col,
-                   void *stream);
+                   const float *data, int64_t ld, int64_t n, double h, unsigned prec,
+                   bool self_rows, bool dense, void *ws, size_t ws_bytes, float *pdf, float *w,
+                   int64_t ldw_row, int64_t ldw_col, void *stream);
 size_t kde1_ws_bytes(int64_t n, int64_t m, bool partials, bool two);
 
 // Fill kernel used instead of hipMemsetAsync (safe under graph capture).

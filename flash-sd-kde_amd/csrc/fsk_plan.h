@@ -32,6 +32,12 @@ constexpr int kScoreLpQT = FSK_LP_QT;
 #endif
 constexpr int kScoreLpQTx6 = FSK_LP_QT_X6;
 inline int score_lp_qt(unsigned prec) { return prec == FSK_PREC_BF16X6 ? kScoreLpQTx6 : kScoreLpQT; }
+// Waves per workgroup of the 16-bit score kernels: the forms whose phi.X takes
+// FP16 phi leave out most tile pairs of spatially ordered data and then do far
+// less work per staged pair, so eight waves share each LDS chunk.
+constexpr int score_lp_waves(unsigned prec) {
+  return prec == FSK_PREC_F16 || prec == FSK_PREC_F16X3S || prec == FSK_PREC_BF16X3S ? 8 : 4;
+}
 // The 16-bit score kernels take data tiles in pairs: their splits are whole pairs.
 constexpr int kLpPairPts = 32;
 // Symmetric score: I block = rows of one workgroup (4 waves x kSymQT tiles).
@@ -104,7 +110,8 @@ inline SplitPlan score16_ordered_plan(int64_t n) {
 inline SplitPlan score16_lp_plan(int64_t n, int64_t m, unsigned prec) {
   int64_t cap = kOrderedPartialCap / (m * 17 * 4 > 0 ? m * 17 * 4 : 1);
   if (cap < 1) cap = 1;
-  return make_split_plan(n, m, 4 * 16 * score_lp_qt(prec), kLpPairPts, 2048, cap);
+  return make_split_plan(n, m, score_lp_waves(prec) * 16 * score_lp_qt(prec), kLpPairPts, 2048,
+                         cap);
 }
 
 inline SplitPlan score16_rows_plan(int64_t n, int64_t m) {

@@ -497,7 +497,8 @@ static int score_check(const char *what, const float *data, int64_t ld, int64_t 
   const bool lowp = prec == FSK_PREC_BF16 || prec == FSK_PREC_F16 || prec == FSK_PREC_BF16X3 ||
                     prec == FSK_PREC_BF16X6 || prec == FSK_PREC_F16X3 || prec == FSK_PREC_F16X3S ||
                     prec == FSK_PREC_BF16X3S || prec == FSK_PREC_F32_VALU;
-  if ((prec != FSK_PREC_F32 && !(lowp && lowp_ok)) || (flags & ~FSK_PREC_MASK)) {
+  if ((prec != FSK_PREC_F32 && !(lowp && lowp_ok)) ||
+      (flags & ~(FSK_PREC_MASK | (lowp_ok ? FSK_FLAG_DENSE : 0u)))) {
     set_error("%s: flags 0x%x not available on this path", what, flags);
     return FSK_EUNSUPPORTED;
   }
@@ -518,8 +519,9 @@ extern "C" int fsk_score16_ordered(const float *data, int64_t ld, int64_t n, dou
     return score16_valu_run(what, data, ld, n, data, ld, n, bandwidth, true, ws, ws_bytes, pdf, w,
                             ldw_row, ldw_col, stream);
   if ((flags & FSK_PREC_MASK) != FSK_PREC_F32)
-    return score16_lp_run(what, data, ld, n, data, ld, n, bandwidth, flags & FSK_PREC_MASK, ws,
-                          ws_bytes, pdf, w, ldw_row, ldw_col, stream);
+    return score16_lp_run(what, data, ld, n, data, ld, n, bandwidth, flags & FSK_PREC_MASK, true,
+                          (flags & FSK_FLAG_DENSE) != 0, ws, ws_bytes, pdf, w, ldw_row, ldw_col,
+                          stream);
   ScoreLayout L;
   size_t need = score_layout(ws, n, true, &L);
   if (!ws || ws_bytes < need) {
@@ -611,7 +613,8 @@ extern "C" int fsk_score16_rows(const float *rows, int64_t ldr, int64_t m, const
     return score16_valu_run(what, rows, ldr, m, data, ld, n, bandwidth, false, ws, ws_bytes, pdf,
                             w, ldw_row, ldw_col, stream);
   if ((flags & FSK_PREC_MASK) != FSK_PREC_F32)
-    return score16_lp_run(what, rows, ldr, m, data, ld, n, bandwidth, flags & FSK_PREC_MASK, ws,
+    return score16_lp_run(what, rows, ldr, m, data, ld, n, bandwidth, flags & FSK_PREC_MASK,
+                          rows == data && ldr == ld && m == n, (flags & FSK_FLAG_DENSE) != 0, ws,
                           ws_bytes, pdf, w, ldw_row, ldw_col, stream);
   ScoreLayout L;
   size_t need = score_layout(ws, n, true, &L, m);

@@ -47,13 +47,28 @@
 //
 // Points are centred on the data mean: w' = sum phi (x - mu); the finish
 // kernel returns w = w' + mu * pdf in FP64.
+//
+// Negligible tiles. In the forms whose phi.X operand is phi rounded to FP16
+// (F16, F16X3S, BF16X3S), every phi <= 2^-26 becomes exactly 0 (below half
+// of FP16's smallest subnormal). The SKIP instances test each row tile's
+// cross terms against a pair of data tiles with one ballot and leave out the
+// exps, conversions, row sums and the phi.X MFMA when every exponent is
+// <= skip_t:
+//   F16          skip_t = -26: row sums and phi.X both use the FP16 phi, so
+//                the results are bitwise those of the dense kernel;
+//   F16X3S,      phi.X as above (the centred sums bitwise); the row sums
+//   BF16X3S      add the unrounded phi, so skip_t = -(ceil(log2 n) + 28)
+//                bounds what a row loses by n 2^skip_t <= 2^-28 of its self
+//                term (rows = data only; other rows run dense).
+// In data put in spatial order, most tile pairs of a large set are far apart
+// and skip.
 #include "fsk_common.h"
 #include "fsk_lowp.h"
 #include "fsk_plan.h"
 
 namespace fsk {
 
-static_assert(kNormPad % (64 * kScoreLpQT) == 0 && kNormPad % (64 * kScoreLpQTx6) == 0,
+static_assert(kNormPad % (128 * kScoreLpQT) == 0 && kNormPad % (64 * kScoreLpQTx6) == 0,
               "records must cover whole row blocks");
 static_assert(kNormPad % kLpPairPts == 0, "records must cover whole tile pairs");
 
@@ -66,6 +81,7 @@ template <int PREC>
 struct LpPair {
   static constexpr bool kX6 = PREC == FSK_PREC_BF16X6;
   static constexpr bool kOnePhi = PREC == FSK_PREC_F16X3S || PREC == FSK_PREC_BF16X3S;
+  static constexpr bool kF16Phi = kOnePhi || PREC == FSK_PREC_F16;  // phi.X takes FP16 phi
   static constexpr bool kX3 = PREC == FSK_PREC_BF16X3 || PREC == FSK_PREC_F16X3 || kOnePhi;
   static constexpr bool kSplitPhi = kX6 || (kX3 && !kOnePhi);
   static constexpr int kCross = kX6 ? 4 : kX3 ? 2 : 1;  // cross-term A operands per tile
@@ -75,10 +91,17 @@ struct LpPair {
   static constexpr int kSlots = 2 * kCross + kX;
   static constexpr int kSlotU16 = 64 * 8;  // 16 bytes per lane
   static constexpr int kPairU16 = kSlots * kSlotU16;
-  // pairs per LDS chunk: up to 16 KiB per buffer, two buffers
+  // pairs per LDS chunk: up to 16 KiB per buffer
   static constexpr int kChunk = 8192 / kPairU16 > 0 ? 8192 / kPairU16 : 1;
   static constexpr int kChunkU16 = kChunk * kPairU16;
   static constexpr int kChunkSlots = kChunk * kSlots;
+  // Waves per workgroup sharing each chunk, and LDS buffers (chunks staged
+  // kBufs - 1 ahead). The forms that leave out negligible tiles spend a few
+  // hundred cycles per staged pair where they skip, against the chunk's
+  // L2 / MALL latency of a microsecond or more: eight waves share a chunk
+  // and it is fetched two chunks ahead.
+  static constexpr int kWaves = score_lp_waves(PREC);
+  static constexpr int kBufs = kF16Phi ? 3 : 2;
 };
 
 // Record halves (lo, hi) of cross-term A operand `s` of a form.
@@ -228,13 +251,29 @@ __device__ __forceinline__ f4 lp_cross(const u16x8 (&A)[LpPair<PREC>::kCross], u
 template <int PREC>
 constexpr int lp_min_waves() { return PREC == FSK_PREC_BF16X6 ? 3 : 4; }
 
-template <int PREC, int QT>
-__global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(lp_min_waves<PREC>())))
+// s_waitcnt vmcnt(n) for n = 0..3, expcnt and lgkmcnt not waited on (gfx9
+// encoding: vmcnt in bits 3:0, expcnt 6:4, lgkmcnt 11:8).
+__device__ __forceinline__ void wait_vmcnt(int n) {
+  if (n <= 0) __builtin_amdgcn_s_waitcnt(0xF70);
+  else if (n == 1) __builtin_amdgcn_s_waitcnt(0xF71);
+  else if (n == 2) __builtin_amdgcn_s_waitcnt(0xF72);
+  else __builtin_amdgcn_s_waitcnt(0xF73);
+}
+
+// Largest of a lane's cross-term exponents.
+__device__ __forceinline__ float max4(f4 d) {
+  return fmaxf(fmaxf(d[0], d[1]), fmaxf(d[2], d[3]));
+}
+
+template <int PREC, int QT, bool SKIP>
+__global__ __launch_bounds__(64 * LpPair<PREC>::kWaves)
+__attribute__((amdgpu_waves_per_eu(lp_min_waves<PREC>())))
 void score16_lp_kernel(
     const unsigned short *__restrict__ blk, const unsigned short *__restrict__ qrec, int64_t m,
     int64_t npairs_data, int64_t split_pairs, int64_t qblocks, int64_t ntasks, int64_t per_xcd,
-    float *__restrict__ pdf_part, float *__restrict__ w_part) {
+    float skip_t, float *__restrict__ pdf_part, float *__restrict__ w_part) {
   using BL = LpPair<PREC>;
+  static_assert(!SKIP || BL::kF16Phi, "tiles are skipped only where phi.X takes FP16 phi");
   constexpr bool X6 = BL::kX6;
   constexpr bool X3 = BL::kX3;
   constexpr bool SPLIT_PHI = BL::kSplitPhi;  // phi.X with phi = phi_h + phi_l
@@ -251,8 +290,11 @@ void score16_lp_kernel(
   constexpr int KC = BL::kCross;
   constexpr int CH = BL::kChunk;
   constexpr int S1 = BL::kSlotU16;
+  constexpr int W = BL::kWaves;
+  constexpr int NB = BL::kBufs;
+  static_assert(BL::kChunkSlots <= 4 * W, "at most 3 newer loads per wave are waited past");
   using RL = RecLayout<PREC>;
-  __shared__ __attribute__((aligned(16))) unsigned short lds[2 * BL::kChunkU16];
+  __shared__ __attribute__((aligned(16))) unsigned short lds[NB * BL::kChunkU16];
 
   // task, split and the data range are wave-uniform: keep them scalar
   const int64_t task = __builtin_amdgcn_readfirstlane(int(xcd_task(blockIdx.x, per_xcd)));
@@ -262,7 +304,7 @@ void score16_lp_kernel(
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int g = lane >> 4, c = lane & 15;
-  const int64_t row0 = qb * (64 * QT) + int64_t(wave) * (16 * QT);
+  const int64_t row0 = qb * (16 * W * QT) + int64_t(wave) * (16 * QT);
 
   u16x8 B1[QT], B2[QT], B3[QT], B4[QT];
   f4 w[QT];
@@ -297,28 +339,35 @@ void score16_lp_kernel(
   const int npair = int(p_end - p_begin);
   const int nchunk = (npair + CH - 1) / CH;
   const unsigned short *src0 = blk + p_begin * BL::kPairU16;
-  // Wave `wave` copies slots wave, wave + 4, ... of a chunk: one 1 KiB slot per
+  // Wave `wave` copies slots wave, wave + W, ... of a chunk: one 1 KiB slot per
   // instruction, lane l's 16 bytes landing at LDS slot base + 16 l.
+  const int my_loads = (BL::kChunkSlots - wave + W - 1) / W;  // per chunk, this wave
   auto stage = [&](int ch, int buf) {
     const unsigned short *src = src0 + int64_t(ch) * BL::kChunkU16 + lane * 8;
 #pragma unroll
-    for (int q = 0; q < (BL::kChunkSlots + 3) / 4; ++q) {
-      const int k = 4 * q + wave;
+    for (int q = 0; q < (BL::kChunkSlots + W - 1) / W; ++q) {
+      const int k = W * q + wave;
       if (k < BL::kChunkSlots)
         __builtin_amdgcn_global_load_lds(src + k * S1,
                                          (lds_void *)(lds + buf * BL::kChunkU16 + k * S1), 16, 0, 0);
     }
   };
 
-  stage(0, 0);
+  for (int c = 0; c < NB - 1 && c < nchunk; ++c) stage(c, c);
   int tc = 0;
   for (int ch = 0; ch < nchunk; ++ch) {
-    // this wave's copies of chunk ch have landed; the barrier makes every
-    // wave's copies visible and retires all reads of the buffer staged next
-    __builtin_amdgcn_s_waitcnt(0);
-    __syncthreads();
-    if (ch + 1 < nchunk) stage(ch + 1, (ch + 1) & 1);
-    const unsigned short *cb = lds + (ch & 1) * BL::kChunkU16 + lane * 8;
+    // This wave's copies of chunk ch have landed once at most those of the
+    // chunks staged after it are outstanding (loads retire in order); the
+    // barrier makes every wave's copies visible, and every wave has finished
+    // reading the buffer staged next (chunk ch - 1's). A plain s_barrier: a
+    // fenced one would also wait for the chunks in flight.
+    const int newer = nchunk - 1 - ch < NB - 2 ? nchunk - 1 - ch : NB - 2;
+    wait_vmcnt(newer * my_loads);
+    __asm__ volatile("" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    __asm__ volatile("" ::: "memory");
+    if (ch + NB - 1 < nchunk) stage(ch + NB - 1, (ch + NB - 1) % NB);
+    const unsigned short *cb = lds + (ch % NB) * BL::kChunkU16 + lane * 8;
     const int np = npair - ch * CH < CH ? npair - ch * CH : CH;
     for (int t = 0; t < np; ++t) {
       const unsigned short *pb = cb + t * BL::kPairU16;
@@ -328,6 +377,36 @@ void score16_lp_kernel(
         Aa[0] = lds_u16x8(pb);
         Ab[0] = lds_u16x8(pb + S1);
         const u16x8 BX = lds_u16x8(pb + 2 * S1);
+        if constexpr (SKIP) {
+          // the cross terms of two row tiles, then per row tile one ballot
+          // decides
+          f4 da[QT], db[QT];
+#pragma unroll
+          for (int R = 0; R < QT; ++R) {
+            if (R % 2 == 0) {
+#pragma unroll
+              for (int r = R; r < R + 2 && r < QT; ++r) {
+                da[r] = lp_cross<PREC>(Aa, B1[r], B2[r], B3[r], B4[r]);
+                db[r] = lp_cross<PREC>(Ab, B1[r], B2[r], B3[r], B4[r]);
+              }
+            }
+            if (__builtin_amdgcn_ballot_w64(fmaxf(max4(da[R]), max4(db[R])) > skip_t)) {
+              float ea[4], eb[4];
+#pragma unroll
+              for (int u = 0; u < 4; ++u) {
+                ea[u] = fexp2_le1(da[R][u]);
+                eb[u] = fexp2_le1(db[R][u]);
+              }
+              const unsigned a0 = pack2<PX>(ea[0], ea[1]), a1 = pack2<PX>(ea[2], ea[3]);
+              const unsigned b0 = pack2<PX>(eb[0], eb[1]), b1 = pack2<PX>(eb[2], eb[3]);
+              pdf[R] = dot2_ones<PX>(a0, pdf[R]);
+              pdf[R] = dot2_ones<PX>(a1, pdf[R]);
+              pdf[R] = dot2_ones<PX>(b0, pdf[R]);
+              pdf[R] = dot2_ones<PX>(b1, pdf[R]);
+              w[R] = mfma16(words(a0, a1, b0, b1), BX, w[R], PX);
+            }
+          }
+        } else {
 #pragma unroll
         for (int R = 0; R < QT; ++R) {
           const f4 da = lp_cross<PREC>(Aa, B1[R], B2[R], B3[R], B4[R]);
@@ -360,6 +439,7 @@ void score16_lp_kernel(
           __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
         }
         __builtin_amdgcn_sched_group_barrier(0x002, 64, 0);
+        }
       } else {
         // split and one-phi forms: tile a, then tile b (one tile's operands
         // live at a time); the split forms keep phi_h of tile a per row tile
@@ -373,6 +453,25 @@ void score16_lp_kernel(
           for (int s = 0; s < KC; ++s) A[s] = lds_u16x8(tb + s * S1);
           const u16x8 BX = lds_u16x8(pb + (2 * KC + tt) * S1);
           const u16x8 BXL = SPLIT_PHI && tt == 1 ? lds_u16x8(pb + (2 * KC + 2) * S1) : BX;
+          if constexpr (SKIP) {
+            f4 d[QT];
+#pragma unroll
+            for (int R = 0; R < QT; ++R) d[R] = lp_cross<PREC>(A, B1[R], B2[R], B3[R], B4[R]);
+#pragma unroll
+            for (int R = 0; R < QT; ++R) {
+              if (__builtin_amdgcn_ballot_w64(max4(d[R]) > skip_t)) {
+                float e[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                  e[u] = fexp2_le1(d[R][u]);
+                  pdf[R] += e[u];
+                }
+                const unsigned p0 = pack2<PX>(e[0], e[1]), p1 = pack2<PX>(e[2], e[3]);
+                w[R] = mfma16(words(p0, p1, p0, p1), BX, w[R], PX);
+              }
+            }
+            continue;
+          }
 #pragma unroll
           for (int R = 0; R < QT; ++R) {
             const f4 d = lp_cross<PREC>(A, B1[R], B2[R], B3[R], B4[R]);
@@ -493,12 +592,34 @@ size_t score16_lp_ws_bytes(int64_t n, int64_t m, unsigned prec) {
   return score_lp_layout(nullptr, n, m, prec, &L);
 }
 
+// The score kernel of form P, with the tile test where the form has one and
+// `skip` asks for it.
+template <int P, int QT>
+static void launch_score_lp(bool skip, float skip_t, int64_t grid, hipStream_t s,
+                            const ScoreLpLayout &L, int64_t m, int64_t split_pairs,
+                            int64_t qblocks, int64_t ntasks, int64_t per_xcd) {
+  if constexpr (LpPair<P>::kF16Phi) {
+    if (skip) {
+      hipLaunchKernelGGL((score16_lp_kernel<P, QT, true>), dim3(grid), dim3(64 * LpPair<P>::kWaves),
+                         0, s, L.blk,
+                         L.qrec, m, L.npairs_data, split_pairs, qblocks, ntasks, per_xcd, skip_t,
+                         L.pdf_part, L.w_part);
+      return;
+    }
+  }
+  hipLaunchKernelGGL((score16_lp_kernel<P, QT, false>), dim3(grid), dim3(64 * LpPair<P>::kWaves),
+                     0, s, L.blk,
+                     L.qrec, m, L.npairs_data, split_pairs, qblocks, ntasks, per_xcd, 0.f,
+                     L.pdf_part, L.w_part);
+}
+
 // Score accumulators of m rows against the KDE of n data points with 16-bit
-// MFMA operands (rows == data for the empirical score of the data set).
+// MFMA operands. self_rows: the rows are the data (the empirical score of the
+// data set), so every row holds its self term; dense: evaluate every tile.
 int score16_lp_run(const char *what, const float *rows_in, int64_t ldr, int64_t m,
-                   const float *data, int64_t ld, int64_t n, double h, unsigned prec, void *ws,
-                   size_t ws_bytes, float *pdf, float *w, int64_t ldw_row, int64_t ldw_col,
-                   void *stream) {
+                   const float *data, int64_t ld, int64_t n, double h, unsigned prec,
+                   bool self_rows, bool dense, void *ws, size_t ws_bytes, float *pdf, float *w,
+                   int64_t ldw_row, int64_t ldw_col, void *stream) {
   ScoreLpLayout L;
   size_t need = score_lp_layout(ws, n, m, prec, &L);
   if (!ws || ws_bytes < need) {
@@ -517,6 +638,18 @@ int score16_lp_run(const char *what, const float *rows_in, int64_t ldr, int64_t 
   const int64_t split_pairs = p.split_len / kLpPairPts;
   int64_t per_xcd;
   const int64_t grid = xcd_grid(ntasks, &per_xcd);
+  // tile test threshold (exponents in log2 units): FP16 phi is exactly 0 at
+  // or below 2^-26; the unrounded row sums of the one-phi forms lose at most
+  // n 2^skip_t <= 2^-28 of a row's self term
+  bool skip = false;
+  float skip_t = 0.f;
+  if (!dense && prec == FSK_PREC_F16) {
+    skip = true;
+    skip_t = -26.f;
+  } else if (!dense && self_rows && (prec == FSK_PREC_F16X3S || prec == FSK_PREC_BF16X3S)) {
+    skip = true;
+    skip_t = -float(std::ceil(std::log2(double(n))) + 28.0);
+  }
 #define FSK_LP(P, PXT, QT)                                                                       \
   do {                                                                                           \
     hipLaunchKernelGGL(prep_lse_kernel<P>, dim3(cdiv(xrows, 256)), dim3(256), 0, s, data, ld, n, \
@@ -532,9 +665,7 @@ int score16_lp_run(const char *what, const float *rows_in, int64_t ldr, int64_t 
                        dim3(cdiv(L.npairs_alloc * LpPair<P>::kSlots * 64, 256)), dim3(256), 0,   \
                        s, L.xrec, L.xt, L.npairs_data, L.npairs_alloc, L.blk);                    \
     prof_begin(FSK_OP_SCORE16_ORDERED, s);                                                       \
-    hipLaunchKernelGGL((score16_lp_kernel<P, QT>), dim3(grid), dim3(kWG), 0, s, L.blk, L.qrec,   \
-                       m, L.npairs_data, split_pairs, p.qblocks, ntasks, per_xcd, L.pdf_part,     \
-                       L.w_part);                                                                \
+    launch_score_lp<P, QT>(skip, skip_t, grid, s, L, m, split_pairs, p.qblocks, ntasks, per_xcd); \
     prof_end(FSK_OP_SCORE16_ORDERED, s);                                                         \
   } while (0)
   switch (prec) {

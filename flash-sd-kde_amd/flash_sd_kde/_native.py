@@ -35,6 +35,7 @@ PREC_F16X3S = 7
 PREC_BF16X3S = 8
 FLAG_LAPLACIAN = 1 << 8
 FLAG_PHI_S = 1 << 9
+FLAG_DENSE = 1 << 10
 
 OP_KDE16_SPLITK = 1
 OP_KDE16_ATOMIC = 2

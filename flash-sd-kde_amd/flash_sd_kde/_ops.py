@@ -205,12 +205,14 @@ def kde1_score_sums(train: torch.Tensor, query: torch.Tensor, bandwidth: float):
 
 
 def score16(train: torch.Tensor, bandwidth: float, *, variant: str = "symmetric",
-            transposed_w: bool = False, precision: str = "fp32"):
+            transposed_w: bool = False, precision: str = "fp32", dense: bool = False):
     """Empirical-score accumulators (pdf (n,), weighted (n,16)).
 
     transposed_w=True stores the weighted sums in a (16, n) buffer and returns
     its transpose, the layout of the reference's flash backend
-    (kernels/flash_sd_kde/empirical.py:180,223)."""
+    (kernels/flash_sd_kde/empirical.py:180,223). dense=True evaluates every
+    tile pair in the forms that otherwise leave out negligible ones (fp16,
+    fp16x3s, bf16x3s)."""
     h = _check_h(bandwidth)
     dev = train.device
     nat.require_gpu(dev, "score16")
@@ -238,6 +240,8 @@ def score16(train: torch.Tensor, bandwidth: float, *, variant: str = "symmetric"
     if flags != nat.PREC_F32 and variant != "ordered":
         raise ValueError("reduced-precision score accumulators use the ordered kernel")
     check_fp16_range(train, train, h, flags)
+    if dense and variant == "ordered":
+        flags |= nat.FLAG_DENSE
     ws, nbytes = nat.workspace(op, n, n, flags, dev)
     st = fn(train.data_ptr(), train.stride(0), n, h, flags, ws.data_ptr(), nbytes, pdf.data_ptr(),
             wbuf.data_ptr(), ld_row, ld_col, nat.stream_handle(dev))
@@ -246,9 +250,10 @@ def score16(train: torch.Tensor, bandwidth: float, *, variant: str = "symmetric"
 
 
 def score16_rows(rows: torch.Tensor, train: torch.Tensor, bandwidth: float, *,
-                 precision: str = "fp32"):
+                 precision: str = "fp32", dense: bool = False):
     """(pdf (m,), weighted (m,16)) of `rows` under the KDE of `train`, in the
-    FP32 form or one of the 16-bit MFMA forms of the ordered score."""
+    FP32 form or one of the 16-bit MFMA forms of the ordered score
+    (dense=True: every tile pair evaluated, as in score16)."""
     h = _check_h(bandwidth)
     dev = train.device
     nat.require_gpu(dev, "score16_rows")
@@ -261,6 +266,8 @@ def score16_rows(rows: torch.Tensor, train: torch.Tensor, bandwidth: float, *,
                      nat.PREC_BF16X3S):
         raise ValueError(f"precision {precision} is not available for the score pass")
     check_fp16_range(train, rows, h, flags)
+    if dense:
+        flags |= nat.FLAG_DENSE
     pdf = torch.empty(m, dtype=torch.float32, device=dev)
     w = torch.empty((m, ND), dtype=torch.float32, device=dev)
     ws, nbytes = nat.workspace(nat.OP_SCORE16_ROWS, n, m, flags, dev)

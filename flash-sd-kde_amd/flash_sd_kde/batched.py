@@ -35,6 +35,7 @@ import torch
 
 from flash_sd_kde import _native as nat
 from flash_sd_kde.bandwidth import scott_mean_std_bandwidth
+from flash_sd_kde.spatial import spatial_order
 
 MAX_SETS = 64
 MAX_BW = 32
@@ -43,29 +44,6 @@ MAX_BW = 32
 # (~204 VGPRs), so larger workgroup counts would only queue behind these.
 CANDIDATES = [(4, 1, 2), (4, 1, 3), (8, 1, 1), (8, 1, 2), (4, 2, 1), (4, 2, 2), (8, 2, 1)]
 DEFAULT_TUNE = (4, 1, 3)
-
-def spatial_order(x: torch.Tensor, bits: int = 10) -> torch.Tensor:
-    """Permutation of the rows of x (n, d) that sorts them by the Morton code
-    of their coordinates along the three leading principal axes (`bits` bits
-    per axis): neighbours in the order are mostly neighbours in space."""
-    n = x.shape[0]
-    if n < 3:
-        return torch.arange(n, device=x.device)
-    y = x.double() - x.double().mean(0)
-    cov = (y.T @ y).cpu().numpy() / (n - 1)
-    _, vecs = np.linalg.eigh(cov)
-    k = min(3, x.shape[1])
-    axes = torch.as_tensor(np.ascontiguousarray(vecs[:, -k:]), device=x.device)
-    p = y @ axes
-    lo, hi = p.min(0).values, p.max(0).values
-    scale = (1 << bits) - 1
-    q = ((p - lo) / (hi - lo).clamp_min(1e-300) * scale).long().clamp(0, scale)
-    code = torch.zeros(n, dtype=torch.int64, device=x.device)
-    for b in range(bits):
-        for a in range(k):
-            code |= ((q[:, a] >> b) & 1) << (k * b + a)
-    return torch.argsort(code, stable=True)
-
 
 _tune_cache: dict[tuple, tuple[tuple[int, int, int], dict]] = {}
 _tune_lock = threading.Lock()

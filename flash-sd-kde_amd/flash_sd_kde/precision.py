@@ -24,6 +24,14 @@ with points centred on the data mean, log-densities by log-sum-exp
 (per-split running maximum, FP64 combine) and FP64 debias. The score and the
 log-density may use different forms (`sdkde_logdensity(score_precision=...)`).
 
+In the forms whose phi.X takes phi rounded to FP16 ("fp16", "fp16x3s",
+"bf16x3s") the score kernel leaves out each row tile x tile pair whose every
+term is negligible (exactly 0 in FP16 for "fp16": bitwise the dense result;
+below n 2^-28 of a row's self term in the FP32 row sums of the other two).
+`emp_score` therefore runs these forms on the data in spatial order
+(`spatial.spatial_order`) and returns the results in the caller's order; the
+order changes the sums only by rounding.
+
 `ref64_*` are FP64 direct-difference GPU kernels used to measure the error of
 every form at sizes the CPU oracle cannot reach.
 """
@@ -33,6 +41,13 @@ from __future__ import annotations
 import torch
 
 from flash_sd_kde import _ops
+from flash_sd_kde.spatial import spatial_order
+
+# forms whose score kernel leaves out negligible tiles, and the smallest data
+# set that is put in spatial order for them (below it the kernel has few
+# tiles to leave out)
+SKIP_PRECISIONS = ("fp16", "fp16x3s", "bf16x3s")
+SPATIAL_MIN_N = 8192
 
 SCORE_PRECISIONS = ("fp32", "fp32_valu", "fp16x3", "fp16x3s", "bf16x6", "bf16x3", "bf16x3s",
                     "fp16", "bf16")
@@ -48,24 +63,42 @@ def kde_logdensity(data, queries, bandwidth: float, *, precision: str = "fp32",
                                  bandwidth, precision=precision)
 
 
-def emp_score(data, bandwidth: float, *, precision: str = "fp32", device="cuda"):
-    """(pdf_sum (n,), weighted (n, 16)) by the deterministic ordered kernel."""
+def _ordered_score(train: torch.Tensor, bandwidth: float, precision: str, spatial: bool,
+                   dense: bool):
+    if spatial and not dense and precision in SKIP_PRECISIONS and train.shape[0] >= SPATIAL_MIN_N:
+        perm = spatial_order(train)
+        pdf_s, w_s = _ops.score16(train[perm], bandwidth, variant="ordered", precision=precision)
+        pdf = torch.empty_like(pdf_s)
+        pdf[perm] = pdf_s
+        w = torch.empty_like(w_s)
+        w[perm] = w_s
+        return pdf, w
+    return _ops.score16(train, bandwidth, variant="ordered", precision=precision, dense=dense)
+
+
+def emp_score(data, bandwidth: float, *, precision: str = "fp32", spatial: bool = True,
+              dense: bool = False, device="cuda"):
+    """(pdf_sum (n,), weighted (n, 16)) by the deterministic ordered kernel.
+    spatial=False keeps the caller's order for the forms that leave out
+    negligible tiles; dense=True evaluates every tile (both only change
+    rounding)."""
     device = torch.device(device)
-    return _ops.score16(_ops.as_points16(data, device), bandwidth, variant="ordered",
-                        precision=precision)
+    return _ordered_score(_ops.as_points16(data, device), bandwidth, precision, spatial, dense)
 
 
 def sdkde_fit_transform(data, bandwidth: float, *, precision: str = "fp32", eps: float = 1e-12,
+                        spatial: bool = True, dense: bool = False,
                         device="cuda") -> torch.Tensor:
     """One-step empirical SD-KDE debiasing with the chosen score precision."""
     device = torch.device(device)
     train = _ops.as_points16(data, device)
-    pdf, w = _ops.score16(train, bandwidth, variant="ordered", precision=precision)
+    pdf, w = _ordered_score(train, bandwidth, precision, spatial, dense)
     return _ops.debias(train, pdf, w, bandwidth, eps)
 
 
 def sdkde_logdensity(data, queries, bandwidth: float, *, precision: str = "fp32",
-                     score_precision: str | None = None, device="cuda") -> torch.Tensor:
+                     score_precision: str | None = None, spatial: bool = True,
+                     dense: bool = False, device="cuda") -> torch.Tensor:
     """SD-KDE log-density of `queries`: debias `data` (score in `score_precision`,
     default `precision`), then the log-sum-exp KDE (cross term in `precision`).
 
@@ -84,7 +117,8 @@ def sdkde_logdensity(data, queries, bandwidth: float, *, precision: str = "fp32"
     sp = precision if score_precision is None else score_precision
     if sp not in SCORE_PRECISIONS:
         raise ValueError(f"score_precision must be one of {SCORE_PRECISIONS}, got {sp!r}")
-    deb = sdkde_fit_transform(data, bandwidth, precision=sp, device=device)
+    deb = sdkde_fit_transform(data, bandwidth, precision=sp, spatial=spatial, dense=dense,
+                              device=device)
     return _ops.kde16_logdensity(deb, _ops.as_points16(queries, device), bandwidth,
                                  precision=precision)
 

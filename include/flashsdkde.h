@@ -68,6 +68,10 @@ extern "C" {
 #define FSK_FLAG_LAPLACIAN (1u << 8) /* phi * (1 + d/2 - |q-x|^2/(2h^2)) */
 #define FSK_FLAG_PHI_S (1u << 9)     /* phi * |q-x|^2/h^2 alone: the Laplacian correction's
                                       * moment, for the two-pass (non-fused) form */
+#define FSK_FLAG_DENSE (1u << 10)    /* score forms whose phi.X takes phi rounded to FP16
+                                      * (F16, F16X3S, BF16X3S): evaluate every tile pair
+                                      * instead of leaving out those whose every term is
+                                      * negligible (fsk_score16_ordered) */
 
 /* ---- operation ids for fsk_workspace_bytes ---- */
 #define FSK_OP_KDE16_SPLITK 1
@@ -242,7 +246,8 @@ int fsk_score16_sym_atomic(const float *data, int64_t ld, int64_t n,
                            void *stream);
 
 /* Same accumulators, deterministic: bounded split-K over the data with
- * FP32 partials and an FP64 combine; bitwise reproducible for fixed n.
+ * FP32 partials and an FP64 combine; bitwise reproducible for fixed n (and
+ * fixed FSK_FLAG_DENSE).
  * Points are centred on their mean (FP64, fixed summation order) before the
  * pair terms are formed, and mu * pdf is added back in FP64.
  * flags may select a 16-bit cross term and phi.X (FSK_PREC_F16X3 / _BF16X6 /
@@ -250,6 +255,14 @@ int fsk_score16_sym_atomic(const float *data, int64_t ld, int64_t n,
  * FP32 form (FP32 MFMA; rows whose norm terms span a narrow range are
  * evaluated as exp2(d_ij) * exp2(c|x_i - mu|^2 + K), which needs no per-pair
  * add, the others directly).
+ * Negligible tiles: in the forms whose phi.X takes phi rounded to FP16, a
+ * wave leaves out a row tile x tile pair (16 x 32 pairs) whose every exponent
+ * is at or below a threshold: 2^-26 for F16 (those phi are exactly 0 in FP16
+ * and the row sums add the same FP16 phi, so the results are bitwise the
+ * dense ones); 2^-(ceil(log2 n) + 28) for F16X3S / BF16X3S (the centred
+ * sums of phi.X bitwise; the FP32 row sums of the unrounded phi lose at most
+ * 2^-28 of a row's self term, and w = centred sums + mu pdf carries that). Data in spatial order (nearby points in nearby rows) let most tile
+ * pairs of a large set skip. FSK_FLAG_DENSE evaluates every tile.
  * Replaces emp_score_16d_ordered_splitk,
  * kernels/emp_score_16d_ordered_splitk.py:162-263 (pass A :14-92, reduce
  * kernels/reduce_partials.py:82-108). */
@@ -263,8 +276,11 @@ int fsk_score16_ordered(const float *data, int64_t ld, int64_t n,
  *   pdf[i] = sum_j exp(-|r_i - x_j|^2/(2h^2)),  w[i,:] = sum_j exp(...) x_j,
  * deterministic split-K with FP64 combine (workspace FSK_OP_SCORE16_ROWS with
  * (n, m, flags)). flags takes the precisions of fsk_score16_ordered: FSK_PREC_F32
- * or a 16-bit MFMA form (BF16, F16, BF16X3, BF16X6, F16X3; the FP16 forms carry
- * the range limit stated there, here over rows and data alike). With rows = data this is
+ * or a 16-bit MFMA form (BF16, F16, BF16X3, BF16X6, F16X3, F16X3S, BF16X3S; the FP16
+ * forms carry the range limit stated there, here over rows and data alike), and
+ * FSK_FLAG_DENSE. Tiles are left out as in fsk_score16_ordered for F16 (bitwise);
+ * F16X3S / BF16X3S leave them out only when rows is data (same pointer, stride
+ * and count), where every row holds its self term. With rows = data this is
  * fsk_score16_ordered; a row subset gives exact score labels for the learned
  * surrogate. The reference evaluates the score at separate points only in 1-D
  * (kde_score_eval, flash_sd_kde/reference.py:83-120). */

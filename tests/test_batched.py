@@ -152,3 +152,15 @@ def test_batched_offset_datasets_match_fp64(gpu):
     for k, x in enumerate(sets):
         ref = fp64.loo_loglik(x, scott_mean_std_bandwidth(x) * factors)
         np.testing.assert_allclose(ll[k], ref, rtol=1e-5)
+
+
+def test_morton_bit_spread_matches_bit_loop():
+    """spatial_order's code puts bit b of axis a at bit 3b + a; the magic-mask
+    spread equals moving each bit one by one."""
+    from flash_sd_kde.spatial import _spread3
+
+    v = torch.arange(1024, dtype=torch.int32)
+    ref = torch.zeros_like(v)
+    for b in range(10):
+        ref |= ((v >> b) & 1) << (3 * b)
+    assert torch.equal(_spread3(v), ref)

@@ -172,3 +172,103 @@ def test_valu_score_matches_fp64(gpu, n):
     np.testing.assert_allclose(pr.cpu().numpy(), pre, rtol=1e-5, atol=1e-30)
     scale = np.maximum(np.abs(wre).max(1, keepdims=True), 1e-30)
     assert np.max(np.abs(wr.cpu().numpy() - wre) / scale) < 1e-5
+
+
+# ---------------- negligible tiles of the FP16-phi score forms ----------------
+
+def _spread_mixture(gpu, n, seed):
+    """The precision-form workload at a smaller n: 8 isotropic clusters in
+    16-D (means ~ N(0, 9 I), sigma 1), put in spatial order, with its
+    Silverman bandwidth: most 16 x 32 tile pairs lie in different clusters."""
+    from flash_sd_kde.bandwidth import silverman_bandwidth_nd
+    from flash_sd_kde.datasets import gmm_isotropic
+    from flash_sd_kde.spatial import spatial_order
+
+    x = gmm_isotropic(n, k=8, mean_scale=3.0, seed=seed)
+    h = silverman_bandwidth_nd(x)
+    xd = torch.as_tensor(x, device=gpu)
+    return xd[spatial_order(xd)].contiguous(), h
+
+
+def _negligible_tile_frac(x, h, t_max):
+    """Share of (16-row tile, 32-point tile pair) blocks of x whose every
+    exponent -|x_i - x_j|^2 log2(e) / (2 h^2) is at or below t_max (FP64)."""
+    xs = x.double()
+    n = xs.shape[0]
+    c = 1.4426950408889634 / (2 * h * h)
+    far = total = 0
+    for r0 in range(0, n - 15, 16 * 7):  # every 7th row tile
+        rows = xs[r0:r0 + 16]
+        d2 = torch.cdist(rows, xs) ** 2  # (16, n)
+        npair = n // 32
+        blk = (-c * d2[:, :npair * 32]).reshape(16, npair, 32).amax(dim=(0, 2))
+        far += int((blk <= t_max).sum())
+        total += npair
+    return far / total
+
+
+def test_fp16_score_skipping_is_bitwise_dense(gpu):
+    """F16 form: a tile pair whose every phi <= 2^-26 adds exact zeros (those
+    phi are 0 in FP16, and both the row sums and phi.X take the FP16 phi), so
+    leaving such tiles out gives bitwise the dense results. The data are
+    chosen so that most tiles are left out; the same holds for rows that are
+    not the data (fsk_score16_rows)."""
+    x, h = _spread_mixture(gpu, 20000, 5)
+    frac = _negligible_tile_frac(x, h, -27.0)
+    record("fp16_skip_negligible_tile_frac", frac=frac)
+    assert frac > 0.5
+    p1, w1 = _ops.score16(x, h, variant="ordered", precision="fp16")
+    p2, w2 = _ops.score16(x, h, variant="ordered", precision="fp16", dense=True)
+    assert torch.equal(p1, p2) and torch.equal(w1, w2)
+    rows = x[::3].contiguous()
+    p1, w1 = _ops.score16_rows(rows, x, h, precision="fp16")
+    p2, w2 = _ops.score16_rows(rows, x, h, precision="fp16", dense=True)
+    assert torch.equal(p1, p2) and torch.equal(w1, w2)
+
+
+@pytest.mark.parametrize("prec", ["fp16x3s", "bf16x3s"])
+def test_one_phi_score_skipping_bounds(gpu, prec):
+    """F16X3S / BF16X3S: phi.X takes the FP16 phi (left-out tiles add exact
+    zeros to the centred sums sum phi (x - mu)), the row sums add the
+    unrounded phi, of which a left-out tile pair holds at most
+    2^-(ceil(log2 n) + 28) per term: pdf within 2^-28 of the dense one plus
+    FP32 rounding of the other sums, and w = sum phi (x - mu) + mu pdf
+    differs from the dense one by mu times pdf's difference."""
+    x, h = _spread_mixture(gpu, 20000, 6)
+    t = -(np.ceil(np.log2(x.shape[0])) + 28)
+    frac = _negligible_tile_frac(x, h, t - 1.0)
+    record(f"{prec}_skip_negligible_tile_frac", frac=frac)
+    assert frac > 0.3
+    p1, w1 = _ops.score16(x, h, variant="ordered", precision=prec)
+    p2, w2 = _ops.score16(x, h, variant="ordered", precision=prec, dense=True)
+    rel = float(((p1.double() - p2.double()).abs() / p2.double()).max())
+    mu = x.double().mean(0)
+    dw = (w1.double() - w2.double()) - mu[None, :] * (p1.double() - p2.double())[:, None]
+    rel_w = float((dw.abs().amax(1) / w2.double().abs().amax(1)).max())
+    record(f"{prec}_skip_rel_diff", pdf=rel, w_centred=rel_w)
+    assert rel <= 1e-6 and rel_w <= 1e-6
+    # rows that are not the data run dense: bitwise the dense results
+    rows = x[1::2].contiguous()
+    p1, w1 = _ops.score16_rows(rows, x, h, precision=prec)
+    p2, w2 = _ops.score16_rows(rows, x, h, precision=prec, dense=True)
+    assert torch.equal(p1, p2) and torch.equal(w1, w2)
+
+
+@pytest.mark.parametrize("prec,tol_pdf,tol_w", [("fp16x3s", 5e-5, 1e-4), ("bf16x3s", 1e-3, 5e-4),
+                                                 ("fp16", 5e-2, 2e-2)])
+def test_spatially_ordered_score_matches_fp64(gpu, prec, tol_pdf, tol_w):
+    """emp_score puts the data of the FP16-phi forms in spatial order and
+    returns the sums in the caller's order: against the FP64 GPU score on
+    data in random order, to the forms' bounds, and against the same form in
+    the caller's order to rounding."""
+    x, h = _spread_mixture(gpu, 12000, 7)
+    x = x[torch.randperm(x.shape[0], generator=torch.Generator().manual_seed(0)).to(gpu)]
+    x = x.contiguous()
+    pdf, w = precision.emp_score(x, h, precision=prec, device=gpu)
+    pe, we = _ops.ref64_score(x.double(), h)
+    e_pdf = float(((pdf.double() - pe).abs() / pe).max())
+    e_w = float((w.double() - we).abs().max() / we.abs().max())
+    record(f"spatial_score_{prec}", pdf_max_rel=e_pdf, w_max_rel=e_w)
+    assert e_pdf <= tol_pdf and e_w <= tol_w
+    p0, w0 = precision.emp_score(x, h, precision=prec, spatial=False, device=gpu)
+    assert float(((pdf - p0).abs() / p0).max()) <= tol_pdf
