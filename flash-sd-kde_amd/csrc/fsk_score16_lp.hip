@@ -50,10 +50,10 @@
 //
 // Negligible tiles. In the forms whose phi.X operand is phi rounded to FP16
 // (F16, F16X3S, BF16X3S), every phi <= 2^-26 becomes exactly 0 (below half
-// of FP16's smallest subnormal). The SKIP instances test each row tile's
-// cross terms against a pair of data tiles with one ballot and leave out the
-// exps, conversions, row sums and the phi.X MFMA when every exponent is
-// <= skip_t:
+// of FP16's smallest subnormal). The SKIP instances test a wave's cross terms
+// against a pair of data tiles (F16) or a tile (the others) with one ballot
+// and leave out the exps, conversions, row sums and the phi.X MFMAs when every
+// exponent of the wave's row tiles against the tile (pair) is <= skip_t:
 //   F16          skip_t = -26: row sums and phi.X both use the FP16 phi, so
 //                the results are bitwise those of the dense kernel;
 //   F16X3S,      phi.X as above (the centred sums bitwise); the row sums
@@ -62,6 +62,18 @@
 //                term (rows = data only; other rows run dense).
 // In data put in spatial order, most tile pairs of a large set are far apart
 // and skip.
+//
+// Far pairs before any MFMA. A pre-pass bounds every (wave's 64 rows, tile
+// pair) block from below: the data's three leading principal axes (block
+// power iteration on the covariance of a 4096-point sample) span a frame P
+// with orthonormal rows, so |q - x| >= |P q - P x| >= the gap D between the
+// blocks' boxes along those axes, and every exponent of the block is at most
+//   -c D^2 + 2^-10 (max a + max b) + 1,
+// where a = c |q - mu|^2, b = c |x - mu|^2 and 2^-10 (a + b) bounds the
+// cross term's error from 16-bit coordinates (2^-11 relative each). Blocks
+// whose bound is <= skip_t get a bit in a bitmap; the kernel leaves them out
+// without reading their operands. The bound holds for any data: the frame
+// only decides how many blocks it proves far.
 #include "fsk_common.h"
 #include "fsk_lowp.h"
 #include "fsk_plan.h"
@@ -260,6 +272,164 @@ __device__ __forceinline__ void wait_vmcnt(int n) {
   else __builtin_amdgcn_s_waitcnt(0xF73);
 }
 
+// ---- far-block bitmap of the forms with FP16 phi ----
+constexpr int kFrameSample = 4096;  // points of the covariance sample
+constexpr int kBoxAxes = 3;
+constexpr int kBoxFloats = 8;       // lo[3], hi[3], max c|x - mu|^2, unused
+constexpr int kFarRows = 16 * kScoreLpQT;  // rows per wave: one bitmap row
+// bitmap sizes above this run without it (the tile test alone)
+constexpr int64_t kFarBitsCap = int64_t(256) << 20;
+
+// One workgroup of 256: the covariance of a strided sample of the centred
+// data in FP64 (fixed order), then 40 rounds of block power iteration with
+// modified Gram-Schmidt for its three leading eigenvectors. frame[a * 16 + f];
+// a direction that vanishes (rank-deficient data) is left zero, which keeps
+// the rows orthonormal or zero.
+__global__ __launch_bounds__(256) void frame_kernel(const float *__restrict__ x, int64_t ld,
+                                                    int64_t n, const double *__restrict__ mu_sum,
+                                                    double inv_n, float *__restrict__ frame) {
+  __shared__ float pts[256 * kD];
+  __shared__ double cov[kD * kD];
+  __shared__ double q[kBoxAxes][kD], z[kBoxAxes][kD];  // iterates (thread 0)
+  const int t = threadIdx.x;
+  const int64_t cnt = n < kFrameSample ? n : kFrameSample;
+  const int64_t stride = n / cnt;
+  const int ei = t < kD * kD ? t / kD : 0, ej = t < kD * kD ? t % kD : 0;
+  double acc = 0.0;
+  for (int64_t c0 = 0; c0 < cnt; c0 += 256) {
+    __syncthreads();
+    for (int e = t; e < 256 * kD; e += 256) {
+      const int64_t k = c0 + e / kD;
+      const int f = e % kD;
+      pts[e] = k < cnt ? float(double(x[k * stride * ld + f]) - mu_sum[f] * inv_n) : 0.f;
+    }
+    __syncthreads();
+    if (t < kD * kD)
+      for (int k = 0; k < 256; ++k) acc += double(pts[k * kD + ei]) * double(pts[k * kD + ej]);
+  }
+  if (t < kD * kD) cov[t] = acc;
+  if (t < kBoxAxes * kD) {
+    const int a = t / kD, f = t % kD;
+    q[a][f] = f == a ? 1.0 : 0.01 * double((7 * f + 3 * a) % 11 + 1);
+  }
+  __syncthreads();
+  for (int it = 0; it < 40; ++it) {
+    // z = cov q, one entry per thread; then Gram-Schmidt on thread 0
+    if (t < kBoxAxes * kD) {
+      const int a = t / kD, f = t % kD;
+      double v = 0.0;
+      for (int g = 0; g < kD; ++g) v += cov[f * kD + g] * q[a][g];
+      z[a][f] = v;
+    }
+    __syncthreads();
+    if (t == 0) {
+      for (int a = 0; a < kBoxAxes; ++a) {
+        for (int b = 0; b < a; ++b) {
+          double d = 0.0;
+          for (int f = 0; f < kD; ++f) d += z[a][f] * z[b][f];
+          for (int f = 0; f < kD; ++f) z[a][f] -= d * z[b][f];
+        }
+        double nn = 0.0;
+        for (int f = 0; f < kD; ++f) nn += z[a][f] * z[a][f];
+        const double inv = nn > 1e-300 && nn < 1e300 ? 1.0 / sqrt(nn) : 0.0;
+        for (int f = 0; f < kD; ++f) q[a][f] = z[a][f] = z[a][f] * inv;
+      }
+    }
+    __syncthreads();
+  }
+  if (t < kBoxAxes * kD) frame[t] = float(q[t / kD][t % kD]);
+}
+
+// Boxes of consecutive groups of g points (tile pairs: 32; row blocks: 64):
+// the lowest and highest projection on each frame axis and the largest
+// c |x - mu|^2 (log2 units) over the group's points j < n. A group with no
+// points gets an empty box (lo = +inf, hi = -inf), which every bound finds far.
+__global__ void box_kernel(const float *__restrict__ x, int64_t ld, int64_t n, int64_t groups,
+                           int g, const double *__restrict__ mu_sum, double inv_n, double c,
+                           const float *__restrict__ frame, float *__restrict__ boxes) {
+  const int64_t b = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (b >= groups) return;
+  float lo[kBoxAxes], hi[kBoxAxes], nrm = 0.f;
+  for (int a = 0; a < kBoxAxes; ++a) {
+    lo[a] = INFINITY;
+    hi[a] = -INFINITY;
+  }
+  for (int64_t j = b * g; j < b * g + g && j < n; ++j) {
+    float y[kD];
+    double s2 = 0.0;
+    for (int f = 0; f < kD; ++f) {
+      const double v = double(x[j * ld + f]) - mu_sum[f] * inv_n;
+      y[f] = float(v);
+      s2 += v * v;
+    }
+    for (int a = 0; a < kBoxAxes; ++a) {
+      float pa = 0.f;
+      for (int f = 0; f < kD; ++f) pa += frame[a * kD + f] * y[f];
+      lo[a] = fminf(lo[a], pa);
+      hi[a] = fmaxf(hi[a], pa);
+    }
+    nrm = fmaxf(nrm, float(c * s2));
+  }
+  float *o = boxes + b * kBoxFloats;
+  for (int a = 0; a < kBoxAxes; ++a) {
+    o[a] = lo[a];
+    o[kBoxAxes + a] = hi[a];
+  }
+  o[6] = nrm;
+  o[7] = 0.f;
+}
+
+// bits[rb * words + p / 64] bit p % 64 = 1 when every exponent of row block
+// rb against tile pair p is provably <= skip_t (or p has no points), for the
+// `waves` row blocks of workgroup g; row groups * waves + g holds their AND
+// (far for every wave of the workgroup). One thread per (g, p); a wave
+// covers 64 consecutive p of one g.
+__global__ void far_bits_kernel(const float *__restrict__ row_boxes,
+                                const float *__restrict__ pair_boxes, int64_t npairs,
+                                int64_t words, int64_t groups, int waves, float c, float skip_t,
+                                unsigned long long *__restrict__ bits) {
+  const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= groups * words * 64) return;  // a multiple of 64: whole waves exit
+  // g is the same on the whole wave: scalar loads of the row boxes
+  const int64_t g = __builtin_amdgcn_readfirstlane(int(e / (words * 64)));
+  const int64_t p = e % (words * 64);
+  float plo[kBoxAxes], phi[kBoxAxes], pn = 0.f;
+  const bool real = p < npairs;
+  for (int a = 0; a < kBoxAxes; ++a) {
+    plo[a] = real ? pair_boxes[p * kBoxFloats + a] : 0.f;
+    phi[a] = real ? pair_boxes[p * kBoxFloats + kBoxAxes + a] : 0.f;
+  }
+  if (real) pn = pair_boxes[p * kBoxFloats + 6];
+  unsigned long long all = ~0ull;
+  for (int k = 0; k < waves; ++k) {
+    const float *r = row_boxes + (g * waves + k) * kBoxFloats;
+    bool far = true;
+    if (real) {
+      float d2 = 0.f;
+      for (int a = 0; a < kBoxAxes; ++a) {
+        const float gap = fmaxf(fmaxf(plo[a] - r[kBoxAxes + a], r[a] - phi[a]), 0.f);
+        d2 += gap * gap;
+      }
+      const float bound = -c * d2 + 0x1p-10f * (r[6] + pn) + 1.f;
+      far = !(bound > skip_t);
+    }
+    const unsigned long long m = __builtin_amdgcn_ballot_w64(far);
+    all &= m;
+    if ((threadIdx.x & 63) == 0) bits[(g * waves + k) * words + p / 64] = m;
+  }
+  if ((threadIdx.x & 63) == 0) bits[(groups * waves + g) * words + p / 64] = all;
+}
+
+// One bitmap word by a scalar load that completes inside the asm block: a
+// uniform 8-byte read that stays off the vector-memory counter the chunk
+// copies are waited on with (a vector load would make the compiler wait for
+// every chunk in flight before reading it).
+__device__ __forceinline__ unsigned long long sload_u64(const unsigned long long *p) {
+  unsigned long long v;
+  __asm__ volatile("s_load_dwordx2 %0, %1, 0x0\n\ts_waitcnt lgkmcnt(0)" : "=s"(v) : "s"(p) : "memory");
+  return v;
+}
+
 // Largest of a lane's cross-term exponents.
 __device__ __forceinline__ float max4(f4 d) {
   return fmaxf(fmaxf(d[0], d[1]), fmaxf(d[2], d[3]));
@@ -271,7 +441,8 @@ __attribute__((amdgpu_waves_per_eu(lp_min_waves<PREC>())))
 void score16_lp_kernel(
     const unsigned short *__restrict__ blk, const unsigned short *__restrict__ qrec, int64_t m,
     int64_t npairs_data, int64_t split_pairs, int64_t qblocks, int64_t ntasks, int64_t per_xcd,
-    float skip_t, float *__restrict__ pdf_part, float *__restrict__ w_part) {
+    float skip_t, const unsigned long long *__restrict__ far_bits, int64_t far_words,
+    float *__restrict__ pdf_part, float *__restrict__ w_part) {
   using BL = LpPair<PREC>;
   static_assert(!SKIP || BL::kF16Phi, "tiles are skipped only where phi.X takes FP16 phi");
   constexpr bool X6 = BL::kX6;
@@ -297,7 +468,14 @@ void score16_lp_kernel(
   __shared__ __attribute__((aligned(16))) unsigned short lds[NB * BL::kChunkU16];
 
   // task, split and the data range are wave-uniform: keep them scalar
-  const int64_t task = __builtin_amdgcn_readfirstlane(int(xcd_task(blockIdx.x, per_xcd)));
+  // Dense instances: each XCD takes a contiguous range of tasks (row blocks
+  // of one data split share its L2). SKIP instances: task = workgroup index,
+  // so the round-robin dispatch gives each XCD every eighth row block; the
+  // work of a row block depends on how many tile pairs lie near it, and
+  // contiguous ranges of spatially ordered rows left some XCDs with most of
+  // the near pairs.
+  const int64_t task = __builtin_amdgcn_readfirstlane(
+      int(SKIP ? int64_t(blockIdx.x) : xcd_task(blockIdx.x, per_xcd)));
   if (task >= ntasks) return;
   const int64_t split = __builtin_amdgcn_readfirstlane(int(task / qblocks));
   const int64_t qb = task - split * qblocks;
@@ -353,49 +531,109 @@ void score16_lp_kernel(
     }
   };
 
-  for (int c = 0; c < NB - 1 && c < nchunk; ++c) stage(c, c);
+  // this wave's row of the far-block bitmap and the workgroup's (far for all
+  // of its waves), read 64 pairs at a time
+  const unsigned long long *far_row =
+      SKIP && far_bits ? far_bits + (qb * W + wave) * far_words : nullptr;
+  const unsigned long long *wg_row =
+      SKIP && far_bits ? far_bits + (qblocks * W + qb) * far_words : nullptr;
+  int64_t widx = -1, gidx = -1;
+  unsigned long long fword = 0, gword = 0;
+  // Chunk c is needed unless every pair of it is far for every wave: a
+  // workgroup-uniform decision (all waves read the same words), so chunks
+  // nobody needs are neither copied nor waited for.
+  auto needed = [&](int c) -> bool {
+    if (!wg_row) return true;
+    const int64_t P0 = p_begin + int64_t(c) * CH;
+    const int cnt = npair - c * CH < CH ? npair - c * CH : CH;
+    for (int t = 0; t < cnt; ++t) {
+      const int64_t P = P0 + t;
+      if ((P >> 6) != gidx) {
+        gidx = P >> 6;
+        gword = sload_u64(wg_row + gidx);
+      }
+      if (!((gword >> (P & 63)) & 1)) return true;
+    }
+    return false;
+  };
+  auto flush = [&]() {
+#pragma unroll
+    for (int R = 0; R < QT; ++R) {
+      pdf64[R] += pdf[R];
+      pdf[R] = 0.f;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) w64[R][r] += w[R][r];
+      w[R] = f4{0.f, 0.f, 0.f, 0.f};
+    }
+  };
   int tc = 0;
-  for (int ch = 0; ch < nchunk; ++ch) {
-    // This wave's copies of chunk ch have landed once at most those of the
-    // chunks staged after it are outstanding (loads retire in order); the
-    // barrier makes every wave's copies visible, and every wave has finished
-    // reading the buffer staged next (chunk ch - 1's). A plain s_barrier: a
-    // fenced one would also wait for the chunks in flight.
-    const int newer = nchunk - 1 - ch < NB - 2 ? nchunk - 1 - ch : NB - 2;
-    wait_vmcnt(newer * my_loads);
-    __asm__ volatile("" ::: "memory");
-    __builtin_amdgcn_s_barrier();
-    __asm__ volatile("" ::: "memory");
-    if (ch + NB - 1 < nchunk) stage(ch + NB - 1, (ch + NB - 1) % NB);
-    const unsigned short *cb = lds + (ch % NB) * BL::kChunkU16 + lane * 8;
+  // the pairs of chunk ch, from LDS buffer buf
+  auto run_chunk = [&](int ch, int buf) {
+    const unsigned short *cb = lds + buf * BL::kChunkU16 + lane * 8;
     const int np = npair - ch * CH < CH ? npair - ch * CH : CH;
     for (int t = 0; t < np; ++t) {
       const unsigned short *pb = cb + t * BL::kPairU16;
-      if constexpr (SINGLE) {
-        // single forms: both tiles at once, one phi.X MFMA [phi_a | phi_b]
-        u16x8 Aa[KC], Ab[KC];
-        Aa[0] = lds_u16x8(pb);
-        Ab[0] = lds_u16x8(pb + S1);
-        const u16x8 BX = lds_u16x8(pb + 2 * S1);
-        if constexpr (SKIP) {
-          // the cross terms of two row tiles, then per row tile one ballot
-          // decides
-          f4 da[QT], db[QT];
+      // pairs the bitmap proves far add exact zeros to phi.X and at most
+      // n 2^skip_t to a row sum: leave them out without reading them
+      bool far = false;
+      if constexpr (SKIP) {
+        if (far_row) {
+          const int64_t P = p_begin + int64_t(ch) * CH + t;
+          if ((P >> 6) != widx) {
+            widx = P >> 6;
+            fword = sload_u64(far_row + widx);
+          }
+          far = (fword >> (P & 63)) & 1;
+        }
+      }
+      if (!far) {
+        if constexpr (SINGLE) {
+          // single forms: both tiles at once, one phi.X MFMA [phi_a | phi_b]
+          u16x8 Aa[KC], Ab[KC];
+          Aa[0] = lds_u16x8(pb);
+          Ab[0] = lds_u16x8(pb + S1);
+          const u16x8 BX = lds_u16x8(pb + 2 * S1);
+          if constexpr (SKIP) {
+            // all cross terms, then one ballot for the wave's row tiles: in
+            // spatially ordered data a tile pair is negligible for nearly all
+            // of a wave's 64 consecutive rows or for none
+            f4 da[QT], db[QT];
 #pragma unroll
-          for (int R = 0; R < QT; ++R) {
-            if (R % 2 == 0) {
+            for (int R = 0; R < QT; ++R) {
+              da[R] = lp_cross<PREC>(Aa, B1[R], B2[R], B3[R], B4[R]);
+              db[R] = lp_cross<PREC>(Ab, B1[R], B2[R], B3[R], B4[R]);
+            }
+            float mx = fmaxf(max4(da[0]), max4(db[0]));
 #pragma unroll
-              for (int r = R; r < R + 2 && r < QT; ++r) {
-                da[r] = lp_cross<PREC>(Aa, B1[r], B2[r], B3[r], B4[r]);
-                db[r] = lp_cross<PREC>(Ab, B1[r], B2[r], B3[r], B4[r]);
+            for (int R = 1; R < QT; ++R) mx = fmaxf(mx, fmaxf(max4(da[R]), max4(db[R])));
+            if (__builtin_amdgcn_ballot_w64(mx > skip_t)) {
+#pragma unroll
+              for (int R = 0; R < QT; ++R) {
+                float ea[4], eb[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                  ea[u] = fexp2_le1(da[R][u]);
+                  eb[u] = fexp2_le1(db[R][u]);
+                }
+                const unsigned a0 = pack2<PX>(ea[0], ea[1]), a1 = pack2<PX>(ea[2], ea[3]);
+                const unsigned b0 = pack2<PX>(eb[0], eb[1]), b1 = pack2<PX>(eb[2], eb[3]);
+                pdf[R] = dot2_ones<PX>(a0, pdf[R]);
+                pdf[R] = dot2_ones<PX>(a1, pdf[R]);
+                pdf[R] = dot2_ones<PX>(b0, pdf[R]);
+                pdf[R] = dot2_ones<PX>(b1, pdf[R]);
+                w[R] = mfma16(words(a0, a1, b0, b1), BX, w[R], PX);
               }
             }
-            if (__builtin_amdgcn_ballot_w64(fmaxf(max4(da[R]), max4(db[R])) > skip_t)) {
+          } else {
+#pragma unroll
+            for (int R = 0; R < QT; ++R) {
+              const f4 da = lp_cross<PREC>(Aa, B1[R], B2[R], B3[R], B4[R]);
+              const f4 db = lp_cross<PREC>(Ab, B1[R], B2[R], B3[R], B4[R]);
               float ea[4], eb[4];
 #pragma unroll
               for (int u = 0; u < 4; ++u) {
-                ea[u] = fexp2_le1(da[R][u]);
-                eb[u] = fexp2_le1(db[R][u]);
+                ea[u] = fexp2_le1(da[u]);
+                eb[u] = fexp2_le1(db[u]);
               }
               const unsigned a0 = pack2<PX>(ea[0], ea[1]), a1 = pack2<PX>(ea[2], ea[3]);
               const unsigned b0 = pack2<PX>(eb[0], eb[1]), b1 = pack2<PX>(eb[2], eb[3]);
@@ -405,110 +643,154 @@ void score16_lp_kernel(
               pdf[R] = dot2_ones<PX>(b1, pdf[R]);
               w[R] = mfma16(words(a0, a1, b0, b1), BX, w[R], PX);
             }
+            // Issue order of one pair (3 LDS reads, 12 MFMAs, 32 exps, 32 other
+            // VALU ops per wave): the compiler's own order clusters the MFMAs and
+            // the exps; interleaving 3 exps and 2 VALU ops per MFMA keeps the
+            // matrix core busy under the transcendental work (FP16 score at
+            // n = 2^20: 125 -> 119 ms; other mixes measured 119-123 ms).
+            __builtin_amdgcn_sched_group_barrier(0x100, 3, 0);  // LDS reads
+            __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);  // MFMA
+#pragma unroll
+            for (int gi = 0; gi < 10; ++gi) {
+              __builtin_amdgcn_sched_group_barrier(0x400, 3, 0);  // transcendental
+              __builtin_amdgcn_sched_group_barrier(0x002, 2, 0);  // VALU
+              __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
+            }
+            __builtin_amdgcn_sched_group_barrier(0x002, 64, 0);
           }
         } else {
+          // split and one-phi forms: tile a, then tile b (one tile's operands
+          // live at a time); the split forms keep phi_h of tile a per row tile
+          // for the shared [phi_h,a | phi_h,b] . [Xl_a | Xl_b] product
+          unsigned ha[QT][2];
 #pragma unroll
-        for (int R = 0; R < QT; ++R) {
-          const f4 da = lp_cross<PREC>(Aa, B1[R], B2[R], B3[R], B4[R]);
-          const f4 db = lp_cross<PREC>(Ab, B1[R], B2[R], B3[R], B4[R]);
-          float ea[4], eb[4];
+          for (int tt = 0; tt < 2; ++tt) {
+            const unsigned short *tb = pb + tt * KC * S1;
+            u16x8 A[KC];
 #pragma unroll
-          for (int u = 0; u < 4; ++u) {
-            ea[u] = fexp2_le1(da[u]);
-            eb[u] = fexp2_le1(db[u]);
-          }
-          const unsigned a0 = pack2<PX>(ea[0], ea[1]), a1 = pack2<PX>(ea[2], ea[3]);
-          const unsigned b0 = pack2<PX>(eb[0], eb[1]), b1 = pack2<PX>(eb[2], eb[3]);
-          pdf[R] = dot2_ones<PX>(a0, pdf[R]);
-          pdf[R] = dot2_ones<PX>(a1, pdf[R]);
-          pdf[R] = dot2_ones<PX>(b0, pdf[R]);
-          pdf[R] = dot2_ones<PX>(b1, pdf[R]);
-          w[R] = mfma16(words(a0, a1, b0, b1), BX, w[R], PX);
-        }
-        // Issue order of one pair (3 LDS reads, 12 MFMAs, 32 exps, 32 other
-        // VALU ops per wave): the compiler's own order clusters the MFMAs and
-        // the exps; interleaving 3 exps and 2 VALU ops per MFMA keeps the
-        // matrix core busy under the transcendental work (FP16 score at
-        // n = 2^20: 125 -> 119 ms; other mixes measured 119-123 ms).
-        __builtin_amdgcn_sched_group_barrier(0x100, 3, 0);  // LDS reads
-        __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);  // MFMA
+            for (int s = 0; s < KC; ++s) A[s] = lds_u16x8(tb + s * S1);
+            const u16x8 BX = lds_u16x8(pb + (2 * KC + tt) * S1);
+            const u16x8 BXL = SPLIT_PHI && tt == 1 ? lds_u16x8(pb + (2 * KC + 2) * S1) : BX;
+            if constexpr (SKIP) {
+              // one ballot per tile for the wave's row tiles, as above
+              f4 d[QT];
 #pragma unroll
-        for (int gi = 0; gi < 10; ++gi) {
-          __builtin_amdgcn_sched_group_barrier(0x400, 3, 0);  // transcendental
-          __builtin_amdgcn_sched_group_barrier(0x002, 2, 0);  // VALU
-          __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
-        }
-        __builtin_amdgcn_sched_group_barrier(0x002, 64, 0);
-        }
-      } else {
-        // split and one-phi forms: tile a, then tile b (one tile's operands
-        // live at a time); the split forms keep phi_h of tile a per row tile
-        // for the shared [phi_h,a | phi_h,b] . [Xl_a | Xl_b] product
-        unsigned ha[QT][2];
+              for (int R = 0; R < QT; ++R) d[R] = lp_cross<PREC>(A, B1[R], B2[R], B3[R], B4[R]);
+              float mx = max4(d[0]);
 #pragma unroll
-        for (int tt = 0; tt < 2; ++tt) {
-          const unsigned short *tb = pb + tt * KC * S1;
-          u16x8 A[KC];
+              for (int R = 1; R < QT; ++R) mx = fmaxf(mx, max4(d[R]));
+              if (__builtin_amdgcn_ballot_w64(mx > skip_t)) {
 #pragma unroll
-          for (int s = 0; s < KC; ++s) A[s] = lds_u16x8(tb + s * S1);
-          const u16x8 BX = lds_u16x8(pb + (2 * KC + tt) * S1);
-          const u16x8 BXL = SPLIT_PHI && tt == 1 ? lds_u16x8(pb + (2 * KC + 2) * S1) : BX;
-          if constexpr (SKIP) {
-            f4 d[QT];
+                for (int R = 0; R < QT; ++R) {
+                  float e[4];
 #pragma unroll
-            for (int R = 0; R < QT; ++R) d[R] = lp_cross<PREC>(A, B1[R], B2[R], B3[R], B4[R]);
+                  for (int u = 0; u < 4; ++u) {
+                    e[u] = fexp2_le1(d[R][u]);
+                    pdf[R] += e[u];
+                  }
+                  const unsigned p0 = pack2<PX>(e[0], e[1]), p1 = pack2<PX>(e[2], e[3]);
+                  w[R] = mfma16(words(p0, p1, p0, p1), BX, w[R], PX);
+                }
+              }
+              continue;
+            }
 #pragma unroll
             for (int R = 0; R < QT; ++R) {
-              if (__builtin_amdgcn_ballot_w64(max4(d[R]) > skip_t)) {
-                float e[4];
+              const f4 d = lp_cross<PREC>(A, B1[R], B2[R], B3[R], B4[R]);
+              float e[4];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                  e[u] = fexp2_le1(d[R][u]);
-                  pdf[R] += e[u];
+              for (int u = 0; u < 4; ++u) {
+                e[u] = fexp2_le1(d[u]);
+                pdf[R] += e[u];
+              }
+              if constexpr (SPLIT_PHI) {
+                unsigned hi[2], lo[2];
+                split_phi(e, hi, lo);
+                w[R] = mfma16(words(hi[0], hi[1], lo[0], lo[1]), BX, w[R], PX);
+                if (tt == 0) {
+                  ha[R][0] = hi[0];
+                  ha[R][1] = hi[1];
+                } else {
+                  w[R] = mfma16(words(ha[R][0], ha[R][1], hi[0], hi[1]), BXL, w[R], PX);
                 }
+              } else {
                 const unsigned p0 = pack2<PX>(e[0], e[1]), p1 = pack2<PX>(e[2], e[3]);
                 w[R] = mfma16(words(p0, p1, p0, p1), BX, w[R], PX);
               }
-            }
-            continue;
-          }
-#pragma unroll
-          for (int R = 0; R < QT; ++R) {
-            const f4 d = lp_cross<PREC>(A, B1[R], B2[R], B3[R], B4[R]);
-            float e[4];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-              e[u] = fexp2_le1(d[u]);
-              pdf[R] += e[u];
-            }
-            if constexpr (SPLIT_PHI) {
-              unsigned hi[2], lo[2];
-              split_phi(e, hi, lo);
-              w[R] = mfma16(words(hi[0], hi[1], lo[0], lo[1]), BX, w[R], PX);
-              if (tt == 0) {
-                ha[R][0] = hi[0];
-                ha[R][1] = hi[1];
-              } else {
-                w[R] = mfma16(words(ha[R][0], ha[R][1], hi[0], hi[1]), BXL, w[R], PX);
-              }
-            } else {
-              const unsigned p0 = pack2<PX>(e[0], e[1]), p1 = pack2<PX>(e[2], e[3]);
-              w[R] = mfma16(words(p0, p1, p0, p1), BX, w[R], PX);
             }
           }
         }
       }
       if (++tc == FLUSH) {
         tc = 0;
-#pragma unroll
-        for (int R = 0; R < QT; ++R) {
-          pdf64[R] += pdf[R];
-          pdf[R] = 0.f;
-#pragma unroll
-          for (int r = 0; r < 4; ++r) w64[R][r] += w[R][r];
-          w[R] = f4{0.f, 0.f, 0.f, 0.f};
-        }
+        flush();
       }
+    }
+  };
+  if constexpr (SKIP) {
+    // Copies run NB - 1 needed chunks ahead: a stretch of chunks nobody needs
+    // costs a scan of their bits, not a copy latency. ring[b] is the chunk
+    // buffer b holds (nchunk: none).
+    int scan = 0;
+    auto next_needed = [&]() -> int {
+      while (scan < nchunk && !needed(scan)) ++scan;
+      return scan < nchunk ? scan++ : nchunk;
+    };
+    int ring[NB];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) ring[b] = nchunk;
+#pragma unroll
+    for (int b = 0; b < NB - 1; ++b) {
+      ring[b] = next_needed();
+      if (ring[b] < nchunk) stage(ring[b], b);
+    }
+    int last = -1;
+    for (int k = 0;; ++k) {
+      const int ch = ring[k % NB];
+      if (ch >= nchunk) break;
+      // pairs of the chunks skipped since the last one (all whole chunks) add
+      // nothing; flushing once if a flush point lies among them keeps the
+      // FP64 sums bitwise those of the dense order
+      const int skipped = (ch - last - 1) * CH;
+      if (skipped > 0) {
+        if (tc + skipped >= FLUSH) flush();
+        tc = (tc + skipped) % FLUSH;
+      }
+      last = ch;
+      // This wave's copies of chunk ch have landed once at most those of the
+      // chunks staged after it are outstanding (loads retire in order); the
+      // barrier makes every wave's copies visible, and every wave has finished
+      // reading the buffer staged next. A plain s_barrier: a fenced one would
+      // also wait for the chunks in flight.
+      int newer = 0;
+#pragma unroll
+      for (int j = 1; j <= NB - 2; ++j)
+        if (ring[(k + j) % NB] < nchunk) newer += my_loads;
+      wait_vmcnt(newer);
+      __asm__ volatile("" ::: "memory");
+      __builtin_amdgcn_s_barrier();
+      __asm__ volatile("" ::: "memory");
+      {
+        const int bn = (k + NB - 1) % NB;
+        ring[bn] = next_needed();
+        if (ring[bn] < nchunk) stage(ring[bn], bn);
+      }
+      run_chunk(ch, k % NB);
+    }
+  } else {
+    // every chunk in order, copied NB - 1 ahead
+    for (int c = 0; c < NB - 1 && c < nchunk; ++c) stage(c, c);
+    for (int ch = 0; ch < nchunk; ++ch) {
+      // this wave's copies of chunk ch have landed once at most those of the
+      // chunks staged after it are outstanding; the barrier makes every
+      // wave's copies visible and retires all reads of the buffer staged next
+      const int newer = nchunk - 1 - ch < NB - 2 ? nchunk - 1 - ch : NB - 2;
+      wait_vmcnt(newer * my_loads);
+      __asm__ volatile("" ::: "memory");
+      __builtin_amdgcn_s_barrier();
+      __asm__ volatile("" ::: "memory");
+      if (ch + NB - 1 < nchunk) stage(ch + NB - 1, (ch + NB - 1) % NB);
+      run_chunk(ch, ch % NB);
     }
   }
 #pragma unroll
@@ -570,6 +852,10 @@ struct ScoreLpLayout {
   unsigned short *xrec, *qrec, *xt, *blk;
   float *pdf_part, *w_part;
   int64_t npairs_data, npairs_alloc;
+  // far-block bitmap (forms with FP16 phi, when within kFarBitsCap)
+  float *frame, *pair_boxes, *row_boxes;
+  unsigned long long *far_bits;
+  int64_t row_blocks, far_words;
 };
 
 static size_t score_lp_layout(void *ws, int64_t n, int64_t m, unsigned prec, ScoreLpLayout *L) {
@@ -584,6 +870,21 @@ static size_t score_lp_layout(void *ws, int64_t n, int64_t m, unsigned prec, Sco
   L->blk = cv.take<unsigned short>(L->npairs_alloc * lp_pair_u16(prec));
   L->pdf_part = cv.take<float>(p.splits * m);
   L->w_part = cv.take<float>(p.splits * m * kD);
+  L->row_blocks = p.qblocks * score_lp_waves(prec);
+  L->far_words = cdiv(L->npairs_data, 64);
+  const bool bits = (prec == FSK_PREC_F16 || prec == FSK_PREC_F16X3S || prec == FSK_PREC_BF16X3S) &&
+                    (L->row_blocks + p.qblocks) * L->far_words * 8 <= kFarBitsCap;
+  L->frame = L->pair_boxes = L->row_boxes = nullptr;
+  L->far_bits = nullptr;
+  if (bits) {
+    L->frame = cv.take<float>(kBoxAxes * kD);
+    L->pair_boxes = cv.take<float>(L->npairs_data * kBoxFloats);
+    L->row_boxes = cv.take<float>(L->row_blocks * kBoxFloats);
+    // the waves' rows, then one row per workgroup (the AND of its waves')
+    L->far_bits = cv.take<unsigned long long>((L->row_blocks + p.qblocks) * L->far_words);
+  } else {
+    L->row_blocks = L->far_words = 0;
+  }
   return cv.used;
 }
 
@@ -595,22 +896,21 @@ size_t score16_lp_ws_bytes(int64_t n, int64_t m, unsigned prec) {
 // The score kernel of form P, with the tile test where the form has one and
 // `skip` asks for it.
 template <int P, int QT>
-static void launch_score_lp(bool skip, float skip_t, int64_t grid, hipStream_t s,
-                            const ScoreLpLayout &L, int64_t m, int64_t split_pairs,
-                            int64_t qblocks, int64_t ntasks, int64_t per_xcd) {
+static void launch_score_lp(bool skip, float skip_t, const unsigned long long *far_bits,
+                            int64_t grid, hipStream_t s, const ScoreLpLayout &L, int64_t m,
+                            int64_t split_pairs, int64_t qblocks, int64_t ntasks,
+                            int64_t per_xcd) {
   if constexpr (LpPair<P>::kF16Phi) {
     if (skip) {
       hipLaunchKernelGGL((score16_lp_kernel<P, QT, true>), dim3(grid), dim3(64 * LpPair<P>::kWaves),
-                         0, s, L.blk,
-                         L.qrec, m, L.npairs_data, split_pairs, qblocks, ntasks, per_xcd, skip_t,
-                         L.pdf_part, L.w_part);
+                         0, s, L.blk, L.qrec, m, L.npairs_data, split_pairs, qblocks, ntasks,
+                         per_xcd, skip_t, far_bits, L.far_words, L.pdf_part, L.w_part);
       return;
     }
   }
   hipLaunchKernelGGL((score16_lp_kernel<P, QT, false>), dim3(grid), dim3(64 * LpPair<P>::kWaves),
-                     0, s, L.blk,
-                     L.qrec, m, L.npairs_data, split_pairs, qblocks, ntasks, per_xcd, 0.f,
-                     L.pdf_part, L.w_part);
+                     0, s, L.blk, L.qrec, m, L.npairs_data, split_pairs, qblocks, ntasks, per_xcd,
+                     0.f, (const unsigned long long *)nullptr, int64_t(0), L.pdf_part, L.w_part);
 }
 
 // Score accumulators of m rows against the KDE of n data points with 16-bit
@@ -650,6 +950,21 @@ int score16_lp_run(const char *what, const float *rows_in, int64_t ldr, int64_t 
     skip = true;
     skip_t = -float(std::ceil(std::log2(double(n))) + 28.0);
   }
+  // far-block bitmap: frame of the data, boxes of tile pairs and of the
+  // waves' row blocks, one bit per (row block, pair)
+  const unsigned long long *far_bits = nullptr;
+  if (skip && L.far_bits && n > 0 && m > 0) {
+    const double cabs = -exp2_coef(h);
+    hipLaunchKernelGGL(frame_kernel, dim3(1), dim3(256), 0, s, data, ld, n, L.mu, inv_n, L.frame);
+    hipLaunchKernelGGL(box_kernel, dim3(cdiv(L.npairs_data, 256)), dim3(256), 0, s, data, ld, n,
+                       L.npairs_data, kLpPairPts, L.mu, inv_n, cabs, L.frame, L.pair_boxes);
+    hipLaunchKernelGGL(box_kernel, dim3(cdiv(L.row_blocks, 256)), dim3(256), 0, s, rows_in, ldr,
+                       m, L.row_blocks, kFarRows, L.mu, inv_n, cabs, L.frame, L.row_boxes);
+    hipLaunchKernelGGL(far_bits_kernel, dim3(cdiv(p.qblocks * L.far_words * 64, 256)), dim3(256),
+                       0, s, L.row_boxes, L.pair_boxes, L.npairs_data, L.far_words, p.qblocks,
+                       score_lp_waves(prec), float(cabs), skip_t, L.far_bits);
+    far_bits = L.far_bits;
+  }
 #define FSK_LP(P, PXT, QT)                                                                       \
   do {                                                                                           \
     hipLaunchKernelGGL(prep_lse_kernel<P>, dim3(cdiv(xrows, 256)), dim3(256), 0, s, data, ld, n, \
@@ -665,7 +980,8 @@ int score16_lp_run(const char *what, const float *rows_in, int64_t ldr, int64_t 
                        dim3(cdiv(L.npairs_alloc * LpPair<P>::kSlots * 64, 256)), dim3(256), 0,   \
                        s, L.xrec, L.xt, L.npairs_data, L.npairs_alloc, L.blk);                    \
     prof_begin(FSK_OP_SCORE16_ORDERED, s);                                                       \
-    launch_score_lp<P, QT>(skip, skip_t, grid, s, L, m, split_pairs, p.qblocks, ntasks, per_xcd); \
+    launch_score_lp<P, QT>(skip, skip_t, far_bits, grid, s, L, m, split_pairs, p.qblocks, ntasks, \
+                           per_xcd);                                                             \
     prof_end(FSK_OP_SCORE16_ORDERED, s);                                                         \
   } while (0)
   switch (prec) {

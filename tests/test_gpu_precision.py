@@ -226,6 +226,27 @@ def test_fp16_score_skipping_is_bitwise_dense(gpu):
     assert torch.equal(p1, p2) and torch.equal(w1, w2)
 
 
+@pytest.mark.parametrize("gap", [5.6, 6.0, 6.4, 7.0, 9.0])
+def test_fp16_far_blocks_at_the_threshold_are_bitwise_dense(gpu, gap):
+    """Two tight clusters whose closest pairs sit around the F16 threshold
+    (c |q - x|^2 = 26 at |q - x| ~ 6 h): blocks the box bound proves far and
+    blocks it cannot prove far both end bitwise as in the dense kernel, for
+    the data set's own score and for rows that are not the data."""
+    rng = np.random.default_rng(int(gap * 10))
+    u = rng.normal(size=16)
+    u /= np.linalg.norm(u)
+    a = rng.normal(size=(3000, 16)) * 0.1 + 0.5 * gap * u
+    b = rng.normal(size=(3000, 16)) * 0.1 - 0.5 * gap * u + 40.0 * (gap > 8.0)
+    x = torch.as_tensor(np.concatenate([a, b]).astype(np.float32), device=gpu)
+    p1, w1 = _ops.score16(x, 1.0, variant="ordered", precision="fp16")
+    p2, w2 = _ops.score16(x, 1.0, variant="ordered", precision="fp16", dense=True)
+    assert torch.equal(p1, p2) and torch.equal(w1, w2)
+    rows = (x[::7] + 0.05).contiguous()
+    p1, w1 = _ops.score16_rows(rows, x, 1.0, precision="fp16")
+    p2, w2 = _ops.score16_rows(rows, x, 1.0, precision="fp16", dense=True)
+    assert torch.equal(p1, p2) and torch.equal(w1, w2)
+
+
 @pytest.mark.parametrize("prec", ["fp16x3s", "bf16x3s"])
 def test_one_phi_score_skipping_bounds(gpu, prec):
     """F16X3S / BF16X3S: phi.X takes the FP16 phi (left-out tiles add exact
