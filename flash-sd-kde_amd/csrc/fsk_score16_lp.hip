@@ -539,22 +539,24 @@ void score16_lp_kernel(
       SKIP && far_bits ? far_bits + (qblocks * W + qb) * far_words : nullptr;
   int64_t widx = -1, gidx = -1;
   unsigned long long fword = 0, gword = 0;
-  // Chunk c is needed unless every pair of it is far for every wave: a
-  // workgroup-uniform decision (all waves read the same words), so chunks
-  // nobody needs are neither copied nor waited for.
-  auto needed = [&](int c) -> bool {
-    if (!wg_row) return true;
-    const int64_t P0 = p_begin + int64_t(c) * CH;
-    const int cnt = npair - c * CH < CH ? npair - c * CH : CH;
-    for (int t = 0; t < cnt; ++t) {
-      const int64_t P = P0 + t;
-      if ((P >> 6) != gidx) {
-        gidx = P >> 6;
-        gword = sload_u64(wg_row + gidx);
-      }
-      if (!((gword >> (P & 63)) & 1)) return true;
+  // Bits of pairs [P, P + cnt) (cnt <= 64) of a bitmap row, bit t for pair
+  // P + t; idx / word cache the row's last word read (rows are walked in
+  // increasing order).
+  auto bits_at = [&](const unsigned long long *row, int64_t P, int cnt, int64_t &idx,
+                     unsigned long long &word) -> unsigned long long {
+    const int64_t w0 = P >> 6;
+    const int sh = int(P & 63);
+    if (w0 != idx) {
+      idx = w0;
+      word = sload_u64(row + w0);
     }
-    return false;
+    unsigned long long b = word >> sh;
+    if (sh + cnt > 64) {
+      idx = w0 + 1;
+      word = sload_u64(row + w0 + 1);
+      b |= word << (64 - sh);
+    }
+    return cnt == 64 ? b : b & ((1ull << cnt) - 1);
   };
   auto flush = [&]() {
 #pragma unroll
@@ -566,165 +568,176 @@ void score16_lp_kernel(
       w[R] = f4{0.f, 0.f, 0.f, 0.f};
     }
   };
+  // the computation of one tile pair, read from LDS at pb
+  auto pair_body = [&](const unsigned short *pb) {
+    if constexpr (SINGLE) {
+      // single forms: both tiles at once, one phi.X MFMA [phi_a | phi_b]
+      u16x8 Aa[KC], Ab[KC];
+      Aa[0] = lds_u16x8(pb);
+      Ab[0] = lds_u16x8(pb + S1);
+      const u16x8 BX = lds_u16x8(pb + 2 * S1);
+      if constexpr (SKIP) {
+        // all cross terms, then one ballot for the wave's row tiles: in
+        // spatially ordered data a tile pair is negligible for nearly all
+        // of a wave's 64 consecutive rows or for none
+        f4 da[QT], db[QT];
+#pragma unroll
+        for (int R = 0; R < QT; ++R) {
+          da[R] = lp_cross<PREC>(Aa, B1[R], B2[R], B3[R], B4[R]);
+          db[R] = lp_cross<PREC>(Ab, B1[R], B2[R], B3[R], B4[R]);
+        }
+        float mx = fmaxf(max4(da[0]), max4(db[0]));
+#pragma unroll
+        for (int R = 1; R < QT; ++R) mx = fmaxf(mx, fmaxf(max4(da[R]), max4(db[R])));
+        if (__builtin_amdgcn_ballot_w64(mx > skip_t)) {
+#pragma unroll
+          for (int R = 0; R < QT; ++R) {
+            float ea[4], eb[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+              ea[u] = fexp2_le1(da[R][u]);
+              eb[u] = fexp2_le1(db[R][u]);
+            }
+            const unsigned a0 = pack2<PX>(ea[0], ea[1]), a1 = pack2<PX>(ea[2], ea[3]);
+            const unsigned b0 = pack2<PX>(eb[0], eb[1]), b1 = pack2<PX>(eb[2], eb[3]);
+            pdf[R] = dot2_ones<PX>(a0, pdf[R]);
+            pdf[R] = dot2_ones<PX>(a1, pdf[R]);
+            pdf[R] = dot2_ones<PX>(b0, pdf[R]);
+            pdf[R] = dot2_ones<PX>(b1, pdf[R]);
+            w[R] = mfma16(words(a0, a1, b0, b1), BX, w[R], PX);
+          }
+        }
+      } else {
+#pragma unroll
+        for (int R = 0; R < QT; ++R) {
+          const f4 da = lp_cross<PREC>(Aa, B1[R], B2[R], B3[R], B4[R]);
+          const f4 db = lp_cross<PREC>(Ab, B1[R], B2[R], B3[R], B4[R]);
+          float ea[4], eb[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            ea[u] = fexp2_le1(da[u]);
+            eb[u] = fexp2_le1(db[u]);
+          }
+          const unsigned a0 = pack2<PX>(ea[0], ea[1]), a1 = pack2<PX>(ea[2], ea[3]);
+          const unsigned b0 = pack2<PX>(eb[0], eb[1]), b1 = pack2<PX>(eb[2], eb[3]);
+          pdf[R] = dot2_ones<PX>(a0, pdf[R]);
+          pdf[R] = dot2_ones<PX>(a1, pdf[R]);
+          pdf[R] = dot2_ones<PX>(b0, pdf[R]);
+          pdf[R] = dot2_ones<PX>(b1, pdf[R]);
+          w[R] = mfma16(words(a0, a1, b0, b1), BX, w[R], PX);
+        }
+        // Issue order of one pair (3 LDS reads, 12 MFMAs, 32 exps, 32 other
+        // VALU ops per wave): the compiler's own order clusters the MFMAs and
+        // the exps; interleaving 3 exps and 2 VALU ops per MFMA keeps the
+        // matrix core busy under the transcendental work (FP16 score at
+        // n = 2^20: 125 -> 119 ms; other mixes measured 119-123 ms).
+        __builtin_amdgcn_sched_group_barrier(0x100, 3, 0);  // LDS reads
+        __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);  // MFMA
+#pragma unroll
+        for (int gi = 0; gi < 10; ++gi) {
+          __builtin_amdgcn_sched_group_barrier(0x400, 3, 0);  // transcendental
+          __builtin_amdgcn_sched_group_barrier(0x002, 2, 0);  // VALU
+          __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
+        }
+        __builtin_amdgcn_sched_group_barrier(0x002, 64, 0);
+      }
+    } else {
+      // split and one-phi forms: tile a, then tile b (one tile's operands
+      // live at a time); the split forms keep phi_h of tile a per row tile
+      // for the shared [phi_h,a | phi_h,b] . [Xl_a | Xl_b] product
+      unsigned ha[QT][2];
+#pragma unroll
+      for (int tt = 0; tt < 2; ++tt) {
+        const unsigned short *tb = pb + tt * KC * S1;
+        u16x8 A[KC];
+#pragma unroll
+        for (int s = 0; s < KC; ++s) A[s] = lds_u16x8(tb + s * S1);
+        const u16x8 BX = lds_u16x8(pb + (2 * KC + tt) * S1);
+        const u16x8 BXL = SPLIT_PHI && tt == 1 ? lds_u16x8(pb + (2 * KC + 2) * S1) : BX;
+        if constexpr (SKIP) {
+          // one ballot per tile for the wave's row tiles, as above
+          f4 d[QT];
+#pragma unroll
+          for (int R = 0; R < QT; ++R) d[R] = lp_cross<PREC>(A, B1[R], B2[R], B3[R], B4[R]);
+          float mx = max4(d[0]);
+#pragma unroll
+          for (int R = 1; R < QT; ++R) mx = fmaxf(mx, max4(d[R]));
+          if (__builtin_amdgcn_ballot_w64(mx > skip_t)) {
+#pragma unroll
+            for (int R = 0; R < QT; ++R) {
+              float e[4];
+#pragma unroll
+              for (int u = 0; u < 4; ++u) {
+                e[u] = fexp2_le1(d[R][u]);
+                pdf[R] += e[u];
+              }
+              const unsigned p0 = pack2<PX>(e[0], e[1]), p1 = pack2<PX>(e[2], e[3]);
+              w[R] = mfma16(words(p0, p1, p0, p1), BX, w[R], PX);
+            }
+          }
+          continue;
+        }
+#pragma unroll
+        for (int R = 0; R < QT; ++R) {
+          const f4 d = lp_cross<PREC>(A, B1[R], B2[R], B3[R], B4[R]);
+          float e[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            e[u] = fexp2_le1(d[u]);
+            pdf[R] += e[u];
+          }
+          if constexpr (SPLIT_PHI) {
+            unsigned hi[2], lo[2];
+            split_phi(e, hi, lo);
+            w[R] = mfma16(words(hi[0], hi[1], lo[0], lo[1]), BX, w[R], PX);
+            if (tt == 0) {
+              ha[R][0] = hi[0];
+              ha[R][1] = hi[1];
+            } else {
+              w[R] = mfma16(words(ha[R][0], ha[R][1], hi[0], hi[1]), BXL, w[R], PX);
+            }
+          } else {
+            const unsigned p0 = pack2<PX>(e[0], e[1]), p1 = pack2<PX>(e[2], e[3]);
+            w[R] = mfma16(words(p0, p1, p0, p1), BX, w[R], PX);
+          }
+        }
+      }
+    }
+  };
   int tc = 0;
-  // the pairs of chunk ch, from LDS buffer buf
+  // every pair of chunk ch, from LDS buffer buf (dense instances)
   auto run_chunk = [&](int ch, int buf) {
     const unsigned short *cb = lds + buf * BL::kChunkU16 + lane * 8;
     const int np = npair - ch * CH < CH ? npair - ch * CH : CH;
     for (int t = 0; t < np; ++t) {
-      const unsigned short *pb = cb + t * BL::kPairU16;
-      // pairs the bitmap proves far add exact zeros to phi.X and at most
-      // n 2^skip_t to a row sum: leave them out without reading them
-      bool far = false;
-      if constexpr (SKIP) {
-        if (far_row) {
-          const int64_t P = p_begin + int64_t(ch) * CH + t;
-          if ((P >> 6) != widx) {
-            widx = P >> 6;
-            fword = sload_u64(far_row + widx);
-          }
-          far = (fword >> (P & 63)) & 1;
-        }
-      }
-      if (!far) {
-        if constexpr (SINGLE) {
-          // single forms: both tiles at once, one phi.X MFMA [phi_a | phi_b]
-          u16x8 Aa[KC], Ab[KC];
-          Aa[0] = lds_u16x8(pb);
-          Ab[0] = lds_u16x8(pb + S1);
-          const u16x8 BX = lds_u16x8(pb + 2 * S1);
-          if constexpr (SKIP) {
-            // all cross terms, then one ballot for the wave's row tiles: in
-            // spatially ordered data a tile pair is negligible for nearly all
-            // of a wave's 64 consecutive rows or for none
-            f4 da[QT], db[QT];
-#pragma unroll
-            for (int R = 0; R < QT; ++R) {
-              da[R] = lp_cross<PREC>(Aa, B1[R], B2[R], B3[R], B4[R]);
-              db[R] = lp_cross<PREC>(Ab, B1[R], B2[R], B3[R], B4[R]);
-            }
-            float mx = fmaxf(max4(da[0]), max4(db[0]));
-#pragma unroll
-            for (int R = 1; R < QT; ++R) mx = fmaxf(mx, fmaxf(max4(da[R]), max4(db[R])));
-            if (__builtin_amdgcn_ballot_w64(mx > skip_t)) {
-#pragma unroll
-              for (int R = 0; R < QT; ++R) {
-                float ea[4], eb[4];
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                  ea[u] = fexp2_le1(da[R][u]);
-                  eb[u] = fexp2_le1(db[R][u]);
-                }
-                const unsigned a0 = pack2<PX>(ea[0], ea[1]), a1 = pack2<PX>(ea[2], ea[3]);
-                const unsigned b0 = pack2<PX>(eb[0], eb[1]), b1 = pack2<PX>(eb[2], eb[3]);
-                pdf[R] = dot2_ones<PX>(a0, pdf[R]);
-                pdf[R] = dot2_ones<PX>(a1, pdf[R]);
-                pdf[R] = dot2_ones<PX>(b0, pdf[R]);
-                pdf[R] = dot2_ones<PX>(b1, pdf[R]);
-                w[R] = mfma16(words(a0, a1, b0, b1), BX, w[R], PX);
-              }
-            }
-          } else {
-#pragma unroll
-            for (int R = 0; R < QT; ++R) {
-              const f4 da = lp_cross<PREC>(Aa, B1[R], B2[R], B3[R], B4[R]);
-              const f4 db = lp_cross<PREC>(Ab, B1[R], B2[R], B3[R], B4[R]);
-              float ea[4], eb[4];
-#pragma unroll
-              for (int u = 0; u < 4; ++u) {
-                ea[u] = fexp2_le1(da[u]);
-                eb[u] = fexp2_le1(db[u]);
-              }
-              const unsigned a0 = pack2<PX>(ea[0], ea[1]), a1 = pack2<PX>(ea[2], ea[3]);
-              const unsigned b0 = pack2<PX>(eb[0], eb[1]), b1 = pack2<PX>(eb[2], eb[3]);
-              pdf[R] = dot2_ones<PX>(a0, pdf[R]);
-              pdf[R] = dot2_ones<PX>(a1, pdf[R]);
-              pdf[R] = dot2_ones<PX>(b0, pdf[R]);
-              pdf[R] = dot2_ones<PX>(b1, pdf[R]);
-              w[R] = mfma16(words(a0, a1, b0, b1), BX, w[R], PX);
-            }
-            // Issue order of one pair (3 LDS reads, 12 MFMAs, 32 exps, 32 other
-            // VALU ops per wave): the compiler's own order clusters the MFMAs and
-            // the exps; interleaving 3 exps and 2 VALU ops per MFMA keeps the
-            // matrix core busy under the transcendental work (FP16 score at
-            // n = 2^20: 125 -> 119 ms; other mixes measured 119-123 ms).
-            __builtin_amdgcn_sched_group_barrier(0x100, 3, 0);  // LDS reads
-            __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);  // MFMA
-#pragma unroll
-            for (int gi = 0; gi < 10; ++gi) {
-              __builtin_amdgcn_sched_group_barrier(0x400, 3, 0);  // transcendental
-              __builtin_amdgcn_sched_group_barrier(0x002, 2, 0);  // VALU
-              __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
-            }
-            __builtin_amdgcn_sched_group_barrier(0x002, 64, 0);
-          }
-        } else {
-          // split and one-phi forms: tile a, then tile b (one tile's operands
-          // live at a time); the split forms keep phi_h of tile a per row tile
-          // for the shared [phi_h,a | phi_h,b] . [Xl_a | Xl_b] product
-          unsigned ha[QT][2];
-#pragma unroll
-          for (int tt = 0; tt < 2; ++tt) {
-            const unsigned short *tb = pb + tt * KC * S1;
-            u16x8 A[KC];
-#pragma unroll
-            for (int s = 0; s < KC; ++s) A[s] = lds_u16x8(tb + s * S1);
-            const u16x8 BX = lds_u16x8(pb + (2 * KC + tt) * S1);
-            const u16x8 BXL = SPLIT_PHI && tt == 1 ? lds_u16x8(pb + (2 * KC + 2) * S1) : BX;
-            if constexpr (SKIP) {
-              // one ballot per tile for the wave's row tiles, as above
-              f4 d[QT];
-#pragma unroll
-              for (int R = 0; R < QT; ++R) d[R] = lp_cross<PREC>(A, B1[R], B2[R], B3[R], B4[R]);
-              float mx = max4(d[0]);
-#pragma unroll
-              for (int R = 1; R < QT; ++R) mx = fmaxf(mx, max4(d[R]));
-              if (__builtin_amdgcn_ballot_w64(mx > skip_t)) {
-#pragma unroll
-                for (int R = 0; R < QT; ++R) {
-                  float e[4];
-#pragma unroll
-                  for (int u = 0; u < 4; ++u) {
-                    e[u] = fexp2_le1(d[R][u]);
-                    pdf[R] += e[u];
-                  }
-                  const unsigned p0 = pack2<PX>(e[0], e[1]), p1 = pack2<PX>(e[2], e[3]);
-                  w[R] = mfma16(words(p0, p1, p0, p1), BX, w[R], PX);
-                }
-              }
-              continue;
-            }
-#pragma unroll
-            for (int R = 0; R < QT; ++R) {
-              const f4 d = lp_cross<PREC>(A, B1[R], B2[R], B3[R], B4[R]);
-              float e[4];
-#pragma unroll
-              for (int u = 0; u < 4; ++u) {
-                e[u] = fexp2_le1(d[u]);
-                pdf[R] += e[u];
-              }
-              if constexpr (SPLIT_PHI) {
-                unsigned hi[2], lo[2];
-                split_phi(e, hi, lo);
-                w[R] = mfma16(words(hi[0], hi[1], lo[0], lo[1]), BX, w[R], PX);
-                if (tt == 0) {
-                  ha[R][0] = hi[0];
-                  ha[R][1] = hi[1];
-                } else {
-                  w[R] = mfma16(words(ha[R][0], ha[R][1], hi[0], hi[1]), BXL, w[R], PX);
-                }
-              } else {
-                const unsigned p0 = pack2<PX>(e[0], e[1]), p1 = pack2<PX>(e[2], e[3]);
-                w[R] = mfma16(words(p0, p1, p0, p1), BX, w[R], PX);
-              }
-            }
-          }
-        }
-      }
+      pair_body(cb + t * BL::kPairU16);
       if (++tc == FLUSH) {
         tc = 0;
         flush();
       }
+    }
+  };
+  // the pairs of chunk ch this wave needs (SKIP instances): the bitmap's far
+  // pairs add exact zeros to phi.X and at most n 2^skip_t to a row sum, so
+  // the walk visits only the set bits of the near mask. The FP32 sums are
+  // flushed before the first pair past each flush point, which keeps them
+  // bitwise those of the dense order (nothing is added in between).
+  int64_t next_flush = FLUSH;
+  auto run_chunk_near = [&](int ch, int buf) {
+    const unsigned short *cb = lds + buf * BL::kChunkU16 + lane * 8;
+    const int np = npair - ch * CH < CH ? npair - ch * CH : CH;
+    const int64_t q0 = int64_t(ch) * CH;  // pair index within the task
+    unsigned long long near = np == 64 ? ~0ull : (1ull << np) - 1;
+    if (far_row) near &= ~bits_at(far_row, p_begin + q0, np, widx, fword);
+    while (near) {
+      const int t = __builtin_ctzll(near);
+      near &= near - 1;
+      const int64_t q = q0 + t;
+      if (q >= next_flush) {
+        flush();
+        next_flush = (q / FLUSH + 1) * FLUSH;
+      }
+      pair_body(cb + t * BL::kPairU16);
     }
   };
   if constexpr (SKIP) {
@@ -733,8 +746,33 @@ void score16_lp_kernel(
     // buffer b holds (nchunk: none).
     int scan = 0;
     auto next_needed = [&]() -> int {
-      while (scan < nchunk && !needed(scan)) ++scan;
-      return scan < nchunk ? scan++ : nchunk;
+      if (!wg_row) return scan < nchunk ? scan++ : nchunk;
+      // A chunk is needed unless every pair of it is far for every wave of
+      // the workgroup; every wave reads the same words, so all agree, and
+      // chunks nobody needs are neither copied nor waited for. The first
+      // pair at or after chunk `scan` that some wave needs is found a 64-pair
+      // word at a time.
+      int64_t P = p_begin + int64_t(scan) * CH;
+      while (P < p_end) {
+        const int64_t w = P >> 6;
+        if (w != gidx) {
+          gidx = w;
+          gword = sload_u64(wg_row + w);
+        }
+        const unsigned long long near = ~gword >> (P & 63);
+        if (near) {
+          P += __builtin_ctzll(near);
+          break;
+        }
+        P = (w + 1) << 6;
+      }
+      if (P >= p_end) {
+        scan = nchunk;
+        return nchunk;
+      }
+      const int c = int((P - p_begin) / CH);
+      scan = c + 1;
+      return c;
     };
     int ring[NB];
 #pragma unroll
@@ -744,19 +782,9 @@ void score16_lp_kernel(
       ring[b] = next_needed();
       if (ring[b] < nchunk) stage(ring[b], b);
     }
-    int last = -1;
     for (int k = 0;; ++k) {
       const int ch = ring[k % NB];
       if (ch >= nchunk) break;
-      // pairs of the chunks skipped since the last one (all whole chunks) add
-      // nothing; flushing once if a flush point lies among them keeps the
-      // FP64 sums bitwise those of the dense order
-      const int skipped = (ch - last - 1) * CH;
-      if (skipped > 0) {
-        if (tc + skipped >= FLUSH) flush();
-        tc = (tc + skipped) % FLUSH;
-      }
-      last = ch;
       // This wave's copies of chunk ch have landed once at most those of the
       // chunks staged after it are outstanding (loads retire in order); the
       // barrier makes every wave's copies visible, and every wave has finished
@@ -775,7 +803,7 @@ void score16_lp_kernel(
         ring[bn] = next_needed();
         if (ring[bn] < nchunk) stage(ring[bn], bn);
       }
-      run_chunk(ch, k % NB);
+      run_chunk_near(ch, k % NB);
     }
   } else {
     // every chunk in order, copied NB - 1 ahead
