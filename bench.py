@@ -426,17 +426,6 @@ def score_variant(prec):
     return "symmetric" if prec == "fp32" else "ordered"
 
 
-def form_score(xd, h, prec, dense=False):
-    """Score accumulators of a form: the FP32 symmetric kernel, else
-    precision.emp_score (the forms with FP16 phi run on the data in spatial
-    order and leave out negligible tile pairs unless dense)."""
-    from flash_sd_kde import _ops, precision
-
-    if score_variant(prec) == "symmetric":
-        return _ops.score16(xd, h, variant="symmetric", precision=prec)
-    return precision.emp_score(xd, h, precision=prec, dense=dense, device=xd.device)
-
-
 def negligible_tile_frac(xd, h, t_max, samples=96):
     """Sampled share of the (16-row tile, 32-point tile pair) blocks of the
     data in spatial order whose every exponent -|x_i - x_j|^2 log2(e)/(2h^2)
@@ -484,9 +473,8 @@ def precision_forms(xd, qd, h, fp32_ms, fp32_step, steps, warmup, n_err, rates=N
         log_norm = -math.log(n) - 16 * math.log(hh) - 8 * math.log(2 * math.pi)
         res = {"fp32": torch.log(fp32_step(hh)[:n_err].double()) + log_norm}
         for name, sp, ep in PRECISION_FORMS:
-            pdf, w = form_score(xd, hh, sp)
-            deb = _ops.debias(xd, pdf, w, hh, 1e-12)
-            res[name] = _ops.kde16_logdensity(deb, sub, hh, precision=ep)
+            res[name] = precision.sdkde_logdensity(xd, sub, hh, precision=ep, score_precision=sp,
+                                                   device=xd.device)
         return res
 
     errs = {}
@@ -515,9 +503,8 @@ def precision_forms(xd, qd, h, fp32_ms, fp32_step, steps, warmup, n_err, rates=N
     dense_ms, tile_frac = {}, {}
     for name, sp, ep in PRECISION_FORMS:
         def step():
-            pdf, w = form_score(xd, h, sp)
-            deb = _ops.debias(xd, pdf, w, h, 1e-12)
-            return _ops.kde16_logdensity(deb, qd, h, precision=ep)
+            return precision.sdkde_logdensity(xd, qd, h, precision=ep, score_precision=sp,
+                                              device=xd.device)
         for _ in range(warmup):
             step()
         torch.cuda.synchronize()
@@ -535,21 +522,25 @@ def precision_forms(xd, qd, h, fp32_ms, fp32_step, steps, warmup, n_err, rates=N
                      "ms_per_step": round(ms, 3), "gpair_evals_per_s": round(pairs / ms / 1e6, 3),
                      "speedup_vs_fp32": round(fp32_ms / ms, 3), "score_kernel_ms": round(score_ms, 3),
                      "kde_kernel_ms": round(kde_ms, 3), **err_fields(name)}
-        if sp in precision.SKIP_PRECISIONS:
-            # the same kernel evaluating every tile pair (FSK_FLAG_DENSE), and
-            # the sampled share of tile pairs below the kernel's threshold
-            if sp not in dense_ms:
+        if sp in precision.SKIP_PRECISIONS or ep in precision.LSE_SKIP_PRECISIONS:
+            # the same kernels evaluating every term (FSK_FLAG_DENSE)
+            if (sp, ep) not in dense_ms:
                 lib.fsk_profile_enable(1)
-                form_score(xd, h, sp, dense=True)
+                precision.sdkde_logdensity(xd, qd, h, precision=ep, score_precision=sp, dense=True,
+                                           device=xd.device)
                 torch.cuda.synchronize()
-                dense_ms[sp] = lib.fsk_profile_last_ms(_native.OP_SCORE16_ORDERED)
+                dense_ms[(sp, ep)] = (lib.fsk_profile_last_ms(_native.OP_SCORE16_ORDERED),
+                                      lib.fsk_profile_last_ms(_native.OP_KDE16_LOGDENSITY))
                 lib.fsk_profile_enable(0)
+            out[name]["kde_kernel_dense_ms"] = round(dense_ms[(sp, ep)][1], 3)
+        if sp in precision.SKIP_PRECISIONS:
+            # the sampled share of tile pairs below the score kernel's threshold
             t_skip = -27.0 if sp == "fp16" else -(math.ceil(math.log2(n)) + 29.0)
-            out[name]["score_kernel_dense_ms"] = round(dense_ms[sp], 3)
+            out[name]["score_kernel_dense_ms"] = round(dense_ms[(sp, ep)][0], 3)
             if t_skip not in tile_frac:
                 tile_frac[t_skip] = negligible_tile_frac(xd, h, t_skip)
             out[name]["negligible_tile_frac_sampled"] = round(tile_frac[t_skip], 4)
-            score_ms = dense_ms[sp]  # the issue fractions below are the dense kernel's
+            score_ms = dense_ms[(sp, ep)][0]  # the issue fractions below are the dense kernel's
         if (rates and rates.get("exp_per_s") and rates.get("fma_per_s")
                 and rates.get("bf16_mfma_flops") and score_ms > 0 and sp in SCORE_ISSUE_MIX):
             # issue time of the score kernel at the rates measured in this run,

@@ -1,0 +1,171 @@
+// fsk_far.hip - far-block bitmap pre-pass (frame, boxes, bits); the bound
+// and the layout are described in fsk_far.h.
+#include "fsk_far.h"
+
+namespace fsk {
+
+// One workgroup of 256: the covariance of a strided sample of the centred
+// data in FP64 (fixed order), then 40 rounds of block power iteration with
+// modified Gram-Schmidt for its three leading eigenvectors. frame[a * 16 + f];
+// a direction that vanishes (rank-deficient data) is left zero, which keeps
+// the rows orthonormal or zero.
+__global__ __launch_bounds__(256) void frame_kernel(const float *__restrict__ x, int64_t ld,
+                                                    int64_t n, const double *__restrict__ mu_sum,
+                                                    double inv_n, float *__restrict__ frame) {
+  __shared__ float pts[256 * kD];
+  __shared__ double cov[kD * kD];
+  __shared__ double q[kBoxAxes][kD], z[kBoxAxes][kD];  // iterates (thread 0)
+  const int t = threadIdx.x;
+  const int64_t cnt = n < kFrameSample ? n : kFrameSample;
+  const int64_t stride = n / cnt;
+  const int ei = t < kD * kD ? t / kD : 0, ej = t < kD * kD ? t % kD : 0;
+  double acc = 0.0;
+  for (int64_t c0 = 0; c0 < cnt; c0 += 256) {
+    __syncthreads();
+    for (int e = t; e < 256 * kD; e += 256) {
+      const int64_t k = c0 + e / kD;
+      const int f = e % kD;
+      pts[e] = k < cnt ? float(double(x[k * stride * ld + f]) - mu_sum[f] * inv_n) : 0.f;
+    }
+    __syncthreads();
+    if (t < kD * kD)
+      for (int k = 0; k < 256; ++k) acc += double(pts[k * kD + ei]) * double(pts[k * kD + ej]);
+  }
+  if (t < kD * kD) cov[t] = acc;
+  if (t < kBoxAxes * kD) {
+    const int a = t / kD, f = t % kD;
+    q[a][f] = f == a ? 1.0 : 0.01 * double((7 * f + 3 * a) % 11 + 1);
+  }
+  __syncthreads();
+  for (int it = 0; it < 40; ++it) {
+    // z = cov q, one entry per thread; then Gram-Schmidt on thread 0
+    if (t < kBoxAxes * kD) {
+      const int a = t / kD, f = t % kD;
+      double v = 0.0;
+      for (int g = 0; g < kD; ++g) v += cov[f * kD + g] * q[a][g];
+      z[a][f] = v;
+    }
+    __syncthreads();
+    if (t == 0) {
+      for (int a = 0; a < kBoxAxes; ++a) {
+        for (int b = 0; b < a; ++b) {
+          double d = 0.0;
+          for (int f = 0; f < kD; ++f) d += z[a][f] * z[b][f];
+          for (int f = 0; f < kD; ++f) z[a][f] -= d * z[b][f];
+        }
+        double nn = 0.0;
+        for (int f = 0; f < kD; ++f) nn += z[a][f] * z[a][f];
+        const double inv = nn > 1e-300 && nn < 1e300 ? 1.0 / sqrt(nn) : 0.0;
+        for (int f = 0; f < kD; ++f) q[a][f] = z[a][f] = z[a][f] * inv;
+      }
+    }
+    __syncthreads();
+  }
+  if (t < kBoxAxes * kD) frame[t] = float(q[t / kD][t % kD]);
+}
+
+// Boxes of consecutive groups of g points (tile pairs: 32; row blocks: 64):
+// the lowest and highest projection on each frame axis, the largest
+// c |x - mu|^2 (log2 units) over the group's points j < n, and the group's
+// threshold: thr, or with lower (per point) the smallest lower[j] - margin.
+// A group with no points gets an empty box (lo = +inf, hi = -inf), which
+// every bound finds far.
+__global__ void box_kernel(const float *__restrict__ x, int64_t ld, int64_t n, int64_t groups,
+                           int g, const double *__restrict__ mu_sum, double inv_n, double c,
+                           const float *__restrict__ frame, float thr,
+                           const float *__restrict__ lower, float margin,
+                           float *__restrict__ boxes) {
+  const int64_t b = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (b >= groups) return;
+  float lo[kBoxAxes], hi[kBoxAxes], nrm = 0.f, t = lower ? INFINITY : thr;
+  for (int a = 0; a < kBoxAxes; ++a) {
+    lo[a] = INFINITY;
+    hi[a] = -INFINITY;
+  }
+  for (int64_t j = b * g; j < b * g + g && j < n; ++j) {
+    float y[kD];
+    double s2 = 0.0;
+    for (int f = 0; f < kD; ++f) {
+      const double v = double(x[j * ld + f]) - mu_sum[f] * inv_n;
+      y[f] = float(v);
+      s2 += v * v;
+    }
+    for (int a = 0; a < kBoxAxes; ++a) {
+      float pa = 0.f;
+      for (int f = 0; f < kD; ++f) pa += frame[a * kD + f] * y[f];
+      lo[a] = fminf(lo[a], pa);
+      hi[a] = fmaxf(hi[a], pa);
+    }
+    nrm = fmaxf(nrm, float(c * s2));
+    if (lower) t = fminf(t, lower[j] - margin);
+  }
+  float *o = boxes + b * kBoxFloats;
+  for (int a = 0; a < kBoxAxes; ++a) {
+    o[a] = lo[a];
+    o[kBoxAxes + a] = hi[a];
+  }
+  o[6] = nrm;
+  o[7] = t;
+}
+
+// bits[rb * words + p / 64] bit p % 64 = 1 when every exponent of row block
+// rb against tile pair p is provably <= the row block's threshold (or p has
+// no points), for the
+// `waves` row blocks of workgroup g; row groups * waves + g holds their AND
+// (far for every wave of the workgroup). One thread per (g, p); a wave
+// covers 64 consecutive p of one g.
+__global__ void far_bits_kernel(const float *__restrict__ row_boxes,
+                                const float *__restrict__ pair_boxes, int64_t npairs,
+                                int64_t words, int64_t groups, int waves, float c,
+                                unsigned long long *__restrict__ bits) {
+  const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= groups * words * 64) return;  // a multiple of 64: whole waves exit
+  // g is the same on the whole wave: scalar loads of the row boxes
+  const int64_t g = __builtin_amdgcn_readfirstlane(int(e / (words * 64)));
+  const int64_t p = e % (words * 64);
+  float plo[kBoxAxes], phi[kBoxAxes], pn = 0.f;
+  const bool real = p < npairs;
+  for (int a = 0; a < kBoxAxes; ++a) {
+    plo[a] = real ? pair_boxes[p * kBoxFloats + a] : 0.f;
+    phi[a] = real ? pair_boxes[p * kBoxFloats + kBoxAxes + a] : 0.f;
+  }
+  if (real) pn = pair_boxes[p * kBoxFloats + 6];
+  unsigned long long all = ~0ull;
+  for (int k = 0; k < waves; ++k) {
+    const float *r = row_boxes + (g * waves + k) * kBoxFloats;
+    bool far = true;
+    if (real) {
+      float d2 = 0.f;
+      for (int a = 0; a < kBoxAxes; ++a) {
+        const float gap = fmaxf(fmaxf(plo[a] - r[kBoxAxes + a], r[a] - phi[a]), 0.f);
+        d2 += gap * gap;
+      }
+      const float bound = -c * d2 + 0x1p-10f * (r[6] + pn) + 1.f;
+      far = !(bound > r[7]);
+    }
+    const unsigned long long m = __builtin_amdgcn_ballot_w64(far);
+    all &= m;
+    if ((threadIdx.x & 63) == 0) bits[(g * waves + k) * words + p / 64] = m;
+  }
+  if ((threadIdx.x & 63) == 0) bits[(groups * waves + g) * words + p / 64] = all;
+}
+
+void launch_far_bits(const float *data, int64_t ld, int64_t n, const float *rows, int64_t ldr,
+                     int64_t m, const double *mu_sum, double inv_n, double c, int64_t npairs,
+                     int64_t groups, int waves, float skip_t, const float *row_lower,
+                     float margin, float *frame, float *pair_boxes, float *row_boxes,
+                     unsigned long long *bits, hipStream_t s) {
+  const int64_t row_blocks = groups * waves;
+  const int64_t words = far_words(npairs);
+  hipLaunchKernelGGL(frame_kernel, dim3(1), dim3(256), 0, s, data, ld, n, mu_sum, inv_n, frame);
+  hipLaunchKernelGGL(box_kernel, dim3(cdiv(npairs, 256)), dim3(256), 0, s, data, ld, n, npairs,
+                     kFarPairPts, mu_sum, inv_n, c, frame, 0.f, (const float *)nullptr, 0.f,
+                     pair_boxes);
+  hipLaunchKernelGGL(box_kernel, dim3(cdiv(row_blocks, 256)), dim3(256), 0, s, rows, ldr, m,
+                     row_blocks, kFarRows, mu_sum, inv_n, c, frame, skip_t, row_lower, margin,
+                     row_boxes);
+  hipLaunchKernelGGL(far_bits_kernel, dim3(cdiv(groups * words * 64, 256)), dim3(256), 0, s,
+                     row_boxes, pair_boxes, npairs, words, groups, waves, float(c), bits);
+}
+
+}  // namespace fsk

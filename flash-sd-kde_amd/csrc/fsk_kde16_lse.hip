@@ -34,6 +34,7 @@
 // FP32 forms: MFMA f32 16x16x4 with the norm terms (and -m) in the
 // accumulator initialiser, or plain FP32 FMAs on the VALU.
 #include "fsk_common.h"
+#include "fsk_far.h"
 #include "fsk_plan.h"
 #include "fsk_lowp.h"
 
@@ -102,20 +103,34 @@ __device__ __forceinline__ float gmax(float v) {
 template <int PREC>
 constexpr int lse_min_waves() { return PREC == FSK_PREC_BF16X6 ? 3 : 4; }
 
-template <int PREC, int QT>
+// SKIP instances (16-bit forms) take per-query lower bounds `lower` of
+// log2 sum_j exp2(t_ij) (from a pass over a sample of the data) and leave out
+// terms below lower - margin, margin = ceil(log2 n) + 28: a query loses at
+// most n 2^(lower - margin) <= 2^-28 of its sum. Pairs of data tiles the
+// far-block bitmap (fsk_far.h) marks are not read; of the others, a tile
+// whose every exponent is below every threshold of the wave's rows adds
+// nothing. Waves whose rows all have lower > direct_min (their largest term
+// is then above 2^-100) run the unshifted sum; the others run the dense
+// loops below.
+template <int PREC, int QT, bool SKIP>
 __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(lse_min_waves<PREC>())))
 void kde16_lse_mfma_kernel(
     const float *__restrict__ xf, const float *__restrict__ bn,
     const unsigned short *__restrict__ xrec, int64_t n_pad, const float *__restrict__ qf,
     const float *__restrict__ qa, const unsigned short *__restrict__ qrec, int64_t m,
     int64_t split_len, int64_t qblocks, int64_t ntasks, int64_t per_xcd,
+    const float *__restrict__ lower, float margin, float direct_min,
+    const unsigned long long *__restrict__ far_bits, int64_t far_words,
     float2 *__restrict__ part) {
   constexpr bool F32 = (PREC == FSK_PREC_F32);
   constexpr bool X3 = (PREC == FSK_PREC_BF16X3 || PREC == FSK_PREC_F16X3);
   constexpr bool X6 = (PREC == FSK_PREC_BF16X6);
   constexpr int P16 = CrossType<PREC>::kP16;
   using RL = RecLayout<PREC>;
-  const int64_t task = xcd_task(blockIdx.x, per_xcd);
+  static_assert(!SKIP || !F32, "the 16-bit forms leave out terms");
+  // SKIP: task = workgroup index, so the round-robin dispatch spreads the
+  // row blocks (whose work now depends on their neighbourhood) over the XCDs
+  const int64_t task = SKIP ? int64_t(blockIdx.x) : xcd_task(blockIdx.x, per_xcd);
   if (task >= ntasks) return;
   const int64_t split = task / qblocks;
   const int64_t qb = task - split * qblocks;
@@ -233,6 +248,69 @@ void kde16_lse_mfma_kernel(
   if (j_end > n_pad) j_end = n_pad;
 
   f4 d[QT];
+  int tc = 0;
+  auto flush = [&]() {
+#pragma unroll
+    for (int R = 0; R < QT; ++R) {
+      s64[R] += s[R];
+      s[R] = 0.f;
+    }
+  };
+  if constexpr (SKIP) {
+    float thr[QT];
+    bool direct = true;
+#pragma unroll
+    for (int R = 0; R < QT; ++R) {
+      int64_t i = row0 + 16 * R + c;
+      if (i > m - 1) i = m - 1;
+      const float lo = lower[i];
+      thr[R] = lo - margin;
+      direct = direct && lo > direct_min;
+    }
+    if (__all(direct)) {
+      // unshifted sums over the pairs of data tiles the bitmap leaves in,
+      // 64 pairs per word (p: pair index, tiles 2p and 2p + 1)
+      const unsigned long long *far_row =
+          far_bits ? far_bits + (qb * (kWG / 64) + wave) * far_words : nullptr;
+      const int64_t p0 = j_begin / (2 * kTile), p1 = (j_end + 2 * kTile - 1) / (2 * kTile);
+      for (int64_t w = p0 >> 6; w <= (p1 - 1) >> 6; ++w) {
+        unsigned long long near = far_row ? ~sload_u64(far_row + w) : ~0ull;
+        if (w == (p0 >> 6)) near &= ~0ull << (p0 & 63);
+        if (w == ((p1 - 1) >> 6) && (p1 & 63)) near &= ~0ull >> (64 - (p1 & 63));
+        while (near) {
+          const int64_t p = (w << 6) + __builtin_ctzll(near);
+          near &= near - 1;
+#pragma unroll
+          for (int tt = 0; tt < 2; ++tt) {
+            const int64_t j0 = p * (2 * kTile) + tt * kTile;
+            if (j0 < j_begin || j0 >= j_end) continue;
+            tile(j0, d);
+            bool hot = false;
+#pragma unroll
+            for (int R = 0; R < QT; ++R) hot = hot || max4(d[R]) > thr[R];
+            if (__any(hot)) {
+#pragma unroll
+              for (int R = 0; R < QT; ++R) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) s[R] += fexp2_le1(d[R][r]);
+              }
+            }
+            if (++tc == kLseFlush) {
+              tc = 0;
+              flush();
+            }
+          }
+        }
+      }
+#pragma unroll
+      for (int R = 0; R < QT; ++R) {
+        double v = sum_groups(s64[R] + s[R]);
+        int64_t i = row0 + 16 * R + c;
+        if (g == 0 && i < m) part[split * m + i] = make_float2(0.f, float(v));
+      }
+      return;
+    }
+  }
   // First tile: exact per-query maximum of this split's first 16 points; a
   // query whose maximum is above -kLseDirect keeps m = 0.
   tile(j_begin, d);
@@ -247,7 +325,7 @@ void kde16_lse_mfma_kernel(
 #pragma unroll
     for (int r = 0; r < 4; ++r) s[R] += fexp2(fminf(d[R][r] - dm, negm[R]));
   }
-  int tc = 1;
+  tc = 1;
   if (__all(direct)) {
     // every query of the wave at m = 0: per pair one clamped exp (exp2 of
     // min(t, 0) as v_exp ... clamp) and one add; no maximum to track
@@ -260,11 +338,7 @@ void kde16_lse_mfma_kernel(
       }
       if (++tc == kLseFlush) {
         tc = 0;
-#pragma unroll
-        for (int R = 0; R < QT; ++R) {
-          s64[R] += s[R];
-          s[R] = 0.f;
-        }
+        flush();
       }
     }
   } else {
@@ -298,11 +372,7 @@ void kde16_lse_mfma_kernel(
       }
       if (++tc == kLseFlush) {
         tc = 0;
-#pragma unroll
-        for (int R = 0; R < QT; ++R) {
-          s64[R] += s[R];
-          s[R] = 0.f;
-        }
+        flush();
       }
     }
   }
@@ -390,11 +460,47 @@ __global__ void lse_combine_kernel(const float2 *__restrict__ part, int64_t S, i
   out[i] = (acc > 0.0) ? kLn2 * (M + log2(acc)) + log_const : -INFINITY;
 }
 
+// log2 of each query's sum from split partials (m_s, s_s): the lower bounds
+// of the SKIP instances when the partials come from a sample of the data
+// (-inf where no term survived).
+__global__ void lse_lower_kernel(const float2 *__restrict__ part, int64_t S, int64_t m,
+                                 float *__restrict__ lower) {
+  int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  double M = -INFINITY;
+  for (int64_t s = 0; s < S; ++s) {
+    float2 p = part[s * m + i];
+    if (p.y > 0.f && double(p.x) > M) M = p.x;
+  }
+  double acc = 0.0;
+  for (int64_t s = 0; s < S; ++s) {
+    float2 p = part[s * m + i];
+    if (p.y > 0.f) acc += double(p.y) * exp2(double(p.x) - M);
+  }
+  lower[i] = acc > 0.0 ? float(M + log2(acc)) : -INFINITY;
+}
+
+// Margin of the SKIP instances below a query's lower bound (log2 units):
+// n terms below 2^-(ceil(log2 n) + 28) of the sum add at most 2^-28 of it.
+inline double lse_margin(int64_t n) { return std::ceil(std::log2(double(n))) + 28.0; }
+
+// The 16-bit log-densities leave out negligible terms (SKIP instances) from
+// this many data points on; the lower bounds come from every k-th point with
+// k = n / kLseSample.
+constexpr int64_t kLseSkipMinN = 16384;
+constexpr int64_t kLseSample = 16384;
+
 struct LseLayout {
   double *mu;
   float *xf, *bn, *qf, *qa;
   unsigned short *xrec, *qrec;
   float2 *part;
+  // SKIP: sample records and partials, lower bounds, far-block bitmap
+  unsigned short *srec;
+  float2 *spart;
+  float *lower, *frame, *pair_boxes, *row_boxes;
+  unsigned long long *far_bits;
+  int64_t stride, ns, npairs;
 };
 
 static bool lse_is16(unsigned prec) {
@@ -421,6 +527,25 @@ static size_t lse_layout(void *ws, int64_t n, int64_t m, unsigned flags, LseLayo
     L->qa = cv.take<float>(m);
   }
   L->part = cv.take<float2>(p.splits * m);
+  L->srec = nullptr;
+  L->spart = nullptr;
+  L->lower = L->frame = L->pair_boxes = L->row_boxes = nullptr;
+  L->far_bits = nullptr;
+  L->stride = L->ns = L->npairs = 0;
+  if (lse_is16(prec) && n >= kLseSkipMinN) {
+    L->stride = n / kLseSample > 1 ? n / kLseSample : 1;
+    L->ns = cdiv(n, L->stride);
+    L->srec = cv.take<unsigned short>(round_up(L->ns, kValuChunk) * rec_size(prec));
+    L->spart = cv.take<float2>(kde16_plan(L->ns, m).splits * m);
+    L->lower = cv.take<float>(m);
+    L->npairs = cdiv(n, kFarPairPts);
+    if (far_bits_count(p.qblocks, kWG / 64, L->npairs) * 8 <= kFarBitsCap) {
+      L->frame = cv.take<float>(kBoxAxes * kD);
+      L->pair_boxes = cv.take<float>(L->npairs * kBoxFloats);
+      L->row_boxes = cv.take<float>(p.qblocks * (kWG / 64) * kBoxFloats);
+      L->far_bits = cv.take<unsigned long long>(far_bits_count(p.qblocks, kWG / 64, L->npairs));
+    }
+  }
   return cv.used;
 }
 
@@ -450,7 +575,7 @@ extern "C" int fsk_kde16_logdensity(const float *data, int64_t ld, int64_t n,
     return FSK_EINVAL;
   }
   const unsigned prec = flags & FSK_PREC_MASK;
-  if (prec > FSK_PREC_BF16X3S || (flags & ~FSK_PREC_MASK)) {
+  if (prec > FSK_PREC_BF16X3S || (flags & ~(FSK_PREC_MASK | FSK_FLAG_DENSE))) {
     set_error("%s: flags 0x%x not available (Laplacian has no log form)", what, flags);
     return FSK_EUNSUPPORTED;
   }
@@ -498,12 +623,62 @@ extern "C" int fsk_kde16_logdensity(const float *data, int64_t ld, int64_t n,
   } else {
     const int64_t ntasks = p.qblocks * p.splits;
     const int64_t grid = xcd_grid(ntasks, &per_xcd);
-#define FSK_LSE(P)                                                                         \
-  hipLaunchKernelGGL((kde16_lse_mfma_kernel<P, kKdeQT>), dim3(grid), dim3(kWG), 0, s, L.xf, \
-                     L.bn, L.xrec, round_up(n, kTile), L.qf, L.qa, L.qrec, m, p.split_len,  \
-                     p.qblocks, ntasks, per_xcd, L.part)
+    const bool skip = L.lower && !(flags & FSK_FLAG_DENSE);
+    if (skip) {
+      // lower bounds: the sums over every stride-th point (a subset of the
+      // data, so below each full sum), from the dense kernel on its records
+      const int64_t ns_pad = round_up(L.ns, kValuChunk);
+      SplitPlan ps = kde16_plan(L.ns, m);
+      int64_t per_xcd_s;
+      const int64_t ntasks_s = ps.qblocks * ps.splits;
+      const int64_t grid_s = xcd_grid(ntasks_s, &per_xcd_s);
+#define FSK_SAMPLE(P)                                                                            \
+  do {                                                                                           \
+    hipLaunchKernelGGL(prep_lse_kernel<P>, dim3(cdiv(ns_pad, 256)), dim3(256), 0, s, data,       \
+                       ld * L.stride, L.ns, ns_pad, L.mu, inv_n, scl, coef_s, 0, (float *)nullptr, \
+                       (float *)nullptr, L.srec);                                                \
+    hipLaunchKernelGGL((kde16_lse_mfma_kernel<P, kKdeQT, false>), dim3(grid_s), dim3(kWG), 0, s, \
+                       L.xf, L.bn, L.srec, round_up(L.ns, kTile), L.qf, L.qa, L.qrec, m,         \
+                       ps.split_len, ps.qblocks, ntasks_s, per_xcd_s, (const float *)nullptr,    \
+                       0.f, 0.f, (const unsigned long long *)nullptr, int64_t(0), L.spart);      \
+  } while (0)
+      switch (prec) {
+        case FSK_PREC_BF16: FSK_SAMPLE(FSK_PREC_BF16); break;
+        case FSK_PREC_F16: FSK_SAMPLE(FSK_PREC_F16); break;
+        case FSK_PREC_BF16X6: FSK_SAMPLE(FSK_PREC_BF16X6); break;
+        case FSK_PREC_F16X3:
+        case FSK_PREC_F16X3S: FSK_SAMPLE(FSK_PREC_F16X3); break;
+        default: FSK_SAMPLE(FSK_PREC_BF16X3); break;
+      }
+#undef FSK_SAMPLE
+      hipLaunchKernelGGL(lse_lower_kernel, dim3(cdiv(m, 256)), dim3(256), 0, s, L.spart,
+                         ps.splits, m, L.lower);
+      if (L.far_bits)
+        launch_far_bits(data, ld, n, query, ldq, m, L.mu, inv_n, -coef, L.npairs, p.qblocks,
+                        kWG / 64, 0.f, L.lower, float(lse_margin(n)), L.frame, L.pair_boxes,
+                        L.row_boxes, L.far_bits, s);
+    }
+    const float direct_min = float(-kLseDirect + std::log2(double(L.ns > 0 ? L.ns : 1)));
+#define FSK_LSE(P)                                                                                \
+  do {                                                                                            \
+    if (skip)                                                                                     \
+      hipLaunchKernelGGL((kde16_lse_mfma_kernel<P, kKdeQT, true>), dim3(grid), dim3(kWG), 0, s,   \
+                         L.xf, L.bn, L.xrec, round_up(n, kTile), L.qf, L.qa, L.qrec, m,           \
+                         p.split_len, p.qblocks, ntasks, per_xcd, L.lower, float(lse_margin(n)),  \
+                         direct_min, L.far_bits, far_words(L.npairs), L.part);                    \
+    else                                                                                          \
+      hipLaunchKernelGGL((kde16_lse_mfma_kernel<P, kKdeQT, false>), dim3(grid), dim3(kWG), 0, s,  \
+                         L.xf, L.bn, L.xrec, round_up(n, kTile), L.qf, L.qa, L.qrec, m,           \
+                         p.split_len, p.qblocks, ntasks, per_xcd, (const float *)nullptr, 0.f,    \
+                         0.f, (const unsigned long long *)nullptr, int64_t(0), L.part);           \
+  } while (0)
+#define FSK_LSE32(P)                                                                             \
+  hipLaunchKernelGGL((kde16_lse_mfma_kernel<P, kKdeQT, false>), dim3(grid), dim3(kWG), 0, s,     \
+                     L.xf, L.bn, L.xrec, round_up(n, kTile), L.qf, L.qa, L.qrec, m, p.split_len, \
+                     p.qblocks, ntasks, per_xcd, (const float *)nullptr, 0.f, 0.f,               \
+                     (const unsigned long long *)nullptr, int64_t(0), L.part)
     switch (prec) {
-      case FSK_PREC_F32: FSK_LSE(FSK_PREC_F32); break;
+      case FSK_PREC_F32: FSK_LSE32(FSK_PREC_F32); break;
       case FSK_PREC_BF16: FSK_LSE(FSK_PREC_BF16); break;
       case FSK_PREC_F16: FSK_LSE(FSK_PREC_F16); break;
       case FSK_PREC_BF16X6: FSK_LSE(FSK_PREC_BF16X6); break;
@@ -512,6 +687,7 @@ extern "C" int fsk_kde16_logdensity(const float *data, int64_t ld, int64_t n,
       default: FSK_LSE(FSK_PREC_BF16X3); break;
     }
 #undef FSK_LSE
+#undef FSK_LSE32
   }
   prof_end(FSK_OP_KDE16_LOGDENSITY, s);
   if ((st = launch_status(what)) != FSK_OK) return st;

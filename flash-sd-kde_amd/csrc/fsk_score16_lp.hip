@@ -63,18 +63,12 @@
 // In data put in spatial order, most tile pairs of a large set are far apart
 // and skip.
 //
-// Far pairs before any MFMA. A pre-pass bounds every (wave's 64 rows, tile
-// pair) block from below: the data's three leading principal axes (block
-// power iteration on the covariance of a 4096-point sample) span a frame P
-// with orthonormal rows, so |q - x| >= |P q - P x| >= the gap D between the
-// blocks' boxes along those axes, and every exponent of the block is at most
-//   -c D^2 + 2^-10 (max a + max b) + 1,
-// where a = c |q - mu|^2, b = c |x - mu|^2 and 2^-10 (a + b) bounds the
-// cross term's error from 16-bit coordinates (2^-11 relative each). Blocks
-// whose bound is <= skip_t get a bit in a bitmap; the kernel leaves them out
-// without reading their operands. The bound holds for any data: the frame
-// only decides how many blocks it proves far.
+// Far pairs before any MFMA: a bitmap from box bounds along the data's
+// principal axes (fsk_far.h) marks the (wave's 64 rows, tile pair) blocks
+// whose every exponent is provably <= skip_t; the kernel leaves them out
+// without reading their operands.
 #include "fsk_common.h"
+#include "fsk_far.h"
 #include "fsk_lowp.h"
 #include "fsk_plan.h"
 
@@ -270,164 +264,6 @@ __device__ __forceinline__ void wait_vmcnt(int n) {
   else if (n == 1) __builtin_amdgcn_s_waitcnt(0xF71);
   else if (n == 2) __builtin_amdgcn_s_waitcnt(0xF72);
   else __builtin_amdgcn_s_waitcnt(0xF73);
-}
-
-// ---- far-block bitmap of the forms with FP16 phi ----
-constexpr int kFrameSample = 4096;  // points of the covariance sample
-constexpr int kBoxAxes = 3;
-constexpr int kBoxFloats = 8;       // lo[3], hi[3], max c|x - mu|^2, unused
-constexpr int kFarRows = 16 * kScoreLpQT;  // rows per wave: one bitmap row
-// bitmap sizes above this run without it (the tile test alone)
-constexpr int64_t kFarBitsCap = int64_t(256) << 20;
-
-// One workgroup of 256: the covariance of a strided sample of the centred
-// data in FP64 (fixed order), then 40 rounds of block power iteration with
-// modified Gram-Schmidt for its three leading eigenvectors. frame[a * 16 + f];
-// a direction that vanishes (rank-deficient data) is left zero, which keeps
-// the rows orthonormal or zero.
-__global__ __launch_bounds__(256) void frame_kernel(const float *__restrict__ x, int64_t ld,
-                                                    int64_t n, const double *__restrict__ mu_sum,
-                                                    double inv_n, float *__restrict__ frame) {
-  __shared__ float pts[256 * kD];
-  __shared__ double cov[kD * kD];
-  __shared__ double q[kBoxAxes][kD], z[kBoxAxes][kD];  // iterates (thread 0)
-  const int t = threadIdx.x;
-  const int64_t cnt = n < kFrameSample ? n : kFrameSample;
-  const int64_t stride = n / cnt;
-  const int ei = t < kD * kD ? t / kD : 0, ej = t < kD * kD ? t % kD : 0;
-  double acc = 0.0;
-  for (int64_t c0 = 0; c0 < cnt; c0 += 256) {
-    __syncthreads();
-    for (int e = t; e < 256 * kD; e += 256) {
-      const int64_t k = c0 + e / kD;
-      const int f = e % kD;
-      pts[e] = k < cnt ? float(double(x[k * stride * ld + f]) - mu_sum[f] * inv_n) : 0.f;
-    }
-    __syncthreads();
-    if (t < kD * kD)
-      for (int k = 0; k < 256; ++k) acc += double(pts[k * kD + ei]) * double(pts[k * kD + ej]);
-  }
-  if (t < kD * kD) cov[t] = acc;
-  if (t < kBoxAxes * kD) {
-    const int a = t / kD, f = t % kD;
-    q[a][f] = f == a ? 1.0 : 0.01 * double((7 * f + 3 * a) % 11 + 1);
-  }
-  __syncthreads();
-  for (int it = 0; it < 40; ++it) {
-    // z = cov q, one entry per thread; then Gram-Schmidt on thread 0
-    if (t < kBoxAxes * kD) {
-      const int a = t / kD, f = t % kD;
-      double v = 0.0;
-      for (int g = 0; g < kD; ++g) v += cov[f * kD + g] * q[a][g];
-      z[a][f] = v;
-    }
-    __syncthreads();
-    if (t == 0) {
-      for (int a = 0; a < kBoxAxes; ++a) {
-        for (int b = 0; b < a; ++b) {
-          double d = 0.0;
-          for (int f = 0; f < kD; ++f) d += z[a][f] * z[b][f];
-          for (int f = 0; f < kD; ++f) z[a][f] -= d * z[b][f];
-        }
-        double nn = 0.0;
-        for (int f = 0; f < kD; ++f) nn += z[a][f] * z[a][f];
-        const double inv = nn > 1e-300 && nn < 1e300 ? 1.0 / sqrt(nn) : 0.0;
-        for (int f = 0; f < kD; ++f) q[a][f] = z[a][f] = z[a][f] * inv;
-      }
-    }
-    __syncthreads();
-  }
-  if (t < kBoxAxes * kD) frame[t] = float(q[t / kD][t % kD]);
-}
-
-// Boxes of consecutive groups of g points (tile pairs: 32; row blocks: 64):
-// the lowest and highest projection on each frame axis and the largest
-// c |x - mu|^2 (log2 units) over the group's points j < n. A group with no
-// points gets an empty box (lo = +inf, hi = -inf), which every bound finds far.
-__global__ void box_kernel(const float *__restrict__ x, int64_t ld, int64_t n, int64_t groups,
-                           int g, const double *__restrict__ mu_sum, double inv_n, double c,
-                           const float *__restrict__ frame, float *__restrict__ boxes) {
-  const int64_t b = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (b >= groups) return;
-  float lo[kBoxAxes], hi[kBoxAxes], nrm = 0.f;
-  for (int a = 0; a < kBoxAxes; ++a) {
-    lo[a] = INFINITY;
-    hi[a] = -INFINITY;
-  }
-  for (int64_t j = b * g; j < b * g + g && j < n; ++j) {
-    float y[kD];
-    double s2 = 0.0;
-    for (int f = 0; f < kD; ++f) {
-      const double v = double(x[j * ld + f]) - mu_sum[f] * inv_n;
-      y[f] = float(v);
-      s2 += v * v;
-    }
-    for (int a = 0; a < kBoxAxes; ++a) {
-      float pa = 0.f;
-      for (int f = 0; f < kD; ++f) pa += frame[a * kD + f] * y[f];
-      lo[a] = fminf(lo[a], pa);
-      hi[a] = fmaxf(hi[a], pa);
-    }
-    nrm = fmaxf(nrm, float(c * s2));
-  }
-  float *o = boxes + b * kBoxFloats;
-  for (int a = 0; a < kBoxAxes; ++a) {
-    o[a] = lo[a];
-    o[kBoxAxes + a] = hi[a];
-  }
-  o[6] = nrm;
-  o[7] = 0.f;
-}
-
-// bits[rb * words + p / 64] bit p % 64 = 1 when every exponent of row block
-// rb against tile pair p is provably <= skip_t (or p has no points), for the
-// `waves` row blocks of workgroup g; row groups * waves + g holds their AND
-// (far for every wave of the workgroup). One thread per (g, p); a wave
-// covers 64 consecutive p of one g.
-__global__ void far_bits_kernel(const float *__restrict__ row_boxes,
-                                const float *__restrict__ pair_boxes, int64_t npairs,
-                                int64_t words, int64_t groups, int waves, float c, float skip_t,
-                                unsigned long long *__restrict__ bits) {
-  const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (e >= groups * words * 64) return;  // a multiple of 64: whole waves exit
-  // g is the same on the whole wave: scalar loads of the row boxes
-  const int64_t g = __builtin_amdgcn_readfirstlane(int(e / (words * 64)));
-  const int64_t p = e % (words * 64);
-  float plo[kBoxAxes], phi[kBoxAxes], pn = 0.f;
-  const bool real = p < npairs;
-  for (int a = 0; a < kBoxAxes; ++a) {
-    plo[a] = real ? pair_boxes[p * kBoxFloats + a] : 0.f;
-    phi[a] = real ? pair_boxes[p * kBoxFloats + kBoxAxes + a] : 0.f;
-  }
-  if (real) pn = pair_boxes[p * kBoxFloats + 6];
-  unsigned long long all = ~0ull;
-  for (int k = 0; k < waves; ++k) {
-    const float *r = row_boxes + (g * waves + k) * kBoxFloats;
-    bool far = true;
-    if (real) {
-      float d2 = 0.f;
-      for (int a = 0; a < kBoxAxes; ++a) {
-        const float gap = fmaxf(fmaxf(plo[a] - r[kBoxAxes + a], r[a] - phi[a]), 0.f);
-        d2 += gap * gap;
-      }
-      const float bound = -c * d2 + 0x1p-10f * (r[6] + pn) + 1.f;
-      far = !(bound > skip_t);
-    }
-    const unsigned long long m = __builtin_amdgcn_ballot_w64(far);
-    all &= m;
-    if ((threadIdx.x & 63) == 0) bits[(g * waves + k) * words + p / 64] = m;
-  }
-  if ((threadIdx.x & 63) == 0) bits[(groups * waves + g) * words + p / 64] = all;
-}
-
-// One bitmap word by a scalar load that completes inside the asm block: a
-// uniform 8-byte read that stays off the vector-memory counter the chunk
-// copies are waited on with (a vector load would make the compiler wait for
-// every chunk in flight before reading it).
-__device__ __forceinline__ unsigned long long sload_u64(const unsigned long long *p) {
-  unsigned long long v;
-  __asm__ volatile("s_load_dwordx2 %0, %1, 0x0\n\ts_waitcnt lgkmcnt(0)" : "=s"(v) : "s"(p) : "memory");
-  return v;
 }
 
 // Largest of a lane's cross-term exponents.
@@ -900,8 +736,10 @@ static size_t score_lp_layout(void *ws, int64_t n, int64_t m, unsigned prec, Sco
   L->w_part = cv.take<float>(p.splits * m * kD);
   L->row_blocks = p.qblocks * score_lp_waves(prec);
   L->far_words = cdiv(L->npairs_data, 64);
+  static_assert(kFarRows == 16 * kScoreLpQT, "a bitmap row holds one wave's rows");
   const bool bits = (prec == FSK_PREC_F16 || prec == FSK_PREC_F16X3S || prec == FSK_PREC_BF16X3S) &&
-                    (L->row_blocks + p.qblocks) * L->far_words * 8 <= kFarBitsCap;
+                    far_bits_count(p.qblocks, score_lp_waves(prec), L->npairs_data) * 8 <=
+                        kFarBitsCap;
   L->frame = L->pair_boxes = L->row_boxes = nullptr;
   L->far_bits = nullptr;
   if (bits) {
@@ -909,7 +747,8 @@ static size_t score_lp_layout(void *ws, int64_t n, int64_t m, unsigned prec, Sco
     L->pair_boxes = cv.take<float>(L->npairs_data * kBoxFloats);
     L->row_boxes = cv.take<float>(L->row_blocks * kBoxFloats);
     // the waves' rows, then one row per workgroup (the AND of its waves')
-    L->far_bits = cv.take<unsigned long long>((L->row_blocks + p.qblocks) * L->far_words);
+    L->far_bits = cv.take<unsigned long long>(
+        far_bits_count(p.qblocks, score_lp_waves(prec), L->npairs_data));
   } else {
     L->row_blocks = L->far_words = 0;
   }
@@ -983,14 +822,9 @@ int score16_lp_run(const char *what, const float *rows_in, int64_t ldr, int64_t 
   const unsigned long long *far_bits = nullptr;
   if (skip && L.far_bits && n > 0 && m > 0) {
     const double cabs = -exp2_coef(h);
-    hipLaunchKernelGGL(frame_kernel, dim3(1), dim3(256), 0, s, data, ld, n, L.mu, inv_n, L.frame);
-    hipLaunchKernelGGL(box_kernel, dim3(cdiv(L.npairs_data, 256)), dim3(256), 0, s, data, ld, n,
-                       L.npairs_data, kLpPairPts, L.mu, inv_n, cabs, L.frame, L.pair_boxes);
-    hipLaunchKernelGGL(box_kernel, dim3(cdiv(L.row_blocks, 256)), dim3(256), 0, s, rows_in, ldr,
-                       m, L.row_blocks, kFarRows, L.mu, inv_n, cabs, L.frame, L.row_boxes);
-    hipLaunchKernelGGL(far_bits_kernel, dim3(cdiv(p.qblocks * L.far_words * 64, 256)), dim3(256),
-                       0, s, L.row_boxes, L.pair_boxes, L.npairs_data, L.far_words, p.qblocks,
-                       score_lp_waves(prec), float(cabs), skip_t, L.far_bits);
+    launch_far_bits(data, ld, n, rows_in, ldr, m, L.mu, inv_n, cabs, L.npairs_data, p.qblocks,
+                    score_lp_waves(prec), skip_t, nullptr, 0.f, L.frame, L.pair_boxes,
+                    L.row_boxes, L.far_bits, s);
     far_bits = L.far_bits;
   }
 #define FSK_LP(P, PXT, QT)                                                                       \

@@ -147,8 +147,10 @@ def check_fp16_range(train: torch.Tensor, query: torch.Tensor, h: float, flags: 
 
 
 def kde16_logdensity(train: torch.Tensor, query: torch.Tensor, bandwidth: float, *,
-                     precision: str = "fp32") -> torch.Tensor:
-    """Normalised log-density by log-sum-exp, float64 (m,)."""
+                     precision: str = "fp32", dense: bool = False) -> torch.Tensor:
+    """Normalised log-density by log-sum-exp, float64 (m,). The 16-bit forms
+    leave out terms below 2^-(ceil(log2 n) + 28) of a query's sum from
+    n = 16384 on; dense=True evaluates every term."""
     h = _check_h(bandwidth)
     dev = train.device
     nat.require_gpu(dev, "kde16_logdensity")
@@ -157,6 +159,8 @@ def kde16_logdensity(train: torch.Tensor, query: torch.Tensor, bandwidth: float,
         raise ValueError("data and queries must be non-empty.")
     flags = precision_flag(precision)
     check_fp16_range(train, query, h, flags)
+    if dense:
+        flags |= nat.FLAG_DENSE
     out = torch.empty(m, dtype=torch.float64, device=dev)
     ws, nbytes = nat.workspace(nat.OP_KDE16_LOGDENSITY, n, m, flags, dev)
     st = nat.lib().fsk_kde16_logdensity(train.data_ptr(), train.stride(0), n, query.data_ptr(),

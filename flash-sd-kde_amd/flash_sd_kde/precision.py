@@ -30,7 +30,10 @@ term is negligible (exactly 0 in FP16 for "fp16": bitwise the dense result;
 below n 2^-28 of a row's self term in the FP32 row sums of the other two).
 `emp_score` therefore runs these forms on the data in spatial order
 (`spatial.spatial_order`) and returns the results in the caller's order; the
-order changes the sums only by rounding.
+order changes the sums only by rounding. The 16-bit log-densities leave out
+terms below 2^-(ceil(log2 n) + 28) of a query's sum (bounded by a first pass
+over a sample of the data), so `kde_logdensity` and `sdkde_logdensity` put
+data and queries in spatial order for them as well.
 
 `ref64_*` are FP64 direct-difference GPU kernels used to measure the error of
 every form at sizes the CPU oracle cannot reach.
@@ -43,10 +46,11 @@ import torch
 from flash_sd_kde import _ops
 from flash_sd_kde.spatial import spatial_order
 
-# forms whose score kernel leaves out negligible tiles, and the smallest data
-# set that is put in spatial order for them (below it the kernel has few
-# tiles to leave out)
+# forms whose score kernel leaves out negligible tiles, forms whose
+# log-density does, and the smallest data set that is put in spatial order
+# for them (below it the kernels have few tiles to leave out)
 SKIP_PRECISIONS = ("fp16", "fp16x3s", "bf16x3s")
+LSE_SKIP_PRECISIONS = ("fp16", "bf16", "fp16x3", "bf16x3", "bf16x6", "fp16x3s", "bf16x3s")
 SPATIAL_MIN_N = 8192
 
 SCORE_PRECISIONS = ("fp32", "fp32_valu", "fp16x3", "fp16x3s", "bf16x6", "bf16x3", "bf16x3s",
@@ -55,12 +59,28 @@ LOGDENSITY_PRECISIONS = ("fp32", "fp32_valu", "fp16x3", "fp16x3s", "bf16x6", "bf
                          "fp16", "bf16")
 
 
+def _logdensity(train: torch.Tensor, query: torch.Tensor, bandwidth: float, precision: str,
+                spatial: bool, dense: bool, train_ordered: bool = False) -> torch.Tensor:
+    if (spatial and not dense and precision in LSE_SKIP_PRECISIONS
+            and train.shape[0] >= SPATIAL_MIN_N):
+        if not train_ordered:
+            train = train[spatial_order(train)]
+        qperm = spatial_order(query)
+        out_s = _ops.kde16_logdensity(train, query[qperm], bandwidth, precision=precision)
+        out = torch.empty_like(out_s)
+        out[qperm] = out_s
+        return out
+    return _ops.kde16_logdensity(train, query, bandwidth, precision=precision, dense=dense)
+
+
 def kde_logdensity(data, queries, bandwidth: float, *, precision: str = "fp32",
-                   device="cuda") -> torch.Tensor:
-    """log p(q) = log(1/(n (2 pi)^8 h^16) sum_j exp(-|q - x_j|^2 / 2h^2)), float64."""
+                   spatial: bool = True, dense: bool = False, device="cuda") -> torch.Tensor:
+    """log p(q) = log(1/(n (2 pi)^8 h^16) sum_j exp(-|q - x_j|^2 / 2h^2)), float64.
+    The 16-bit forms run on data and queries in spatial order (spatial=False:
+    the caller's order) and leave out negligible terms (dense=True: none)."""
     device = torch.device(device)
-    return _ops.kde16_logdensity(_ops.as_points16(data, device), _ops.as_points16(queries, device),
-                                 bandwidth, precision=precision)
+    return _logdensity(_ops.as_points16(data, device), _ops.as_points16(queries, device),
+                       bandwidth, precision, spatial, dense)
 
 
 def _ordered_score(train: torch.Tensor, bandwidth: float, precision: str, spatial: bool,
@@ -117,10 +137,18 @@ def sdkde_logdensity(data, queries, bandwidth: float, *, precision: str = "fp32"
     sp = precision if score_precision is None else score_precision
     if sp not in SCORE_PRECISIONS:
         raise ValueError(f"score_precision must be one of {SCORE_PRECISIONS}, got {sp!r}")
-    deb = sdkde_fit_transform(data, bandwidth, precision=sp, spatial=spatial, dense=dense,
+    train = _ops.as_points16(data, device)
+    query = _ops.as_points16(queries, device)
+    if (spatial and not dense and train.shape[0] >= SPATIAL_MIN_N
+            and (sp in SKIP_PRECISIONS or precision in LSE_SKIP_PRECISIONS)):
+        # score, debias and log-density all on the data in spatial order
+        xs = train[spatial_order(train)]
+        pdf, w = _ops.score16(xs, bandwidth, variant="ordered", precision=sp)
+        deb = _ops.debias(xs, pdf, w, bandwidth, 1e-12)
+        return _logdensity(deb, query, bandwidth, precision, True, False, train_ordered=True)
+    deb = sdkde_fit_transform(train, bandwidth, precision=sp, spatial=spatial, dense=dense,
                               device=device)
-    return _ops.kde16_logdensity(deb, _ops.as_points16(queries, device), bandwidth,
-                                 precision=precision)
+    return _logdensity(deb, query, bandwidth, precision, spatial, dense)
 
 
 def ref64_sdkde_logdensity(data, queries, bandwidth: float, *, eps: float = 1e-12,

@@ -26,23 +26,31 @@ def _spread3(v: torch.Tensor) -> torch.Tensor:
 def spatial_order(x: torch.Tensor, bits: int = 10) -> torch.Tensor:
     """Permutation of the rows of x (n, d) that sorts them by the Morton code
     of their coordinates along the three leading principal axes (`bits` bits
-    per axis, at most 10): bit b of axis a goes to code bit 3b + a."""
+    per axis, at most 10, over the mean +- 4 standard deviations along each;
+    bit b of axis a goes to code bit 3b + a). The axes come from the
+    covariance of a strided sample of at most ~65k rows."""
     n = x.shape[0]
     if n < 3:
         return torch.arange(n, device=x.device)
     if not 1 <= bits <= 10:
         raise ValueError(f"bits must be in [1, 10], got {bits}")
     xf = x.float()
-    y = xf - xf.mean(0)
-    cov = (y.T @ y).double().cpu().numpy() / (n - 1)
-    _, vecs = np.linalg.eigh(cov)
+    # the frame from a strided sample of at most ~65k rows (it only orders)
+    ys = xf[::max(1, n // 65536)]
+    mu = ys.mean(0)
+    yc = ys - mu
+    cov = (yc.T @ yc).double().cpu().numpy() / max(ys.shape[0] - 1, 1)
+    ev, vecs = np.linalg.eigh(cov)
     k = min(3, x.shape[1])
     axes = torch.as_tensor(np.ascontiguousarray(vecs[:, -k:]), dtype=torch.float32,
                            device=x.device)
-    p = y @ axes
-    lo, hi = torch.aminmax(p, dim=0)
+    # cells over mu +- 4 standard deviations per axis; rows beyond fall into
+    # the end cells
+    sd = np.sqrt(np.maximum(ev[-k:], 0.0))
+    half = torch.as_tensor(np.maximum(4.0 * sd, 1e-30), dtype=torch.float32, device=x.device)
+    p = (xf - mu) @ axes
     scale = (1 << bits) - 1
-    q = ((p - lo) * (scale / (hi - lo).clamp_min(1e-30))).to(torch.int32).clamp_(0, scale)
+    q = ((p + half) * (scale / (2.0 * half))).clamp_(0, scale).to(torch.int32)
     code = torch.zeros(n, dtype=torch.int32, device=x.device)
     for a in range(k):
         code |= _spread3(q[:, a]) << a

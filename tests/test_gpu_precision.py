@@ -293,3 +293,44 @@ def test_spatially_ordered_score_matches_fp64(gpu, prec, tol_pdf, tol_w):
     assert e_pdf <= tol_pdf and e_w <= tol_w
     p0, w0 = precision.emp_score(x, h, precision=prec, spatial=False, device=gpu)
     assert float(((pdf - p0).abs() / p0).max()) <= tol_pdf
+
+
+# ---------------- negligible terms of the 16-bit log-densities ----------------
+
+@pytest.mark.parametrize("prec", ["fp16x3", "bf16x6", "fp16"])
+def test_logdensity_skipping_bounds(gpu, prec):
+    """From n = 16384 on, the 16-bit log-densities bound each query's sum from
+    below by a pass over every k-th data point and leave out terms below
+    2^-(ceil(log2 n) + 28) of that bound (far blocks by the box bitmap, tiles
+    by the kernel's own test): the log-density moves by at most 2^-28 plus
+    FP32 rounding of the other terms, against the dense kernel; data and
+    queries in spatial order (the Python layer's default) and in random order
+    (fewer terms left out, same bound)."""
+    from flash_sd_kde.datasets import gmm_isotropic
+
+    x, h = _spread_mixture(gpu, 20000, 8)
+    q = torch.as_tensor(gmm_isotropic(4000, k=8, mean_scale=3.0, seed=9), device=gpu)
+    ld = _ops.kde16_logdensity(x, q, h, precision=prec, dense=True)
+    lp = precision.kde_logdensity(x, q, h, precision=prec, device=gpu)
+    lr = _ops.kde16_logdensity(x, q, h, precision=prec)
+    d_sp = float((lp - ld).abs().max())
+    d_rand = float((lr - ld).abs().max())
+    record(f"logdensity_skip_{prec}", spatial=d_sp, caller_order=d_rand)
+    # FP32 partial sums run over 512 terms between FP64 flushes: leaving terms
+    # out shifts their rounding by ~1e-6 relative
+    assert d_sp <= 2e-6 and d_rand <= 2e-6
+
+
+@pytest.mark.parametrize("prec,tol", [("fp16x3", 1e-5), ("bf16x6", 1e-5)])
+def test_spatial_logdensity_matches_fp64(gpu, prec, tol):
+    """kde_logdensity of the FP32-class 16-bit forms with terms left out,
+    against the FP64 GPU log-density (relative)."""
+    from flash_sd_kde.datasets import gmm_isotropic
+
+    x, h = _spread_mixture(gpu, 20000, 10)
+    q = torch.as_tensor(gmm_isotropic(3000, k=8, mean_scale=3.0, seed=11), device=gpu)
+    lp = precision.kde_logdensity(x, q, h, precision=prec, device=gpu)
+    ref = _ops.ref64_logdensity(x.double(), q.double(), h)
+    rel = float(((lp - ref).abs() / ref.abs()).max())
+    record(f"spatial_logdensity_{prec}", max_rel=rel)
+    assert rel <= tol
