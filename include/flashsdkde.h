@@ -68,10 +68,11 @@ extern "C" {
 #define FSK_FLAG_LAPLACIAN (1u << 8) /* phi * (1 + d/2 - |q-x|^2/(2h^2)) */
 #define FSK_FLAG_PHI_S (1u << 9)     /* phi * |q-x|^2/h^2 alone: the Laplacian correction's
                                       * moment, for the two-pass (non-fused) form */
-#define FSK_FLAG_DENSE (1u << 10)    /* score forms whose phi.X takes phi rounded to FP16
-                                      * (F16, F16X3S, BF16X3S): evaluate every tile pair
-                                      * instead of leaving out those whose every term is
-                                      * negligible (fsk_score16_ordered) */
+#define FSK_FLAG_DENSE (1u << 10)    /* evaluate every term: no negligible tile pairs left
+                                      * out in the score forms whose phi.X takes phi rounded
+                                      * to FP16 (F16, F16X3S, BF16X3S; fsk_score16_ordered,
+                                      * fsk_score16_rows) or in the 16-bit log-densities
+                                      * (fsk_kde16_logdensity) */
 
 /* ---- operation ids for fsk_workspace_bytes ---- */
 #define FSK_OP_KDE16_SPLITK 1
@@ -193,6 +194,13 @@ int fsk_kde16_splitk_noalloc(const float *data, int64_t ld, int64_t n,
  * out[i] = log( 1/(n (2 pi)^8 h^16) * sum_j exp(-|q_i - x_j|^2/(2h^2)) ),
  * computed as per-split (max, sum) pairs in base 2 and an FP64 combine.
  * Data are centred on the data mean before the cross term.
+ * The 16-bit forms leave out negligible terms from n = 16384 on: a pass over
+ * every (n / 16384)-th data point bounds each query's sum from below, and
+ * terms below 2^-(ceil(log2 n) + 28) of that bound are not evaluated (pairs
+ * of data tiles a box bound proves far are not read; other tiles whose every
+ * term is below the wave's thresholds add nothing), so a sum loses at most
+ * 2^-28 of itself. Data and queries in spatial order let most blocks skip.
+ * FSK_FLAG_DENSE evaluates every term.
  * No reference counterpart (the reference returns log(density + eps),
  * flash_sd_kde/estimator.py:114); FP64 semantics follow
  * experiments/error_suite_16d/reference.py:32-82. */
