@@ -481,7 +481,8 @@ size_t score16_rows_ws_bytes(int64_t n, int64_t m) {
 }
 
 static int score_check(const char *what, const float *data, int64_t ld, int64_t n, double h,
-                       unsigned flags, const float *pdf, const float *w, bool lowp_ok) {
+                       unsigned flags, const float *pdf, const float *w, bool lowp_ok,
+                       unsigned extra_flags = 0u) {
   int st;
   if ((st = check_points16(what, data, ld, n)) != FSK_OK) return st;
   if ((st = check_elem_rows(what, n)) != FSK_OK) return st;
@@ -498,7 +499,7 @@ static int score_check(const char *what, const float *data, int64_t ld, int64_t 
                     prec == FSK_PREC_BF16X6 || prec == FSK_PREC_F16X3 || prec == FSK_PREC_F16X3S ||
                     prec == FSK_PREC_BF16X3S || prec == FSK_PREC_F32_VALU;
   if ((prec != FSK_PREC_F32 && !(lowp && lowp_ok)) ||
-      (flags & ~(FSK_PREC_MASK | (lowp_ok ? FSK_FLAG_DENSE : 0u)))) {
+      (flags & ~(FSK_PREC_MASK | extra_flags | (lowp_ok ? FSK_FLAG_DENSE : 0u)))) {
     set_error("%s: flags 0x%x not available on this path", what, flags);
     return FSK_EUNSUPPORTED;
   }
@@ -605,7 +606,7 @@ extern "C" int fsk_score16_rows(const float *rows, int64_t ldr, int64_t m, const
                                 size_t ws_bytes, float *pdf, float *w, int64_t ldw_row,
                                 int64_t ldw_col, void *stream) {
   const char *what = "fsk_score16_rows";
-  int st = score_check(what, data, ld, n, bandwidth, flags, pdf, w, true);
+  int st = score_check(what, data, ld, n, bandwidth, flags, pdf, w, true, FSK_FLAG_ROWS_IN_DATA);
   if (st != FSK_OK) return st;
   if ((st = check_points16(what, rows, ldr, m)) != FSK_OK) return st;
   if ((st = check_elem_rows(what, m)) != FSK_OK) return st;
@@ -614,7 +615,9 @@ extern "C" int fsk_score16_rows(const float *rows, int64_t ldr, int64_t m, const
                             w, ldw_row, ldw_col, stream);
   if ((flags & FSK_PREC_MASK) != FSK_PREC_F32)
     return score16_lp_run(what, rows, ldr, m, data, ld, n, bandwidth, flags & FSK_PREC_MASK,
-                          rows == data && ldr == ld && m == n, (flags & FSK_FLAG_DENSE) != 0, ws,
+                          (rows == data && ldr == ld && m == n) ||
+                              (flags & FSK_FLAG_ROWS_IN_DATA) != 0,
+                          (flags & FSK_FLAG_DENSE) != 0, ws,
                           ws_bytes, pdf, w, ldw_row, ldw_col, stream);
   ScoreLayout L;
   size_t need = score_layout(ws, n, true, &L, m);

@@ -36,6 +36,7 @@ PREC_BF16X3S = 8
 FLAG_LAPLACIAN = 1 << 8
 FLAG_PHI_S = 1 << 9
 FLAG_DENSE = 1 << 10
+FLAG_ROWS_IN_DATA = 1 << 11
 
 OP_KDE16_SPLITK = 1
 OP_KDE16_ATOMIC = 2

@@ -254,10 +254,11 @@ def score16(train: torch.Tensor, bandwidth: float, *, variant: str = "symmetric"
 
 
 def score16_rows(rows: torch.Tensor, train: torch.Tensor, bandwidth: float, *,
-                 precision: str = "fp32", dense: bool = False):
+                 precision: str = "fp32", dense: bool = False, rows_in_data: bool = False):
     """(pdf (m,), weighted (m,16)) of `rows` under the KDE of `train`, in the
     FP32 form or one of the 16-bit MFMA forms of the ordered score
-    (dense=True: every tile pair evaluated, as in score16)."""
+    (dense=True: every tile pair evaluated, as in score16). rows_in_data=True
+    states that every row is one of the data points (FSK_FLAG_ROWS_IN_DATA)."""
     h = _check_h(bandwidth)
     dev = train.device
     nat.require_gpu(dev, "score16_rows")
@@ -272,6 +273,8 @@ def score16_rows(rows: torch.Tensor, train: torch.Tensor, bandwidth: float, *,
     check_fp16_range(train, rows, h, flags)
     if dense:
         flags |= nat.FLAG_DENSE
+    if rows_in_data and flags != nat.PREC_F32 and flags != nat.PREC_F32_VALU:
+        flags |= nat.FLAG_ROWS_IN_DATA
     pdf = torch.empty(m, dtype=torch.float32, device=dev)
     w = torch.empty((m, ND), dtype=torch.float32, device=dev)
     ws, nbytes = nat.workspace(nat.OP_SCORE16_ROWS, n, m, flags, dev)

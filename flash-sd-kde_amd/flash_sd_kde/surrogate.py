@@ -31,6 +31,7 @@ import torch
 
 from flash_sd_kde import _native as nat
 from flash_sd_kde import _ops
+from flash_sd_kde.spatial import spatial_order
 
 DIMS = (16, 256, 256, 256, 16)
 
@@ -390,13 +391,15 @@ class ScoreMLP:
 
 
 def exact_shift(rows: torch.Tensor, data: torch.Tensor, h: float, eps: float = 1e-12, *,
-                precision: str = "fp32"):
+                precision: str = "fp32", rows_in_data: bool = False):
     """delta = w/(pdf + eps) - x of `rows` under the KDE of `data` (score kernel
     in `precision`, FP64 debias): x + delta/2 is the exact one-step SD-KDE
-    point. A 16-bit form whose range the inputs exceed (ValueError from the
-    FP16 range check) falls back to the FP32 form."""
+    point. rows_in_data: every row is one of the data points (the 16-bit
+    forms may then leave out negligible tile pairs). A 16-bit form whose
+    range the inputs exceed (ValueError from the FP16 range check) falls back
+    to the FP32 form."""
     try:
-        pdf, w = _ops.score16_rows(rows, data, h, precision=precision)
+        pdf, w = _ops.score16_rows(rows, data, h, precision=precision, rows_in_data=rows_in_data)
     except ValueError:
         if precision == "fp32":
             raise
@@ -431,12 +434,18 @@ def surrogate_fit_transform(data, bandwidth: float, *, n_label: int = 262144,
     n_label = min(n_label, max(1, n // 2))
     n_holdout = min(n_holdout, max(1, (n - n_label) // 2))
     n_probe = min(n_probe, max(1, n - n_label - n_holdout))
-    lab = x[perm[:n_label]].contiguous()
     hold = x[perm[n_label:n_label + n_holdout]].contiguous()
     probe = x[perm[n_label + n_holdout:n_label + n_holdout + n_probe]].contiguous()
 
     t0 = time.perf_counter()
-    target = exact_shift(lab, x, h, precision=label_precision)
+    # labels: a random subset of the data, both in spatial order, so the
+    # label kernel leaves out the tile pairs whose terms are negligible
+    order = spatial_order(x)
+    xs = x[order]
+    inv = torch.empty_like(order)
+    inv[order] = torch.arange(n, device=device)
+    lab = xs[torch.sort(inv[perm[:n_label]]).values].contiguous()
+    target = exact_shift(lab, xs, h, precision=label_precision, rows_in_data=True)
     torch.cuda.synchronize(device)
     info["label_s"] = time.perf_counter() - t0
     t0 = time.perf_counter()

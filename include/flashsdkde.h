@@ -68,6 +68,10 @@ extern "C" {
 #define FSK_FLAG_LAPLACIAN (1u << 8) /* phi * (1 + d/2 - |q-x|^2/(2h^2)) */
 #define FSK_FLAG_PHI_S (1u << 9)     /* phi * |q-x|^2/h^2 alone: the Laplacian correction's
                                       * moment, for the two-pass (non-fused) form */
+#define FSK_FLAG_ROWS_IN_DATA (1u << 11) /* fsk_score16_rows: every row is also one of the
+                                          * data points, so each row's sums hold its self
+                                          * term (lets F16X3S / BF16X3S leave out negligible
+                                          * tile pairs as for rows = data) */
 #define FSK_FLAG_DENSE (1u << 10)    /* evaluate every term: no negligible tile pairs left
                                       * out in the score forms whose phi.X takes phi rounded
                                       * to FP16 (F16, F16X3S, BF16X3S; fsk_score16_ordered,
@@ -288,7 +292,8 @@ int fsk_score16_ordered(const float *data, int64_t ld, int64_t n,
  * forms carry the range limit stated there, here over rows and data alike), and
  * FSK_FLAG_DENSE. Tiles are left out as in fsk_score16_ordered for F16 (bitwise);
  * F16X3S / BF16X3S leave them out only when rows is data (same pointer, stride
- * and count), where every row holds its self term. With rows = data this is
+ * and count) or the caller sets FSK_FLAG_ROWS_IN_DATA (every row is one of the
+ * data points), where every row holds its self term. With rows = data this is
  * fsk_score16_ordered; a row subset gives exact score labels for the learned
  * surrogate. The reference evaluates the score at separate points only in 1-D
  * (kde_score_eval, flash_sd_kde/reference.py:83-120). */

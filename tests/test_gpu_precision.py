@@ -273,6 +273,13 @@ def test_one_phi_score_skipping_bounds(gpu, prec):
     p1, w1 = _ops.score16_rows(rows, x, h, precision=prec)
     p2, w2 = _ops.score16_rows(rows, x, h, precision=prec, dense=True)
     assert torch.equal(p1, p2) and torch.equal(w1, w2)
+    # unless the caller states that every row is a data point: the same bound
+    p3, w3 = _ops.score16_rows(rows, x, h, precision=prec, rows_in_data=True)
+    rel = float(((p3.double() - p2.double()).abs() / p2.double()).max())
+    dw = (w3.double() - w2.double()) - mu[None, :] * (p3.double() - p2.double())[:, None]
+    rel_w = float((dw.abs().amax(1) / w2.double().abs().amax(1)).max())
+    record(f"{prec}_rows_in_data_skip_rel_diff", pdf=rel, w_centred=rel_w)
+    assert rel <= 1e-6 and rel_w <= 1e-6
 
 
 @pytest.mark.parametrize("prec,tol_pdf,tol_w", [("fp16x3s", 5e-5, 1e-4), ("bf16x3s", 1e-3, 5e-4),
