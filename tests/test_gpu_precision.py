@@ -341,3 +341,21 @@ def test_spatial_logdensity_matches_fp64(gpu, prec, tol):
     rel = float(((lp - ref).abs() / ref.abs()).max())
     record(f"spatial_logdensity_{prec}", max_rel=rel)
     assert rel <= tol
+
+
+@pytest.mark.parametrize("n,m", [(16385, 1), (16400, 257), (20001, 3)])
+def test_logdensity_skipping_ragged(gpu, n, m):
+    """The skipping log-density at ragged sizes: n just past the threshold
+    and not a multiple of the 16-point tile or the 32-point pair, a single
+    query and partial row blocks; against the dense kernel (the 2^-28 bound
+    plus FP32 rounding of the partial sums)."""
+    from flash_sd_kde.datasets import gmm_isotropic
+
+    x = torch.as_tensor(gmm_isotropic(n, k=8, mean_scale=3.0, seed=n), device=gpu)
+    q = torch.as_tensor(gmm_isotropic(m, k=8, mean_scale=3.0, seed=m + 7), device=gpu)
+    h = 1.4
+    for prec in ("fp16x3", "bf16"):
+        lp = precision.kde_logdensity(x, q, h, precision=prec, device=gpu)
+        ld = _ops.kde16_logdensity(x, q, h, precision=prec, dense=True)
+        assert torch.isfinite(lp).all()
+        assert float((lp - ld).abs().max()) <= 2e-6
