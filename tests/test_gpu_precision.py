@@ -348,7 +348,10 @@ def test_logdensity_skipping_ragged(gpu, n, m):
     """The skipping log-density at ragged sizes: n just past the threshold
     and not a multiple of the 16-point tile or the 32-point pair, a single
     query and partial row blocks; against the dense kernel (the 2^-28 bound
-    plus FP32 rounding of the partial sums)."""
+    plus FP32 rounding of the partial sums: a lane's partial runs over up to
+    128 terms, at most 128 * 2^-24 = 7.6e-6 relative, and leaving terms out
+    reorders them; far queries whose sums hold many comparable terms come
+    closest)."""
     from flash_sd_kde.datasets import gmm_isotropic
 
     x = torch.as_tensor(gmm_isotropic(n, k=8, mean_scale=3.0, seed=n), device=gpu)
@@ -358,4 +361,4 @@ def test_logdensity_skipping_ragged(gpu, n, m):
         lp = precision.kde_logdensity(x, q, h, precision=prec, device=gpu)
         ld = _ops.kde16_logdensity(x, q, h, precision=prec, dense=True)
         assert torch.isfinite(lp).all()
-        assert float((lp - ld).abs().max()) <= 2e-6
+        assert float((lp - ld).abs().max()) <= 1e-5
