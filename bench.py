@@ -312,24 +312,35 @@ def surrogate_form(fp32_pair_rate):
 
 
 def kde1_visited_pairs(data, queries, h):
-    """Pairs the 1-D kernels evaluate: per block of 2048 sorted queries, the
-    1024-point chunks of the sorted data within 12.8 / k of the block's first
-    and last query, k = sqrt(log2(e) / (2 h^2)) (beyond that every term is
-    exactly 0 in FP32); the same rule as the library's range kernel, in FP64."""
+    """Pairs the 1-D plain and score kernels evaluate: per block of 2048
+    sorted queries, the 1024-point chunks of the sorted data that meet the
+    union of the queries' reaches [q - r, q + r], r = min(sqrt(d_nn^2 +
+    T / k^2), 12.8 / k) with d_nn the distance to the nearest data point,
+    T = ceil(log2 n) + 32 and k = sqrt(log2(e) / (2 h^2)) (terms more than
+    2^-T below a query's largest are left out, and beyond 12.8 / k every term
+    is exactly 0 in FP32); the library's rule, in FP64."""
     import numpy as np
 
     xs = np.sort(np.asarray(data, dtype=np.float32)).astype(np.float64)
     qs = np.sort(np.asarray(queries, dtype=np.float32)).astype(np.float64)
-    reach = 12.8 / math.sqrt(math.log2(math.e) / (2.0 * h * h))
+    k = math.sqrt(math.log2(math.e) / (2.0 * h * h))
     n, rows, chunk = xs.size, 2048, 1024
+    t = math.ceil(math.log2(n)) + 32.0
+    pos = np.searchsorted(xs, qs, side="left")
+    right = np.where(pos < n, xs[np.minimum(pos, n - 1)] - qs, np.inf)
+    left = np.where(pos > 0, qs - xs[np.maximum(pos - 1, 0)], np.inf)
+    dnn = np.minimum(right, left)
+    r = np.minimum(np.sqrt(dnn * dnn + t / (k * k)), 12.8 / k)
     n_pad = -(-n // chunk) * chunk
-    first, last = qs[::rows], qs[np.minimum(np.arange(0, qs.size, rows) + rows, qs.size) - 1]
-    jlo = np.searchsorted(xs, first - reach, side="left")
-    jhi = np.searchsorted(xs, last + reach, side="right")
+    starts = np.arange(0, qs.size, rows)
+    lo = np.minimum.reduceat(qs - r, starts)
+    hi = np.maximum.reduceat(qs + r, starts)
+    jlo = np.searchsorted(xs, lo, side="left")
+    jhi = np.searchsorted(xs, hi, side="right")
     c0 = (jlo // chunk) * chunk
     c1 = np.minimum(-(-jhi // chunk) * chunk, n_pad)
     width = np.where(jhi > jlo, c1 - c0, 0)
-    nrows = np.minimum(np.arange(0, qs.size, rows) + rows, qs.size) - np.arange(0, qs.size, rows)
+    nrows = np.minimum(starts + rows, qs.size) - starts
     return float(np.sum(width * nrows))
 
 

@@ -20,9 +20,13 @@
 // exp2(-t) < 2^-161, which is exactly 0 in FP32 (the smallest denormal is
 // 2^-149); so each query block only visits the data chunks that meet
 // [q_lo - 12.8/k, q_hi + 12.8/k] (binary search in FP64), and every pair it
-// skips would have added exactly +0. The sums are those of all pairs, in
-// another order. Results are written at the sorted positions and scattered
-// back to the callers' order.
+// skips would have added exactly +0. The plain and score sums go further:
+// a query's largest term is its nearest data point's, so its terms more than
+// 2^-T below that one (T = ceil(log2 n) + 32) add at most n 2^-T <= 2^-32 of
+// its sum; each query reaches sqrt(d_nn^2 + T/k^2) (kde1_reach_kernel), and a
+// block visits the chunks that meet the union of its queries' reaches.
+// Results are written at the sorted positions and scattered back to the
+// callers' order.
 #include "fsk_common.h"
 #include "fsk_plan.h"
 
@@ -56,17 +60,59 @@ __global__ void iota_i32(int *p, int64_t n) {
   if (i < n) p[i] = int(i);
 }
 
+// Reach of each sorted query for the plain and score sums: terms more than
+// 2^-T below the query's largest term (its nearest data point's) are left
+// out, T = ceil(log2 n) + 32, so a sum loses at most n 2^-T <= 2^-32 of
+// itself: qreach[i] = sqrt(d_nn^2 + T / k^2), at most the exact-zero reach.
+__global__ void kde1_reach_kernel(const float *__restrict__ xs, int64_t n,
+                                  const float *__restrict__ qs, int64_t m, double t_over_k2,
+                                  double zero_reach, float *__restrict__ qreach) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const double q = qs[i];
+  int64_t a = 0, z = n;  // first j with xs[j] >= q
+  while (a < z) {
+    const int64_t mid = (a + z) >> 1;
+    if (double(xs[mid]) < q) a = mid + 1; else z = mid;
+  }
+  double d = INFINITY;
+  if (a < n) d = double(xs[a]) - q;
+  if (a > 0) d = fmin(d, q - double(xs[a - 1]));
+  const double r = sqrt(d * d + t_over_k2);
+  qreach[i] = float(fmin(r, zero_reach));
+}
+
 // Data-chunk range [ranges[2b], ranges[2b+1]) of query block b: the chunks
-// that hold the sorted data points within kReach1d of the block's sorted
-// queries (empty when there are none).
-__global__ void kde1_range_kernel(const float *__restrict__ xs, int64_t n, int64_t n_pad,
-                                  const float *__restrict__ qs, int64_t m, int64_t qblocks,
-                                  double reach, int64_t *__restrict__ ranges) {
-  const int64_t b = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (b >= qblocks) return;
+// that hold the sorted data points within reach of the block's sorted
+// queries (empty when there are none). One workgroup per block; with qreach
+// each query has its own reach, else every query has `reach`.
+__global__ __launch_bounds__(256) void kde1_range_kernel(
+    const float *__restrict__ xs, int64_t n, int64_t n_pad, const float *__restrict__ qs,
+    int64_t m, int64_t qblocks, double reach, const float *__restrict__ qreach,
+    int64_t *__restrict__ ranges) {
+  __shared__ double red_lo[256], red_hi[256];
+  const int64_t b = blockIdx.x;
   const int64_t i0 = b * k1dRowsPerWG;
   const int64_t i1 = (i0 + k1dRowsPerWG < m ? i0 + k1dRowsPerWG : m) - 1;
-  const double lo = double(qs[i0]) - reach, hi = double(qs[i1]) + reach;
+  double lo = INFINITY, hi = -INFINITY;
+  for (int64_t i = i0 + threadIdx.x; i <= i1; i += 256) {
+    const double r = qreach ? double(qreach[i]) : reach;
+    lo = fmin(lo, double(qs[i]) - r);
+    hi = fmax(hi, double(qs[i]) + r);
+  }
+  red_lo[threadIdx.x] = lo;
+  red_hi[threadIdx.x] = hi;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (int(threadIdx.x) < w) {
+      red_lo[threadIdx.x] = fmin(red_lo[threadIdx.x], red_lo[threadIdx.x + w]);
+      red_hi[threadIdx.x] = fmax(red_hi[threadIdx.x], red_hi[threadIdx.x + w]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
+  lo = red_lo[0];
+  hi = red_hi[0];
   int64_t a = 0, z = n;  // first j with xs[j] >= lo
   while (a < z) {
     const int64_t mid = (a + z) >> 1;
@@ -198,7 +244,7 @@ __global__ __launch_bounds__(256) void kde1_kernel(const float *__restrict__ xs,
 }
 
 struct Kde1Layout {
-  float *xs, *qs, *res, *res2, *part, *part2;
+  float *xs, *qs, *res, *res2, *part, *part2, *qreach;
   int *qiota, *qperm;
   int64_t *ranges;
   void *tmp;
@@ -219,6 +265,7 @@ static size_t kde1_layout(void *ws, int64_t n, int64_t m, bool partials, bool tw
   L->qiota = cv.take<int>(m);
   L->qperm = cv.take<int>(m);
   L->ranges = cv.take<int64_t>(2 * p.qblocks);
+  L->qreach = cv.take<float>(m);
   L->res = cv.take<float>(m);
   L->res2 = two ? cv.take<float>(m) : nullptr;
   L->part = partials ? cv.take<float>(p.splits * m) : nullptr;
@@ -283,8 +330,18 @@ static int kde1_run(const char *what, int mode, bool atomic, const float *data, 
     hipLaunchKernelGGL(pad1_kernel, dim3(cdiv(n_pad - n, 256)), dim3(256), 0, s, L.xs, n, n_pad,
                        L.qs, m, kReach1d / k);
   SplitPlan p = kde1_plan(n, m);
-  hipLaunchKernelGGL(kde1_range_kernel, dim3(cdiv(p.qblocks, 256)), dim3(256), 0, s, L.xs, n,
-                     n_pad, L.qs, m, p.qblocks, kReach1d / k, L.ranges);
+  // plain and score sums: per-query reach from the nearest data point (the
+  // Laplacian factor and the moment change sign or grow with distance: they
+  // keep the exact-zero reach)
+  const bool bounded = mode == 0 || mode == 2;
+  if (bounded) {
+    const double T = std::ceil(std::log2(double(n))) + 32.0;
+    hipLaunchKernelGGL(kde1_reach_kernel, dim3(cdiv(m, 256)), dim3(256), 0, s, L.xs, n, L.qs, m,
+                       T / (k * k), kReach1d / k, L.qreach);
+  }
+  hipLaunchKernelGGL(kde1_range_kernel, dim3(p.qblocks), dim3(256), 0, s, L.xs, n, n_pad, L.qs, m,
+                     p.qblocks, kReach1d / k, bounded ? (const float *)L.qreach : nullptr,
+                     L.ranges);
   const int64_t ntasks = p.qblocks * p.splits;
   int64_t per_xcd;
   const int64_t grid = xcd_grid(ntasks, &per_xcd);
