@@ -218,7 +218,10 @@ int fsk_kde16_logdensity(const float *data, int64_t ld, int64_t n,
  * All 1-D entry points sort the scaled data and the queries on the device
  * (radix sort in the workspace) and let each block of sorted queries visit
  * only the data within the distance beyond which every term is exactly 0 in
- * FP32; n and m are at most 2^31 - 1, and their workspace size is queried
+ * FP32. The plain sums (no flag) and fsk_kde1_score reach less far: a
+ * query's terms more than 2^-(ceil(log2 n) + 32) below its largest (its
+ * nearest data point's) are left out, at most 2^-32 of its sum. n and m are
+ * at most 2^31 - 1, and their workspace size is queried
  * from the current device (fsk_workspace_bytes returns 0 without one). The
  * sorts record memset nodes, so the 1-D entry points refuse HIP graph
  * capture (FSK_EUNSUPPORTED); every other entry point is capturable.
