@@ -114,17 +114,18 @@ __global__ __launch_bounds__(256) void loo_center_kernel(const float *__restrict
   }
 }
 
-// Bandwidth skip groups. A term exp2(c_b delta) with c_b delta <= -64 is
-// below 2^-64, and S_ib >= 1 (the nearest neighbour's term is exp2(0)), so
-// all such terms of a row together change S_ib by less than n 2^-64 relative
-// (2^-46 at n = 2^18): far below one FP32 rounding. Per data tile a group of
-// kLooGroup bandwidths is evaluated only if some lane of the wave has a delta
-// below 64 / |c_b| for the group's widest member (one compare and ballot per
-// group); every other group is skipped. At the grid's narrow end (0.05 h, c_b
-// 400 times that at h) most tiles skip most groups.
+// Bandwidth skip groups. For a dataset of n points let T = ceil(log2 n) +
+// kLooSkipMargin. A term exp2(c_b delta) with c_b delta <= -T is below 2^-T,
+// and S_ib >= 1 (the nearest neighbour's term is exp2(0)), so all such terms
+// of a row together change S_ib by less than n 2^-T <= 2^-28 relative: below
+// one FP32 rounding of S_ib, and at most 2^-28 on log S_ib. Per data tile a
+// group of kLooGroup bandwidths is evaluated only if some lane of the wave has
+// a delta below T / |c_b| for the group's widest member (one compare and
+// ballot per group); every other group is skipped. At the grid's narrow end
+// (0.05 h, c_b 400 times that at h) most tiles skip most groups.
 constexpr int kLooGroup = 4;
 constexpr int kLooGroups = kLooMaxBw / kLooGroup;
-constexpr float kLooSkipExp = 64.f;
+constexpr float kLooSkipMargin = 28.f;
 
 // Register swaps between lane halves (32 lanes) and between neighbouring
 // 16-lane rows. v_permlane32_swap a, b swaps the upper 32 lanes of a with the
@@ -216,8 +217,10 @@ __global__ __launch_bounds__(64 * WAVES) void loo_grid_kernel(
     float cf[kLooMaxBw];
 #pragma unroll
     for (int b = 0; b < kLooMaxBw; ++b) cf[b] = coef_sh[b];
-    // skip threshold of each group: delta < 64 / min|c_b| over its bandwidths
-    // (0, never active, for a group past n_bw)
+    // skip threshold of each group: delta < T / min|c_b| over its bandwidths
+    // with T = ceil(log2 n) + kLooSkipMargin (0, never active, for a group
+    // past n_bw)
+    const float skip_exp = float(n > 1 ? 64 - __clzll((unsigned long long)(n - 1)) : 0) + kLooSkipMargin;
     float thr[kLooGroups];
 #pragma unroll
     for (int q = 0; q < kLooGroups; ++q) {
@@ -225,7 +228,7 @@ __global__ __launch_bounds__(64 * WAVES) void loo_grid_kernel(
 #pragma unroll
       for (int e = 0; e < kLooGroup; ++e)
         if (q * kLooGroup + e < nbw) mn = fminf(mn, -cf[q * kLooGroup + e]);
-      thr[q] = kLooSkipExp / mn;
+      thr[q] = skip_exp / mn;
     }
 
     // this wave's 16 RT rows (padded coordinates; the rows past n are idle)

@@ -17,7 +17,8 @@ batch signature. The whole sequence (queue reset, centring, persistent
 kernel, finish) is captured once in a HIP graph and replayed by run().
 
 The kernel skips, per 16-point data tile and group of four bandwidths, terms
-below 2^-64 of a row's nearest-neighbour term. It skips the more the closer
+below 2^-(ceil(log2 n) + 28) of a row's nearest-neighbour term (n the
+dataset's size; together below 2^-28 of the row's sum). It skips the more the closer
 together the points of a tile lie, so each dataset is first put in a spatial
 order (`spatial_order`): the Morton (Z-order) code of its coordinates along
 its three leading principal axes. The log-likelihoods are sums over all rows
@@ -116,7 +117,8 @@ class LooBandwidthGrid:
         """Time every tile/launch-shape candidate on this batch; keep the
         fastest. All candidates must produce the same log-likelihoods: they
         differ in the order of FP64 atomic additions, and in which terms below
-        2^-64 of a row's nearest-neighbour term their tiles skip."""
+        2^-(ceil(log2 n) + 28) of a row's nearest-neighbour term their tiles
+        skip."""
         key = self.signature()
         with _tune_lock:
             if key in _tune_cache:
@@ -146,7 +148,8 @@ class LooBandwidthGrid:
     def exps_evaluated(self) -> int:
         """Exponentials one launch evaluates with the chosen shape (the
         logical count is pair_count() * n_bw; the kernel skips tile groups
-        whose terms are all below 2^-64 of the nearest-neighbour term)."""
+        whose terms are all below 2^-(ceil(log2 n) + 28) of the
+        nearest-neighbour term)."""
         cnt = torch.zeros(1, dtype=torch.int64, device=self.device)
         self._launch(exps_done=cnt)
         return int(cnt.item())

@@ -335,9 +335,9 @@ int fsk_sdkde_debias_f64(const float *x, int64_t ldx, int64_t n,
  * One persistent launch over all datasets (work queue, largest first). The
  * tunables, chosen by the Python autotuner: waves_per_wg in {4, 8}, row_tiles
  * in {1, 2} (16 or 32 rows per wave: the task tile is 16 row_tiles
- * waves_per_wg rows) and wg_per_cu in 1..8. Terms below 2^-64 of a row's
- * nearest-neighbour term are skipped per data tile and group of 4 bandwidths
- * (their sum is below n 2^-64 relative). exps_done (DEVICE, nullable): the
+ * waves_per_wg rows) and wg_per_cu in 1..8. Terms below 2^-(ceil(log2 n_d) + 28) of
+ * a row's nearest-neighbour term are skipped per data tile and group of 4
+ * bandwidths (their sum is below 2^-28 of the row's sum). exps_done (DEVICE, nullable): the
  * count of exponentials evaluated is ADDED to it (the caller zeroes it).
  * Limits: n_sets <= 64, n_bw <= 32, every n_d >= 2.
  * Capturable: no allocation, no host synchronization, fills by kernel. */

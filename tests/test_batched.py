@@ -57,7 +57,7 @@ def test_spatial_order_is_a_permutation_that_keeps_clusters_together():
 @pytest.mark.gpu
 def test_batched_spatial_order_changes_only_rounding(gpu):
     """The spatial order of the points only changes the summation order and
-    which negligible (< 2^-64 relative) terms are skipped: the log-likelihoods
+    which negligible (< 2^-(ceil(log2 n) + 28) relative) terms are skipped: the log-likelihoods
     agree with the given order, and fewer exponentials are evaluated."""
     sets = _sets([3000, 700, 65], seed=6)
     factors = np.geomspace(0.05, 2.0, 32)
