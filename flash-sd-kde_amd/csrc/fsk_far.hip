@@ -6,7 +6,7 @@ namespace fsk {
 
 // One workgroup of 256: the covariance of a strided sample of the centred
 // data in FP64 (fixed order), then 40 rounds of block power iteration with
-// modified Gram-Schmidt for its three leading eigenvectors. frame[a * 16 + f];
+// modified Gram-Schmidt for its kBoxAxes leading eigenvectors. frame[a * 16 + f];
 // a direction that vanishes (rank-deficient data) is left zero, which keeps
 // the rows orthonormal or zero.
 __global__ __launch_bounds__(256) void frame_kernel(const float *__restrict__ x, int64_t ld,
@@ -104,8 +104,8 @@ __global__ void box_kernel(const float *__restrict__ x, int64_t ld, int64_t n, i
     o[a] = lo[a];
     o[kBoxAxes + a] = hi[a];
   }
-  o[6] = nrm;
-  o[7] = t;
+  o[kBoxNorm] = nrm;
+  o[kBoxThr] = t;
 }
 
 // bits[rb * words + p / 64] bit p % 64 = 1 when every exponent of row block
@@ -129,7 +129,7 @@ __global__ void far_bits_kernel(const float *__restrict__ row_boxes,
     plo[a] = real ? pair_boxes[p * kBoxFloats + a] : 0.f;
     phi[a] = real ? pair_boxes[p * kBoxFloats + kBoxAxes + a] : 0.f;
   }
-  if (real) pn = pair_boxes[p * kBoxFloats + 6];
+  if (real) pn = pair_boxes[p * kBoxFloats + kBoxNorm];
   unsigned long long all = ~0ull;
   for (int k = 0; k < waves; ++k) {
     const float *r = row_boxes + (g * waves + k) * kBoxFloats;
@@ -140,8 +140,8 @@ __global__ void far_bits_kernel(const float *__restrict__ row_boxes,
         const float gap = fmaxf(fmaxf(plo[a] - r[kBoxAxes + a], r[a] - phi[a]), 0.f);
         d2 += gap * gap;
       }
-      const float bound = -c * d2 + 0x1p-10f * (r[6] + pn) + 1.f;
-      far = !(bound > r[7]);
+      const float bound = -c * d2 + 0x1p-10f * (r[kBoxNorm] + pn) + 1.f;
+      far = !(bound > r[kBoxThr]);
     }
     const unsigned long long m = __builtin_amdgcn_ballot_w64(far);
     all &= m;
