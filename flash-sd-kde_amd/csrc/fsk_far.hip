@@ -4,17 +4,25 @@
 
 namespace fsk {
 
+// Sum over the 16 lanes of a lane group (every lane gets the total).
+__device__ __forceinline__ double sum16(double v) {
+#pragma unroll
+  for (int o = 8; o >= 1; o >>= 1) v += __shfl_xor(v, o, 16);
+  return v;
+}
+
 // One workgroup of 256: the covariance of a strided sample of the centred
 // data in FP64 (fixed order), then 40 rounds of block power iteration with
-// modified Gram-Schmidt for its kBoxAxes leading eigenvectors. frame[a * 16 + f];
-// a direction that vanishes (rank-deficient data) is left zero, which keeps
-// the rows orthonormal or zero.
+// modified Gram-Schmidt for its kBoxAxes leading eigenvectors, on the first
+// wave: lane f (of each 16-lane group) holds coordinate f of every iterate,
+// and the dot products are 16-lane sums. frame[a * 16 + f]; a direction that
+// vanishes (rank-deficient data) is left zero, which keeps the rows
+// orthonormal or zero.
 __global__ __launch_bounds__(256) void frame_kernel(const float *__restrict__ x, int64_t ld,
                                                     int64_t n, const double *__restrict__ mu_sum,
                                                     double inv_n, float *__restrict__ frame) {
   __shared__ float pts[256 * kD];
   __shared__ double cov[kD * kD];
-  __shared__ double q[kBoxAxes][kD], z[kBoxAxes][kD];  // iterates (thread 0)
   const int t = threadIdx.x;
   const int64_t cnt = n < kFrameSample ? n : kFrameSample;
   const int64_t stride = n / cnt;
@@ -32,36 +40,36 @@ __global__ __launch_bounds__(256) void frame_kernel(const float *__restrict__ x,
       for (int k = 0; k < 256; ++k) acc += double(pts[k * kD + ei]) * double(pts[k * kD + ej]);
   }
   if (t < kD * kD) cov[t] = acc;
-  if (t < kBoxAxes * kD) {
-    const int a = t / kD, f = t % kD;
-    q[a][f] = f == a ? 1.0 : 0.01 * double((7 * f + 3 * a) % 11 + 1);
-  }
   __syncthreads();
+  if (t >= 64) return;
+  const int f = t & (kD - 1);
+  double crow[kD], q[kBoxAxes];
+#pragma unroll
+  for (int g = 0; g < kD; ++g) crow[g] = cov[f * kD + g];
+#pragma unroll
+  for (int a = 0; a < kBoxAxes; ++a) q[a] = f == a ? 1.0 : 0.01 * double((7 * f + 3 * a) % 11 + 1);
   for (int it = 0; it < 40; ++it) {
-    // z = cov q, one entry per thread; then Gram-Schmidt on thread 0
-    if (t < kBoxAxes * kD) {
-      const int a = t / kD, f = t % kD;
+    double z[kBoxAxes];
+#pragma unroll
+    for (int a = 0; a < kBoxAxes; ++a) {
       double v = 0.0;
-      for (int g = 0; g < kD; ++g) v += cov[f * kD + g] * q[a][g];
-      z[a][f] = v;
+#pragma unroll
+      for (int g = 0; g < kD; ++g) v += crow[g] * __shfl(q[a], g, 16);
+      z[a] = v;
     }
-    __syncthreads();
-    if (t == 0) {
-      for (int a = 0; a < kBoxAxes; ++a) {
-        for (int b = 0; b < a; ++b) {
-          double d = 0.0;
-          for (int f = 0; f < kD; ++f) d += z[a][f] * z[b][f];
-          for (int f = 0; f < kD; ++f) z[a][f] -= d * z[b][f];
-        }
-        double nn = 0.0;
-        for (int f = 0; f < kD; ++f) nn += z[a][f] * z[a][f];
-        const double inv = nn > 1e-300 && nn < 1e300 ? 1.0 / sqrt(nn) : 0.0;
-        for (int f = 0; f < kD; ++f) q[a][f] = z[a][f] = z[a][f] * inv;
-      }
+#pragma unroll
+    for (int a = 0; a < kBoxAxes; ++a) {
+#pragma unroll
+      for (int b = 0; b < a; ++b) z[a] -= sum16(z[a] * z[b]) * z[b];
+      const double nn = sum16(z[a] * z[a]);
+      z[a] *= nn > 1e-300 && nn < 1e300 ? 1.0 / sqrt(nn) : 0.0;
     }
-    __syncthreads();
+#pragma unroll
+    for (int a = 0; a < kBoxAxes; ++a) q[a] = z[a];
   }
-  if (t < kBoxAxes * kD) frame[t] = float(q[t / kD][t % kD]);
+  if (t < kD)
+#pragma unroll
+    for (int a = 0; a < kBoxAxes; ++a) frame[a * kD + f] = float(q[a]);
 }
 
 // Boxes of consecutive groups of g points (tile pairs: 32; row blocks: 64):
