@@ -247,6 +247,36 @@ def test_fp16_far_blocks_at_the_threshold_are_bitwise_dense(gpu, gap):
     assert torch.equal(p1, p2) and torch.equal(w1, w2)
 
 
+@pytest.mark.parametrize("shape", ["identical", "line", "plane"])
+def test_skipping_on_rank_deficient_data(gpu, shape):
+    """Data of rank 0, 1 and 2 leave directions of the far-block frame without
+    variance (they come out zero, or as any unit vector orthogonal to the
+    others; the bound holds for every orthonormal frame): the FP16 score
+    still ends bitwise as the dense kernel, and the 16-bit log-density within
+    the dense kernel's rounding."""
+    rng = np.random.default_rng(21)
+    n = 20000
+    base = rng.normal(size=16).astype(np.float32)
+    if shape == "identical":
+        x = np.tile(base, (n, 1))
+    else:
+        k = 1 if shape == "line" else 2
+        dirs = rng.normal(size=(k, 16))
+        x = (base + (rng.normal(size=(n, k)) * 2.0) @ dirs).astype(np.float32)
+    x = torch.as_tensor(x, device=gpu)
+    h = 1.0
+    p1, w1 = _ops.score16(x, h, variant="ordered", precision="fp16")
+    p2, w2 = _ops.score16(x, h, variant="ordered", precision="fp16", dense=True)
+    assert torch.equal(p1, p2) and torch.equal(w1, w2)
+    q = (x[::5] + 0.01).contiguous()
+    ld = _ops.kde16_logdensity(x, q, h, precision="fp16x3", dense=True)
+    ls = _ops.kde16_logdensity(x, q, h, precision="fp16x3")
+    assert bool(torch.isfinite(ls).all())
+    diff = float((ls - ld).abs().max())
+    record(f"rank_deficient_skip_{shape}", logdensity_vs_dense=diff)
+    assert diff <= 2e-6
+
+
 @pytest.mark.parametrize("prec", ["fp16x3s", "bf16x3s"])
 def test_one_phi_score_skipping_bounds(gpu, prec):
     """F16X3S / BF16X3S: phi.X takes the FP16 phi (left-out tiles add exact
