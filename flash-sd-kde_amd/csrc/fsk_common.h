@@ -149,6 +149,7 @@ void prof_end(int op, hipStream_t s);
 // Workspace sizes of the individual operations (defined next to each op).
 size_t kde16_ws_bytes(int64_t n, int64_t m, bool with_partials);
 size_t kde16_lse_ws_bytes(int64_t n, int64_t m, unsigned flags);
+size_t spatial_ws_bytes();
 size_t score16_ws_bytes(int64_t n, bool ordered);
 size_t score16_lp_ws_bytes(int64_t n, int64_t m, unsigned prec);
 size_t score16_rows_ws_bytes(int64_t n, int64_t m);

@@ -207,6 +207,7 @@ extern "C" size_t fsk_workspace_bytes(int op, int64_t n, int64_t m, unsigned fla
       return (flags & FSK_PREC_MASK) == FSK_PREC_F32
                  ? score16_ws_bytes(n, true)
                  : score16_lp_ws_bytes(n, n, flags & FSK_PREC_MASK);
+    case FSK_OP_SPATIAL_CODES16: return spatial_ws_bytes();
     default: return 0;
   }
 }

@@ -1,6 +1,8 @@
 // fsk_far.hip - far-block bitmap pre-pass (frame, boxes, bits); the bound
 // and the layout are described in fsk_far.h.
 #include "fsk_far.h"
+#include "fsk_lowp.h"
+#include "fsk_plan.h"
 
 namespace fsk {
 
@@ -17,10 +19,12 @@ __device__ __forceinline__ double sum16(double v) {
 // wave: lane f (of each 16-lane group) holds coordinate f of every iterate,
 // and the dot products are 16-lane sums. frame[a * 16 + f]; a direction that
 // vanishes (rank-deficient data) is left zero, which keeps the rows
-// orthonormal or zero.
+// orthonormal or zero. sd (nullable): the sample's standard deviation along
+// each row, sqrt(q^T C q / count).
 __global__ __launch_bounds__(256) void frame_kernel(const float *__restrict__ x, int64_t ld,
                                                     int64_t n, const double *__restrict__ mu_sum,
-                                                    double inv_n, float *__restrict__ frame) {
+                                                    double inv_n, float *__restrict__ frame,
+                                                    float *__restrict__ sd) {
   __shared__ float pts[256 * kD];
   __shared__ double cov[kD * kD];
   const int t = threadIdx.x;
@@ -70,6 +74,16 @@ __global__ __launch_bounds__(256) void frame_kernel(const float *__restrict__ x,
   if (t < kD)
 #pragma unroll
     for (int a = 0; a < kBoxAxes; ++a) frame[a * kD + f] = float(q[a]);
+  if (sd) {
+#pragma unroll
+    for (int a = 0; a < kBoxAxes; ++a) {
+      double v = 0.0;
+#pragma unroll
+      for (int g = 0; g < kD; ++g) v += crow[g] * __shfl(q[a], g, 16);
+      const double lam = sum16(q[a] * v) / double(cnt);
+      if (t == 0) sd[a] = float(sqrt(lam > 0.0 ? lam : 0.0));
+    }
+  }
 }
 
 // Boxes of consecutive groups of g points (tile pairs: 32; row blocks: 64):
@@ -165,7 +179,8 @@ void launch_far_bits(const float *data, int64_t ld, int64_t n, const float *rows
                      unsigned long long *bits, hipStream_t s) {
   const int64_t row_blocks = groups * waves;
   const int64_t words = far_words(npairs);
-  hipLaunchKernelGGL(frame_kernel, dim3(1), dim3(256), 0, s, data, ld, n, mu_sum, inv_n, frame);
+  hipLaunchKernelGGL(frame_kernel, dim3(1), dim3(256), 0, s, data, ld, n, mu_sum, inv_n, frame,
+                     (float *)nullptr);
   hipLaunchKernelGGL(box_kernel, dim3(cdiv(npairs, 256)), dim3(256), 0, s, data, ld, n, npairs,
                      kFarPairPts, mu_sum, inv_n, c, frame, 0.f, (const float *)nullptr, 0.f,
                      pair_boxes);
@@ -176,4 +191,71 @@ void launch_far_bits(const float *data, int64_t ld, int64_t n, const float *rows
                      row_boxes, pair_boxes, npairs, words, groups, waves, float(c), bits);
 }
 
+// Morton code of each point: its coordinates along the frame's first three
+// rows (the leading axis first), each quantized to `bits` bits over the mean
+// +- 4 sd along the row (points beyond fall into the end cells); bit b of the
+// coordinate along row r goes to code bit 3b + (2 - r). A row whose sd is
+// below 1e-6 of the leading one (rank-deficient data: its coordinates are
+// rounding noise) contributes zeros.
+__global__ void spatial_codes_kernel(const float *__restrict__ x, int64_t ld, int64_t n,
+                                     const double *__restrict__ mu_sum, double inv_n,
+                                     const float *__restrict__ frame, const float *__restrict__ sd,
+                                     int bits, unsigned *__restrict__ codes) {
+  const int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  float y[kD];
+#pragma unroll
+  for (int f = 0; f < kD; ++f) y[f] = float(double(x[j * ld + f]) - mu_sum[f] * inv_n);
+  const float scale = float((1 << bits) - 1);
+  unsigned code = 0;
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    float p = 0.f;
+#pragma unroll
+    for (int f = 0; f < kD; ++f) p += frame[r * kD + f] * y[f];
+    const float half = fmaxf(4.f * sd[r], 1e-30f);
+    const unsigned c = sd[r] > 1e-6f * sd[0]
+                           ? unsigned(fminf(fmaxf((p + half) * (scale / (2.f * half)), 0.f), scale))
+                           : 0u;
+    for (int b = 0; b < bits; ++b) code |= ((c >> b) & 1u) << (3 * b + 2 - r);
+  }
+  codes[j] = code;
+}
+
+size_t spatial_ws_bytes() {
+  Carve cv(nullptr);
+  cv.take<double>(kMeanWs);
+  cv.take<float>(kBoxAxes * kD);
+  cv.take<float>(kBoxAxes);
+  return cv.used;
+}
+
 }  // namespace fsk
+
+extern "C" int fsk_spatial_codes16(const float *x, int64_t ld, int64_t n, int bits,
+                                   uint32_t *codes, void *workspace, size_t workspace_bytes,
+                                   void *stream) {
+  using namespace fsk;
+  if (!x || !codes || n <= 0 || ld < kD || bits < 1 || bits > 10) {
+    set_error("fsk_spatial_codes16: bad arguments (n=%lld ld=%lld bits=%d)", (long long)n,
+              (long long)ld, bits);
+    return FSK_EINVAL;
+  }
+  const size_t need = spatial_ws_bytes();
+  if (!workspace || workspace_bytes < need) {
+    set_error("fsk_spatial_codes16: workspace of %zu bytes needed, got %zu", need,
+              workspace_bytes);
+    return FSK_EWORKSPACE;
+  }
+  Carve cv(workspace);
+  double *mu = cv.take<double>(kMeanWs);
+  float *frame = cv.take<float>(kBoxAxes * kD);
+  float *sd = cv.take<float>(kBoxAxes);
+  hipStream_t s = as_stream(stream);
+  launch_mean16(x, ld, n, mu, s);
+  hipLaunchKernelGGL(frame_kernel, dim3(1), dim3(256), 0, s, x, ld, n, mu, 1.0 / double(n), frame,
+                     sd);
+  hipLaunchKernelGGL(spatial_codes_kernel, dim3(cdiv(n, 256)), dim3(256), 0, s, x, ld, n, mu,
+                     1.0 / double(n), frame, sd, bits, codes);
+  return launch_status("fsk_spatial_codes16");
+}

@@ -50,6 +50,7 @@ OP_LOO_GRID = 9
 OP_SCORE16_ROWS = 10
 OP_KDE_ND_SPLITK = 11
 OP_SCORE_ND_ORDERED = 12
+OP_SPATIAL_CODES16 = 13
 ND_MAX_DIM = 64
 
 _p = ctypes.c_void_p
@@ -77,6 +78,7 @@ SIGNATURES = {
     "fsk_score16_ordered": [_p, _i64, _i64, _d, _u, _p, _sz, _p, _p, _i64, _i64, _p],
     "fsk_score16_rows": [_p, _i64, _i64, _p, _i64, _i64, _d, _u, _p, _sz, _p, _p, _i64, _i64, _p],
     "fsk_reduce_partials_f64": [_p, _i64, _i64, _i64, _p, _p],
+    "fsk_spatial_codes16": [_p, _i64, _i64, ctypes.c_int, _p, _p, _sz, _p],
     "fsk_reduce_weighted_partials_f64": [_p, _i64, _i64, _i64, _p, _i64, _i64, _p],
     "fsk_sdkde_debias_f64": [_p, _i64, _i64, _p, _p, _i64, _i64, _d, _d, _p, _i64, _p, _p],
     "fsk_ref64_workspace_bytes": [_i64, _i64],

@@ -285,6 +285,24 @@ def score16_rows(rows: torch.Tensor, train: torch.Tensor, bandwidth: float, *,
     return pdf, w
 
 
+def spatial_codes16(x: torch.Tensor, bits: int = 10) -> torch.Tensor:
+    """Morton codes (int32, n) of the rows of a float32 (n, 16) CUDA tensor
+    along their three leading principal axes (fsk_spatial_codes16): no host
+    synchronization."""
+    n = x.shape[0]
+    if x.dim() != 2 or x.shape[1] != ND or x.dtype != torch.float32 or not x.is_cuda:
+        raise ValueError("spatial_codes16 takes a float32 (n, 16) CUDA tensor")
+    if x.stride(1) != 1 or x.stride(0) < ND:
+        x = x.contiguous()
+    dev = x.device
+    codes = torch.empty(n, dtype=torch.int32, device=dev)
+    ws, nbytes = nat.workspace(nat.OP_SPATIAL_CODES16, n, 0, 0, dev)
+    st = nat.lib().fsk_spatial_codes16(x.data_ptr(), x.stride(0), n, bits, codes.data_ptr(),
+                                       ws.data_ptr(), nbytes, nat.stream_handle(dev))
+    nat.check(st, "fsk_spatial_codes16")
+    return codes
+
+
 def debias(train: torch.Tensor, pdf: torch.Tensor, weighted: torch.Tensor, bandwidth: float,
            eps: float, *, return_score: bool = False):
     """x + (h^2/2) * ((w/(pdf+eps) - x)/h^2), evaluated in FP64 on the GPU."""

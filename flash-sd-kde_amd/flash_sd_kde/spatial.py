@@ -41,13 +41,21 @@ def spatial_order(x: torch.Tensor, bits: int = 10) -> torch.Tensor:
     """Permutation of the rows of x (n, d) that sorts them by the Morton code
     of their coordinates along the three leading principal axes (`bits` bits
     per axis, at most 10, over the mean +- 4 standard deviations along each;
-    bit b of axis a goes to code bit 3b + a). The axes come from the
-    covariance of a strided sample of at most ~65k rows."""
+    bit b of axis a goes to code bit 3b + a; an axis whose standard deviation
+    is below 1e-6 of the leading one, rounding noise in rank-deficient data,
+    contributes zeros). The axes come from the
+    covariance of a strided sample of at most ~65k rows. A float32 (n, 16)
+    CUDA tensor takes the library's codes instead (fsk_spatial_codes16: the
+    axes from a 4096-point sample by power iteration on the GPU, no host
+    synchronization), then one argsort."""
     n = x.shape[0]
     if n < 3:
         return torch.arange(n, device=x.device)
     if not 1 <= bits <= 10:
         raise ValueError(f"bits must be in [1, 10], got {bits}")
+    if x.is_cuda and x.dim() == 2 and x.shape[1] == 16 and x.dtype == torch.float32:
+        from flash_sd_kde import _ops
+        return torch.argsort(_ops.spatial_codes16(x, bits), stable=True)
     xf = x.float()
     # the frame from a strided sample of at most ~65k rows (it only orders)
     ys = xf[::max(1, n // 65536)]
@@ -65,6 +73,8 @@ def spatial_order(x: torch.Tensor, bits: int = 10) -> torch.Tensor:
     p = (xf - mu) @ axes
     scale = (1 << bits) - 1
     q = ((p + half) * (scale / (2.0 * half))).clamp_(0, scale).to(torch.int32)
+    flat = torch.as_tensor(sd <= 1e-6 * sd.max(), device=x.device)
+    q[:, flat] = 0
     lut = _spread_table(x.device)
     code = lut[q[:, 0]]
     for a in range(1, k):

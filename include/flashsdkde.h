@@ -91,6 +91,7 @@ extern "C" {
 #define FSK_OP_SCORE16_ROWS 10
 #define FSK_OP_KDE_ND_SPLITK 11    /* workspace: fsk_nd_workspace_bytes */
 #define FSK_OP_SCORE_ND_ORDERED 12 /* workspace: fsk_nd_workspace_bytes */
+#define FSK_OP_SPATIAL_CODES16 13
 
 const char *fsk_last_error(void);
 int fsk_version(void);
@@ -303,6 +304,19 @@ int fsk_score16_ordered(const float *data, int64_t ld, int64_t n,
 int fsk_score16_rows(const float *rows, int64_t ldr, int64_t m, const float *data, int64_t ld,
                      int64_t n, double bandwidth, unsigned flags, void *ws, size_t ws_bytes,
                      float *pdf, float *w, int64_t ldw_row, int64_t ldw_col, void *stream);
+
+/* Morton (Z-order) codes of n 16-D points (no counterpart in the reference;
+ * it puts data and queries in spatial order for the kernels above that leave
+ * out negligible terms, without a host synchronization): the coordinates
+ * along the points' three leading principal axes (block power iteration on
+ * the FP64 covariance of a strided sample of at most 4096 centred points),
+ * each quantized to `bits` bits (1..10) over the mean +- 4 standard
+ * deviations along the axis (points beyond fall into the end cells); bit b
+ * of the coordinate along the k-th leading axis goes to code bit 3b + 2 - k.
+ * codes: n values (DEVICE). Workspace: fsk_workspace_bytes(
+ * FSK_OP_SPATIAL_CODES16, n, 0, 0). */
+int fsk_spatial_codes16(const float *x, int64_t ld, int64_t n, int bits, uint32_t *codes,
+                        void *workspace, size_t workspace_bytes, void *stream);
 
 /* FP64 split-K combine: out[i] = (float) sum_s (double) partials[s*ld_split + i].
  * Replaces reduce_pdf_partials, kernels/reduce_partials.py:61-79. */

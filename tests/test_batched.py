@@ -55,6 +55,39 @@ def test_spatial_order_is_a_permutation_that_keeps_clusters_together():
 
 
 @pytest.mark.gpu
+def test_spatial_order_on_the_gpu(gpu):
+    """The library's Morton codes (fsk_spatial_codes16, used for float32
+    (n, 16) CUDA tensors): a permutation that keeps two clusters apart, and on
+    points along one line the order follows the line up to one cell of the
+    10-bit grid over the mean +- 4 standard deviations (either direction: the
+    sign of an axis is arbitrary)."""
+    from flash_sd_kde.spatial import spatial_order
+
+    rng = np.random.default_rng(4)
+    a = rng.normal(size=(3000, 16)) + 40.0
+    b = rng.normal(size=(2000, 16)) - 40.0
+    x = np.concatenate([a, b])[rng.permutation(5000)]
+    perm = spatial_order(torch.as_tensor(x, dtype=torch.float32, device=gpu)).cpu().numpy()
+    assert np.array_equal(np.sort(perm), np.arange(5000))
+    side = x[perm, 0] > 0
+    assert np.count_nonzero(side[1:] != side[:-1]) == 1
+
+    u = rng.normal(size=16)
+    u /= np.linalg.norm(u)
+    t = rng.normal(size=20000) * 3.0
+    line = (t[:, None] * u + 0.5).astype(np.float32)
+    perm = spatial_order(torch.as_tensor(line, device=gpu)).cpu().numpy()
+    assert np.array_equal(np.sort(perm), np.arange(20000))
+    ts = t[perm] if t[perm][0] < t[perm][-1] else -t[perm]
+    ts = ts[np.abs(ts - t.mean()) < 3.5 * t.std()]  # past 4 sd: the end cells
+    cell = 8.0 * t.std() / 1023
+    assert np.all(np.diff(ts) >= -1.5 * cell)
+    with pytest.raises(ValueError):
+        from flash_sd_kde import _ops
+        _ops.spatial_codes16(torch.as_tensor(line, device=gpu), bits=11)
+
+
+@pytest.mark.gpu
 def test_batched_spatial_order_changes_only_rounding(gpu):
     """The spatial order of the points only changes the summation order and
     which negligible (< 2^-(ceil(log2 n) + 28) relative) terms are skipped: the log-likelihoods
