@@ -6,11 +6,24 @@
 
 namespace fsk {
 
-// Sum over the 16 lanes of a lane group (every lane gets the total).
+// A double from lane perm(l) of the same 16-lane row, by DPP on both halves
+// (CTRL: 0xB1 / 0x4E swap lanes 1 / 2 apart in each quad, 0x141 mirrors each
+// half-row, 0x140 the row).
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_mov_dpp(int(b), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp(int(b >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double((long long)(unsigned)lo | ((long long)hi << 32));
+}
+
+// Sum over the 16 lanes of a lane group (every lane gets the same total:
+// each step adds two values that their partner lane adds in the other order).
 __device__ __forceinline__ double sum16(double v) {
-#pragma unroll
-  for (int o = 8; o >= 1; o >>= 1) v += __shfl_xor(v, o, 16);
-  return v;
+  v += dpp_f64<0xB1>(v);
+  v += dpp_f64<0x4E>(v);
+  v += dpp_f64<0x141>(v);
+  return v + dpp_f64<0x140>(v);
 }
 
 // One workgroup of 256: the covariance of a strided sample of the centred
